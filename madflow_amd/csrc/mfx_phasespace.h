@@ -1,0 +1,91 @@
+// RAMBO phase space as per-lane FP64 device code.
+//   python_package/madflow/phasespace.py:108-212 (Kleiss, Stirling, Ellis,
+//   Comput. Phys. Commun. 40 (1986) 359): isotropic massless q_i from 4 uniforms each,
+//   conformal boost + scale to sum p = (sqrts, 0), flat weight; massive particles by
+//   rescaling the 3-momenta with xi solving sum_i sqrt(m_i^2 + xi^2 E_i^2) = sqrts.
+// Unlike the reference's batch-global tf.while_loop (the batch stops when one event
+// converges, :90-103), every lane runs its own Newton iteration to convergence and
+// the energies are recomputed from the final xi.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfx {
+
+constexpr int RAMBO_MAXN = 8;
+constexpr double MFX_PI = 3.14159265358979323846;
+
+struct RamboConst {
+  int n;
+  int massive;
+  double masses[RAMBO_MAXN];
+  double total_mass;
+  double logw0;   // log(pi/2)(n-1) - 2 lgamma(n-1) - log(n-1)
+  double inorm;   // 1 / (2 pi)^(3n-4)
+};
+
+// x: 4n uniforms (cos theta, phi, and two for the energy) per particle; returns weight.
+__device__ __forceinline__ double rambo_event(const RamboConst& rc, const double* x, double sqrts,
+                                              double (*p)[4]) {
+  const int n = rc.n;
+  double Q[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    const double costh = 2.0 * x[4 * i] - 1.0;
+    const double sinth = sqrt(1.0 - costh * costh);
+    double sphi, cphi;
+    sincos(2.0 * MFX_PI * x[4 * i + 1], &sphi, &cphi);
+    const double e = -log(x[4 * i + 2] * x[4 * i + 3]);
+    p[i][0] = e;
+    p[i][1] = e * sinth * sphi;
+    p[i][2] = e * sinth * cphi;
+    p[i][3] = e * costh;
+    for (int k = 0; k < 4; ++k) Q[k] += p[i][k];
+  }
+  const double M = sqrt(Q[0] * Q[0] - (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]));
+  const double iM = 1.0 / M;
+  const double xs = sqrts * iM;
+  const double b1 = -Q[1] * iM, b2 = -Q[2] * iM, b3 = -Q[3] * iM;
+  const double gamma = Q[0] * iM;
+  const double a = 1.0 / (1.0 + gamma);
+  for (int i = 0; i < n; ++i) {
+    const double bq = p[i][1] * b1 + p[i][2] * b2 + p[i][3] * b3;
+    const double t = bq * a + p[i][0];
+    p[i][0] = (p[i][0] * gamma + bq) * xs;
+    p[i][1] = (p[i][1] + b1 * t) * xs;
+    p[i][2] = (p[i][2] + b2 * t) * xs;
+    p[i][3] = (p[i][3] + b3 * t) * xs;
+  }
+  double logw = rc.logw0 + (2 * n - 4) * log(sqrts);
+  if (rc.massive) {
+    // Newton for xi, starting above the root (f(xi0) >= 0 by Minkowski's inequality)
+    double xi = sqrt(1.0 - (rc.total_mass / sqrts) * (rc.total_mass / sqrts));
+    for (int it = 0; it < 50; ++it) {
+      const double x2 = xi * xi;
+      double f0 = -sqrts, g0 = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double E2 = p[i][0] * p[i][0];
+        const double nE = sqrt(rc.masses[i] * rc.masses[i] + E2 * x2);
+        f0 += nE;
+        g0 += E2 / nE;
+      }
+      if (fabs(f0) <= 1e-14 * sqrts) break;
+      xi -= f0 / (xi * g0);
+    }
+    const double x2 = xi * xi;
+    double wt2 = 1.0, wt3 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double v = p[i][0] * xi;
+      const double nE = sqrt(rc.masses[i] * rc.masses[i] + p[i][0] * p[i][0] * x2);
+      wt2 *= v / nE;
+      wt3 += v * v / nE;
+      p[i][0] = nE;
+      p[i][1] *= xi;
+      p[i][2] *= xi;
+      p[i][3] *= xi;
+    }
+    logw += (2 * n - 3) * log(xi) + log(wt2 / wt3 * sqrts);
+  }
+  return exp(logw) * rc.inorm;
+}
+
+}  // namespace mfx

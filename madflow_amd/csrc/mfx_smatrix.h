@@ -1,0 +1,168 @@
+// Fused smatrix kernels: one launch evaluates, per event (one event per lane), every
+// helicity row, the whole diagram program, the jamp accumulation and the colour sum,
+// then writes |M|^2 averaged over helicities/colours — nothing intermediate touches
+// HBM.  This replaces both the TensorFlow graph of
+//   madgraph_plugin/template_files/matrix_method_python.inc:80-138  (smatrix/matrix)
+// and the generated custom op, which launched one kernel per helicity
+//   python_package/madflow/custom_op/constants.py:164-198, generation.py:96-105.
+#pragma once
+#include "mfx_helas.h"
+
+namespace mfx {
+
+struct SmatrixArgs {
+  const double* all_ps;  // (nev, nexternal, 4) f64, E px py pz
+  int64_t nev;
+  double mdl_MT, mdl_WT;
+  double gc[3][2];       // frozen couplings GC_10, GC_11, GC_12 (re, im)
+  const double* gc_ev;   // optional per-event couplings [3][nev][2] (running alpha_s)
+  double sqh;            // value of sqrt(1/2) used by vxxxxx
+  int helv[8];           // explicit helicity vector (matrix(hel)), when use_helv
+  int use_helv;
+  double* out;           // (nev,) f64
+  double* jamp_out;      // optional (nev, ncolor, 2) f64, only with use_helv
+};
+
+template <class R>
+__device__ __forceinline__ Params<R> load_params(const SmatrixArgs& a, int64_t e) {
+  Params<R> prm;
+  prm.mdl_MT = R(a.mdl_MT);
+  prm.mdl_WT = R(a.mdl_WT);
+  prm.sqh = R(a.sqh);
+  if (a.gc_ev) {
+    const double* g = a.gc_ev;
+    prm.GC_10 = cx<R>(R(g[2 * e]), R(g[2 * e + 1]));
+    prm.GC_11 = cx<R>(R(g[2 * (a.nev + e)]), R(g[2 * (a.nev + e) + 1]));
+    prm.GC_12 = cx<R>(R(g[2 * (2 * a.nev + e)]), R(g[2 * (2 * a.nev + e) + 1]));
+  } else {
+    prm.GC_10 = cx<R>(R(a.gc[0][0]), R(a.gc[0][1]));
+    prm.GC_11 = cx<R>(R(a.gc[1][0]), R(a.gc[1][1]));
+    prm.GC_12 = cx<R>(R(a.gc[2][0]), R(a.gc[2][1]));
+  }
+  return prm;
+}
+
+// sum_ij J_i cf_ij conj(J_j) / den with a real symmetric cf:
+//   sum_i cf_ii |J_i|^2 + 2 sum_{i<j} cf_ij Re(J_i conj(J_j))
+template <class T, class R>
+__device__ __forceinline__ R colour_sum(const cx<R> (&J)[T::NCOLOR]) {
+  constexpr int NC = T::NCOLOR;
+  R res = R(0);
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    R zr = T::cf(R(), i, i) * J[i].re;
+    R zi = T::cf(R(), i, i) * J[i].im;
+#pragma unroll
+    for (int j = i + 1; j < NC; ++j) {
+      const R c = R(2) * T::cf(R(), i, j);
+      zr += c * J[j].re;
+      zi += c * J[j].im;
+    }
+    res += zr * J[i].re + zi * J[i].im;
+  }
+  return res * R(1.0 / T::CF_DEN);
+}
+
+template <class T, class R>
+__device__ __forceinline__ void load_momenta(const double* __restrict__ all_ps, int64_t e, R (&p)[T::NEXTERNAL][4]) {
+  const double* src = all_ps + e * (T::NEXTERNAL * 4);
+#pragma unroll
+  for (int l = 0; l < T::NEXTERNAL; ++l) {
+    const double2 a = *reinterpret_cast<const double2*>(src + 4 * l);
+    const double2 b = *reinterpret_cast<const double2*>(src + 4 * l + 2);
+    p[l][0] = R(a.x);
+    p[l][1] = R(a.y);
+    p[l][2] = R(b.x);
+    p[l][3] = R(b.y);
+  }
+}
+
+// The momentum-only parts of the program (slot momenta, propagator denominators)
+// are the same for every helicity row; if the compiler hoists them out of the
+// helicity loop they all stay live across it and spill.  An empty asm that "may
+// modify" the momenta keeps them inside the loop at no instruction cost.
+template <class R> __device__ __forceinline__ void opaque(R& x) { asm volatile("" : "+v"(x)); }
+
+// Hot path: all helicity rows for nev events.
+template <class T, class R>
+__global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nev; e += stride) {
+    R p[T::NEXTERNAL][4];
+    load_momenta<T, R>(a.all_ps, e, p);
+    const Params<R> prm = load_params<R>(a, e);
+    R ans = R(0);
+#pragma unroll 1
+    for (int h = 0; h < T::NCOMB; ++h) {
+#pragma unroll
+      for (int l = 0; l < T::NEXTERNAL; ++l)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) opaque(p[l][k]);
+      cx<R> jamp[T::NCOLOR];
+      T::template amplitudes<R>(p, T::hel_bits(h), nullptr, prm, jamp);
+      ans += colour_sum<T, R>(jamp);
+    }
+    a.out[e] = double(ans) / T::DENOMINATOR;
+  }
+}
+
+// matrix(all_ps, hel, ...): one explicit helicity vector (may hold 4 = BRST check for a
+// gluon); optionally also returns the jamps.
+template <class T, class R>
+__global__ void __launch_bounds__(256) k_matrix_hel(SmatrixArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nev; e += stride) {
+    R p[T::NEXTERNAL][4];
+    load_momenta<T, R>(a.all_ps, e, p);
+    const Params<R> prm = load_params<R>(a, e);
+    cx<R> jamp[T::NCOLOR];
+    T::template amplitudes<R>(p, 0u, a.helv, prm, jamp);
+    a.out[e] = double(colour_sum<T, R>(jamp));
+    if (a.jamp_out) {
+#pragma unroll
+      for (int c = 0; c < T::NCOLOR; ++c) {
+        a.jamp_out[(e * T::NCOLOR + c) * 2] = double(jamp[c].re);
+        a.jamp_out[(e * T::NCOLOR + c) * 2 + 1] = double(jamp[c].im);
+      }
+    }
+  }
+}
+
+struct ProcessEntry {
+  const char* name;
+  int nexternal, ncomb, ncolor, ndiags, ngraphs;
+  double denominator;
+  hipError_t (*launch)(const SmatrixArgs& a, int precision, int per_helicity, hipStream_t s);
+};
+
+template <class T>
+hipError_t launch_process(const SmatrixArgs& a, int precision, int per_helicity, hipStream_t s) {
+  if (a.nev <= 0) return hipSuccess;
+  const int block = 256;
+  int64_t nblk = (a.nev + block - 1) / block;
+  if (nblk > (1 << 30)) nblk = 1 << 30;
+  dim3 grid((unsigned)nblk), blk(block);
+  if (precision == 32) {
+    if (per_helicity) hipLaunchKernelGGL((k_matrix_hel<T, float>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_smatrix<T, float>), grid, blk, 0, s, a);
+  } else {
+    if (per_helicity) hipLaunchKernelGGL((k_matrix_hel<T, double>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_smatrix<T, double>), grid, blk, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mfx
+
+namespace mfx {
+// Registry filled by the process translation units at library load time.
+int register_process(const ProcessEntry& e);
+}  // namespace mfx
+
+#define MFX_INSTANTIATE_PROCESS(NAME)                                                                   \
+  namespace mfx {                                                                                      \
+  static const int registered_##NAME = register_process(                                               \
+      ProcessEntry{NAME::Traits::PROC_NAME, NAME::Traits::NEXTERNAL, NAME::Traits::NCOMB,               \
+                   NAME::Traits::NCOLOR, NAME::Traits::NDIAGS, NAME::Traits::NGRAPHS,                   \
+                   NAME::Traits::DENOMINATOR, &launch_process<NAME::Traits>});                          \
+  }

@@ -1,0 +1,152 @@
+"""Host side of the drop-in process modules (``madflow_amd/processes/matrix_1_<proc>.py``).
+
+``MatrixBase`` implements what the reference's generated class does in TensorFlow
+(``madgraph_plugin/template_files/matrix_method_python.inc:80-138``) by handing
+device buffers to the fused HIP kernel (``mfx_smatrix`` / ``mfx_matrix``):
+
+* ``smatrix(all_ps, *params)``: ``all_ps`` is a ``(nev, nexternal, 4)`` float64 tensor
+  on the GPU; ``params`` are ``mdl_MT, mdl_WT`` (floats) and the couplings (complex
+  scalars, or complex tensors of length 1 or nev as returned by ``Model.evaluate``).
+  Returns a float64 ``(nev,)`` tensor on the same device.
+* ``matrix(all_ps, hel, *params)``: one helicity row, colour-summed, not averaged.
+
+Precision is a runtime choice (``precision=64`` FP64 build, ``32`` FP32 build) of the
+same kernels; ``ref_compat=True`` uses the reference's float32-rounded sqrt(1/2)
+(``python_package/madflow/wavefunctions_flow.py:10``) so FP64 results match it to
+round-off.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .parameters import make_model
+
+_PROC_PARAMS = {
+    "gg_ttx": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
+    "gg_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
+    "gg_ttxgg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
+    "gg_ttxggg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
+}
+_GC_SLOT = {"GC_10": 0, "GC_11": 1, "GC_12": 2}
+
+
+def model_for(process_name, param_card_path=None):
+    return make_model(_PROC_PARAMS[process_name], param_card_path)
+
+
+def process_index(name):
+    lib = _lib.load()
+    idx = lib.mfx_process_index(name.encode())
+    if idx < 0:
+        raise _lib.MfxError("process %s is not compiled into %s" % (name, _lib.LIB_PATH))
+    return idx
+
+
+def _as_scalar(x, name):
+    if isinstance(x, torch.Tensor):
+        if x.numel() != 1:
+            raise ValueError("%s must be a scalar" % name)
+        return x.reshape(()).item()
+    return x
+
+
+class MatrixBase:
+    process_name = None
+    params = ()
+    precision = 64
+    ref_compat = True
+
+    def __init__(self, precision=None, ref_compat=None):
+        if precision is not None:
+            self.precision = int(precision)
+        if ref_compat is not None:
+            self.ref_compat = bool(ref_compat)
+        self._idx = None
+
+    @property
+    def index(self):
+        if self._idx is None:
+            self._idx = process_index(self.process_name)
+        return self._idx
+
+    # ---------------------------------------------------------------- helpers
+    def _prepare(self, all_ps, params):
+        all_ps = _lib.require_device_tensor(all_ps, torch.float64, "all_ps")
+        if all_ps.dim() != 3 or all_ps.shape[1] != self.nexternal or all_ps.shape[2] != 4:
+            raise ValueError("all_ps must have shape (nev, %d, 4), got %s" % (self.nexternal, tuple(all_ps.shape)))
+        nev = all_ps.shape[0]
+        if len(params) != len(self.params):
+            raise TypeError("%s.smatrix expects %d model parameters %s" % (
+                type(self).__name__, len(self.params), list(self.params)))
+        vals = dict(zip(self.params, params))
+        mw = np.array([float(_as_scalar(vals["mdl_MT"], "mdl_MT")), float(_as_scalar(vals["mdl_WT"], "mdl_WT"))])
+        gc = np.zeros(6)
+        per_event = None
+        for name, slot in _GC_SLOT.items():
+            if name not in vals:
+                continue
+            v = vals[name]
+            if isinstance(v, torch.Tensor) and v.numel() > 1:
+                if v.numel() != nev:
+                    raise ValueError("%s must have length 1 or nev=%d" % (name, nev))
+                if per_event is None:
+                    per_event = torch.zeros((3, nev, 2), dtype=torch.float64, device=all_ps.device)
+                vv = v.to(device=all_ps.device, dtype=torch.complex128).reshape(-1)
+                per_event[slot, :, 0] = vv.real
+                per_event[slot, :, 1] = vv.imag
+            else:
+                c = complex(_as_scalar(v, name))
+                gc[2 * slot] = c.real
+                gc[2 * slot + 1] = c.imag
+        if per_event is not None:
+            # broadcast frozen couplings into the per-event array too
+            for name, slot in _GC_SLOT.items():
+                if name in vals and not (isinstance(vals[name], torch.Tensor) and vals[name].numel() > 1):
+                    per_event[slot, :, 0] = gc[2 * slot]
+                    per_event[slot, :, 1] = gc[2 * slot + 1]
+        return all_ps, nev, mw, gc, per_event
+
+    def _smatrix(self, all_ps, params):
+        all_ps, nev, mw, gc, per_event = self._prepare(all_ps, params)
+        out = torch.empty(nev, dtype=torch.float64, device=all_ps.device)
+        if nev == 0:
+            return out
+        lib = _lib.load()
+        _lib.check(lib.mfx_smatrix(
+            self.index, ctypes.c_void_p(all_ps.data_ptr()), nev, mw.ctypes.data_as(ctypes.c_void_p),
+            gc.ctypes.data_as(ctypes.c_void_p),
+            ctypes.c_void_p(per_event.data_ptr()) if per_event is not None else None,
+            ctypes.c_void_p(out.data_ptr()), self.precision, int(self.ref_compat), _lib.stream_ptr(all_ps.device)))
+        return out
+
+    def _matrix(self, all_ps, hel, params, return_jamp=False):
+        all_ps, nev, mw, gc, per_event = self._prepare(all_ps, params)
+        if isinstance(hel, torch.Tensor):
+            hel = hel.detach().cpu().numpy()
+        helv = np.ascontiguousarray(np.asarray(hel, dtype=np.float64).round().astype(np.int32))
+        if helv.shape != (self.nexternal,):
+            raise ValueError("hel must have %d entries" % self.nexternal)
+        out = torch.empty(nev, dtype=torch.float64, device=all_ps.device)
+        jamp = torch.empty((nev, self.ncolor, 2), dtype=torch.float64, device=all_ps.device) if return_jamp else None
+        if nev > 0:
+            lib = _lib.load()
+            _lib.check(lib.mfx_matrix(
+                self.index, ctypes.c_void_p(all_ps.data_ptr()), nev, helv.ctypes.data_as(ctypes.c_void_p),
+                mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
+                ctypes.c_void_p(per_event.data_ptr()) if per_event is not None else None,
+                ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(jamp.data_ptr()) if jamp is not None else None,
+                self.precision, int(self.ref_compat), _lib.stream_ptr(all_ps.device)))
+        if return_jamp:
+            return out, torch.complex(jamp[..., 0], jamp[..., 1])
+        return out
+
+
+def load_matrix(process_name, **kw):
+    """Instantiate the drop-in class of a generated process module by name."""
+    import importlib
+    mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % process_name)
+    cls = getattr(mod, "Matrix_1_%s" % process_name)
+    return cls(**kw), mod

@@ -1,0 +1,79 @@
+"""Device-side phase-space helpers over the C ABI (Philox uniforms + RAMBO).
+
+``rambo(xrand, n, sqrts, masses)`` mirrors ``python_package/madflow/phasespace.py:145-212``
+(returns momenta (nev, n, 4) and weights (nev,)); ``sample_momenta`` builds full
+``all_ps`` tensors (incoming gluons along +-z at fixed partonic sqrt(s)) for the
+generated processes.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PROCESS_KIN = {
+    # name: (sqrt(s) of the benchmark configuration, outgoing masses)
+    "gg_ttx": (1000.0, [173.0, 173.0]),
+    "gg_ttxg": (2000.0, [173.0, 173.0, 0.0]),
+    "gg_ttxgg": (3000.0, [173.0, 173.0, 0.0, 0.0]),
+    "gg_ttxggg": (5000.0, [173.0, 173.0, 0.0, 0.0, 0.0]),
+}
+
+
+def philox_uniforms(nev, ndim, seed, iteration, ev0=0, device="cuda"):
+    out = torch.empty((nev, ndim), dtype=torch.float64, device=device)
+    if nev:
+        _lib.check(_lib.load().mfx_philox_uniforms(ev0, nev, ndim, seed, iteration, ctypes.c_void_p(out.data_ptr()),
+                                                   _lib.stream_ptr(out.device)))
+    return out
+
+
+def rambo(xrand, n_particles, sqrts, masses=None):
+    """RAMBO on device: xrand (nev, >= 4n) float64 GPU tensor -> (p (nev,n,4), wt (nev,))."""
+    xrand = _lib.require_device_tensor(xrand, torch.float64, "xrand")
+    nev = xrand.shape[0]
+    dev = xrand.device
+    if isinstance(sqrts, torch.Tensor) and sqrts.numel() > 1:
+        s = _lib.require_device_tensor(sqrts.reshape(-1), torch.float64, "sqrts")
+        sstride = 1
+    else:
+        s = torch.full((1,), float(sqrts), dtype=torch.float64, device=dev)
+        sstride = 0
+    m = None
+    if masses is not None and np.sum(masses) != 0:
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        if m.shape != (n_particles,):
+            raise ValueError("masses must have one entry per outgoing particle")
+        if isinstance(sqrts, (int, float)) and m.sum() > sqrts:
+            raise ValueError(f"Not enough energy ({sqrts}) to generate particles of mass: {list(masses)}")
+    p = torch.empty((nev, n_particles, 4), dtype=torch.float64, device=dev)
+    wt = torch.empty(nev, dtype=torch.float64, device=dev)
+    if nev:
+        _lib.check(_lib.load().mfx_rambo(
+            ctypes.c_void_p(xrand.data_ptr()), nev, xrand.shape[1], n_particles, ctypes.c_void_p(s.data_ptr()),
+            sstride, m.ctypes.data_as(ctypes.c_void_p) if m is not None else None, ctypes.c_void_p(p.data_ptr()),
+            ctypes.c_void_p(wt.data_ptr()), _lib.stream_ptr(dev)))
+    return p, wt
+
+
+def with_incoming(p_out, sqrts):
+    """Prepend the two incoming massless partons along +z / -z (phasespace.py:309-313)."""
+    nev = p_out.shape[0]
+    e = float(sqrts) / 2.0
+    pin = torch.zeros((nev, 2, 4), dtype=torch.float64, device=p_out.device)
+    pin[:, 0, 0] = e
+    pin[:, 0, 3] = e
+    pin[:, 1, 0] = e
+    pin[:, 1, 3] = -e
+    return torch.cat([pin, p_out], dim=1).contiguous()
+
+
+def sample_momenta(name, nev, seed=1, iteration=0, device="cuda"):
+    """All-particle momenta for ``name`` at its benchmark sqrt(s) from Philox + RAMBO."""
+    sqrts, masses = PROCESS_KIN[name]
+    n = len(masses)
+    x = philox_uniforms(nev, 4 * n, seed, iteration, device=device)
+    p, _ = rambo(x, n, sqrts, masses)
+    return with_incoming(p, sqrts)

@@ -1,0 +1,32 @@
+"""ORACLE — test infrastructure only.
+
+Writes madflow_amd/processes/flops_per_me.json: the exact number of real floating-point
+operations (add/sub, mul, div, sqrt; an FMA counts 2) that the oracle executes for one
+smatrix evaluation of each process (all helicities, all diagrams, jamps, colour sum).
+This is the algorithmic work per matrix element used for the roofline figures.
+Run: python -m oracle.count_flops
+"""
+
+import json
+import os
+
+from oracle import oracle as O
+
+OUT = os.path.join(os.path.dirname(O.HERE), "madflow_amd", "processes", "flops_per_me.json")
+
+
+def main():
+    res = {}
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"):
+        try:
+            O.load_process(name)
+        except FileNotFoundError:
+            continue
+        res[name] = O.flops_per_me(name)
+        print(name, res[name])
+    with open(OUT, "w") as f:
+        json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,242 @@
+"""Independent brute-force |M|^2 for g g > t t~ + n g (test infrastructure, numpy).
+
+Shares nothing with the generator or the oracle's ALOHA code except the external
+wavefunctions (pinned separately by the Dirac-equation / polarisation tests):
+
+* colour is explicit: Gell-Mann generators T = lambda/2, f^{abc} = -2i Tr([T^a,T^b]T^c),
+  every external colour index is a tensor axis, and |M|^2 is summed over all of them;
+* Lorentz algebra is explicit: 4x4 Dirac matrices in the chiral (HELAS) basis;
+* the amplitude is built by colour-dressed Berends-Giele recursion over subsets of the
+  gluons (all Feynman diagrams at once, no diagram list, no colour basis, no jamps);
+* Feynman rules are the SM UFO ones (vertex = coupling x colour x Lorentz):
+  q-q-g  GC_11 T^a gamma^mu;  g-g-g  GC_10 f^{abc} VVV1;  g-g-g-g  GC_12 sum_k c_k VVVV_k;
+  propagators -i g_{mu nu}/p^2 and i (p-slash + m)/(p^2 - m^2 + i m Gamma).
+  Momenta of the vertex legs are the outgoing momenta of the attached subsets (the
+  slot convention of the HELAS wavefunctions).
+"""
+
+import itertools
+
+import numpy as np
+
+from oracle import oracle as O
+
+# --------------------------------------------------------------------- colour
+_L = np.zeros((8, 3, 3), dtype=complex)
+_L[0][0, 1] = _L[0][1, 0] = 1
+_L[1][0, 1] = -1j
+_L[1][1, 0] = 1j
+_L[2][0, 0] = 1
+_L[2][1, 1] = -1
+_L[3][0, 2] = _L[3][2, 0] = 1
+_L[4][0, 2] = -1j
+_L[4][2, 0] = 1j
+_L[5][1, 2] = _L[5][2, 1] = 1
+_L[6][1, 2] = -1j
+_L[6][2, 1] = 1j
+_L[7][0, 0] = _L[7][1, 1] = 1 / np.sqrt(3)
+_L[7][2, 2] = -2 / np.sqrt(3)
+T = _L / 2
+F = np.zeros((8, 8, 8))
+for a, b, c in itertools.product(range(8), repeat=3):
+    comm = T[a] @ T[b] - T[b] @ T[a]
+    F[a, b, c] = (-2j * np.trace(comm @ T[c])).real
+
+# f(-1,1,2) f(3,4,-1), f(-1,1,3) f(2,4,-1), f(-1,1,4) f(2,3,-1) with legs (1,2,3,4)
+C4 = [np.einsum("eab,cde->abcd", F, F),
+      np.einsum("eac,bde->abcd", F, F),
+      np.einsum("ead,bce->abcd", F, F)]
+
+# ---------------------------------------------------------------------- Dirac
+_s = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]), np.array([[1, 0], [0, -1]], complex)]
+_I2 = np.eye(2)
+_Z2 = np.zeros((2, 2))
+G = [np.block([[_Z2, _I2], [_I2, _Z2]])] + [np.block([[_Z2, s], [-s, _Z2]]) for s in _s]
+METRIC = np.diag([1.0, -1.0, -1.0, -1.0])
+
+
+def slash(v):
+    """gamma^mu v_mu for an upper-index 4-vector v (complex allowed)."""
+    vl = METRIC @ v
+    return sum(G[m] * vl[m] for m in range(4))
+
+
+def mdot(a, b):
+    return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3]
+
+
+# ------------------------------------------------------------------ recursion
+def amplitude_tensor(p, hel, mt, wt, gs, sqh):
+    """Full colour tensor M[a_g1..a_gn, i_t, j_tbar] for one event and helicity row.
+
+    p: (nexternal, 4) momenta (legs 0,1 incoming gluons, 2 top, 3 anti-top, 4.. gluons)."""
+    n = p.shape[0]
+    gl = [0, 1] + list(range(4, n))
+    ng = len(gl)
+    GC10, GC11, GC12 = -gs, 1j * gs, 1j * gs * gs
+    ref = sqh != np.sqrt(0.5)
+    eps, K = {}, {}
+    for i, leg in enumerate(gl):
+        w = O.wavefunction("v", p[leg:leg + 1], 0.0, hel[leg], -1 if leg < 2 else 1, ref_compat=ref)[0]
+        eps[i] = w[2:6]
+        K[i] = (-1 if leg < 2 else 1) * p[leg]
+    ubar = O.wavefunction("o", p[2:3], mt, hel[2], 1)[0][2:6]    # row spinor of the outgoing top
+    vcol = O.wavefunction("i", p[3:4], mt, hel[3], -1)[0][2:6]   # column spinor of the outgoing anti-top
+    pt = p[2]
+
+    # gluon currents J[S]: array (4, 8, 8**|S|); colour axes of S in increasing order
+    J = {}
+    for i in range(ng):
+        arr = np.zeros((4, 8, 8), complex)
+        for a in range(8):
+            arr[:, a, a] = eps[i]
+        J[(i,)] = arr
+
+    def KS(S):
+        return sum(K[i] for i in S)
+
+    def merge(parts, arrs):
+        """Combine the colour axes of several children into the sorted order of their union."""
+        S = tuple(sorted(sum(parts, ())))
+        return S
+
+    def reorder(tensor, child_sets, S):
+        # tensor has axes (..., sets concatenated in child order); reorder to sorted S
+        order = sum(child_sets, ())
+        perm = [order.index(x) for x in S]
+        lead = tensor.ndim - len(order)
+        return np.transpose(tensor, list(range(lead)) + [lead + q for q in perm])
+
+    for size in range(2, ng + 1):
+        for S in itertools.combinations(range(ng), size):
+            KSv = KS(S)
+            acc = np.zeros((4, 8) + (8,) * size, complex)
+            first = S[0]
+            rest = S[1:]
+            # two children: S1 contains the first element
+            for r in range(0, len(rest)):
+                for comb in itertools.combinations(rest, r):
+                    S1 = (first,) + comb
+                    S2 = tuple(x for x in rest if x not in comb)
+                    A = J[S1].reshape((4, 8) + (8,) * len(S1))
+                    B = J[S2].reshape((4, 8) + (8,) * len(S2))
+                    P1 = -KSv
+                    P2 = KS(S1)
+                    P3 = KS(S2)
+                    # L^mu = e2^mu (P1.e3) - e2^mu (P2.e3) - e3^mu (P1.e2) + e3^mu (P3.e2) + (e2.e3)(P2-P3)^mu
+                    Am = np.tensordot(METRIC, A, axes=(1, 0))
+                    Bm = np.tensordot(METRIC, B, axes=(1, 0))
+                    e3P = np.tensordot(P1 - P2, Bm, axes=(0, 0))       # (8, cols2)
+                    e2P = np.tensordot(P3 - P1, Am, axes=(0, 0))       # (8, cols1)
+                    d23 = np.tensordot(A, Bm, axes=(0, 0))             # (8,cols1,8,cols2)
+                    n1, n2 = len(S1), len(S2)
+                    ax1 = "".join("ABCDEFG"[q] for q in range(n1))
+                    ax2 = "".join("HIJKLMN"[q] for q in range(n2))
+                    L = (np.einsum("mb%s,c%s->mb%sc%s" % (ax1, ax2, ax1, ax2), A, e3P)
+                         + np.einsum("mc%s,b%s->mb%sc%s" % (ax2, ax1, ax1, ax2), B, e2P)
+                         + np.einsum("b%sc%s,m->mb%sc%s" % (ax1, ax2, ax1, ax2), d23, P2 - P3))
+                    term = GC10 * np.einsum("abc,mb%sc%s->ma%s%s" % (ax1, ax2, ax1, ax2), F, L)
+                    acc += reorder(term, [S1, S2], S)
+            # three children (four-gluon vertex)
+            if size >= 3:
+                for part in _partitions3(S):
+                    S1, S2, S3 = part
+                    A = J[S1].reshape((4, 8) + (8,) * len(S1))
+                    B = J[S2].reshape((4, 8) + (8,) * len(S2))
+                    Cc = J[S3].reshape((4, 8) + (8,) * len(S3))
+                    n1, n2, n3 = len(S1), len(S2), len(S3)
+                    a1 = "".join("ABCDEFG"[q] for q in range(n1))
+                    a2 = "".join("HIJKLMN"[q] for q in range(n2))
+                    a3 = "OPQRSTU"[:n3]
+                    Bm = np.tensordot(METRIC, B, axes=(1, 0))
+                    Cm = np.tensordot(METRIC, Cc, axes=(1, 0))
+                    d23 = np.tensordot(A, Bm, axes=(0, 0))    # (b,a1,c,a2)
+                    d24 = np.tensordot(A, Cm, axes=(0, 0))    # (b,a1,d,a3)
+                    d34 = np.tensordot(B, Cm, axes=(0, 0))    # (c,a2,d,a3)
+                    out = "mb%sc%sd%s" % (a1, a2, a3)
+                    t_c_d23 = np.einsum("md%s,b%sc%s->%s" % (a3, a1, a2, out), Cc, d23)   # e4 (e2.e3)
+                    t_b_d24 = np.einsum("mc%s,b%sd%s->%s" % (a2, a1, a3, out), B, d24)    # e3 (e2.e4)
+                    t_a_d34 = np.einsum("mb%s,c%sd%s->%s" % (a1, a2, a3, out), A, d34)    # e2 (e3.e4)
+                    Ls = [t_c_d23 - t_b_d24,      # VVVV1: g14 g23 - g13 g24
+                          t_c_d23 - t_a_d34,      # VVVV3: g14 g23 - g12 g34
+                          t_b_d24 - t_a_d34]      # VVVV4: g13 g24 - g12 g34
+                    for k in range(3):
+                        term = GC12 * np.einsum("abcd,%s->ma%s%s%s" % (out, a1, a2, a3), C4[k], Ls[k])
+                        acc += reorder(term, [S1, S2, S3], S)
+            if size < ng or True:
+                k2 = mdot(KSv, KSv)
+                acc = acc * (-1j / k2)
+            J[S] = acc.reshape(4, 8, -1)
+
+    # quark line from the top: U[S] has axes (spinor 4, open colour 3, top colour 3, 8**|S|)
+    U = {(): np.einsum("b,kl->bkl", ubar, np.eye(3)).reshape(4, 3, 3, 1)}
+
+    def vertex_right(Urow, Jc, S0, Sc):
+        """Urow (4,3,3,cols0) x GC11 T^c gamma.J -> (4,3,3,cols0,colsc)."""
+        Jm = Jc.reshape(4, 8, -1)
+        # gamma^mu J_mu: (4x4) per colour/col
+        gJ = np.einsum("mxy,m...->xy...", np.array([G[m] * METRIC[m, m] for m in range(4)]), Jm)  # (4,4,8,colsc)
+        tmp = np.einsum("bkl...,bcaZ->ckla...Z", Urow, gJ) if False else None
+        out = np.einsum("bklA,byaZ->yklAaZ", Urow, gJ)            # (4,3,3,cols0,8,colsc)
+        out = np.einsum("yklAaZ,akm->ymlAZ", out, T)              # colour flows k -> m
+        return GC11 * out
+
+    for size in range(1, ng + 1):
+        for S in itertools.combinations(range(ng), size):
+            acc = None
+            for r in range(1, size + 1):
+                for Sl in itertools.combinations(S, r):    # last attached cluster
+                    S0 = tuple(x for x in S if x not in Sl)
+                    v = vertex_right(U[S0], J[Sl], S0, Sl)      # (4,3,3,cols0,colsl)
+                    v = v.reshape(4, 3, 3, *((8,) * (len(S0) + len(Sl))))
+                    v = reorder(v, [S0, Sl], S).reshape(4, 3, 3, -1)
+                    acc = v if acc is None else acc + v
+            if size < ng:
+                P = pt + KS(S)
+                prop = 1j * (slash(P.astype(complex)) + mt * np.eye(4)) / (mdot(P, P) - mt * mt + 1j * mt * wt)
+                acc = np.einsum("bklA,bc->cklA", acc, prop)
+                U[S] = acc
+            else:
+                full = acc
+    # the last vertex was already applied for size == ng; close with v(tbar)
+    # recompute closing: M = sum_{Sl} U[S0] GC11 T gamma.J v
+    M = None
+    Sall = tuple(range(ng))
+    for r in range(1, ng + 1):
+        for Sl in itertools.combinations(Sall, r):
+            S0 = tuple(x for x in Sall if x not in Sl)
+            v = vertex_right(U[S0], J[Sl], S0, Sl)
+            v = np.einsum("yklAZ,y->klAZ", v, vcol)
+            v = v.reshape(3, 3, *((8,) * ng))
+            v = reorder(v, [S0, Sl], Sall)
+            M = v if M is None else M + v
+    return M   # (i_open=tbar colour j, i_top, a_1..a_ng)
+
+
+def _partitions3(S):
+    S = list(S)
+    first = S[0]
+    rest = S[1:]
+    out = []
+    for r in range(0, len(rest)):
+        for comb in itertools.combinations(rest, r):
+            S1 = (first,) + comb
+            R = [x for x in rest if x not in comb]
+            if len(R) < 2:
+                continue
+            f2 = R[0]
+            rr = R[1:]
+            for r2 in range(0, len(rr)):
+                for comb2 in itertools.combinations(rr, r2):
+                    S2 = (f2,) + comb2
+                    S3 = tuple(x for x in rr if x not in comb2)
+                    if S3:
+                        out.append((S1, S2, S3))
+    return out
+
+
+def matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False):
+    gs = 2 * np.sqrt(np.pi * alpha_s)
+    sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
+    M = amplitude_tensor(p, hel, mt, wt, gs, sqh)
+    return float(np.sum(np.abs(M) ** 2))

@@ -87,6 +87,50 @@ int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double
 int mfx_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration, double* out,
                         void* stream);
 
+/* ------------------------------------------------------ VEGAS integration */
+/* Phase-space / integrand configuration (plain C struct). */
+typedef struct mfx_ps_params {
+  int n_out;            /* outgoing particles (t, t~, gluons) */
+  int ndim;             /* VEGAS dimensions: 4*n_out (RAMBO) or 3*n_out-4 (RAMBO on diet) */
+  int nbins;            /* VEGAS bins per dimension (<= 64) */
+  int mapping;          /* 0 RAMBO (phasespace.py:145), 1 RAMBO on diet (minimal dimensions) */
+  double sqrts;         /* fixed partonic sqrt(s) in GeV; incoming partons along +-z */
+  double masses[8];     /* outgoing masses */
+  double gev2pb;        /* unit conversion applied with the flux 1/(2 s) (phasespace.py:315-317) */
+  double pt_min;        /* cuts on outgoing legs >= cut_first_leg: pT > pt_min */
+  double abs_eta_max;   /*   |eta| < abs_eta_max (0 disables) */
+  double dr_min;        /*   Delta R(i,j) > dr_min between those legs (0 disables) */
+  int cut_first_leg;    /* first external leg index the cuts apply to (n_out + 2: no cuts) */
+  uint32_t seed;        /* Philox key */
+} mfx_ps_params;
+
+/* Device workspace needed by mfx_ps_generate / mfx_vegas_accumulate for nev events. */
+int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins);
+
+/* One VEGAS iteration's events: Philox uniforms (counter ev0+e, iteration) -> VEGAS map
+ * (grid: device [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520).  Writes the
+ * full momenta all_ps (nev, n_out+2, 4), wps = phase-space weight x VEGAS jacobian x
+ * gev2pb/(2s) (0 for cut events), flag (1 = pass), slot (dense index or -1) and the
+ * passing events' momenta densely into dense (the input of mfx_smatrix, the reference's
+ * boolean_mask compaction).  *npass_host receives the number of passing events (this
+ * synchronises the stream). */
+int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, void* workspace,
+                    int64_t workspace_bytes, int64_t* npass_host, void* stream);
+
+/* Accumulate one iteration: f = me[slot] * wps (or f_in if given; the reference's
+ * scatter_nd back to full length, madflow_exec.py:466-468).  result (device, 6 + ndim*nbins
+ * doubles): [sum f, sum f^2, max f, #f != 0, 0, 0, histogram of f^2 per dimension and bin].
+ * Sums are compensated FP64 in a fixed order (bitwise reproducible).  f_out: optional
+ * per-event weights. */
+int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const int* slot,
+                         const double* me, const double* wps, const double* f_in, double* f_out, void* workspace,
+                         int64_t workspace_bytes, double* result, void* stream);
+
+/* VEGAS grid refinement from the f^2 histogram: smoothing, ((1-r)/ln(1/r))^alpha damping
+ * and equal-weight rebinning, per dimension (device grid updated in place). */
+int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, double alpha, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,9 +89,9 @@ __global__ void k_aloha(int routine, const double* __restrict__ ins, int n_in, i
     case 1: FFV1_1(a, b, cp, m, wd, o); break;
     case 2: FFV1_2(a, b, cp, m, wd, o); break;
     case 3: VVV1P0_1(a, b, cp, m, wd, o); break;
-    case 4: VVVV1P0_1(a, b, c, cp, m, wd, o); break;
-    case 5: VVVV3P0_1(a, b, c, cp, m, wd, o); break;
-    case 6: VVVV4P0_1(a, b, c, cp, m, wd, o); break;
+    case 4: VVVVP0_1s<1>(a, b, c, cp, m, wd, o); break;
+    case 5: VVVVP0_1s<3>(a, b, c, cp, m, wd, o); break;
+    case 6: VVVVP0_1s<4>(a, b, c, cp, m, wd, o); break;
     case 7: FFV1P0_3(a, b, cp, m, wd, o); break;
     case 8: amp = VVV1_0(a, b, c, cp); is_amp = true; break;
     case 9: amp = VVVV1_0(a, b, c, d, cp); is_amp = true; break;

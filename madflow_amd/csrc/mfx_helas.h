@@ -169,9 +169,13 @@ template <class R> __device__ __forceinline__ void mom_in(const W<R>& w, R (&P)[
 template <class R> __device__ __forceinline__ void mom_out(const W<R>& w, R (&P)[4]) {
   P[0] = -w[0].re; P[1] = -w[1].re; P[2] = -w[1].im; P[3] = -w[0].im;
 }
-// COUP / (P^2 - M^2 + i M W)
-template <class R> __device__ __forceinline__ cx<R> prop_denom(cx<R> coup, const R (&P)[4], R M, R Wd) {
-  return cdiv(coup, cx<R>(P[0] * P[0] - P[1] * P[1] - P[2] * P[2] - P[3] * P[3] - M * M, M * Wd));
+// P^2 - M^2 from the slot momentum of the current
+template <class R> __device__ __forceinline__ R den_from_slots(const R (&P)[4], R M) {
+  return P[0] * P[0] - P[1] * P[1] - P[2] * P[2] - P[3] * P[3] - M * M;
+}
+// COUP / (P^2 - M^2 + i M W), with P^2 - M^2 given
+template <class R> __device__ __forceinline__ cx<R> prop_denom(cx<R> coup, R den2, R M, R Wd) {
+  return cdiv(coup, cx<R>(den2, M * Wd));
 }
 template <class R> __device__ __forceinline__ cx<R> mdot(const W<R>& a, const W<R>& b) {
   return a[2] * b[2] - a[3] * b[3] - a[4] * b[4] - a[5] * b[5];
@@ -201,12 +205,12 @@ __device__ __forceinline__ cx<R> FFV1_0(const W<R>& F1, const W<R>& F2, const W<
 
 // ------------------------------------------------------------------------ FFV1_1
 template <class R>
-__device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COUP, R M1, R W1, W<R>& F1) {
+__device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& F1) {
   F1[0] = F2[0] + V3[0];
   F1[1] = F2[1] + V3[1];
   R P[4];
   mom_out(F1, P);
-  const cx<R> denom = prop_denom(COUP, P, M1, W1);
+  const cx<R> denom = prop_denom(COUP, den2, M1, W1);
   const LC<R> v = lightcone(V3);
   const R pp0 = P[0] + P[3], pm0 = P[0] - P[3];
   const cx<R> q(P[1], P[2]), qb(P[1], -P[2]);
@@ -219,12 +223,12 @@ __device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COU
 
 // ------------------------------------------------------------------------ FFV1_2
 template <class R>
-__device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COUP, R M2, R W2, W<R>& F2) {
+__device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COUP, R M2, R W2, R den2, W<R>& F2) {
   F2[0] = F1[0] + V3[0];
   F2[1] = F1[1] + V3[1];
   R P[4];
   mom_out(F2, P);
-  const cx<R> denom = prop_denom(COUP, P, M2, W2);
+  const cx<R> denom = prop_denom(COUP, den2, M2, W2);
   const LC<R> v = lightcone(V3);
   const R pp0 = P[0] + P[3], pm0 = P[0] - P[3];
   const cx<R> q(P[1], P[2]), qb(P[1], -P[2]);
@@ -237,7 +241,7 @@ __device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COU
 
 // ---------------------------------------------------------------------- VVV1P0_1
 template <class R>
-__device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> COUP, R M1, R W1, W<R>& V1) {
+__device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& V1) {
   R P2[4], P3[4], P1[4];
   mom_in(V2, P2);
   mom_in(V3, P3);
@@ -246,7 +250,7 @@ __device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> C
   mom_out(V1, P1);
   const cx<R> TMP1 = mdotp(V3, P1), TMP2 = mdotp(V3, P2), TMP3 = mdotp(V2, P1), TMP4 = mdotp(V2, P3);
   const cx<R> TMP5 = mdot(V3, V2);
-  const cx<R> di = mul_i(prop_denom(COUP, P1, M1, W1));
+  const cx<R> di = mul_i(prop_denom(COUP, den2, M1, W1));
   const cx<R> c2 = TMP2 - TMP1, c3 = TMP3 - TMP4;
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) V1[2 + mu] = di * (TMP5 * (P3[mu] - P2[mu]) + V2[2 + mu] * c2 + V3[2 + mu] * c3);
@@ -256,12 +260,10 @@ __device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> C
 // which = 1: L = V4 (V2.V3) - V3 (V2.V4); 3: V4 (V2.V3) - V2 (V3.V4); 4: V3 (V2.V4) - V2 (V3.V4)
 template <int WHICH, class R>
 __device__ __forceinline__ void VVVVP0_1(const W<R>& V2, const W<R>& V3, const W<R>& V4, cx<R> COUP, R M1, R W1,
-                                         W<R>& V1) {
+                                         R den2, W<R>& V1) {
   V1[0] = V2[0] + V3[0] + V4[0];
   V1[1] = V2[1] + V3[1] + V4[1];
-  R P1[4];
-  mom_out(V1, P1);
-  const cx<R> dmi = mul_mi(prop_denom(COUP, P1, M1, W1));
+  const cx<R> dmi = mul_mi(prop_denom(COUP, den2, M1, W1));
   if (WHICH == 1) {
     const cx<R> x = dmi * mdot(V2, V3), y = dmi * mdot(V2, V4);
 #pragma unroll
@@ -277,16 +279,19 @@ __device__ __forceinline__ void VVVVP0_1(const W<R>& V2, const W<R>& V3, const W
   }
 }
 template <class R>
-__device__ __forceinline__ void VVVV1P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, W<R>& o) {
-  VVVVP0_1<1>(a, b, c, C, M, Wd, o);
+__device__ __forceinline__ void VVVV1P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, R den2,
+                                          W<R>& o) {
+  VVVVP0_1<1>(a, b, c, C, M, Wd, den2, o);
 }
 template <class R>
-__device__ __forceinline__ void VVVV3P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, W<R>& o) {
-  VVVVP0_1<3>(a, b, c, C, M, Wd, o);
+__device__ __forceinline__ void VVVV3P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, R den2,
+                                          W<R>& o) {
+  VVVVP0_1<3>(a, b, c, C, M, Wd, den2, o);
 }
 template <class R>
-__device__ __forceinline__ void VVVV4P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, W<R>& o) {
-  VVVVP0_1<4>(a, b, c, C, M, Wd, o);
+__device__ __forceinline__ void VVVV4P0_1(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, R den2,
+                                          W<R>& o) {
+  VVVVP0_1<4>(a, b, c, C, M, Wd, den2, o);
 }
 
 // ---------------------------------------------------------------------- FFV1P0_3
@@ -296,7 +301,7 @@ __device__ __forceinline__ void FFV1P0_3(const W<R>& F1, const W<R>& F2, cx<R> C
   V3[1] = F1[1] + F2[1];
   R P3[4];
   mom_out(V3, P3);
-  const cx<R> dmi = mul_mi(prop_denom(COUP, P3, M3, W3));
+  const cx<R> dmi = mul_mi(prop_denom(COUP, den_from_slots(P3, M3), M3, W3));
   const cx<R> J0 = F1[2] * F2[4] + F1[3] * F2[5] + F1[4] * F2[2] + F1[5] * F2[3];
   const cx<R> J1 = F1[4] * F2[3] + F1[5] * F2[2] - F1[2] * F2[5] - F1[3] * F2[4];
   const cx<R> J2 = mul_i(F1[3] * F2[4] + F1[4] * F2[3] - F1[2] * F2[5] - F1[5] * F2[2]);
@@ -339,6 +344,30 @@ __device__ __forceinline__ cx<R> VVVV3_0(const W<R>& a, const W<R>& b, const W<R
 template <class R>
 __device__ __forceinline__ cx<R> VVVV4_0(const W<R>& a, const W<R>& b, const W<R>& c, const W<R>& d, cx<R> C) {
   return VVVV_0<4>(a, b, c, d, C);
+}
+
+// Slot-momentum variants of the off-shell routines (P^2 - M^2 from the current's own
+// momentum, as the reference computes it); used by the building-block C entry points.
+template <class R>
+__device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> C, R M, R Wd, W<R>& F1) {
+  R P[4] = {-(F2[0].re + V3[0].re), -(F2[1].re + V3[1].re), -(F2[1].im + V3[1].im), -(F2[0].im + V3[0].im)};
+  FFV1_1(F2, V3, C, M, Wd, den_from_slots(P, M), F1);
+}
+template <class R>
+__device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> C, R M, R Wd, W<R>& F2) {
+  R P[4] = {-(F1[0].re + V3[0].re), -(F1[1].re + V3[1].re), -(F1[1].im + V3[1].im), -(F1[0].im + V3[0].im)};
+  FFV1_2(F1, V3, C, M, Wd, den_from_slots(P, M), F2);
+}
+template <class R>
+__device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> C, R M, R Wd, W<R>& V1) {
+  R P[4] = {-(V2[0].re + V3[0].re), -(V2[1].re + V3[1].re), -(V2[1].im + V3[1].im), -(V2[0].im + V3[0].im)};
+  VVV1P0_1(V2, V3, C, M, Wd, den_from_slots(P, M), V1);
+}
+template <int WHICH, class R>
+__device__ __forceinline__ void VVVVP0_1s(const W<R>& a, const W<R>& b, const W<R>& c, cx<R> C, R M, R Wd, W<R>& o) {
+  R P[4] = {-(a[0].re + b[0].re + c[0].re), -(a[1].re + b[1].re + c[1].re), -(a[1].im + b[1].im + c[1].im),
+            -(a[0].im + b[0].im + c[0].im)};
+  VVVVP0_1<WHICH>(a, b, c, C, M, Wd, den_from_slots(P, M), o);
 }
 
 }  // namespace mfx

@@ -1,0 +1,482 @@
+// VEGAS integration on the device: the event loop of the reference's integrand
+// (python_package/madflow/scripts/madflow_exec.py:419-472, utilities.py:82-86) with the
+// VEGAS map / histogram / refinement that vegasflow provides (Lepage, J.Comput.Phys. 27
+// (1978) 192), as HIP kernels.  Per iteration:
+//   k_generate   Philox uniforms -> VEGAS map -> RAMBO or RAMBO-on-diet -> cuts ->
+//                full momenta + phase-space weight + pass flag          (one event / lane)
+//   k_scan       block counts -> exclusive offsets                       (one block)
+//   k_compact    passing events -> dense all_ps for the smatrix kernel   (deterministic order)
+//   (mfx_smatrix on the dense batch)
+//   k_accumulate f = |M|^2 * wps per event; compensated FP64 sums of f, f^2; per-dimension
+//                bin histograms of f^2 in a fixed order (bitwise reproducible)
+//   k_reduce     fixed-order compensated reduction of the block partials
+//   k_refine     grid refinement (smoothing, ((1-r)/ln(1/r))^alpha damping, rebinning)
+#include "mfx_phasespace.h"
+#include "mfx_rng.h"
+#include "mfx_api_common.h"
+#include "../../include/madflow_amd.h"
+#include <math.h>
+#include <string.h>
+
+namespace mfx {
+
+constexpr int GEN_BLOCK = 256;
+constexpr int ACC_BLOCK = 256;
+constexpr int ACC_GRID = 1024;      // fixed: the reduction order must not depend on nev
+constexpr int MAXDIM = 32;
+constexpr int MAXBINS = 64;
+constexpr int NSCAL = 6;            // sum, comp, sum2, comp2, max, n_nonzero
+
+struct GenConst {
+  RamboConst rc;
+  int ndim, nbins, mapping, nexternal;
+  double sqrts, norm;              // norm = GeV^-2 -> pb conversion / (2 s)
+  double pt_min, abs_eta_max, dr_min;
+  int cut_first, cut_last;         // outgoing-leg range the gluon cuts apply to
+  uint32_t seed;
+};
+
+// --------------------------------------------------------------------- helpers
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// Neumaier-compensated accumulator
+struct KSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    double t, e;
+    two_sum(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void merge(const KSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+};
+
+__device__ __forceinline__ void uniforms(const GenConst& g, uint64_t ev, uint32_t iteration, double* y) {
+  for (int j = 0; 2 * j < g.ndim; ++j) {
+    double u0, u1;
+    philox_pair(ev, (uint32_t)j, g.seed, iteration, u0, u1);
+    y[2 * j] = u0;
+    if (2 * j + 1 < g.ndim) y[2 * j + 1] = u1;
+  }
+}
+
+__device__ __forceinline__ double vmap(const GenConst& g, const double* __restrict__ grid, const double* y, double* x,
+                                       int* bins) {
+  double jac = 1.0;
+  const int nb = g.nbins;
+  for (int d = 0; d < g.ndim; ++d) {
+    const double xn = y[d] * nb;
+    int b = (int)xn;
+    if (b >= nb) b = nb - 1;
+    const double lo = grid[d * (nb + 1) + b], hi = grid[d * (nb + 1) + b + 1];
+    const double w = hi - lo;
+    x[d] = lo + w * (xn - b);
+    jac *= w * nb;
+    if (bins) bins[d] = b;
+  }
+  return jac;
+}
+
+__device__ __forceinline__ double eta_of(const double* p) {
+  const double pa = sqrt(p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+  return 0.5 * log((pa + p[3]) / (pa - p[3]));
+}
+
+__device__ __forceinline__ bool pass_cuts(const GenConst& g, double (*p)[4]) {
+  for (int i = g.cut_first; i <= g.cut_last; ++i) {
+    const double pt = sqrt(p[i][1] * p[i][1] + p[i][2] * p[i][2]);
+    if (!(pt > g.pt_min)) return false;
+    if (g.abs_eta_max > 0.0 && !(fabs(eta_of(p[i])) < g.abs_eta_max)) return false;
+  }
+  if (g.dr_min > 0.0) {
+    for (int i = g.cut_first; i <= g.cut_last; ++i)
+      for (int j = i + 1; j <= g.cut_last; ++j) {
+        const double de = eta_of(p[i]) - eta_of(p[j]);
+        double dp = fabs(atan2(p[i][2], p[i][1]) - atan2(p[j][2], p[j][1]));
+        if (dp > MFX_PI) dp = 2.0 * MFX_PI - dp;
+        if (!(de * de + dp * dp > g.dr_min * g.dr_min)) return false;
+      }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- generation
+__global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double* __restrict__ grid, uint64_t ev0,
+                                                        int64_t nev, uint32_t iteration, double* __restrict__ all_ps,
+                                                        double* __restrict__ wps, uint8_t* __restrict__ flag,
+                                                        int* __restrict__ block_count) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  bool ok = false;
+  if (e < nev) {
+    double y[MAXDIM], x[MAXDIM];
+    uniforms(g, ev0 + e, iteration, y);
+    const double jac = vmap(g, grid, y, x, nullptr);
+    double p[RAMBO_MAXN][4];
+    const double w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p) : diet_event(g.rc, x, g.sqrts, p);
+    ok = pass_cuts(g, p) && isfinite(w);
+    double* out = all_ps + e * (g.nexternal * 4);
+    const double eh = 0.5 * g.sqrts;
+    out[0] = eh; out[1] = 0.0; out[2] = 0.0; out[3] = eh;
+    out[4] = eh; out[5] = 0.0; out[6] = 0.0; out[7] = -eh;
+    for (int i = 0; i < g.rc.n; ++i)
+      for (int k = 0; k < 4; ++k) out[8 + 4 * i + k] = p[i][k];
+    wps[e] = ok ? w * jac * g.norm : 0.0;
+    flag[e] = ok ? 1 : 0;
+  }
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const unsigned long long bal = __ballot(ok);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&cnt, __popcll(bal));
+  __syncthreads();
+  if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
+}
+
+// exclusive scan of the block counts, one block; total in offsets[nblk]
+__global__ void k_scan(const int* __restrict__ counts, int nblk, int* __restrict__ offsets) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nblk + 1023) / 1024;
+  const int lo = t * per, hi = min(nblk, lo + per);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+    offsets[nblk] = acc;
+  }
+  __syncthreads();
+  int acc = part[t];
+  for (int i = lo; i < hi; ++i) {
+    offsets[i] = acc;
+    acc += counts[i];
+  }
+}
+
+__global__ void __launch_bounds__(GEN_BLOCK) k_compact(int nexternal, int64_t nev, const double* __restrict__ all_ps,
+                                                       const uint8_t* __restrict__ flag, const int* __restrict__ offsets,
+                                                       double* __restrict__ dense, int* __restrict__ slot) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  const bool ok = e < nev && flag[e];
+  __shared__ int wave_base[GEN_BLOCK / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(ok);
+  if (lane == 0) wave_base[wave] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < GEN_BLOCK / 64; ++w) {
+      const int v = wave_base[w];
+      wave_base[w] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  if (e >= nev) return;
+  if (ok) {
+    const int rank = offsets[blockIdx.x] + wave_base[wave] + __popcll(bal & ((1ull << lane) - 1ull));
+    slot[e] = rank;
+    const double* src = all_ps + e * (nexternal * 4);
+    double* dst = dense + (int64_t)rank * (nexternal * 4);
+    for (int k = 0; k < nexternal * 4; ++k) dst[k] = src[k];
+  } else {
+    slot[e] = -1;
+  }
+}
+
+// ---------------------------------------------------------------- accumulation
+// partial layout per block: NSCAL scalars then ndim*nbins histogram (plain sums) and the
+// same number of compensation terms.
+__global__ void __launch_bounds__(ACC_BLOCK) k_accumulate(GenConst g, uint64_t ev0, int64_t nev, uint32_t iteration,
+                                                          const int* __restrict__ slot, const double* __restrict__ me,
+                                                          const double* __restrict__ wps, const double* __restrict__ fext,
+                                                          double* __restrict__ fout, double* __restrict__ partials) {
+  extern __shared__ double sh[];
+  const int D = g.ndim, NB = g.nbins, HB = D * NB;
+  const int nwave = ACC_BLOCK / 64;
+  double* hist = sh;                                  // [nwave][HB]
+  double* wv = sh + nwave * HB;                       // [nwave][64]
+  uint8_t* wb = (uint8_t*)(wv + nwave * 64);          // [nwave][64][D] bin indices
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < nwave * HB; i += ACC_BLOCK) hist[i] = 0.0;
+  __syncthreads();
+  KSum s1, s2;
+  double fmx = 0.0, nz = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * ACC_BLOCK;
+  // every wave walks whole 64-event tiles so that lane order fixes the summation order
+  for (int64_t base = (int64_t)blockIdx.x * ACC_BLOCK + wave * 64; base < nev; base += stride) {
+    const int64_t e = base + lane;
+    double f = 0.0;
+    int bins[MAXDIM];
+    if (e < nev) {
+      if (fext) f = fext[e];
+      else {
+        const int s = slot[e];
+        f = s >= 0 ? me[s] * wps[e] : 0.0;
+      }
+      if (!isfinite(f)) f = 0.0;          // a NaN/inf weight must not poison the grid
+      double y[MAXDIM];
+      uniforms(g, ev0 + e, iteration, y);
+      for (int d = 0; d < D; ++d) {
+        int b = (int)(y[d] * NB);
+        bins[d] = b >= NB ? NB - 1 : b;
+      }
+      if (fout) fout[e] = f;
+      s1.add(f);
+      s2.add(f * f);
+      fmx = fmax(fmx, f);
+      nz += f != 0.0 ? 1.0 : 0.0;
+    } else {
+      for (int d = 0; d < D; ++d) bins[d] = 0;
+    }
+    // stage (f^2, bins) of the tile; lane d then adds all 64 events into dimension d
+    wv[wave * 64 + lane] = f * f;
+    for (int d = 0; d < D; ++d) wb[(wave * 64 + lane) * D + d] = (uint8_t)bins[d];
+    // the wave's LDS writes complete before any lane reads them (and reads before the
+    // next tile overwrites them); the trip count is wave-uniform, so no block barrier
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (lane < D) {
+      double* h = hist + wave * HB + lane * NB;
+      for (int j = 0; j < 64; ++j) h[wb[(wave * 64 + j) * D + lane]] += wv[wave * 64 + j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  // block reduction of the scalars in a fixed tree order
+  __shared__ double red[ACC_BLOCK][NSCAL];
+  red[threadIdx.x][0] = s1.s;
+  red[threadIdx.x][1] = s1.c;
+  red[threadIdx.x][2] = s2.s;
+  red[threadIdx.x][3] = s2.c;
+  red[threadIdx.x][4] = fmx;
+  red[threadIdx.x][5] = nz;
+  __syncthreads();
+  for (int off = ACC_BLOCK / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      KSum a{red[threadIdx.x][0], red[threadIdx.x][1]}, b{red[threadIdx.x + off][0], red[threadIdx.x + off][1]};
+      a.merge(b);
+      KSum c{red[threadIdx.x][2], red[threadIdx.x][3]}, d{red[threadIdx.x + off][2], red[threadIdx.x + off][3]};
+      c.merge(d);
+      red[threadIdx.x][0] = a.s;
+      red[threadIdx.x][1] = a.c;
+      red[threadIdx.x][2] = c.s;
+      red[threadIdx.x][3] = c.c;
+      red[threadIdx.x][4] = fmax(red[threadIdx.x][4], red[threadIdx.x + off][4]);
+      red[threadIdx.x][5] += red[threadIdx.x + off][5];
+    }
+    __syncthreads();
+  }
+  double* out = partials + (size_t)blockIdx.x * (NSCAL + 2 * HB);
+  if (threadIdx.x < NSCAL) out[threadIdx.x] = red[0][threadIdx.x];
+  for (int i = threadIdx.x; i < HB; i += ACC_BLOCK) {
+    KSum k;
+    for (int w = 0; w < nwave; ++w) k.add(hist[w * HB + i]);
+    out[NSCAL + i] = k.s;
+    out[NSCAL + HB + i] = k.c;
+  }
+}
+
+// result: [sum f, sum f^2, max f, n_nonzero, 0, 0] then ndim*nbins histogram
+__global__ void k_reduce(int nparts, int hb, const double* __restrict__ partials, double* __restrict__ result) {
+  const int stride = NSCAL + 2 * hb;
+  for (int i = threadIdx.x; i < hb + 1; i += blockDim.x) {
+    if (i == hb) {
+      KSum a, b;
+      double mx = 0.0, nz = 0.0;
+      for (int p = 0; p < nparts; ++p) {
+        const double* q = partials + (size_t)p * stride;
+        a.merge(KSum{q[0], q[1]});
+        b.merge(KSum{q[2], q[3]});
+        mx = fmax(mx, q[4]);
+        nz += q[5];
+      }
+      result[0] = a.s + a.c;
+      result[1] = b.s + b.c;
+      result[2] = mx;
+      result[3] = nz;
+      result[4] = 0.0;
+      result[5] = 0.0;
+    } else {
+      KSum k;
+      for (int p = 0; p < nparts; ++p) {
+        const double* q = partials + (size_t)p * stride;
+        k.merge(KSum{q[NSCAL + i], q[NSCAL + hb + i]});
+      }
+      result[NSCAL + i] = k.s + k.c;
+    }
+  }
+}
+
+// One block per dimension; thread 0 rebins (NB <= 64 entries: serial is cheapest).
+__global__ void k_refine(double* __restrict__ grid, const double* __restrict__ hist, int nbins, double alpha) {
+  if (threadIdx.x != 0) return;
+  const int d = blockIdx.x;
+  double* edges = grid + d * (nbins + 1);
+  const double* h = hist + d * nbins;
+  double sm[MAXBINS], reg[MAXBINS], ne[MAXBINS + 1];
+  if (nbins < 2) return;
+  sm[0] = (h[0] + h[1]) / 2.0;
+  sm[nbins - 1] = (h[nbins - 2] + h[nbins - 1]) / 2.0;
+  for (int i = 1; i < nbins - 1; ++i) sm[i] = (h[i - 1] + h[i] + h[i + 1]) / 3.0;
+  double sum = 0.0;
+  for (int i = 0; i < nbins; ++i) sum += sm[i];
+  if (!(sum > 0.0) || !isfinite(sum)) return;
+  double rsum = 0.0;
+  for (int i = 0; i < nbins; ++i) {
+    const double r = sm[i] / sum;
+    reg[i] = (r > 0.0 && r < 1.0) ? pow((1.0 - r) / log(1.0 / r), alpha) : 0.0;
+    rsum += reg[i];
+  }
+  if (!(rsum > 0.0)) return;
+  const double ave = rsum / nbins;
+  ne[0] = 0.0;
+  ne[nbins] = 1.0;
+  int k = -1;
+  double acc = 0.0;
+  for (int j = 1; j < nbins; ++j) {
+    while (acc < ave && k < nbins - 1) {
+      ++k;
+      acc += reg[k];
+    }
+    acc -= ave;
+    const double lo = edges[k], hi = edges[k + 1];
+    ne[j] = hi - (hi - lo) * acc / reg[k];
+  }
+  for (int j = 0; j <= nbins; ++j) edges[j] = ne[j];
+}
+
+static int make_gen(const mfx_ps_params* p, GenConst& g) {
+  if (!p) return fail(-3, "null parameters%s", "");
+  const int n = p->n_out;
+  if (n < 2 || n > RAMBO_MAXN) return fail(-1, "n_out must be in [2, 8]%s", "");
+  const int need = p->mapping == 0 ? 4 * n : 3 * n - 4;
+  if (p->ndim != need) return fail(-2, "ndim must be 4*n_out (RAMBO) or 3*n_out-4 (diet)%s", "");
+  if (p->ndim > MAXDIM || p->nbins < 2 || p->nbins > MAXBINS) return fail(-2, "ndim/nbins out of range%s", "");
+  memset(&g, 0, sizeof(g));
+  g.rc.n = n;
+  g.rc.massive = 0;
+  g.rc.total_mass = 0.0;
+  for (int i = 0; i < n; ++i) {
+    g.rc.masses[i] = p->masses[i];
+    g.rc.total_mass += p->masses[i];
+    if (p->masses[i] != 0.0) g.rc.massive = 1;
+  }
+  if (g.rc.total_mass >= p->sqrts) return fail(-2, "not enough energy for the outgoing masses%s", "");
+  g.rc.logw0 = log(MFX_PI / 2.0) * (n - 1) - 2.0 * lgamma((double)(n - 1)) - log((double)(n - 1));
+  g.rc.inorm = 1.0 / pow(2.0 * MFX_PI, 3 * n - 4);
+  g.ndim = p->ndim;
+  g.nbins = p->nbins;
+  g.mapping = p->mapping;
+  g.nexternal = n + 2;
+  g.sqrts = p->sqrts;
+  g.norm = p->gev2pb / (2.0 * p->sqrts * p->sqrts);
+  g.pt_min = p->pt_min;
+  g.abs_eta_max = p->abs_eta_max;
+  g.dr_min = p->dr_min;
+  g.cut_first = p->cut_first_leg - 2;
+  g.cut_last = n - 1;
+  g.seed = p->seed;
+  if (p->cut_first_leg < 2 || p->cut_first_leg > n + 2) return fail(-2, "cut_first_leg out of range%s", "");
+  return 0;
+}
+
+}  // namespace mfx
+
+using namespace mfx;
+
+extern "C" {
+
+int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins) {
+  const int64_t nblk = (nev + GEN_BLOCK - 1) / GEN_BLOCK;
+  int64_t bytes = 0;
+  bytes += (nblk + 1) * 4 * 2;                                  // counts, offsets
+  bytes = (bytes + 255) / 256 * 256;
+  bytes += (int64_t)ACC_GRID * (NSCAL + 2 * ndim * nbins) * 8;  // accumulate partials
+  return bytes + 256;
+}
+
+int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, void* workspace,
+                    int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  GenConst g;
+  int rc = make_gen(prm, g);
+  if (rc) return rc;
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (nev > 0x7fffffff) return fail(-2, "nev must fit in 31 bits%s", "");
+  if (workspace_bytes < mfx_integrand_workspace_size(nev, prm->ndim, prm->nbins))
+    return fail(-8, "workspace too small (see mfx_integrand_workspace_size)%s", "");
+  if (nev == 0) {
+    if (npass_host) *npass_host = 0;
+    return 0;
+  }
+  if (!grid || !all_ps || !wps || !flag || !slot || !dense || !workspace) return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
+  int* counts = (int*)workspace;
+  int* offsets = counts + nblk + 1;
+  hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, all_ps, wps, flag,
+                     counts);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
+  hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
+                     slot);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_status(e);
+  if (npass_host) {
+    int tot = 0;
+    e = hipMemcpyAsync(&tot, offsets + nblk, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e);
+    *npass_host = tot;
+  }
+  return 0;
+}
+
+int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const int* slot,
+                         const double* me, const double* wps, const double* f_in, double* f_out, void* workspace,
+                         int64_t workspace_bytes, double* result, void* stream) {
+  GenConst g;
+  int rc = make_gen(prm, g);
+  if (rc) return rc;
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (workspace_bytes < mfx_integrand_workspace_size(nev, prm->ndim, prm->nbins))
+    return fail(-8, "workspace too small (see mfx_integrand_workspace_size)%s", "");
+  if (!result || !workspace) return fail(-3, "null pointer%s", "");
+  if (!f_in && nev > 0 && (!slot || !me || !wps)) return fail(-3, "need f_in or (slot, me, wps)%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nblk = (nev + GEN_BLOCK - 1) / GEN_BLOCK;
+  int64_t off = (nblk + 1) * 4 * 2;
+  off = (off + 255) / 256 * 256;
+  double* partials = (double*)((char*)workspace + off);
+  const int hb = g.ndim * g.nbins;
+  const int nwave = ACC_BLOCK / 64;
+  const size_t shmem = (size_t)nwave * hb * 8 + nwave * 64 * 8 + (size_t)nwave * 64 * g.ndim;
+  if (shmem + sizeof(double) * ACC_BLOCK * NSCAL > 60 * 1024) return fail(-2, "ndim*nbins too large for LDS%s", "");
+  hipLaunchKernelGGL(k_accumulate, dim3(ACC_GRID), dim3(ACC_BLOCK), shmem, s, g, ev0, nev, iteration, slot, me, wps,
+                     f_in, f_out, partials);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, ACC_GRID, hb, partials, result);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, double alpha, void* stream) {
+  if (ndim < 1 || ndim > MAXDIM || nbins < 2 || nbins > MAXBINS) return fail(-2, "ndim/nbins out of range%s", "");
+  if (!grid || !hist) return fail(-3, "null pointer%s", "");
+  hipLaunchKernelGGL(k_refine, dim3(ndim), dim3(64), 0, (hipStream_t)stream, grid, hist, nbins, alpha);
+  return hip_status(hipGetLastError());
+}
+
+}  // extern "C"

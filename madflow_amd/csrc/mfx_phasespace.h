@@ -24,6 +24,8 @@ struct RamboConst {
   double inorm;   // 1 / (2 pi)^(3n-4)
 };
 
+__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4]);
+
 // x: 4n uniforms (cos theta, phi, and two for the energy) per particle; returns weight.
 __device__ __forceinline__ double rambo_event(const RamboConst& rc, const double* x, double sqrts,
                                               double (*p)[4]) {
@@ -56,7 +58,17 @@ __device__ __forceinline__ double rambo_event(const RamboConst& rc, const double
     p[i][3] = (p[i][3] + b3 * t) * xs;
   }
   double logw = rc.logw0 + (2 * n - 4) * log(sqrts);
-  if (rc.massive) {
+  if (rc.massive) logw += massive_rescale(rc, sqrts, p);
+  return exp(logw) * rc.inorm;
+}
+
+// Massive momenta from massless ones (phasespace.py:195-212): rescale the 3-momenta by
+// xi with sum_i sqrt(m_i^2 + xi^2 E_i^2) = sqrts; returns the log of the weight factor
+// xi^(2n-3) prod(|p_i|/E_i) / sum(|p_i|^2/E_i) * sqrts.
+__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4]) {
+  const int n = rc.n;
+  double logw = 0.0;
+  {
     // Newton for xi, starting above the root (f(xi0) >= 0 by Minkowski's inequality)
     double xi = sqrt(1.0 - (rc.total_mass / sqrts) * (rc.total_mass / sqrts));
     for (int it = 0; it < 50; ++it) {
@@ -85,6 +97,59 @@ __device__ __forceinline__ double rambo_event(const RamboConst& rc, const double
     }
     logw += (2 * n - 3) * log(xi) + log(wt2 / wt3 * sqrts);
   }
+  return logw;
+}
+
+// Solve F(u) = r for u in [0,1], F(u) = (k+1) u^k - k u^(k+1)  (k >= 1): the CDF of
+// u = M_{i+1}^2 / M_i^2 in the massless n-body factorisation (Newton with bisection guard).
+__device__ __host__ inline double diet_solve(double r, int k) {
+  double lo = 0.0, hi = 1.0;
+  double u = pow(r, 1.0 / k);  // leading behaviour near u = 0
+  for (int it = 0; it < 60; ++it) {
+    const double uk = pow(u, k - 1);
+    const double F = (k + 1) * uk * u - k * uk * u * u - r;
+    if (F > 0) hi = u; else lo = u;
+    const double dF = (k + 1) * k * uk * (1.0 - u);
+    double un = (dF > 0) ? u - F / dF : 0.5 * (lo + hi);
+    if (!(un > lo && un < hi)) un = 0.5 * (lo + hi);
+    if (fabs(un - u) <= 1e-15 * u) { u = un; break; }
+    u = un;
+  }
+  return u;
+}
+
+// "RAMBO on diet" (S. Platzer, arXiv:1308.2922): n massless momenta from 3n-4 uniforms
+// with the same flat weight as RAMBO, by a chain of two-body decays Q_i -> p_i + Q_{i+1}
+// whose invariant masses are drawn from their exact marginal densities; massive
+// particles are then obtained with RAMBO's xi rescaling.  x[0..n-3] fix the masses,
+// x[n-2+2i], x[n-1+2i] the decay angles of step i.
+__device__ __forceinline__ double diet_event(const RamboConst& rc, const double* x, double sqrts, double (*p)[4]) {
+  const int n = rc.n;
+  double Q[4] = {sqrts, 0.0, 0.0, 0.0};
+  double M = sqrts;
+  for (int i = 0; i < n - 1; ++i) {
+    double Mn = 0.0;
+    if (i < n - 2) Mn = M * sqrt(diet_solve(x[i], n - 2 - i));
+    const double q = (M * M - Mn * Mn) / (2.0 * M);
+    const double costh = 2.0 * x[n - 2 + 2 * i] - 1.0;
+    const double sinth = sqrt(fmax(0.0, 1.0 - costh * costh));
+    double sphi, cphi;
+    sincos(2.0 * MFX_PI * x[n - 1 + 2 * i], &sphi, &cphi);
+    // momentum in the rest frame of Q, then boost with Q/M
+    const double k0 = q, k1 = q * sinth * cphi, k2 = q * sinth * sphi, k3 = q * costh;
+    const double b1 = Q[1] / M, b2 = Q[2] / M, b3 = Q[3] / M, g = Q[0] / M;
+    const double bk = b1 * k1 + b2 * k2 + b3 * k3;
+    const double c = bk / (g + 1.0) + k0;
+    p[i][0] = g * k0 + bk;
+    p[i][1] = k1 + b1 * c;
+    p[i][2] = k2 + b2 * c;
+    p[i][3] = k3 + b3 * c;
+    for (int j = 0; j < 4; ++j) Q[j] -= p[i][j];
+    M = Mn;
+  }
+  for (int j = 0; j < 4; ++j) p[n - 1][j] = Q[j];
+  double logw = rc.logw0 + (2 * n - 4) * log(sqrts);
+  if (rc.massive) logw += massive_rescale(rc, sqrts, p);
   return exp(logw) * rc.inorm;
 }
 

@@ -63,6 +63,26 @@ __device__ __forceinline__ R colour_sum(const cx<R> (&J)[T::NCOLOR]) {
   return res * R(1.0 / T::CF_DEN);
 }
 
+// Per-event invariants inv[k] = 2 k_i.k_j (i < j) of the outgoing momenta k_l = sign_l p_l,
+// in FP64 from the FP64 inputs: propagator denominators are sums of these, which keeps
+// P^2 - M^2 accurate in the FP32 build where it would cancel if formed from components.
+template <class T, class R>
+__device__ __forceinline__ void load_invariants(const double* __restrict__ all_ps, int64_t e, R (&inv)[T::NINV]) {
+  const double* src = all_ps + e * (T::NEXTERNAL * 4);
+  double k[T::NEXTERNAL][4];
+#pragma unroll
+  for (int l = 0; l < T::NEXTERNAL; ++l)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) k[l][c] = T::leg_sign(l) * src[4 * l + c];
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < T::NEXTERNAL; ++i)
+#pragma unroll
+    for (int j = i + 1; j < T::NEXTERNAL; ++j) {
+      inv[idx++] = R(2.0 * (k[i][0] * k[j][0] - k[i][1] * k[j][1] - k[i][2] * k[j][2] - k[i][3] * k[j][3]));
+    }
+}
+
 template <class T, class R>
 __device__ __forceinline__ void load_momenta(const double* __restrict__ all_ps, int64_t e, R (&p)[T::NEXTERNAL][4]) {
   const double* src = all_ps + e * (T::NEXTERNAL * 4);
@@ -89,7 +109,9 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nev; e += stride) {
     R p[T::NEXTERNAL][4];
+    R inv[T::NINV];
     load_momenta<T, R>(a.all_ps, e, p);
+    load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     R ans = R(0);
 #pragma unroll 1
@@ -98,8 +120,10 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
       for (int l = 0; l < T::NEXTERNAL; ++l)
 #pragma unroll
         for (int k = 0; k < 4; ++k) opaque(p[l][k]);
+#pragma unroll
+      for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
       cx<R> jamp[T::NCOLOR];
-      T::template amplitudes<R>(p, T::hel_bits(h), nullptr, prm, jamp);
+      T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
       ans += colour_sum<T, R>(jamp);
     }
     a.out[e] = double(ans) / T::DENOMINATOR;
@@ -113,10 +137,12 @@ __global__ void __launch_bounds__(256) k_matrix_hel(SmatrixArgs a) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nev; e += stride) {
     R p[T::NEXTERNAL][4];
+    R inv[T::NINV];
     load_momenta<T, R>(a.all_ps, e, p);
+    load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     cx<R> jamp[T::NCOLOR];
-    T::template amplitudes<R>(p, 0u, a.helv, prm, jamp);
+    T::template amplitudes<R>(p, inv, 0u, a.helv, prm, jamp);
     a.out[e] = double(colour_sum<T, R>(jamp));
     if (a.jamp_out) {
 #pragma unroll

@@ -132,6 +132,8 @@ class Program:
         self.namps = 0
         self.memo = {}
         self.jamp_terms = []                # per amplitude: {ordering tuple: (re, im)}
+        # external legs whose momenta flow through each wavefunction
+        self.legs = {l: frozenset([l]) for l in range(nexternal)}
 
     def _new_wf(self):
         w = self.nwf
@@ -142,8 +144,10 @@ class Program:
         if key in self.memo:
             return self.memo[key]
         w = self._new_wf()
+        legs = frozenset().union(*[self.legs[a] for a in args])
+        self.legs[w] = legs
         self.ops.append(dict(out=w, kind="wf", routine=routine, args=list(args),
-                             coup=coup, mass=mass, width=width))
+                             coup=coup, mass=mass, width=width, legs=sorted(legs)))
         self.memo[key] = w
         return w
 

@@ -1,0 +1,222 @@
+"""Device VEGAS integration of the matrix elements (the integration loop madflow runs
+through vegasflow: scripts/madflow_exec.py:474-527, utilities.py:56-90).
+
+``MEIntegrand`` fuses, per iteration, Philox uniforms -> VEGAS map -> RAMBO -> cuts ->
+compaction (``mfx_ps_generate``), the smatrix kernel on the passing events, and the
+compensated accumulation of sum f, sum f^2 and the per-dimension histograms
+(``mfx_vegas_accumulate``); the grid is refined on the device (``mfx_vegas_refine``).
+
+``VegasFlow`` mirrors the vegasflow interface used by the reference
+(``VegasFlow(n_dim, n_events)``, ``compile(integrand)``, ``run_integration(n_iter)``,
+``freeze_grid()``): iterations are combined with 1/sigma^2 weights.
+"""
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .matrix import load_matrix
+
+GEV2PB = 389379365.6
+BINS_MAX = 50
+ALPHA = 1.5
+
+
+class PsParams(ctypes.Structure):
+    _fields_ = [
+        ("n_out", ctypes.c_int), ("ndim", ctypes.c_int), ("nbins", ctypes.c_int), ("mapping", ctypes.c_int),
+        ("sqrts", ctypes.c_double), ("masses", ctypes.c_double * 8), ("gev2pb", ctypes.c_double),
+        ("pt_min", ctypes.c_double), ("abs_eta_max", ctypes.c_double), ("dr_min", ctypes.c_double),
+        ("cut_first_leg", ctypes.c_int), ("seed", ctypes.c_uint32),
+    ]
+
+
+def _sig(lib):
+    vp, i64, u64, u32, i, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER(PsParams)
+    lib.mfx_integrand_workspace_size.argtypes = [i64, i, i]
+    lib.mfx_integrand_workspace_size.restype = i64
+    lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_ps_generate.restype = i
+    lib.mfx_vegas_accumulate.argtypes = [P, u64, i64, u32, vp, vp, vp, vp, vp, vp, i64, vp, vp]
+    lib.mfx_vegas_accumulate.restype = i
+    lib.mfx_vegas_refine.argtypes = [vp, vp, i, i, d, vp]
+    lib.mfx_vegas_refine.restype = i
+
+
+PROCESS_SETUP = {
+    # name: n_out, masses, default sqrt(s)
+    "gg_ttx": (2, [173.0, 173.0]),
+    "gg_ttxg": (3, [173.0, 173.0, 0.0]),
+    "gg_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
+    "gg_ttxggg": (5, [173.0, 173.0, 0.0, 0.0, 0.0]),
+}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class MEIntegrand:
+    """sigma = int dPhi |M|^2 / (2 s) * GeV^-2->pb for one process at fixed sqrt(s)."""
+
+    def __init__(self, process, sqrts, precision=64, mapping="rambo", pt_min=0.0, abs_eta_max=0.0, dr_min=0.0,
+                 seed=1, nbins=BINS_MAX, alpha_s=0.118, device="cuda", events_limit=1 << 22, ref_compat=True,
+                 width=None):
+        n_out, masses = PROCESS_SETUP[process]
+        self.process = process
+        self.device = torch.device(device)
+        self.matrix, mod = load_matrix(process, precision=precision, ref_compat=ref_compat)
+        model = mod.get_model_param()
+        model.freeze_alpha_s(alpha_s)
+        self.params = list(model.evaluate())
+        if width is not None:
+            self.params[1] = float(width)
+        p = PsParams()
+        p.n_out = n_out
+        p.mapping = 0 if mapping == "rambo" else 1
+        p.ndim = 4 * n_out if p.mapping == 0 else 3 * n_out - 4
+        p.nbins = nbins
+        p.sqrts = float(sqrts)
+        for k, m in enumerate(masses):
+            p.masses[k] = m
+        p.gev2pb = GEV2PB
+        p.pt_min = pt_min
+        p.abs_eta_max = abs_eta_max
+        p.dr_min = dr_min
+        p.cut_first_leg = 4 if (pt_min > 0 or abs_eta_max > 0 or dr_min > 0) else n_out + 2
+        p.seed = seed
+        self.p = p
+        self.ndim = p.ndim
+        self.nbins = nbins
+        self.nexternal = n_out + 2
+        self.lib = _lib.load()
+        _sig(self.lib)
+        self.grid = torch.linspace(0, 1, nbins + 1, dtype=torch.float64, device=self.device).repeat(self.ndim, 1)
+        self.grid = self.grid.contiguous()
+        self.events_limit = int(events_limit)
+        self._bufs = None
+
+    def _alloc(self, n):
+        if self._bufs is not None and self._bufs["n"] >= n:
+            return self._bufs
+        d = self.device
+        ws = self.lib.mfx_integrand_workspace_size(n, self.ndim, self.nbins)
+        self._bufs = dict(
+            n=n,
+            all_ps=torch.empty((n, self.nexternal, 4), dtype=torch.float64, device=d),
+            dense=torch.empty((n, self.nexternal, 4), dtype=torch.float64, device=d),
+            wps=torch.empty(n, dtype=torch.float64, device=d),
+            flag=torch.empty(n, dtype=torch.uint8, device=d),
+            slot=torch.empty(n, dtype=torch.int32, device=d),
+            ws=torch.empty(ws, dtype=torch.uint8, device=d),
+            result=torch.empty(6 + self.ndim * self.nbins, dtype=torch.float64, device=d),
+            f=torch.empty(n, dtype=torch.float64, device=d),
+        )
+        return self._bufs
+
+    def chunk(self, ev0, n, iteration, keep_f=False):
+        """One chunk of events; returns (result vector on host, npass, f tensor or None)."""
+        b = self._alloc(n)
+        s = _lib.stream_ptr(self.device)
+        npass = ctypes.c_int64(0)
+        _lib.check(self.lib.mfx_ps_generate(
+            ctypes.byref(self.p), _ptr(self.grid), ev0, n, iteration, _ptr(b["all_ps"]), _ptr(b["wps"]),
+            _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+        npass = npass.value
+        me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+        _lib.check(self.lib.mfx_vegas_accumulate(
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
+        return b["result"].cpu().numpy().copy(), npass, (b["f"][:n] if keep_f else None)
+
+    def iteration(self, n_events, iteration, train=True, keep_f=False):
+        """One VEGAS iteration of n_events; returns a dict with I, var, max weight, counts."""
+        tot = None
+        npass = 0
+        fs = []
+        ev0 = 0
+        while ev0 < n_events:
+            n = min(self.events_limit, n_events - ev0)
+            r, k, f = self.chunk(ev0, n, iteration, keep_f)
+            tot = r if tot is None else _merge(tot, r)
+            npass += k
+            if keep_f:
+                fs.append(f.clone())
+            ev0 += n
+        N = float(n_events)
+        mean = tot[0] / N
+        var = max((tot[1] / N - mean * mean) / (N - 1.0), 0.0)
+        if train:
+            hist = torch.from_numpy(tot[6:].copy()).to(self.device)
+            _lib.check(self.lib.mfx_vegas_refine(_ptr(self.grid), _ptr(hist), self.ndim, self.nbins, ALPHA,
+                                                 _lib.stream_ptr(self.device)))
+        out = dict(I=mean, var=var, sigma=math.sqrt(var), wmax=tot[2], nonzero=tot[3], npass=npass, n=n_events,
+                   sum_f=tot[0])
+        if keep_f:
+            out["f"] = torch.cat(fs)
+        return out
+
+
+def _merge(a, b):
+    c = a.copy()
+    c[0] = math.fsum([a[0], b[0]])
+    c[1] = math.fsum([a[1], b[1]])
+    c[2] = max(a[2], b[2])
+    c[3] = a[3] + b[3]
+    c[6:] = a[6:] + b[6:]
+    return c
+
+
+class VegasFlow:
+    """vegasflow-style driver: VegasFlow(n_dim, n_events).compile(integrand);
+    run_integration(n_iter) -> (result, error); freeze_grid()."""
+
+    def __init__(self, n_dim=None, n_events=1000000, events_limit=None):
+        self.n_dim = n_dim
+        self.n_events = int(n_events)
+        self.events_limit = events_limit
+        self.integrand = None
+        self.frozen = False
+        self.iteration_index = 0
+        self.history = []
+
+    def compile(self, integrand):
+        if not isinstance(integrand, MEIntegrand):
+            raise TypeError("compile() takes an MEIntegrand (device-fused integrand)")
+        if self.n_dim is not None and self.n_dim != integrand.ndim:
+            raise ValueError("n_dim %d does not match the integrand's %d" % (self.n_dim, integrand.ndim))
+        if self.events_limit:
+            integrand.events_limit = int(self.events_limit)
+        self.integrand = integrand
+
+    def freeze_grid(self):
+        self.frozen = True
+
+    def run_integration(self, n_iter, log=None):
+        res = []
+        for _ in range(n_iter):
+            t0 = time.perf_counter()
+            r = self.integrand.iteration(self.n_events, self.iteration_index, train=not self.frozen)
+            r["seconds"] = time.perf_counter() - t0
+            self.iteration_index += 1
+            res.append(r)
+            self.history.append(r)
+            if log:
+                log("iteration %d: %.6g +- %.3g  (%.2fs, %d/%d pass)" % (
+                    self.iteration_index, r["I"], r["sigma"], r["seconds"], r["npass"], r["n"]))
+        return combine(res)
+
+
+def combine(results):
+    """1/sigma^2-weighted combination of iterations -> (result, error)."""
+    w = [1.0 / r["var"] for r in results if r["var"] > 0]
+    if not w:
+        return results[-1]["I"], 0.0
+    num = math.fsum(r["I"] / r["var"] for r in results if r["var"] > 0)
+    den = math.fsum(w)
+    return num / den, math.sqrt(1.0 / den)

@@ -245,6 +245,14 @@ def philox_uniforms(nev, ndim, seed, iteration, ev0=0):
     return out
 
 
+def philox_raw(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().mfo_philox_raw(_p(c), _p(k), _p(out))
+    return out
+
+
 def vegas_map(edges, y):
     edges = np.ascontiguousarray(edges, dtype=np.float64)
     ndim, nb1 = edges.shape
@@ -264,6 +272,25 @@ def vegas_refine(edges, hist, alpha=1.5):
     ndim, nb1 = edges.shape
     lib().mfo_vegas_refine(_p(edges), _p(hist), ctypes.c_int(ndim), ctypes.c_int(nb1 - 1), ctypes.c_double(alpha))
     return edges
+
+
+def generate(cfg, edges, ev0, nev, iteration):
+    """Oracle of mfx_ps_generate: returns (all_ps, wps, flag) for events ev0..ev0+nev-1."""
+    n = cfg["n_out"]
+    edges = np.ascontiguousarray(edges, dtype=np.float64)
+    masses = np.zeros(8)
+    masses[:n] = cfg["masses"][:n]
+    all_ps = np.zeros((nev, n + 2, 4))
+    wps = np.zeros(nev)
+    flag = np.zeros(nev, dtype=np.uint8)
+    lib().mfo_generate(ctypes.c_int(n), ctypes.c_int(cfg["ndim"]), ctypes.c_int(cfg["nbins"]),
+                       ctypes.c_int(cfg["mapping"]), ctypes.c_double(cfg["sqrts"]), _p(masses),
+                       ctypes.c_double(cfg["gev2pb"]), ctypes.c_double(cfg["pt_min"]),
+                       ctypes.c_double(cfg["abs_eta_max"]), ctypes.c_double(cfg["dr_min"]),
+                       ctypes.c_int(cfg["cut_first_leg"]), ctypes.c_uint32(cfg["seed"]), _p(edges),
+                       ctypes.c_uint64(ev0), ctypes.c_int64(nev), ctypes.c_uint32(iteration), _p(all_ps), _p(wps),
+                       _p(flag))
+    return all_ps, wps, flag
 
 
 def eta(p):

@@ -206,6 +206,14 @@ int mfo_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint
   return 0;
 }
 
+int mfo_philox_raw(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  u32x4 c;
+  for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
+  u32x4 r = philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+  return 0;
+}
+
 int mfo_vegas_map(const double* edges, int ndim, int nbins, const double* y, int64_t nev, double* x, double* jac,
                   int* bins) {
   VegasGrid g;
@@ -223,6 +231,52 @@ int mfo_vegas_map(const double* edges, int ndim, int nbins, const double* y, int
 
 int mfo_vegas_refine(double* edges, const double* hist, int ndim, int nbins, double alpha) {
   for (int d = 0; d < ndim; ++d) vegas_refine_dim(edges + (size_t)d * (nbins + 1), hist + (size_t)d * nbins, nbins, alpha);
+  return 0;
+}
+
+// Oracle of one VEGAS iteration's event generation: same Philox stream, VEGAS map,
+// RAMBO (mapping 0) or RAMBO on diet (mapping 1), cuts on legs cut_first_leg..n_out+1
+// (pT > pt_min, |eta| < abs_eta_max, Delta R > dr_min), wps = w * jac * gev2pb / (2 s).
+int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, const double* masses, double gev2pb,
+                 double pt_min, double abs_eta_max, double dr_min, int cut_first_leg, uint32_t seed,
+                 const double* edges, uint64_t ev0, int64_t nev, uint32_t iteration, double* all_ps, double* wps,
+                 uint8_t* flag) {
+  VegasGrid g;
+  g.ndim = ndim;
+  g.nbins = nbins;
+  g.edges.assign(edges, edges + (size_t)ndim * (nbins + 1));
+  std::vector<double> y(ndim), x(ndim), p(4 * n_out);
+  std::vector<int> bins(ndim);
+  bool massive = false;
+  for (int i = 0; i < n_out; ++i) massive |= masses[i] != 0;
+  const int next = n_out + 2;
+  for (int64_t e = 0; e < nev; ++e) {
+    philox_uniforms(ev0 + e, ndim, seed, iteration, y.data());
+    double jac = vegas_map(g, y.data(), x.data(), bins.data());
+    double lw;
+    if (mapping == 0) {
+      lw = rambo_massless_event(x.data(), n_out, sqrts, M_PI, p.data());
+    } else {
+      lw = diet_massless(x.data(), n_out, sqrts, M_PI, p.data());
+    }
+    if (massive) lw += massive_factor(p.data(), n_out, masses, sqrts);
+    double w = std::exp(lw) / std::pow(2 * M_PI, 3 * n_out - 4);
+    bool ok = std::isfinite(w);
+    for (int i = cut_first_leg - 2; i < n_out; ++i) {
+      const double* q = &p[4 * i];
+      if (!(pt_of(q) > pt_min)) ok = false;
+      if (abs_eta_max > 0 && !(std::fabs(eta_of(q)) < abs_eta_max)) ok = false;
+      if (dr_min > 0)
+        for (int j = i + 1; j < n_out; ++j)
+          if (!(delta_r(q, &p[4 * j]) > dr_min)) ok = false;
+    }
+    double* o = all_ps + e * next * 4;
+    o[0] = sqrts / 2; o[1] = 0; o[2] = 0; o[3] = sqrts / 2;
+    o[4] = sqrts / 2; o[5] = 0; o[6] = 0; o[7] = -sqrts / 2;
+    for (int k = 0; k < 4 * n_out; ++k) o[8 + k] = p[k];
+    wps[e] = ok ? w * jac * gev2pb / (2 * sqrts * sqrts) : 0.0;
+    flag[e] = ok ? 1 : 0;
+  }
   return 0;
 }
 

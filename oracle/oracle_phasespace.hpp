@@ -209,6 +209,75 @@ inline void rambo_batch(const double* xrand, int64_t nev, int xstride, int n, co
   }
 }
 
+// Massive rescaling of massless momenta p (n x 4) in place (phasespace.py:195-212),
+// per-event Newton; returns the log weight factor.
+inline double massive_factor(double* p, int n, const double* masses, double sqrts) {
+  double tm = 0;
+  for (int i = 0; i < n; ++i) tm += masses[i];
+  double xf = std::sqrt(1 - (tm / sqrts) * (tm / sqrts));
+  for (int it = 0; it < 50; ++it) {
+    double x2 = xf * xf, f0 = 0, g0 = 0;
+    for (int i = 0; i < n; ++i) {
+      double E = p[4 * i];
+      double nE = std::sqrt(masses[i] * masses[i] + E * E * x2);
+      f0 += nE;
+      g0 += E * E / nE;
+    }
+    f0 -= sqrts;
+    if (std::fabs(f0) <= 1e-14 * sqrts) break;
+    xf = xf - f0 / (xf * g0);
+  }
+  double wt2 = 1.0, wt3 = 0.0, x2 = xf * xf;
+  for (int i = 0; i < n; ++i) {
+    double* q = p + 4 * i;
+    double v = q[0] * xf;
+    double nE = std::sqrt(masses[i] * masses[i] + q[0] * q[0] * x2);
+    wt2 *= v / nE;
+    wt3 += v * v / nE;
+    q[0] = nE;
+    for (int k = 1; k < 4; ++k) q[k] *= xf;
+  }
+  return (2 * n - 3) * std::log(xf) + std::log(wt2 / wt3 * sqrts);
+}
+
+// RAMBO on diet (S. Platzer, arXiv:1308.2922): 3n-4 uniforms -> n massless momenta with
+// RAMBO's flat weight, via sequential two-body decays.  u = M_{i+1}^2/M_i^2 has CDF
+// (k+1) u^k - k u^(k+1), k = n-2-i; solved here by plain bisection (60 halvings).
+inline double diet_massless(const double* x, int n, double sqrts, double pi, double* p) {
+  double Q[4] = {sqrts, 0, 0, 0};
+  double M = sqrts;
+  for (int i = 0; i < n - 1; ++i) {
+    double Mn = 0.0;
+    if (i < n - 2) {
+      int k = n - 2 - i;
+      double lo = 0, hi = 1;
+      for (int it = 0; it < 60; ++it) {
+        double u = 0.5 * (lo + hi);
+        double F = (k + 1) * std::pow(u, k) - k * std::pow(u, k + 1);
+        if (F > x[i]) hi = u; else lo = u;
+      }
+      Mn = M * std::sqrt(0.5 * (lo + hi));
+    }
+    double q = (M * M - Mn * Mn) / (2 * M);
+    double ct = 2 * x[n - 2 + 2 * i] - 1, st = std::sqrt(std::max(0.0, 1 - ct * ct));
+    double ph = 2 * pi * x[n - 1 + 2 * i];
+    double k[4] = {q, q * st * std::cos(ph), q * st * std::sin(ph), q * ct};
+    // boost from the rest frame of Q
+    double b[3] = {Q[1] / M, Q[2] / M, Q[3] / M}, g = Q[0] / M;
+    double bk = b[0] * k[1] + b[1] * k[2] + b[2] * k[3];
+    double c = bk / (g + 1) + k[0];
+    double* pi_ = p + 4 * i;
+    pi_[0] = g * k[0] + bk;
+    for (int j = 0; j < 3; ++j) pi_[1 + j] = k[1 + j] + b[j] * c;
+    for (int j = 0; j < 4; ++j) Q[j] -= pi_[j];
+    M = Mn;
+  }
+  for (int j = 0; j < 4; ++j) p[4 * (n - 1) + j] = Q[j];
+  double wt = std::log(pi / 2.0) * (n - 1) - 2.0 * std::lgamma((double)(n - 1)) - std::log((double)(n - 1)) +
+              (2 * n - 4) * std::log(sqrts);
+  return wt;
+}
+
 // ------------------------------------------------------------------------- cuts
 inline double pt_of(const double* p) { return std::sqrt(p[1] * p[1] + p[2] * p[2]); }
 inline double eta_of(const double* p) {

@@ -1,0 +1,120 @@
+"""GPU VEGAS integrand pipeline against the oracle: generation (Philox -> VEGAS map ->
+RAMBO / RAMBO-on-diet -> cuts), compaction, compensated accumulation, refinement, and
+the integrated g g > t t~ cross section against the closed form."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from madflow_amd.vegas import MEIntegrand, VegasFlow, GEV2PB
+
+CASES = [
+    ("gg_ttxg", dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5), "rambo"),
+    ("gg_ttxgg", dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4), "rambo"),
+    ("gg_ttxgg", dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4), "diet"),
+    ("gg_ttx", dict(sqrts=1000.0), "diet"),
+]
+
+
+def _cfg(ig):
+    p = ig.p
+    return dict(n_out=p.n_out, ndim=p.ndim, nbins=p.nbins, mapping=p.mapping, sqrts=p.sqrts,
+                masses=list(p.masses), gev2pb=p.gev2pb, pt_min=p.pt_min, abs_eta_max=p.abs_eta_max,
+                dr_min=p.dr_min, cut_first_leg=p.cut_first_leg, seed=p.seed)
+
+
+def _warp_grid(ig, rng):
+    """A non-uniform monotone grid to exercise the VEGAS map."""
+    g = np.sort(rng.random((ig.ndim, ig.nbins - 1)), axis=1)
+    g = np.concatenate([np.zeros((ig.ndim, 1)), g, np.ones((ig.ndim, 1))], axis=1)
+    ig.grid.copy_(torch.from_numpy(g))
+    return g
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,kw,mapping", CASES)
+def test_generate_parity(dev, name, kw, mapping):
+    print("case generate", name, kw, mapping, flush=True)
+    ig = MEIntegrand(name, precision=64, mapping=mapping, seed=5, **kw)
+    edges = _warp_grid(ig, np.random.default_rng(1))
+    n, ev0, it = 4096, 12345, 3
+    ig._alloc(n)
+    ig.chunk(ev0, n, it)
+    b = ig._bufs
+    ps_ref, wps_ref, flag_ref = O.generate(_cfg(ig), edges, ev0, n, it)
+    flag = b["flag"][:n].cpu().numpy()
+    assert np.array_equal(flag, flag_ref)
+    assert np.max(np.abs(b["all_ps"][:n].cpu().numpy() - ps_ref)) < 1e-9 * kw["sqrts"]
+    w = b["wps"][:n].cpu().numpy()
+    nz = wps_ref != 0
+    assert np.array_equal(w != 0, nz)
+    assert np.max(np.abs(w[nz] / wps_ref[nz] - 1)) < 1e-11
+    # compaction keeps event order
+    slot = b["slot"][:n].cpu().numpy()
+    idx = np.nonzero(flag)[0]
+    assert np.array_equal(slot[idx], np.arange(len(idx)))
+    assert np.all(slot[flag == 0] == -1)
+    np.testing.assert_array_equal(b["dense"][:len(idx)].cpu().numpy(), b["all_ps"][:n].cpu().numpy()[idx])
+
+
+@pytest.mark.gpu
+def test_accumulate_and_refine(dev):
+    print("case accumulate", flush=True)
+    ig = MEIntegrand("gg_ttxgg", precision=32, mapping="diet", seed=9, sqrts=3000.0, pt_min=30.0, dr_min=0.4)
+    n = 300001
+    rng = np.random.default_rng(3)
+    f = rng.lognormal(size=n) * (rng.random(n) > 0.3)
+    f[17] = np.nan
+    ft = torch.from_numpy(f).to(dev)
+    b = ig._alloc(n)
+    import ctypes
+    from madflow_amd import _lib
+    res = []
+    for _ in range(2):
+        _lib.check(ig.lib.mfx_vegas_accumulate(ctypes.byref(ig.p), 7, n, 2, None, None, None,
+                                               ctypes.c_void_p(ft.data_ptr()), None, ctypes.c_void_p(b["ws"].data_ptr()),
+                                               b["ws"].numel(), ctypes.c_void_p(b["result"].data_ptr()),
+                                               _lib.stream_ptr(dev)))
+        res.append(b["result"].cpu().numpy().copy())
+    assert np.array_equal(res[0], res[1])            # bitwise reproducible
+    r = res[0]
+    fc = np.where(np.isfinite(f), f, 0.0)
+    assert abs(r[0] / math.fsum(fc) - 1) < 1e-14
+    assert abs(r[1] / math.fsum(fc * fc) - 1) < 1e-14
+    assert r[2] == fc.max() and r[3] == np.count_nonzero(fc)
+    y = O.philox_uniforms(n, ig.ndim, ig.p.seed, 2, ev0=7)
+    bins = np.minimum((y * ig.nbins).astype(int), ig.nbins - 1)
+    hist = np.stack([np.bincount(bins[:, d], weights=fc * fc, minlength=ig.nbins) for d in range(ig.ndim)])
+    np.testing.assert_allclose(r[6:].reshape(ig.ndim, ig.nbins), hist, rtol=1e-12)
+    # refinement on the device equals the oracle's
+    edges0 = ig.grid.cpu().numpy().copy()
+    ref = O.vegas_refine(edges0, hist, alpha=1.5)
+    hist_t = torch.from_numpy(r[6:].copy()).to(dev)
+    _lib.check(ig.lib.mfx_vegas_refine(ctypes.c_void_p(ig.grid.data_ptr()), ctypes.c_void_p(hist_t.data_ptr()),
+                                       ig.ndim, ig.nbins, 1.5, _lib.stream_ptr(dev)))
+    np.testing.assert_allclose(ig.grid.cpu().numpy(), ref, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mapping", ["rambo", "diet"])
+def test_ttbar_cross_section(dev, mapping):
+    """gg -> tt~ at sqrt(s) = 1 TeV, alpha_s = 0.118, m_t = 173, Gamma_t = 0: the closed
+    form gives 11.3683 pb; VEGAS with 1e6 points x 5 iterations must agree within 2 sigma."""
+    print("case sigma ttbar", mapping, flush=True)
+    ig = MEIntegrand("gg_ttx", precision=64, mapping=mapping, seed=1, sqrts=1000.0, ref_compat=False, width=0.0)
+    vf = VegasFlow(ig.ndim, 1000000)
+    vf.compile(ig)
+    vf.run_integration(2)
+    vf.freeze_grid()
+    res, err = vf.run_integration(5)
+    s, m, a = 1e6, 173.0, 0.118
+    rho = 4 * m * m / s
+    b = math.sqrt(1 - rho)
+    closed = math.pi * a * a / (3 * s) * ((1 + rho + rho * rho / 16) * math.log((1 + b) / (1 - b))
+                                          - b * (7 / 4 + 31 * rho / 16)) * GEV2PB
+    print("sigma", res, "+-", err, "closed", closed, flush=True)
+    assert abs(res - closed) < 2 * err
+    assert err / res < 2e-3

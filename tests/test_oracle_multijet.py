@@ -1,0 +1,92 @@
+"""Pins the oracle's multi-gluon processes, for which the reference ships no vectors:
+
+* colour-dressed brute force (tests/bruteforce.py: explicit SU(3) colour, Dirac matrices,
+  Berends-Giele recursion) per helicity row;
+* gauge (BRST) invariance: a gluon polarisation replaced by its momentum kills every jamp;
+* Lorentz invariance of |M|^2 under random boosts and rotations;
+* Bose symmetry under exchange of final-state gluons;
+* the helicity sum equals the sum of matrix() over the rows.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import bruteforce as BF
+from tests.helpers import phase_space_points
+
+ROWS = {"gg_ttx": [1, 5, 10], "gg_ttxg": [3, 17], "gg_ttxgg": [6]}
+
+
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg"])
+def test_bruteforce_per_helicity(name):
+    ps, _ = phase_space_points(name, 1, seed=31)
+    d = O.load_process(name)
+    rp, coup = O.default_params()
+    for row in ROWS[name]:
+        hel = d["helicities"][row]
+        bf = BF.matrix_hel(ps[0], hel)
+        orc = O.matrix_hel(name, ps, hel, rp, coup)[0]
+        assert abs(orc / bf - 1) < 1e-11, (row, orc, bf)
+
+
+@pytest.mark.parametrize("name", ["gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
+def test_brst_gauge_invariance(name):
+    ps, _ = phase_space_points(name, 3, seed=32)
+    d = O.load_process(name)
+    rp, coup = O.default_params(wt=0.0)
+    rows = [0, 7, len(d["helicities"]) // 2 + 3]
+    gluons = [0, 1] + list(range(4, d["nexternal"]))
+    for row in rows:
+        hel = list(d["helicities"][row])
+        _, jp = O.matrix_hel(name, ps, hel, rp, coup, want_jamp=True)
+        scale = np.abs(jp).max()
+        for g in gluons[:3]:
+            h = list(hel)
+            h[g] = 4
+            _, jb = O.matrix_hel(name, ps, h, rp, coup, want_jamp=True)
+            assert np.abs(jb).max() < 1e-11 * scale, (row, g)
+
+
+def _boost(ps, beta):
+    b = np.asarray(beta)
+    b2 = b @ b
+    g = 1 / np.sqrt(1 - b2)
+    L = np.eye(4)
+    L[0, 0] = g
+    L[0, 1:] = L[1:, 0] = g * b
+    L[1:, 1:] += (g - 1) * np.outer(b, b) / b2
+    return ps @ L.T
+
+
+@pytest.mark.parametrize("name", ["gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
+def test_lorentz_invariance(name):
+    nev = 1 if name == "gg_ttxggg" else 4
+    ps, _ = phase_space_points(name, nev, seed=33)
+    # the fixed-width Breit-Wigner breaks gauge invariance, and a boost changes the
+    # gauge of the polarisation vectors: invariance holds exactly only at zero width
+    rp, coup = O.default_params(wt=0.0)
+    a = O.smatrix(name, ps, rp, coup)
+    b = O.smatrix(name, _boost(ps, [0.1, -0.2, 0.3]), rp, coup)
+    np.testing.assert_allclose(a, b, rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["gg_ttxgg", "gg_ttxggg"])
+def test_bose_symmetry(name):
+    ps, _ = phase_space_points(name, 3, seed=34)
+    rp, coup = O.default_params()
+    q = ps.copy()
+    q[:, [4, 5]] = ps[:, [5, 4]]
+    np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), O.smatrix(name, q, rp, coup), rtol=1e-11)
+    q = ps.copy()
+    q[:, [0, 1]] = ps[:, [1, 0]]
+    np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), O.smatrix(name, q, rp, coup), rtol=1e-11)
+
+
+def test_helicity_sum():
+    name = "gg_ttxg"
+    ps, _ = phase_space_points(name, 5, seed=35)
+    d = O.load_process(name)
+    rp, coup = O.default_params()
+    tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in d["helicities"]) / d["denominator"]
+    np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), tot, rtol=1e-13)

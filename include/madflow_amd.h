@@ -131,6 +131,30 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
  * and equal-weight rebinning, per dimension (device grid updated in place). */
 int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, double alpha, void* stream);
 
+/* ------------------------------------------------ learned proposal (flow) */
+/* Rational-quadratic spline coupling flow on [0,1]^D (no counterpart in the reference:
+ * the north-star learned proposal).  masks: host int[L][D], 1 = dimension transformed
+ * by that layer.  Parameters: one fp32 vector (sizes[0] floats) plus transposed bf16
+ * copies of the hidden/output weights (sizes[1] elements) made by mfx_flow_cast.
+ * sizes[] = {nparams, nbf16, NPmax, then per layer (offset, count)}. */
+int mfx_flow_sizes(int D, int L, const int* masks, int64_t* sizes);
+int mfx_flow_cast(int D, int L, const int* masks, const float* params, uint16_t* params_bf16, void* stream);
+/* x -> u with log|du/dx| accumulated into logdet (inverse = 0), or u -> x with
+ * log|dx/du| (inverse = 1).  save: NULL, or mfx_flow_save_bytes of device memory that
+ * receives what mfx_flow_backward needs (forward direction only). */
+int mfx_flow_apply(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
+                   const float* x_in, float* x_out, float* logdet, int inverse, float* save, void* stream);
+int64_t mfx_flow_save_bytes(int D, int L, const int* masks, int64_t n);
+int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n);
+/* Gradient of sum_i gl_i * logdet_i (+ gx . u) w.r.t. the parameters (grad, fp32) and
+ * the inputs (gx in/out), from a saved forward pass.  partial: workspace of
+ * mfx_flow_partial_bytes; the reduction over samples has a fixed order. */
+int mfx_flow_backward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
+                      const float* save, const float* gl, float* gx, float* partial, float* grad, void* stream);
+/* Adam update of the fp32 parameters (gradient clipped to +-clip, non-finite -> 0). */
+int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
+                  float beta2, float eps, int step, float clip, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,192 @@
+"""Learned importance-sampling proposal: rational-quadratic spline coupling flow on the
+VEGAS-mapped unit hypercube (device kernels in csrc/mfx_flow.hip).
+
+    flow = RQSFlow(D=8, layers=8)
+    flow.train(y, w, steps=..., batch=...)      # weighted maximum likelihood on (point, weight) pairs
+    y, logq = flow.sample(u)                    # u ~ U(0,1)^D  ->  y, log q(y)
+    logq = flow.log_prob(y)
+
+Training maximises sum_i w_i log q(y_i) / sum_i w_i with w_i = |M|^2 x (phase-space and
+VEGAS jacobians) at points y_i drawn uniformly on the VEGAS cube, i.e. it minimises
+KL(p || q) for p proportional to the integrand pulled back to the cube.  Every sample
+of the proposal is reweighted by f(y)/q(y), so the estimate stays unbiased whatever the
+quality of the fit; plain VEGAS (q = 1) is the fallback.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+H = 128
+K = 16
+NPD = 3 * K + 1
+DMAX = 16
+MIND = 1e-3
+
+
+def default_masks(D, L):
+    """Layer l transforms the dimensions whose bit (l//2 mod nbits) equals (l+1) % 2."""
+    nbits = max(1, int(math.ceil(math.log2(D))))
+    masks = np.zeros((L, D), dtype=np.int32)
+    for l in range(L):
+        bit = (l // 2) % nbits
+        want = (l + 1) % 2
+        for d in range(D):
+            masks[l, d] = 1 if ((d >> bit) & 1) == want else 0
+        if masks[l].sum() in (0, D):
+            masks[l] = np.arange(D) % 2 == (l % 2)
+    return masks
+
+
+def _sig(lib):
+    vp, i64, i, f = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float
+    lib.mfx_flow_sizes.argtypes = [i, i, vp, vp]
+    lib.mfx_flow_sizes.restype = i
+    lib.mfx_flow_cast.argtypes = [i, i, vp, vp, vp, vp]
+    lib.mfx_flow_cast.restype = i
+    lib.mfx_flow_apply.argtypes = [i, i, vp, vp, vp, i64, vp, vp, vp, i, vp, vp]
+    lib.mfx_flow_apply.restype = i
+    lib.mfx_flow_save_bytes.argtypes = [i, i, vp, i64]
+    lib.mfx_flow_save_bytes.restype = i64
+    lib.mfx_flow_partial_bytes.argtypes = [i, i, vp, i64]
+    lib.mfx_flow_partial_bytes.restype = i64
+    lib.mfx_flow_backward.argtypes = [i, i, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
+    lib.mfx_flow_backward.restype = i
+    lib.mfx_flow_adam.argtypes = [vp, vp, vp, vp, i64, f, f, f, f, i, f, vp]
+    lib.mfx_flow_adam.restype = i
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class RQSFlow:
+    def __init__(self, D, layers=8, seed=0, device="cuda", masks=None):
+        self.D = D
+        self.L = layers
+        self.device = torch.device(device)
+        self.masks = np.ascontiguousarray(masks if masks is not None else default_masks(D, layers), dtype=np.int32)
+        self.lib = _lib.load()
+        _sig(self.lib)
+        sizes = np.zeros(3 + 2 * layers, dtype=np.int64)
+        _lib.check(self.lib.mfx_flow_sizes(D, layers, self._m(), sizes.ctypes.data_as(ctypes.c_void_p)))
+        self.nparams, self.nbf16, self.NPmax = int(sizes[0]), int(sizes[1]), int(sizes[2])
+        self.layer_off = [(int(sizes[3 + 2 * l]), int(sizes[4 + 2 * l])) for l in range(layers)]
+        self.params = torch.from_numpy(self.init_params(seed)).to(self.device)
+        self.pbf16 = torch.empty(self.nbf16, dtype=torch.int16, device=self.device)
+        self.m1 = torch.zeros_like(self.params)
+        self.m2 = torch.zeros_like(self.params)
+        self.step = 0
+        self._cast()
+
+    def _m(self):
+        return self.masks.ctypes.data_as(ctypes.c_void_p)
+
+    def layout(self, l):
+        """Offsets of W1 [DMAX][H], b1, W2 [H][H], b2, W3 [H][NP], b3 inside layer l."""
+        off, n = self.layer_off[l]
+        nB = int(self.masks[l].sum())
+        NP = ((nB * NPD + 15) // 16) * 16
+        o = {}
+        o["W1"] = off
+        o["b1"] = o["W1"] + DMAX * H
+        o["W2"] = o["b1"] + H
+        o["b2"] = o["W2"] + H * H
+        o["W3"] = o["b2"] + H
+        o["b3"] = o["W3"] + H * NP
+        o["NP"] = NP
+        o["nB"] = nB
+        return o
+
+    def init_params(self, seed):
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.nparams, dtype=np.float32)
+        for l in range(self.L):
+            o = self.layout(l)
+            nA = self.D - o["nB"]
+            w1 = rng.uniform(-1, 1, size=(DMAX, H)) / math.sqrt(nA)
+            w1[nA:] = 0
+            p[o["W1"]:o["W1"] + DMAX * H] = w1.reshape(-1)
+            p[o["W2"]:o["W2"] + H * H] = (rng.uniform(-1, 1, size=(H, H)) / math.sqrt(H)).reshape(-1)
+            # last layer zero: uniform bins; derivative logits so that every knot slope is 1
+            b3 = np.zeros(o["NP"], dtype=np.float32)
+            dlogit = math.log(math.expm1(1.0 - MIND))
+            for b in range(o["nB"]):
+                b3[b * NPD + 2 * K: b * NPD + 3 * K + 1] = dlogit
+            p[o["b3"]:o["b3"] + o["NP"]] = b3
+        return p
+
+    def _cast(self):
+        _lib.check(self.lib.mfx_flow_cast(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16),
+                                          _lib.stream_ptr(self.device)))
+
+    # --------------------------------------------------------------- evaluation
+    def _apply(self, x, inverse, save=None):
+        x = _lib.require_device_tensor(x, torch.float32, "x")
+        n = x.shape[0]
+        out = torch.empty_like(x)
+        ld = torch.zeros(n, dtype=torch.float32, device=x.device)
+        _lib.check(self.lib.mfx_flow_apply(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), n, _p(x),
+                                           _p(out), _p(ld), int(inverse), _p(save), _lib.stream_ptr(x.device)))
+        return out, ld
+
+    def forward(self, y):
+        """y -> u and log|det du/dy| (= log q(y) for a uniform base)."""
+        return self._apply(y, False)
+
+    def log_prob(self, y):
+        return self._apply(y, False)[1]
+
+    def sample(self, u):
+        """u -> y and log q(y)."""
+        y, ld_inv = self._apply(u, True)
+        return y, -ld_inv
+
+    # ----------------------------------------------------------------- training
+    def grad(self, y, w):
+        """Gradient of L = -sum_i w_i log q(y_i) / sum_i w_i (returns L, grad)."""
+        y = _lib.require_device_tensor(y, torch.float32, "y")
+        n = y.shape[0]
+        save = torch.empty(self.lib.mfx_flow_save_bytes(self.D, self.L, self._m(), n) // 4 + 1,
+                           dtype=torch.float32, device=y.device)
+        u, ld = self._apply(y, False, save=save)
+        wsum = w.sum()
+        gl = (-w / wsum).to(torch.float32).contiguous()
+        gx = torch.zeros_like(y)
+        partial = torch.empty(self.lib.mfx_flow_partial_bytes(self.D, self.L, self._m(), n) // 4 + 1,
+                              dtype=torch.float32, device=y.device)
+        grad = torch.zeros_like(self.params)
+        _lib.check(self.lib.mfx_flow_backward(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), n, _p(save), _p(gl),
+                                              _p(gx), _p(partial), _p(grad), _lib.stream_ptr(y.device)))
+        loss = -(w.to(torch.float64) * ld.to(torch.float64)).sum() / wsum.to(torch.float64)
+        return loss, grad
+
+    def adam_step(self, grad, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=10.0):
+        self.step += 1
+        _lib.check(self.lib.mfx_flow_adam(_p(self.params), _p(grad), _p(self.m1), _p(self.m2), self.nparams, lr,
+                                          betas[0], betas[1], eps, self.step, clip, _lib.stream_ptr(self.device)))
+        self._cast()
+
+    def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True):
+        """Minibatch weighted maximum likelihood on device-resident pairs (y, w)."""
+        n = y.shape[0]
+        g = torch.Generator(device=y.device)
+        g.manual_seed(seed)
+        losses = []
+        for it in range(steps):
+            idx = torch.randint(0, n, (batch,), generator=g, device=y.device)
+            yb = y[idx].contiguous()
+            wb = w[idx].contiguous()
+            if float(wb.sum()) <= 0:
+                continue
+            cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
+            loss, grad = self.grad(yb, wb)
+            self.adam_step(grad, lr=cur_lr)
+            if log and (it % max(1, steps // 10) == 0 or it == steps - 1):
+                log("flow step %d loss %.5f" % (it, float(loss)))
+            losses.append(float(loss)) if it % 50 == 0 else None
+        return losses

@@ -107,25 +107,28 @@ typedef struct mfx_ps_params {
 /* Device workspace needed by mfx_ps_generate / mfx_vegas_accumulate for nev events. */
 int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins);
 
-/* One VEGAS iteration's events: Philox uniforms (counter ev0+e, iteration) -> VEGAS map
- * (grid: device [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520).  Writes the
+/* One VEGAS iteration's events: Philox uniforms (counter ev0+e, iteration), or the
+ * points y_in (nev, ndim) f32 of an external proposal, -> VEGAS map (grid: device
+ * [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520).  w_in (optional, nev)
+ * multiplies the weights (e.g. 1/q of the proposal).  Writes the
  * full momenta all_ps (nev, n_out+2, 4), wps = phase-space weight x VEGAS jacobian x
  * gev2pb/(2s) (0 for cut events), flag (1 = pass), slot (dense index or -1) and the
  * passing events' momenta densely into dense (the input of mfx_smatrix, the reference's
  * boolean_mask compaction).  *npass_host receives the number of passing events (this
  * synchronises the stream). */
 int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
-                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, void* workspace,
-                    int64_t workspace_bytes, int64_t* npass_host, void* stream);
+                    const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
+                    double* dense, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
 
 /* Accumulate one iteration: f = me[slot] * wps (or f_in if given; the reference's
  * scatter_nd back to full length, madflow_exec.py:466-468).  result (device, 6 + ndim*nbins
  * doubles): [sum f, sum f^2, max f, #f != 0, 0, 0, histogram of f^2 per dimension and bin].
- * Sums are compensated FP64 in a fixed order (bitwise reproducible).  f_out: optional
+ * Sums are compensated FP64 in a fixed order (bitwise reproducible); the histogram bins
+ * come from y_in if given, else from the Philox stream.  f_out: optional
  * per-event weights. */
-int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const int* slot,
-                         const double* me, const double* wps, const double* f_in, double* f_out, void* workspace,
-                         int64_t workspace_bytes, double* result, void* stream);
+int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const float* y_in,
+                         const int* slot, const double* me, const double* wps, const double* f_in, double* f_out,
+                         void* workspace, int64_t workspace_bytes, double* result, void* stream);
 
 /* VEGAS grid refinement from the f^2 histogram: smoothing, ((1-r)/ln(1/r))^alpha damping
  * and equal-weight rebinning, per dimension (device grid updated in place). */

@@ -108,14 +108,19 @@ __device__ __forceinline__ bool pass_cuts(const GenConst& g, double (*p)[4]) {
 
 // ---------------------------------------------------------------- generation
 __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double* __restrict__ grid, uint64_t ev0,
-                                                        int64_t nev, uint32_t iteration, double* __restrict__ all_ps,
+                                                        int64_t nev, uint32_t iteration, const float* __restrict__ y_in,
+                                                        const double* __restrict__ w_in, double* __restrict__ all_ps,
                                                         double* __restrict__ wps, uint8_t* __restrict__ flag,
                                                         int* __restrict__ block_count) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
   bool ok = false;
   if (e < nev) {
     double y[MAXDIM], x[MAXDIM];
-    uniforms(g, ev0 + e, iteration, y);
+    if (y_in) {
+      for (int d = 0; d < g.ndim; ++d) y[d] = (double)y_in[e * g.ndim + d];
+    } else {
+      uniforms(g, ev0 + e, iteration, y);
+    }
     const double jac = vmap(g, grid, y, x, nullptr);
     double p[RAMBO_MAXN][4];
     const double w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p) : diet_event(g.rc, x, g.sqrts, p);
@@ -126,7 +131,9 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
     out[4] = eh; out[5] = 0.0; out[6] = 0.0; out[7] = -eh;
     for (int i = 0; i < g.rc.n; ++i)
       for (int k = 0; k < 4; ++k) out[8 + 4 * i + k] = p[i][k];
-    wps[e] = ok ? w * jac * g.norm : 0.0;
+    const double wext = w_in ? w_in[e] : 1.0;
+    ok = ok && isfinite(wext);
+    wps[e] = ok ? w * jac * g.norm * wext : 0.0;
     flag[e] = ok ? 1 : 0;
   }
   __shared__ int cnt;
@@ -200,7 +207,8 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_compact(int nexternal, int64_t ne
 // partial layout per block: NSCAL scalars then ndim*nbins histogram (plain sums) and the
 // same number of compensation terms.
 __global__ void __launch_bounds__(ACC_BLOCK) k_accumulate(GenConst g, uint64_t ev0, int64_t nev, uint32_t iteration,
-                                                          const int* __restrict__ slot, const double* __restrict__ me,
+                                                          const float* __restrict__ y_in, const int* __restrict__ slot,
+                                                          const double* __restrict__ me,
                                                           const double* __restrict__ wps, const double* __restrict__ fext,
                                                           double* __restrict__ fout, double* __restrict__ partials) {
   extern __shared__ double sh[];
@@ -228,7 +236,11 @@ __global__ void __launch_bounds__(ACC_BLOCK) k_accumulate(GenConst g, uint64_t e
       }
       if (!isfinite(f)) f = 0.0;          // a NaN/inf weight must not poison the grid
       double y[MAXDIM];
-      uniforms(g, ev0 + e, iteration, y);
+      if (y_in) {
+        for (int d = 0; d < D; ++d) y[d] = (double)y_in[e * D + d];
+      } else {
+        uniforms(g, ev0 + e, iteration, y);
+      }
       for (int d = 0; d < D; ++d) {
         int b = (int)(y[d] * NB);
         bins[d] = b >= NB ? NB - 1 : b;
@@ -411,8 +423,8 @@ int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins) {
 }
 
 int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
-                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, void* workspace,
-                    int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+                    const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
+                    double* dense, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -429,8 +441,8 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
   const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
-  hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, all_ps, wps, flag,
-                     counts);
+  hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, y_in, w_in, all_ps,
+                     wps, flag, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);
@@ -446,9 +458,9 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
   return 0;
 }
 
-int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const int* slot,
-                         const double* me, const double* wps, const double* f_in, double* f_out, void* workspace,
-                         int64_t workspace_bytes, double* result, void* stream) {
+int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const float* y_in,
+                         const int* slot, const double* me, const double* wps, const double* f_in, double* f_out,
+                         void* workspace, int64_t workspace_bytes, double* result, void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -466,8 +478,8 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
   const int nwave = ACC_BLOCK / 64;
   const size_t shmem = (size_t)nwave * hb * 8 + nwave * 64 * 8 + (size_t)nwave * 64 * g.ndim;
   if (shmem + sizeof(double) * ACC_BLOCK * NSCAL > 60 * 1024) return fail(-2, "ndim*nbins too large for LDS%s", "");
-  hipLaunchKernelGGL(k_accumulate, dim3(ACC_GRID), dim3(ACC_BLOCK), shmem, s, g, ev0, nev, iteration, slot, me, wps,
-                     f_in, f_out, partials);
+  hipLaunchKernelGGL(k_accumulate, dim3(ACC_GRID), dim3(ACC_BLOCK), shmem, s, g, ev0, nev, iteration, y_in, slot, me,
+                     wps, f_in, f_out, partials);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, ACC_GRID, hb, partials, result);
   return hip_status(hipGetLastError());
 }

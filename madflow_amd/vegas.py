@@ -40,9 +40,10 @@ def _sig(lib):
     P = ctypes.POINTER(PsParams)
     lib.mfx_integrand_workspace_size.argtypes = [i64, i, i]
     lib.mfx_integrand_workspace_size.restype = i64
-    lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64),
+                                    vp]
     lib.mfx_ps_generate.restype = i
-    lib.mfx_vegas_accumulate.argtypes = [P, u64, i64, u32, vp, vp, vp, vp, vp, vp, i64, vp, vp]
+    lib.mfx_vegas_accumulate.argtypes = [P, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_vegas_accumulate.restype = i
     lib.mfx_vegas_refine.argtypes = [vp, vp, i, i, d, vp]
     lib.mfx_vegas_refine.restype = i
@@ -100,6 +101,7 @@ class MEIntegrand:
         self.grid = self.grid.contiguous()
         self.events_limit = int(events_limit)
         self._bufs = None
+        self.me_timer = None      # list of (start event, end event, n) around smatrix launches
 
     def _alloc(self, n):
         if self._bufs is not None and self._bufs["n"] >= n:
@@ -119,18 +121,30 @@ class MEIntegrand:
         )
         return self._bufs
 
-    def chunk(self, ev0, n, iteration, keep_f=False):
-        """One chunk of events; returns (result vector on host, npass, f tensor or None)."""
+    def chunk(self, ev0, n, iteration, keep_f=False, y_in=None, w_in=None):
+        """One chunk of events; returns (result vector on host, npass, f tensor or None).
+
+        y_in (n, ndim) float32 / w_in (n,) float64: points and extra weights of an external
+        proposal (the flow) instead of the Philox stream."""
         b = self._alloc(n)
         s = _lib.stream_ptr(self.device)
         npass = ctypes.c_int64(0)
         _lib.check(self.lib.mfx_ps_generate(
-            ctypes.byref(self.p), _ptr(self.grid), ev0, n, iteration, _ptr(b["all_ps"]), _ptr(b["wps"]),
-            _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+            ctypes.byref(self.p), _ptr(self.grid), ev0, n, iteration, _ptr(y_in), _ptr(w_in), _ptr(b["all_ps"]),
+            _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["ws"]), b["ws"].numel(),
+            ctypes.byref(npass), s))
         npass = npass.value
-        me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+        if self.me_timer is not None and npass:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
+            e1.record()
+            self.me_timer.append((e0, e1, npass))
+        else:
+            me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
         _lib.check(self.lib.mfx_vegas_accumulate(
-            ctypes.byref(self.p), ev0, n, iteration, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
             _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
         return b["result"].cpu().numpy().copy(), npass, (b["f"][:n] if keep_f else None)
 

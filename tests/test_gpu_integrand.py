@@ -74,7 +74,7 @@ def test_accumulate_and_refine(dev):
     from madflow_amd import _lib
     res = []
     for _ in range(2):
-        _lib.check(ig.lib.mfx_vegas_accumulate(ctypes.byref(ig.p), 7, n, 2, None, None, None,
+        _lib.check(ig.lib.mfx_vegas_accumulate(ctypes.byref(ig.p), 7, n, 2, None, None, None, None,
                                                ctypes.c_void_p(ft.data_ptr()), None, ctypes.c_void_p(b["ws"].data_ptr()),
                                                b["ws"].numel(), ctypes.c_void_p(b["result"].data_ptr()),
                                                _lib.stream_ptr(dev)))

@@ -38,26 +38,28 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dense, npass, params, budget_s=15.0):
-    """Time the CPU oracle on the first passing events of the last iteration."""
+def cpu_baseline(dense, npass, params, budget_s=12.0):
+    """Time the CPU oracle (FP64 C++, OpenMP over events) on the first passing events of
+    the last iteration: doubles the sample until one timed call takes budget_s / 3."""
     import numpy as np
     from oracle import oracle as O
     cores = min(16, os.cpu_count() or 1)
-    ps = dense[:min(npass, 4096)].cpu().numpy()
-    rp = np.array([params[0], params[1]], dtype=np.float64)
-    coup = {"GC_10": complex(params[2].item()), "GC_11": complex(params[3].item()),
-            "GC_12": complex(params[4].item())}
-    n = 64
-    t = 0.0
+    nmax = min(npass, 1 << 20)
+    ps = dense[:nmax].cpu().numpy()
+    rp = np.array([float(params[0]), float(params[1])], dtype=np.float64)
+    c = [complex(x.item()) if hasattr(x, "item") else complex(x) for x in params[2:5]]
+    coup = {"GC_10": c[0], "GC_11": c[1], "GC_12": c[2]}
+    n = 256
     while True:
         t0 = time.perf_counter()
         O.smatrix("gg_ttxgg", ps[:n], rp, coup, ref_compat=True, nthreads=cores)
         t = time.perf_counter() - t0
-        if t > budget_s / 4 or n >= len(ps):
+        if t > budget_s / 3 or n >= nmax:
             break
-        n = min(len(ps), n * 4)
+        n = min(nmax, n * 2)
     return {"value": n / t, "unit": "ME evals/s", "cores": cores, "kind": "port",
-            "sample": "%d post-cut gg->ttgg events of the benchmark stream, FP64 oracle smatrix, %.1f s" % (n, t)}
+            "sample": "%d post-cut gg->ttgg events of the benchmark stream, FP64 oracle smatrix "
+                      "(C++, OpenMP over events), %.1f s" % (n, t)}
 
 
 def run_plain(a, torch):
@@ -92,7 +94,7 @@ def run_plain(a, torch):
     # run length needed for sigma/sigma = 1e-3 at the measured error per timed point
     t_target = wall * (rel / 1e-3) ** 2
     unw_rate = n_unw / wall
-    return dict(ig=ig, sigma=sigma, err=err, rel=rel, wall=wall, t_warm=t_warm, npass=npass, me_ms=me_ms, me_n=me_n,
+    return dict(npass_last=ig.last_chunk_npass, ig=ig, sigma=sigma, err=err, rel=rel, wall=wall, t_warm=t_warm, npass=npass, me_ms=me_ms, me_n=me_n,
                 n_unw=n_unw, unw_rate=unw_rate, t_target=t_target, eff=sum_w / (npass * wmax) if npass else 0.0)
 
 
@@ -232,7 +234,7 @@ def main(argv=None):
     }
     if not a.no_cpu_baseline:
         b = p["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(b["dense"], min(b["n"], 4096), p["ig"].params)
+        res["cpu_baseline"] = cpu_baseline(b["dense"], p["npass_last"], p["ig"].params)
     if a.compare_flow:
         res["flow_comparison"] = run_flow_compare(a, torch, p)
     print(json.dumps(res))

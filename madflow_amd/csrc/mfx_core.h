@@ -49,4 +49,13 @@ struct Params {
   R sqh;
 };
 
+// Arithmetic forms per precision.  FP32 (value = true) uses cancellation-free forms:
+// propagator P^2 - M^2 from per-event invariants 2 k_i.k_j computed in FP64 (the m_t^2
+// terms cancel exactly, no cancellation in t-channel denominators) and |p| + p_z of
+// massive spinors as pT^2 / (|p| - p_z) for p_z < 0.  FP64 (value = false) uses the
+// reference's own expressions (Lorentz square of the current's slot momentum, |p| + p_z),
+// so the FP64 build reproduces the reference to rounding.
+template <class R> struct StableForms { static constexpr bool value = false; };
+template <> struct StableForms<float> { static constexpr bool value = true; };
+
 }  // namespace mfx

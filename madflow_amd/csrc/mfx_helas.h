@@ -22,6 +22,17 @@ namespace mfx {
 template <class R> __device__ __forceinline__ R sel(bool c, R a, R b) { return c ? a : b; }
 template <class R> __device__ __forceinline__ R tfsign(R y) { return y > R(0) ? R(1) : (y < R(0) ? R(-1) : R(0)); }
 
+// |p| + p_z of a massive spinor.  FP64: the reference's expression max(pp + pz, 0).
+// FP32: for pz < 0 the equivalent pT^2 / (pp - pz), which has no cancellation when the
+// particle moves close to -z (pp + pz -> 0 loses all FP32 digits there).
+template <class R> __device__ __forceinline__ R plus_component(const R (&p)[4], R pp) {
+  if constexpr (StableForms<R>::value) {
+    return p[3] >= R(0) ? pp + p[3] : (p[1] * p[1] + p[2] * p[2]) / (pp - p[3]);
+  } else {
+    return fmax(pp + p[3], R(0));
+  }
+}
+
 // ------------------------------------------------------------------------ ixxxxx
 template <class R>
 __device__ __forceinline__ void ixxxxx(const R (&p)[4], R fmass, R nhel, R nsf, W<R>& fi) {
@@ -44,7 +55,7 @@ __device__ __forceinline__ void ixxxxx(const R (&p)[4], R fmass, R nhel, R nsf, 
     const R om1 = fmass / om0;
     const R sfomeg0 = sf0 * (hpos ? om1 : om0);
     const R sfomeg1 = sf1 * (hpos ? om0 : om1);
-    const R pp3 = fmax(pp + p[3], R(0));
+    const R pp3 = plus_component(p, pp);
     const R d = sqrt(R(2) * pp * pp3);
     const bool z = pp3 == R(0);
     const cx<R> chi1(sel(z, -nh, nh * p[1] / d), sel(z, R(0), p[2] / d));
@@ -93,7 +104,7 @@ __device__ __forceinline__ void oxxxxx(const R (&p)[4], R fmass, R nhel, R nsf, 
     const R om1 = fmass / om0;
     const R sfomeg0 = sf0 * (hpos ? om1 : om0);
     const R sfomeg1 = sf1 * (hpos ? om0 : om1);
-    const R pp3 = fmax(pp + p[3], R(0));
+    const R pp3 = plus_component(p, pp);
     const R d = sqrt(R(2) * pp * pp3);
     const bool z = pp3 == R(0);
     const cx<R> chi1(sel(z, -nh, nh * p[1] / d), sel(z, R(0), -p[2] / d));

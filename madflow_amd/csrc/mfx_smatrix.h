@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
     R p[T::NEXTERNAL][4];
     R inv[T::NINV];
     load_momenta<T, R>(a.all_ps, e, p);
-    load_invariants<T, R>(a.all_ps, e, inv);
+    if constexpr (StableForms<R>::value) load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     R ans = R(0);
 #pragma unroll 1
@@ -120,8 +120,10 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
       for (int l = 0; l < T::NEXTERNAL; ++l)
 #pragma unroll
         for (int k = 0; k < 4; ++k) opaque(p[l][k]);
+      if constexpr (StableForms<R>::value) {
 #pragma unroll
-      for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
+        for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
+      }
       cx<R> jamp[T::NCOLOR];
       T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
       ans += colour_sum<T, R>(jamp);
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(256) k_matrix_hel(SmatrixArgs a) {
     R p[T::NEXTERNAL][4];
     R inv[T::NINV];
     load_momenta<T, R>(a.all_ps, e, p);
-    load_invariants<T, R>(a.all_ps, e, inv);
+    if constexpr (StableForms<R>::value) load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     cx<R> jamp[T::NCOLOR];
     T::template amplitudes<R>(p, inv, 0u, a.helv, prm, jamp);

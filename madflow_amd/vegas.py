@@ -102,6 +102,7 @@ class MEIntegrand:
         self.events_limit = int(events_limit)
         self._bufs = None
         self.me_timer = None      # list of (start event, end event, n) around smatrix launches
+        self.last_chunk_npass = 0
 
     def _alloc(self, n):
         if self._bufs is not None and self._bufs["n"] >= n:
@@ -134,6 +135,7 @@ class MEIntegrand:
             _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["ws"]), b["ws"].numel(),
             ctypes.byref(npass), s))
         npass = npass.value
+        self.last_chunk_npass = npass
         if self.me_timer is not None and npass:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)

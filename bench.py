@@ -1,23 +1,29 @@
-"""Benchmark: g g > t t~ g g on one MI355X (the configuration the headline metric is quoted on).
+"""Benchmark: g g > t t~ g g on one MI355X.
 
-Workload: LO g g > t t~ g g at fixed partonic sqrt(s) = 3 TeV, SM couplings (m_t = 173,
-Gamma_t = 1.4915, alpha_s = 0.118), cuts pT(g) > 30 GeV and Delta R(g,g) > 0.4, RAMBO
-phase space (16 VEGAS dimensions, 50 bins each), 1.6e7 points per VEGAS iteration, FP32
-wavefunctions/amplitudes/colour sum, FP64 phase space and VEGAS accumulation.
+Workload (default): LO g g > t t~ g g at a fixed partonic sqrt(s) = 3 TeV. SM couplings:
+m_t = 173, Gamma_t = 1.4915, alpha_s = 0.118. Cuts: pT(g) > 30 GeV and Delta R(g,g) > 0.4.
+Phase space is RAMBO on diet on the 8-dimensional VEGAS cube (50 bins per dimension),
+with 1.6e7 points per VEGAS iteration. Wavefunctions, amplitudes and the colour sum are
+FP32; phase space and VEGAS accumulation are FP64.
 
-One step = one VEGAS iteration (frozen grid after the warm-up iterations): Philox
-uniforms -> VEGAS map -> RAMBO -> cuts -> compaction -> fused smatrix kernel ->
-compensated accumulation, all device resident.
+One step is one VEGAS iteration on the frozen grid, after the warm-up iterations have
+adapted it: Philox uniforms -> VEGAS map -> phase space -> cuts -> compaction -> fused
+smatrix kernel -> compensated accumulation. Everything is device resident.
 
-Reported:
-  value                  ME evaluations per second over the timed steps (whole pipeline)
-  unweighted_events_per_s  N_unw / wall time, N_unw = sum w / max w over the timed steps,
-                           for the run continued (extrapolated at the measured per-step error)
-                           until sigma/sigma = 1e-3
-  roofline               the smatrix kernel: oracle flop count per ME x MEs / kernel time
-  cpu_baseline           the CPU oracle (C++, OpenMP) on a bounded sample of the same events
+Reported (one JSON line):
+  value                      ME evaluations per second over the K timed steps (whole pipeline)
+  me_kernel_evals_per_s      the smatrix kernel alone (HIP events around its launches)
+  roofline                   smatrix kernel: oracle flop count per ME x MEs / kernel time
+  unweighted                 N_unw = sum w / max w over the frozen-grid sample, per second:
+                             "steady" over the timed steps, and "to_target" with the warm-up
+                             counted and the run continued (extrapolated) to sigma/sigma = 1e-3
+  flow_comparison            plain VEGAS with FP64 matrix elements vs VEGAS + learned
+                             RQ-spline proposal with FP32 matrix elements, both to
+                             sigma/sigma = 1e-3 (set-up counted), and their steady rates
+  cpu_baseline               the CPU oracle (C++ FP64, OpenMP over events) on a bounded
+                             sample of the same post-cut events
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--compare-flow]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-flow] [--no-cpu-baseline]
 """
 
 import argparse
@@ -32,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 packed, FP64)
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
+KIN = dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4)
 
 
 def log(*a):
@@ -40,7 +47,7 @@ def log(*a):
 
 def cpu_baseline(dense, npass, params, budget_s=12.0):
     """Time the CPU oracle (FP64 C++, OpenMP over events) on the first passing events of
-    the last iteration: doubles the sample until one timed call takes budget_s / 3."""
+    the last chunk: doubles the sample until one timed call takes budget_s / 3."""
     import numpy as np
     from oracle import oracle as O
     cores = min(16, os.cpu_count() or 1)
@@ -62,137 +69,180 @@ def cpu_baseline(dense, npass, params, budget_s=12.0):
                       "(C++, OpenMP over events), %.1f s" % (n, t)}
 
 
-def run_plain(a, torch):
-    from madflow_amd.vegas import MEIntegrand, VegasFlow, combine
-    ig = MEIntegrand("gg_ttxgg", sqrts=3000.0, precision=a.precision, mapping=a.mapping, pt_min=30.0, dr_min=0.4,
-                     seed=3, events_limit=a.events_limit)
+def _unweighted(res, setup_s, sample_s, target):
+    """sigma, error, N_unw = sum w / max w, and unweighted events per second: steady (the
+    sample alone) and to the target (set-up counted; if the sample stopped short of the
+    target, its length is extrapolated at the measured variance)."""
+    from madflow_amd.vegas import combine
+    s, e = combine(res)
+    rel = e / s if s else float("inf")
+    sum_w = math.fsum(r["sum_f"] for r in res)
+    wmax = max(r["wmax"] for r in res)
+    n_unw = sum_w / wmax if wmax > 0 else 0.0
+    npts = sum(r["n"] for r in res)
+    t_sample = sample_s * max(1.0, (rel / target) ** 2)
+    n_unw_target = n_unw * t_sample / sample_s
+    return dict(sigma_pb=s, err_pb=e, rel_err=rel, iterations=len(res), points=npts, setup_s=setup_s,
+                sample_s=sample_s, reached_target=rel <= target, seconds_to_target=setup_s + t_sample,
+                n_unweighted=n_unw, unweighting_efficiency=sum_w / (npts * wmax) if wmax > 0 else 0.0,
+                unweighted_per_s_steady=n_unw / sample_s,
+                unweighted_per_s_to_target=n_unw_target / (setup_s + t_sample))
+
+
+def _run_to_target(iterate, target, max_iter, tag):
+    from madflow_amd.vegas import combine
+    res = []
+    for k in range(max_iter):
+        res.append(iterate(k))
+        s, e = combine(res)
+        log("%s iteration %d: %.6g +- %.3g (combined rel %.2e)" % (tag, k + 1, res[-1]["I"],
+                                                                    math.sqrt(res[-1]["var"]), e / s if s else 0))
+        if s and e / s <= target:
+            break
+    return res
+
+
+def run_main(a, torch):
+    """Plain VEGAS at the main precision: W warm-up iterations, then exactly K timed ones."""
+    from madflow_amd.vegas import MEIntegrand, VegasFlow
+    ig = MEIntegrand("gg_ttxgg", precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
+                     **KIN)
     vf = VegasFlow(ig.ndim, a.nev)
     vf.compile(ig)
     log("warm-up: %d VEGAS iterations of %d points (grid adaptation)" % (a.warmup, a.nev))
-    t_w0 = time.perf_counter()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     vf.run_integration(a.warmup, log=log)
     torch.cuda.synchronize()
-    t_warm = time.perf_counter() - t_w0
+    t_warm = time.perf_counter() - t0
     vf.freeze_grid()
     ig.me_timer = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = []
-    for k in range(a.steps):
-        r = vf.run_integration(1, log=log)
-        res.append(vf.history[-1])
+    for _ in range(a.steps):
+        vf.run_integration(1, log=log)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    sigma, err = combine(res)
+    res = vf.history[-a.steps:]
     me_ms = sum(e0.elapsed_time(e1) for e0, e1, _ in ig.me_timer)
     me_n = sum(n for _, _, n in ig.me_timer)
-    npass = sum(r["npass"] for r in res)
-    sum_w = math.fsum(r["sum_f"] for r in res)
-    wmax = max(r["wmax"] for r in res)
-    n_unw = sum_w / wmax if wmax > 0 else 0.0
-    rel = err / sigma if sigma else float("inf")
-    # run length needed for sigma/sigma = 1e-3 at the measured error per timed point
-    t_target = wall * (rel / 1e-3) ** 2
-    unw_rate = n_unw / wall
-    return dict(npass_last=ig.last_chunk_npass, ig=ig, sigma=sigma, err=err, rel=rel, wall=wall, t_warm=t_warm, npass=npass, me_ms=me_ms, me_n=me_n,
-                n_unw=n_unw, unw_rate=unw_rate, t_target=t_target, eff=sum_w / (npass * wmax) if npass else 0.0)
+    launches = len(ig.me_timer)
+    ig.me_timer = None
+    return dict(ig=ig, res=res, wall=wall, t_warm=t_warm, me_ms=me_ms, me_n=me_n, launches=launches,
+                npass=sum(r["npass"] for r in res), unw=_unweighted(res, t_warm, wall, a.target))
 
 
-def run_flow_compare(a, torch, plain):
-    """Learned proposal on the 8-dimensional RAMBO-on-diet cube vs plain VEGAS (FP64 ME)
-    at equal wall time: unweighted events per second."""
-    from madflow_amd.vegas import MEIntegrand, VegasFlow, combine
+def run_flow_compare(a, torch, main):
+    """Plain VEGAS with FP64 matrix elements against VEGAS + learned proposal with FP32
+    matrix elements on the same cube, each run to sigma/sigma = target.  The flow path
+    reuses the FP32 grid of the main run (its warm-up time is counted in its set-up),
+    then generates the training pairs, trains, and samples y ~ q with weights f(y)/q(y)."""
+    from madflow_amd.vegas import MEIntegrand, VegasFlow
     from madflow_amd.flow import RQSFlow
-    out = {}
-    # plain FP64 VEGAS path (reference precision) on the same cube
-    ig64 = MEIntegrand("gg_ttxgg", sqrts=3000.0, precision=64, mapping="diet", pt_min=30.0, dr_min=0.4, seed=5,
-                       events_limit=a.events_limit)
+    from madflow_amd.phasespace_gpu import philox_uniforms
+    sync = torch.cuda.synchronize
+    # ---- plain VEGAS, FP64 matrix elements
+    ig64 = MEIntegrand("gg_ttxgg", precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit, **KIN)
     vf = VegasFlow(ig64.ndim, a.nev)
     vf.compile(ig64)
-    vf.run_integration(a.warmup)
+    sync()
+    t0 = time.perf_counter()
+    vf.run_integration(a.warmup, log=log)
     vf.freeze_grid()
-    torch.cuda.synchronize()
+    sync()
+    setup64 = time.perf_counter() - t0
+
+    def it64(k):
+        vf.run_integration(1)
+        return vf.history[-1]
     t0 = time.perf_counter()
-    res = [vf.run_integration(1) and vf.history[-1] for _ in range(a.steps)]
-    torch.cuda.synchronize()
-    w64 = time.perf_counter() - t0
-    s64, e64 = combine(res)
-    unw64 = math.fsum(r["sum_f"] for r in res) / max(r["wmax"] for r in res) / w64
-    out["plain_fp64_vegas"] = dict(sigma=s64, err=e64, unweighted_per_s=unw64, seconds=w64)
-    # training data: (y, f) pairs on the frozen FP32 grid
-    ig = MEIntegrand("gg_ttxgg", sqrts=3000.0, precision=32, mapping="diet", pt_min=30.0, dr_min=0.4, seed=7,
-                     events_limit=a.events_limit)
-    ig.grid.copy_(ig64.grid)
+    res64 = _run_to_target(it64, a.target, a.max_iter, "plain fp64")
+    sync()
+    plain = _unweighted(res64, setup64, time.perf_counter() - t0, a.target)
+    del ig64, vf
+    # ---- VEGAS + flow, FP32 matrix elements, on the main run's frozen grid
+    ig = main["ig"]
+    D = ig.ndim
+    sync()
+    t0 = time.perf_counter()
     ntrain = a.flow_pairs
-    y = torch.empty((ntrain, ig.ndim), dtype=torch.float32, device="cuda")
+    y = torch.empty((ntrain, D), dtype=torch.float32, device="cuda")
     f = torch.empty(ntrain, dtype=torch.float64, device="cuda")
-    from madflow_amd.phasespace_gpu import philox_uniforms
-    t0 = time.perf_counter()
     done = 0
     while done < ntrain:
         n = min(a.events_limit, ntrain - done)
-        yy = philox_uniforms(n, ig.ndim, 99, 0, ev0=done).float()
-        _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=yy.contiguous())
+        yy = philox_uniforms(n, D, 99, 0, ev0=done).float().contiguous()
+        _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=yy)
         y[done:done + n] = yy
         f[done:done + n] = fb
         done += n
-    flow = RQSFlow(ig.ndim, layers=8, seed=1)
-    flow.train(y, f, steps=a.flow_steps, batch=a.flow_batch, lr=2e-3, log=log)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    # sampling with the flow: u -> y, weight 1/q(y)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = []
-    for k in range(a.steps):
-        tot_f, tot_f2, wmax, npass = 0.0, 0.0, 0.0, 0
-        sums = []
-        done = 0
+    sync()
+    t_pairs = time.perf_counter() - t0
+    flow = RQSFlow(D, layers=8, seed=1)
+    flow.train(y, f, steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log)
+    del y, f
+    sync()
+    t_train = time.perf_counter() - t0 - t_pairs
+    setup_flow = main["t_warm"] + t_pairs + t_train
+
+    def itflow(k):
+        sums, done = [], 0
         while done < a.nev:
             n = min(a.events_limit, a.nev - done)
-            u = philox_uniforms(n, ig.ndim, 1234, 100 + k, ev0=done).float()
+            u = philox_uniforms(n, D, 1234, 100 + k, ev0=done).float()
             ys, logq = flow.sample(u)
-            winv = torch.exp(-logq.double())
-            r, npk, _ = ig.chunk(done, n, 100 + k, y_in=ys.contiguous(), w_in=winv.contiguous())
+            winv = torch.exp(-logq.double()).contiguous()
+            r, _, _ = ig.chunk(done, n, 100 + k, y_in=ys.contiguous(), w_in=winv)
             sums.append(r)
-            npass += npk
             done += n
         s1 = math.fsum(r[0] for r in sums)
         s2 = math.fsum(r[1] for r in sums)
-        wmax = max(r[2] for r in sums)
         N = float(a.nev)
         mean = s1 / N
-        var = max((s2 / N - mean * mean) / (N - 1), 0.0)
-        res.append(dict(I=mean, var=var, sum_f=s1, wmax=wmax))
-    torch.cuda.synchronize()
-    wf = time.perf_counter() - t0
-    sf, ef = combine(res)
-    unwf = math.fsum(r["sum_f"] for r in res) / max(r["wmax"] for r in res) / wf
-    out["flow_fp32"] = dict(sigma=sf, err=ef, unweighted_per_s=unwf, seconds=wf, train_seconds=t_train,
-                            train_pairs=ntrain, train_steps=a.flow_steps)
-    out["gain_unweighted_per_s"] = unwf / unw64 if unw64 > 0 else None
-    out["agree_sigma"] = abs(sf - s64) / math.hypot(ef, e64)
-    return out
+        return dict(I=mean, var=max((s2 / N - mean * mean) / (N - 1), 0.0), sum_f=s1,
+                    wmax=max(r[2] for r in sums), n=a.nev)
+    t0 = time.perf_counter()
+    resf = _run_to_target(itflow, a.target, a.max_iter, "flow fp32")
+    sync()
+    learned = _unweighted(resf, setup_flow, time.perf_counter() - t0, a.target)
+    learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"])
+
+    def gain(k):
+        return learned[k] / plain[k] if plain[k] > 0 else None
+    return {
+        "workload": "gg->ttgg, sqrt(s)=3 TeV, pT(g)>30, dR(g,g)>0.4, %d-dim %s VEGAS cube, %d points/iteration, "
+                    "target sigma/sigma=%g" % (D, a.mapping, a.nev, a.target),
+        "plain_vegas_fp64": plain,
+        "vegas_flow_fp32": learned,
+        "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
+                 "train_pairs": ntrain, "train_steps": a.flow_steps, "batch": a.flow_batch},
+        "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
+        "gain_unweighted_per_s_steady": gain("unweighted_per_s_steady"),
+        "sigma_pull": (learned["sigma_pb"] - plain["sigma_pb"]) / math.hypot(learned["err_pb"], plain["err_pb"]),
+    }
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nev", type=int, default=16_000_000)
     ap.add_argument("--events-limit", type=int, default=1 << 22)
-    ap.add_argument("--precision", type=int, default=32)
-    ap.add_argument("--mapping", default="rambo")
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
+    ap.add_argument("--mapping", default="diet", choices=["diet", "rambo"])
+    ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--max-iter", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--compare-flow", action="store_true")
+    ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
     ap.add_argument("--flow-steps", type=int, default=20000)
     ap.add_argument("--flow-batch", type=int, default=4096)
+    ap.add_argument("--flow-lr", type=float, default=1e-3)
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build measures a single MI355X "
-              "(the multi-GPU sharding of the event loop is a design note, not a measured path)", file=sys.stderr)
+        print("bench.py measures one GPU: the event loop shards by Philox counter range, but this build "
+              "runs and reports a single MI355X only (--gpus 1)", file=sys.stderr)
         return 2
     import torch
     if not torch.cuda.is_available():
@@ -200,13 +250,15 @@ def main(argv=None):
         return 2
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
         flops = json.load(fh)["gg_ttxgg"]["total"]
-    p = run_plain(a, torch)
-    kernel_s = p["me_ms"] * 1e-3
-    achieved = flops * p["me_n"] / kernel_s / 1e12
+    m = run_main(a, torch)
+    kernel_s = m["me_ms"] * 1e-3
+    achieved = flops * m["me_n"] / kernel_s / 1e12
     peak = PEAK_TFLOPS[a.precision]
-    me_rate = p["npass"] / p["wall"]
-    log("sigma = %.6g +- %.3g pb (rel %.2e)  ME/s %.4g  kernel-only ME/s %.4g  unweighted/s %.4g  eff %.3g" % (
-        p["sigma"], p["err"], p["rel"], me_rate, p["me_n"] / kernel_s, p["unw_rate"], p["eff"]))
+    me_rate = m["npass"] / m["wall"]
+    u = m["unw"]
+    log("sigma = %.6g +- %.3g pb (rel %.2e)  ME/s %.4g  kernel ME/s %.4g  unweighted/s steady %.4g  eff %.3g" % (
+        u["sigma_pb"], u["err_pb"], u["rel_err"], me_rate, m["me_n"] / kernel_s, u["unweighted_per_s_steady"],
+        u["unweighting_efficiency"]))
     res = {
         "metric": METRIC,
         "value": me_rate,
@@ -214,29 +266,31 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": p["wall"] / a.steps * 1e3,
+        "ms_per_step": m["wall"] / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if a.precision == 32 else "f64",
         "data": "synthetic (Philox4x32-10 points, seed 3)",
-        "config": {"workload": "gg->ttgg LO, sqrt(s)=3 TeV, pT(g)>30, dR(g,g)>0.4, RAMBO 16-dim VEGAS, "
-                               "%d points/iteration, FP32 amplitudes, FP64 VEGAS sums" % a.nev,
-                   "points_per_step": a.nev, "passing_per_step": p["npass"] // a.steps},
-        "unweighted_events_per_s": p["unw_rate"],
-        "unweighting_efficiency": p["eff"],
-        "sigma_pb": p["sigma"], "sigma_err_pb": p["err"],
-        "seconds_to_rel_err_1e-3": p["t_target"],
-        "me_kernel_evals_per_s": p["me_n"] / kernel_s,
+        "config": {"workload": "gg->ttgg LO, sqrt(s)=3 TeV, pT(g)>30 GeV, dR(g,g)>0.4, %s phase space on a "
+                               "%d-dim 50-bin VEGAS grid, %d points/iteration, %s amplitudes and colour sum, FP64 "
+                               "phase space and VEGAS sums" % (a.mapping, m["ig"].ndim, a.nev,
+                                                               "FP32" if a.precision == 32 else "FP64"),
+                   "points_per_step": a.nev, "passing_per_step": m["npass"] // a.steps},
+        "me_kernel_evals_per_s": m["me_n"] / kernel_s,
+        "unweighted": u,
+        "unweighted_events_per_s": u["unweighted_per_s_to_target"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "flops_per_me": flops, "kernel_ms_per_step": p["me_ms"] / a.steps},
+                     "kernel": "k_smatrix<gg_ttxgg,%s>" % ("float" if a.precision == 32 else "double"),
+                     "flops_per_me": flops, "mes_per_launch": m["me_n"] / max(1, m["launches"]),
+                     "kernel_ms_per_launch": m["me_ms"] / max(1, m["launches"])},
     }
     if not a.no_cpu_baseline:
-        b = p["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(b["dense"], p["npass_last"], p["ig"].params)
-    if a.compare_flow:
-        res["flow_comparison"] = run_flow_compare(a, torch, p)
+        b = m["ig"]._bufs
+        res["cpu_baseline"] = cpu_baseline(b["dense"], m["ig"].last_chunk_npass, m["ig"].params)
+    if not a.no_flow:
+        res["flow_comparison"] = run_flow_compare(a, torch, m)
     print(json.dumps(res))
     return 0
 

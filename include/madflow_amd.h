@@ -42,24 +42,34 @@ int mfx_process_info(int proc, int* info);
  * Replaces Matrix_<proc>.smatrix (matrix_method_python.inc:80-104) and the custom op
  * Matrix<proc> (custom_op/constants.py:104-113): sum over every helicity row of the
  * colour-summed |M|^2, divided by the process denominator.
- *   all_ps      (nev, nexternal, 4) f64 device, (E, px, py, pz)
+ *   all_ps      (nev, nexternal, 4) f64 device, (E, px, py, pz), 16-byte aligned
  *   mass_width  host [mdl_MT, mdl_WT]
  *   gc          host [3][2] frozen couplings GC_10, GC_11, GC_12 (re, im); used when gc_ev is NULL
  *   gc_ev       NULL or device [3][nev][2] per-event couplings (running alpha_s)
  *   out         (nev,) f64 device
  *   precision   64 (FP64 build) or 32 (FP32 wavefunctions/amplitudes/colour sum)
  *   ref_compat  1: sqrt(1/2) rounded through float32 like the reference (wavefunctions_flow.py:10)
+ *   workspace   device scratch owned by the caller, 256-byte aligned (NULL/0 for the fused
+ *               processes).  Processes evaluated by the colour-ordered kernels stage their
+ *               jamps there; any size of at least one 16-event tile works, the library
+ *               splits events and helicity rows to fit.
  */
 int mfx_smatrix(int proc, const double* all_ps, int64_t nev, const double* mass_width, const double* gc,
-                const double* gc_ev, double* out, int precision, int ref_compat, void* stream);
+                const double* gc_ev, double* out, int precision, int ref_compat, void* workspace,
+                int64_t workspace_bytes, void* stream);
+
+/* Bytes of workspace for one pass over nev events (per_helicity = 1: for mfx_matrix);
+ * 0 when the process needs none.  No counterpart in the reference: its TF op allocated
+ * its intermediates itself (custom_op/generation.py:206-211). */
+int64_t mfx_smatrix_workspace_size(int proc, int64_t nev, int precision, int per_helicity);
 
 /* matrix(all_ps, hel, ...) of one helicity vector (matrix_method_python.inc:106-138):
  * colour-summed |M|^2 of that row (no averaging).  hel: host int[nexternal], entries
  * +-1, or 4 on a gluon for the BRST/gauge check (wavefunctions_flow.py:146-152).
- * jamp_out: NULL or device (nev, ncolor, 2) f64. */
+ * jamp_out: NULL or device (nev, ncolor, 2) f64.  workspace as for mfx_smatrix. */
 int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, const double* mass_width,
                const double* gc, const double* gc_ev, double* out, double* jamp_out, int precision,
-               int ref_compat, void* stream);
+               int ref_compat, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------ HELAS / ALOHA building blocks */
 /* External wavefunction for nev momenta p (nev,4): kind 0 vxxxxx, 1 oxxxxx, 2 ixxxxx

@@ -31,8 +31,9 @@ _SIGNATURES = {
     "mfx_process_name": ([c_int], ctypes.c_char_p),
     "mfx_process_index": ([ctypes.c_char_p], c_int),
     "mfx_process_info": ([c_int, c_vp], c_int),
-    "mfx_smatrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),
-    "mfx_matrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),
+    "mfx_smatrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp], c_int),
+    "mfx_smatrix_workspace_size": ([c_int, c_i64, c_int, c_int], c_i64),
+    "mfx_matrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp], c_int),
     "mfx_wavefunction": ([c_int, c_vp, c_i64, c_dbl, c_int, c_int, c_vp, c_int, c_int, c_vp], c_int),
     "mfx_aloha": ([c_int, c_vp, c_int, c_i64, c_vp, c_dbl, c_dbl, c_vp, c_int, c_vp], c_int),
     "mfx_rambo": ([c_vp, c_i64, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp], c_int),

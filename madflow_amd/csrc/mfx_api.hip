@@ -4,6 +4,7 @@
 #include "mfx_api_common.h"
 #include "../../include/madflow_amd.h"
 #include <string.h>
+#include <stdio.h>
 
 namespace mfx {
 static const int MAXPROC = 16;
@@ -164,17 +165,45 @@ int mfx_process_info(int proc, int* info) {
   return 0;
 }
 
+// workspace bytes for one pass over nev events (0: the process needs none)
+int64_t mfx_smatrix_workspace_size(int proc, int64_t nev, int precision, int per_helicity) {
+  if (proc < 0 || proc >= NPROC) return fail(-1, "unknown process index%s", "");
+  if (precision != 32 && precision != 64) return fail(-5, "precision must be 32 or 64%s", "");
+  const ProcessEntry& p = process(proc);
+  if (!p.ws_per_row || nev <= 0) return 0;
+  return nev * (per_helicity ? 1 : p.ncomb) * p.ws_per_row(precision);
+}
+
+static int check_workspace(const ProcessEntry& p, int precision, int per_helicity, int64_t nev, const void* ws,
+                           int64_t ws_bytes) {
+  if (!p.ws_per_row || nev <= 0) return 0;
+  const int64_t need = 16 * p.ws_per_row(precision);   // one 16-event tile of one helicity row
+  if (!ws || ws_bytes < need) {
+    char msg[192];
+    snprintf(msg, sizeof(msg), "process %s needs a device workspace of at least %lld bytes "
+             "(mfx_smatrix_workspace_size gives the size for one pass)", p.name, (long long)need);
+    return fail(-8, "%s", msg);
+  }
+  if (((uintptr_t)ws) % 256 != 0) return fail(-6, "workspace must be 256-byte aligned%s", "");
+  (void)per_helicity;
+  return 0;
+}
+
 int mfx_smatrix(int proc, const double* all_ps, int64_t nev, const double* mass_width, const double* gc,
-                const double* gc_ev, double* out, int precision, int ref_compat, void* stream) {
+                const double* gc_ev, double* out, int precision, int ref_compat, void* workspace,
+                int64_t workspace_bytes, void* stream) {
   SmatrixArgs a;
   int rc = fill_args(proc, all_ps, nev, mass_width, gc, gc_ev, out, precision, ref_compat, a);
   if (rc) return rc;
-  return hip_status(process(proc).launch(a, precision, 0, (hipStream_t)stream));
+  const ProcessEntry& p = process(proc);
+  rc = check_workspace(p, precision, 0, nev, workspace, workspace_bytes);
+  if (rc) return rc;
+  return hip_status(p.launch(a, precision, 0, workspace, workspace_bytes, (hipStream_t)stream));
 }
 
 int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, const double* mass_width,
                const double* gc, const double* gc_ev, double* out, double* jamp_out, int precision,
-               int ref_compat, void* stream) {
+               int ref_compat, void* workspace, int64_t workspace_bytes, void* stream) {
   SmatrixArgs a;
   int rc = fill_args(proc, all_ps, nev, mass_width, gc, gc_ev, out, precision, ref_compat, a);
   if (rc) return rc;
@@ -184,9 +213,11 @@ int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, cons
     if (hel[l] != 1 && hel[l] != -1 && hel[l] != 4) return fail(-7, "helicities must be +-1 (or 4 on a gluon)%s", "");
     a.helv[l] = hel[l];
   }
+  rc = check_workspace(p, precision, 1, nev, workspace, workspace_bytes);
+  if (rc) return rc;
   a.use_helv = 1;
   a.jamp_out = jamp_out;
-  return hip_status(p.launch(a, precision, 1, (hipStream_t)stream));
+  return hip_status(p.launch(a, precision, 1, workspace, workspace_bytes, (hipStream_t)stream));
 }
 
 int mfx_wavefunction(int kind, const double* p, int64_t nev, double mass, int nhel, int nsign, double* out,

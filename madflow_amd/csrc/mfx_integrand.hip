@@ -66,6 +66,22 @@ __device__ __forceinline__ void uniforms(const GenConst& g, uint64_t ev, uint32_
   }
 }
 
+// A point of an external proposal: clamped into [0, 1] so that bin indices stay in
+// range whatever the proposal returns; a non-finite coordinate maps to 0 and the event
+// is rejected (returns false).
+__device__ __forceinline__ bool external_point(const float* __restrict__ src, int ndim, double* y) {
+  bool ok = true;
+  for (int d = 0; d < ndim; ++d) {
+    double v = (double)src[d];
+    if (!isfinite(v)) {
+      ok = false;
+      v = 0.0;
+    }
+    y[d] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+  }
+  return ok;
+}
+
 __device__ __forceinline__ double vmap(const GenConst& g, const double* __restrict__ grid, const double* y, double* x,
                                        int* bins) {
   double jac = 1.0;
@@ -116,15 +132,16 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
   bool ok = false;
   if (e < nev) {
     double y[MAXDIM], x[MAXDIM];
+    bool y_ok = true;
     if (y_in) {
-      for (int d = 0; d < g.ndim; ++d) y[d] = (double)y_in[e * g.ndim + d];
+      y_ok = external_point(y_in + e * g.ndim, g.ndim, y);
     } else {
       uniforms(g, ev0 + e, iteration, y);
     }
     const double jac = vmap(g, grid, y, x, nullptr);
     double p[RAMBO_MAXN][4];
     const double w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p) : diet_event(g.rc, x, g.sqrts, p);
-    ok = pass_cuts(g, p) && isfinite(w);
+    ok = y_ok && pass_cuts(g, p) && isfinite(w);
     double* out = all_ps + e * (g.nexternal * 4);
     const double eh = 0.5 * g.sqrts;
     out[0] = eh; out[1] = 0.0; out[2] = 0.0; out[3] = eh;
@@ -237,7 +254,7 @@ __global__ void __launch_bounds__(ACC_BLOCK) k_accumulate(GenConst g, uint64_t e
       if (!isfinite(f)) f = 0.0;          // a NaN/inf weight must not poison the grid
       double y[MAXDIM];
       if (y_in) {
-        for (int d = 0; d < D; ++d) y[d] = (double)y_in[e * D + d];
+        external_point(y_in + e * D, D, y);
       } else {
         uniforms(g, ev0 + e, iteration, y);
       }

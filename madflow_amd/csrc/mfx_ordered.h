@@ -1,0 +1,413 @@
+// Colour-ordered evaluation of g g > t t~ + n gluons, for processes whose colour flows
+// (n! jamps) do not fit the register file next to a Feynman-diagram program.
+//
+// The jamps are the coefficients of the fundamental-string colour basis
+// (T^{a_s1} ... T^{a_sn})_{i_t j_tbar}, the basis of the generated modules'
+// jamp lines (madgraph_plugin/PyOut_exporter.py:334-375).  For one ordering s of the
+// gluons the jamp is the colour-ordered amplitude, computed by the Berends-Giele
+// recursion over consecutive ranges of s (no Feynman diagram is enumerated):
+//
+//   J(i..j) = 1/P^2 [ GC_10 sum_{i<=k<j} N3(J(i..k), J(k+1..j))
+//                     + (-i GC_12) sum_{i<=k<l<j} N4(J(i..k), J(k+1..l), J(l+1..j)) ]
+//   O(0) = top,  O(j) = sum_{i<j} FFV1_1(O(i), J(i..j-1))
+//   jamp_s = - sum_{i<n} FFV1_0(anti-top, O(i), J(i..n-1))
+//
+// N3 is the VVV1 Lorentz structure of VVV1P0_1 without its propagator; the colour-
+// ordered coefficient of the f^{abc} vertex (-i for the ordered word) cancels its i.
+// N4 = 2 V3 (V2.V4) - V2 (V3.V4) - V4 (V2.V3) is the combination VVVV4 - VVVV1 of the
+// SM four-gluon Lorentz structures that the word order selects.  The -1 is the
+// fermion-flow factor of the generated jamps.  Summed over the Feynman diagrams
+// consistent with an ordering, these give exactly the jamp of that ordering.
+//
+// Kernel 1 (k_ordered_jamps): one event per lane, every helicity row, every ordering;
+// each finished jamp is stored to a workspace [helicity][colour][event] (coalesced),
+// so no lane ever holds more than one ordering's currents.
+// Kernel 2 (k_colour_mfma): |M|^2 = sum_h Re(J_h^dagger CF J_h) / (CF_DEN * DENOMINATOR)
+// as a batched dense contraction on the matrix cores: per helicity, a 16-event tile of
+// Re J and Im J (16 x NCOLOR) times the integer colour matrix (NCOLOR x NCOLOR)
+// with v_mfma_{f32,f64}_16x16x4, then the row-wise dot product with J.
+#pragma once
+#include "mfx_smatrix.h"
+
+namespace mfx {
+
+typedef float mfx_f32x4 __attribute__((ext_vector_type(4)));
+typedef double mfx_f64x4 __attribute__((ext_vector_type(4)));
+
+// Lorentz structure of VVV1P0_1 without coupling or propagator; V1 slots = V2 + V3.
+template <class R>
+__device__ __forceinline__ void vvv_numerator(const W<R>& V2, const W<R>& V3, W<R>& N) {
+  R P2[4], P3[4], P1[4];
+  mom_in(V2, P2);
+  mom_in(V3, P3);
+  N[0] = V2[0] + V3[0];
+  N[1] = V2[1] + V3[1];
+  mom_out(N, P1);
+  const cx<R> TMP1 = mdotp(V3, P1), TMP2 = mdotp(V3, P2), TMP3 = mdotp(V2, P1), TMP4 = mdotp(V2, P3);
+  const cx<R> TMP5 = mdot(V3, V2);
+  const cx<R> c2 = TMP2 - TMP1, c3 = TMP3 - TMP4;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) N[2 + mu] = TMP5 * (P3[mu] - P2[mu]) + V2[2 + mu] * c2 + V3[2 + mu] * c3;
+}
+
+// 2 V3 (V2.V4) - V2 (V3.V4) - V4 (V2.V3), added into N[2..5].
+template <class R>
+__device__ __forceinline__ void vvvv_ordered_add(const W<R>& V2, const W<R>& V3, const W<R>& V4, W<R>& N) {
+  const cx<R> d24 = mdot(V2, V4), d34 = mdot(V3, V4), d23 = mdot(V2, V3);
+  const cx<R> t = d24 + d24;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) N[2 + mu] += V3[2 + mu] * t - V2[2 + mu] * d34 - V4[2 + mu] * d23;
+}
+
+template <class R>
+__device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
+#pragma unroll
+  for (int k = 2; k < 6; ++k) acc[k] += t[k];
+}
+
+// One colour-ordered amplitude.  g: gluon wavefunctions in the order of the word;
+// pinv[a][b] (a < b) = 2 k_a.k_b of the gluons at word positions a, b and
+// tinv[a] = 2 k_top.k_a (FP32 forms only); prm couplings.
+template <int NG, class R>
+__device__ __forceinline__ cx<R> ordered_amplitude(const W<R> (&g)[NG], const W<R>& top, const W<R>& atop,
+                                                   const R (&pinv)[NG][NG], const R (&tinv)[NG],
+                                                   const Params<R>& prm) {
+  constexpr bool INV = StableForms<R>::value;
+  W<R> J[NG][NG];
+#pragma unroll
+  for (int i = 0; i < NG; ++i) J[i][i] = g[i];
+  // -i GC_12 multiplies the four-gluon structures (GC_12 = i gs^2: a real gs^2)
+  const cx<R> c4 = mul_mi(prm.GC_12);
+#pragma unroll
+  for (int len = 2; len <= NG; ++len) {
+#pragma unroll
+    for (int i = 0; i + len <= NG; ++i) {
+      const int j = i + len - 1;
+      W<R> N3;
+      vvv_numerator(J[i][i], J[i + 1][j], N3);
+#pragma unroll
+      for (int k = i + 1; k < j; ++k) {
+        W<R> t;
+        vvv_numerator(J[i][k], J[k + 1][j], t);
+        add_components(N3, t);
+      }
+      W<R> N4;
+      N4[0] = N3[0];
+      N4[1] = N3[1];
+#pragma unroll
+      for (int q = 2; q < 6; ++q) N4[q] = cx<R>(R(0), R(0));
+#pragma unroll
+      for (int k = i; k < j; ++k)
+#pragma unroll
+        for (int l = k + 1; l < j; ++l) vvvv_ordered_add(J[i][k], J[k + 1][l], J[l + 1][j], N4);
+      // propagator 1/P^2 of the massless gluon
+      R den2;
+      if constexpr (INV) {
+        den2 = R(0);
+#pragma unroll
+        for (int a = i; a <= j; ++a)
+#pragma unroll
+          for (int b = a + 1; b <= j; ++b) den2 += pinv[a][b];
+      } else {
+        R P[4];
+        mom_out(N3, P);
+        den2 = den_from_slots(P, R(0));
+      }
+      const R rden = R(1) / den2;
+      const cx<R> k3 = prm.GC_10 * rden, k4 = c4 * rden;
+      W<R>& o = J[i][j];
+      o[0] = N3[0];
+      o[1] = N3[1];
+#pragma unroll
+      for (int q = 2; q < 6; ++q) o[q] = k3 * N3[q] + k4 * N4[q];
+    }
+  }
+  // top line: O(j) carries the top and the first j gluons of the word
+  W<R> O[NG];
+  O[0] = top;
+#pragma unroll
+  for (int j = 1; j < NG; ++j) {
+    R den2 = R(0);
+    if constexpr (INV) {
+#pragma unroll
+      for (int a = 0; a < j; ++a) {
+        den2 += tinv[a];
+#pragma unroll
+        for (int b = a + 1; b < j; ++b) den2 += pinv[a][b];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < j; ++i) {
+      W<R> t;
+      if constexpr (INV) FFV1_1(O[i], J[i][j - 1], prm.GC_11, prm.mdl_MT, prm.mdl_WT, den2, t);
+      else FFV1_1(O[i], J[i][j - 1], prm.GC_11, prm.mdl_MT, prm.mdl_WT, t);
+      if (i == 0) O[j] = t;
+      else add_components(O[j], t);
+    }
+  }
+  cx<R> amp(R(0), R(0));
+#pragma unroll
+  for (int i = 0; i < NG; ++i) amp += FFV1_0(atop, O[i], J[i][NG - 1], prm.GC_11);
+  return cx<R>(-amp.re, -amp.im);
+}
+
+struct OrderedArgs {
+  SmatrixArgs s;
+  int64_t ev0;          // first event of this sub-chunk
+  int64_t n;            // events in this sub-chunk
+  int h0, nhel;         // helicity rows h0 .. h0+nhel-1 (or 1 row for an explicit vector)
+  void* jbuf;           // [nhel][NCOLOR][n] complex R
+};
+
+// Per-lane staging in LDS (lane-major: slot s of lane t at s * BLOCK + t), used as an
+// indexable store: the gluon wavefunctions and the pair invariants are read back in the
+// order of each colour ordering (the ordering index is wave-uniform).
+template <class T, class R>
+struct OrderedStage {
+  static constexpr int NG = T::NG;
+  static constexpr int NWF = NG * 12;                  // 5 gluons x 6 complex
+  static constexpr int NINV = StableForms<R>::value ? (NG + 1) * NG / 2 : 0;   // gluon-gluon + top-gluon
+  static constexpr int NSLOT = NWF + NINV;
+};
+
+template <class T, class R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
+  constexpr int NG = T::NG, NE = T::NEXTERNAL, NC = T::NCOLOR;
+  using S = OrderedStage<T, R>;
+  __shared__ R lds[S::NSLOT * BLOCK];
+  const int t = threadIdx.x;
+  const int64_t el = (int64_t)blockIdx.x * BLOCK + t;
+  if (el >= a.n) return;                 // no block-level synchronisation below
+  const int64_t e = a.ev0 + el;
+  R p[NE][4];
+  load_momenta<T, R>(a.s.all_ps, e, p);
+  const Params<R> prm = load_params<R>(a.s, e);
+  if constexpr (S::NINV > 0) {
+    // 2 k_x.k_y in FP64 for x, y among (gluon slots 0..NG-1, top = slot NG)
+    const double* src = a.s.all_ps + e * (NE * 4);
+    double k[NG + 1][4];
+#pragma unroll
+    for (int s = 0; s <= NG; ++s) {
+      const int leg = s < NG ? T::gluon_leg(s) : T::TOP_LEG;
+      const double sg = T::leg_sign(leg);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) k[s][c] = sg * src[4 * leg + c];
+    }
+#pragma unroll
+    for (int x = 0; x <= NG; ++x)
+#pragma unroll
+      for (int y = x + 1; y <= NG; ++y) {
+        const double v = 2.0 * (k[x][0] * k[y][0] - k[x][1] * k[y][1] - k[x][2] * k[y][2] - k[x][3] * k[y][3]);
+        lds[(S::NWF + T::pair_index(x, y)) * BLOCK + t] = R(v);
+      }
+  }
+  cx<R>* jb = reinterpret_cast<cx<R>*>(a.jbuf);
+#pragma unroll 1
+  for (int hh = 0; hh < a.nhel; ++hh) {
+    const unsigned hb = a.s.use_helv ? 0u : T::hel_bits(a.h0 + hh);
+    const int* helv = a.s.use_helv ? a.s.helv : nullptr;
+    W<R> top, atop;
+    T::template fermions<R>(p, hb, helv, prm, top, atop);
+#pragma unroll
+    for (int s = 0; s < NG; ++s) {
+      W<R> gw;
+      T::template gluon<R>(p, s, hb, helv, prm, gw);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        lds[(s * 12 + 2 * q) * BLOCK + t] = gw[q].re;
+        lds[(s * 12 + 2 * q + 1) * BLOCK + t] = gw[q].im;
+      }
+    }
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) {
+      int perm[NG];
+#pragma unroll
+      for (int k = 0; k < NG; ++k) perm[k] = T::perm(c, k);
+      W<R> g[NG];
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          g[k][q].re = lds[(perm[k] * 12 + 2 * q) * BLOCK + t];
+          g[k][q].im = lds[(perm[k] * 12 + 2 * q + 1) * BLOCK + t];
+        }
+      R pinv[NG][NG], tinv[NG];
+      if constexpr (S::NINV > 0) {
+#pragma unroll
+        for (int x = 0; x < NG; ++x) {
+          tinv[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
+#pragma unroll
+          for (int y = x + 1; y < NG; ++y) pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
+        }
+      }
+      const cx<R> amp = ordered_amplitude<NG, R>(g, top, atop, pinv, tinv, prm);
+      jb[((int64_t)hh * NC + c) * a.n + el] = amp;
+    }
+  }
+}
+
+// MFMA 16x16x4 on one wave: D += A(16x4) * B(4x16); lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds D[4*(l/16) + r][l%16], r = 0..3.
+__device__ __forceinline__ mfx_f32x4 mfma16x16x4(float a, float b, mfx_f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mfx_f64x4 mfma16x16x4(double a, double b, mfx_f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <class R> struct Vec4;
+template <> struct Vec4<float> { typedef mfx_f32x4 type; };
+template <> struct Vec4<double> { typedef mfx_f64x4 type; };
+
+// |M|^2 of the events of one sub-chunk from their jamps: one wave per 16-event tile,
+// four tiles per block; the integer colour matrix (padded to NCP columns) sits in LDS.
+template <class T, class R>
+__global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n, int nhel, double scale,
+                                                     int accumulate, double* out) {
+  constexpr int NC = T::NCOLOR;
+  constexpr int NCP = (NC + 15) / 16 * 16;    // padded output columns
+  constexpr int KS = NC / 4;                  // k-steps (NCOLOR is a multiple of 4)
+  static_assert(NC % 4 == 0, "NCOLOR must be a multiple of 4");
+  typedef typename Vec4<R>::type V4;
+  __shared__ float cf[NC * NCP];   // integer numerators: exact in float
+  for (int i = threadIdx.x; i < NC * NCP; i += 256) {
+    const int r = i / NCP, c = i % NCP;
+    cf[i] = c < NC ? T::cf(float(), r, c) : 0.0f;
+  }
+  __syncthreads();
+  const cx<R>* J = reinterpret_cast<const cx<R>*>(jbuf);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int64_t ntiles = (n + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t e0 = tile * 16;
+    const int64_t ea = e0 + lr;                 // event of this lane's A row
+    R acc[4] = {R(0), R(0), R(0), R(0)};
+    for (int h = 0; h < nhel; ++h) {
+      const cx<R>* Jh = J + (int64_t)h * NC * n;
+      R are[KS], aim[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const cx<R> v = ea < n ? Jh[(int64_t)(4 * s + lq) * n + ea] : cx<R>(R(0), R(0));
+        are[s] = v.re;
+        aim[s] = v.im;
+      }
+#pragma unroll 1
+      for (int nt = 0; nt < NCP / 16; ++nt) {
+        V4 dre = {R(0), R(0), R(0), R(0)}, dim = {R(0), R(0), R(0), R(0)};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const R b = R(cf[(4 * s + lq) * NCP + nt * 16 + lr]);
+          dre = mfma16x16x4(are[s], b, dre);
+          dim = mfma16x16x4(aim[s], b, dim);
+        }
+        const int col = nt * 16 + lr;
+        if (col < NC) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t ed = e0 + 4 * lq + r;
+            if (ed < n) {
+              const cx<R> x = Jh[(int64_t)col * n + ed];
+              acc[r] += x.re * dre[r] + x.im * dim[r];
+            }
+          }
+        }
+      }
+    }
+    // sum over the 16 columns held by the lanes of each quarter-wave
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) acc[r] += __shfl_xor(acc[r], off, 64);
+    }
+    if (lr == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t ed = e0 + 4 * lq + r;
+        if (ed < n) out[ed] = (accumulate ? out[ed] : 0.0) + double(acc[r]) * scale;
+      }
+    }
+  }
+}
+
+// jamps of a single helicity row [NCOLOR][n] -> (n, NCOLOR, 2) f64
+template <class R>
+__global__ void k_jamp_transpose(const void* jbuf, int64_t n, int ncolor, double* jamp_out) {
+  const cx<R>* J = reinterpret_cast<const cx<R>*>(jbuf);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ncolor) return;
+  const int64_t e = i / ncolor;
+  const int c = (int)(i % ncolor);
+  const cx<R> v = J[(int64_t)c * n + e];
+  jamp_out[2 * i] = double(v.re);
+  jamp_out[2 * i + 1] = double(v.im);
+}
+
+// bytes of jamp workspace per event and helicity row
+template <class T>
+int64_t ordered_ws_per_row(int precision) {
+  return (int64_t)T::NCOLOR * 2 * (precision == 32 ? 4 : 8);
+}
+
+// Sub-chunks of events and groups of helicity rows sized to the caller's workspace:
+// the largest helicity group that still leaves a sub-chunk of min(nev, 65536) events
+// (enough waves for every SIMD), then as many events as the workspace holds.
+template <class T>
+hipError_t launch_ordered(const SmatrixArgs& s, int precision, int per_helicity, void* ws, int64_t ws_bytes,
+                          hipStream_t st) {
+  if (s.nev <= 0) return hipSuccess;
+  const int nrows = per_helicity ? 1 : T::NCOMB;
+  const int64_t row = ordered_ws_per_row<T>(precision);
+  const int64_t want = s.nev < 65536 ? s.nev : 65536;
+  int hg = nrows;
+  while (hg > 1 && ws_bytes / (row * hg) < want) hg /= 2;
+  const int64_t nsub = ws ? ws_bytes / (row * hg) : 0;
+  if (nsub < 16) return hipErrorInvalidValue;   // the entry point checks the size first
+  const double scale = 1.0 / (T::CF_DEN * (per_helicity ? 1.0 : T::DENOMINATOR));
+  for (int64_t off = 0; off < s.nev; off += nsub) {
+    const int64_t n = s.nev - off < nsub ? s.nev - off : nsub;
+    const int64_t tiles = (n + 15) / 16;
+    int64_t cblk = (tiles + 3) / 4;
+    if (cblk > 65536) cblk = 65536;
+    for (int h0 = 0; h0 < nrows; h0 += hg) {
+      OrderedArgs a;
+      a.s = s;
+      a.ev0 = off;
+      a.n = n;
+      a.h0 = h0;
+      a.nhel = hg;
+      a.jbuf = ws;
+      const int acc = h0 > 0;
+      if (precision == 32) {
+        hipLaunchKernelGGL((k_ordered_jamps<T, float, 64>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_colour_mfma<T, float>), dim3((unsigned)cblk), dim3(256), 0, st, (const void*)ws, n, hg,
+                           scale, acc, s.out + off);
+      } else {
+        hipLaunchKernelGGL((k_ordered_jamps<T, double, 64>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_colour_mfma<T, double>), dim3((unsigned)cblk), dim3(256), 0, st, (const void*)ws, n,
+                           hg, scale, acc, s.out + off);
+      }
+    }
+    if (per_helicity && s.jamp_out) {
+      const unsigned nb = (unsigned)((n * T::NCOLOR + 255) / 256);
+      if (precision == 32)
+        hipLaunchKernelGGL((k_jamp_transpose<float>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, T::NCOLOR,
+                           s.jamp_out + off * T::NCOLOR * 2);
+      else
+        hipLaunchKernelGGL((k_jamp_transpose<double>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, T::NCOLOR,
+                           s.jamp_out + off * T::NCOLOR * 2);
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+}  // namespace mfx
+
+#define MFX_INSTANTIATE_ORDERED(NAME)                                                                   \
+  namespace mfx {                                                                                      \
+  static const int registered_##NAME = register_process(ProcessEntry{                                  \
+      NAME::Traits::PROC_NAME, NAME::Traits::NEXTERNAL, NAME::Traits::NCOMB, NAME::Traits::NCOLOR,      \
+      NAME::Traits::NDIAGS, NAME::Traits::NGRAPHS, NAME::Traits::DENOMINATOR, &launch_ordered<NAME::Traits>, \
+      &ordered_ws_per_row<NAME::Traits>});                                                           \
+  }

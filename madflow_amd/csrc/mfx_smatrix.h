@@ -160,11 +160,15 @@ struct ProcessEntry {
   const char* name;
   int nexternal, ncomb, ncolor, ndiags, ngraphs;
   double denominator;
-  hipError_t (*launch)(const SmatrixArgs& a, int precision, int per_helicity, hipStream_t s);
+  // ws / ws_bytes: caller-owned device workspace (ignored by the fused kernels)
+  hipError_t (*launch)(const SmatrixArgs& a, int precision, int per_helicity, void* ws, int64_t ws_bytes,
+                       hipStream_t s);
+  // workspace bytes per event and helicity row; nullptr: the process needs none
+  int64_t (*ws_per_row)(int precision);
 };
 
 template <class T>
-hipError_t launch_process(const SmatrixArgs& a, int precision, int per_helicity, hipStream_t s) {
+hipError_t launch_process(const SmatrixArgs& a, int precision, int per_helicity, void*, int64_t, hipStream_t s) {
   if (a.nev <= 0) return hipSuccess;
   const int block = 256;
   int64_t nblk = (a.nev + block - 1) / block;
@@ -192,5 +196,5 @@ int register_process(const ProcessEntry& e);
   static const int registered_##NAME = register_process(                                               \
       ProcessEntry{NAME::Traits::PROC_NAME, NAME::Traits::NEXTERNAL, NAME::Traits::NCOMB,               \
                    NAME::Traits::NCOLOR, NAME::Traits::NDIAGS, NAME::Traits::NGRAPHS,                   \
-                   NAME::Traits::DENOMINATOR, &launch_process<NAME::Traits>});                          \
+                   NAME::Traits::DENOMINATOR, &launch_process<NAME::Traits>, nullptr});                 \
   }

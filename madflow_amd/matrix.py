@@ -109,17 +109,39 @@ class MatrixBase:
                     per_event[slot, :, 1] = gc[2 * slot + 1]
         return all_ps, nev, mw, gc, per_event
 
+    # device workspace for processes evaluated by the colour-ordered kernels: at most this
+    # many bytes are kept (the library splits events and helicity rows to fit)
+    workspace_limit = 8 << 30
+
+    def _workspace(self, nev, per_helicity, device):
+        """(pointer, bytes) of a cached device workspace, grown on demand; (None, 0) when the
+        process needs none."""
+        lib = _lib.load()
+        need = int(lib.mfx_smatrix_workspace_size(self.index, nev, self.precision, int(per_helicity)))
+        if need < 0:
+            _lib.check(need)
+        if need == 0:
+            return None, 0
+        size = min(need, self.workspace_limit)
+        ws = getattr(self, "_ws", None)
+        if ws is None or ws.numel() < size or ws.device != device:
+            self._ws = None
+            ws = self._ws = torch.empty(size, dtype=torch.uint8, device=device)
+        return ctypes.c_void_p(ws.data_ptr()), ws.numel()
+
     def _smatrix(self, all_ps, params):
         all_ps, nev, mw, gc, per_event = self._prepare(all_ps, params)
         out = torch.empty(nev, dtype=torch.float64, device=all_ps.device)
         if nev == 0:
             return out
         lib = _lib.load()
+        ws, ws_bytes = self._workspace(nev, False, all_ps.device)
         _lib.check(lib.mfx_smatrix(
             self.index, ctypes.c_void_p(all_ps.data_ptr()), nev, mw.ctypes.data_as(ctypes.c_void_p),
             gc.ctypes.data_as(ctypes.c_void_p),
             ctypes.c_void_p(per_event.data_ptr()) if per_event is not None else None,
-            ctypes.c_void_p(out.data_ptr()), self.precision, int(self.ref_compat), _lib.stream_ptr(all_ps.device)))
+            ctypes.c_void_p(out.data_ptr()), self.precision, int(self.ref_compat), ws, ws_bytes,
+            _lib.stream_ptr(all_ps.device)))
         return out
 
     def _matrix(self, all_ps, hel, params, return_jamp=False):
@@ -133,12 +155,13 @@ class MatrixBase:
         jamp = torch.empty((nev, self.ncolor, 2), dtype=torch.float64, device=all_ps.device) if return_jamp else None
         if nev > 0:
             lib = _lib.load()
+            ws, ws_bytes = self._workspace(nev, True, all_ps.device)
             _lib.check(lib.mfx_matrix(
                 self.index, ctypes.c_void_p(all_ps.data_ptr()), nev, helv.ctypes.data_as(ctypes.c_void_p),
                 mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
                 ctypes.c_void_p(per_event.data_ptr()) if per_event is not None else None,
                 ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(jamp.data_ptr()) if jamp is not None else None,
-                self.precision, int(self.ref_compat), _lib.stream_ptr(all_ps.device)))
+                self.precision, int(self.ref_compat), ws, ws_bytes, _lib.stream_ptr(all_ps.device)))
         if return_jamp:
             return out, torch.complex(jamp[..., 0], jamp[..., 1])
         return out

@@ -24,24 +24,34 @@ def test_library_exports_header_symbols():
 def test_registry_and_argument_checks():
     lib = _lib.load()
     names = [lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())]
-    assert names[:3] == ["gg_ttx", "gg_ttxg", "gg_ttxgg"]
+    assert names[:2] == ["gg_ttx", "gg_ttxg"]
+    assert set(names) >= {"gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"}
     info = np.zeros(5, dtype=np.int32)
-    assert lib.mfx_process_info(2, info.ctypes.data_as(ctypes.c_void_p)) == 0
+    ttgg = lib.mfx_process_index(b"gg_ttxgg")
+    assert lib.mfx_process_info(ttgg, info.ctypes.data_as(ctypes.c_void_p)) == 0
     assert list(info) == [6, 64, 24, 123, 159]
+    assert lib.mfx_process_info(lib.mfx_process_index(b"gg_ttxggg"), info.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert list(info) == [7, 128, 120, 1240, 1890]
     assert lib.mfx_process_index(b"nope") < 0
     mw = np.array([173.0, 1.5])
     gc = np.zeros(6)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     # negative nev, bad precision, unknown process: rejected before any launch
-    assert lib.mfx_smatrix(0, None, -1, mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
-                           None, None, 64, 1, None) < 0
-    assert lib.mfx_smatrix(0, None, 0, mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
-                           None, None, 16, 1, None) < 0
+    assert lib.mfx_smatrix(0, None, -1, vp(mw), vp(gc), None, None, 64, 1, None, 0, None) < 0
+    assert lib.mfx_smatrix(0, None, 0, vp(mw), vp(gc), None, None, 16, 1, None, 0, None) < 0
     assert b"precision" in lib.mfx_last_error()
-    assert lib.mfx_smatrix(99, None, 0, mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
-                           None, None, 64, 1, None) < 0
+    assert lib.mfx_smatrix(99, None, 0, vp(mw), vp(gc), None, None, 64, 1, None, 0, None) < 0
     # empty batches are valid no-ops
-    assert lib.mfx_smatrix(0, None, 0, mw.ctypes.data_as(ctypes.c_void_p), gc.ctypes.data_as(ctypes.c_void_p),
-                           None, None, 64, 1, None) == 0
+    assert lib.mfx_smatrix(0, None, 0, vp(mw), vp(gc), None, None, 64, 1, None, 0, None) == 0
+    # the fused processes need no workspace; the colour-ordered ones say how much
+    assert lib.mfx_smatrix_workspace_size(ttgg, 1000, 32, 0) == 0
+    t5 = lib.mfx_process_index(b"gg_ttxggg")
+    assert lib.mfx_smatrix_workspace_size(t5, 1000, 32, 0) == 1000 * 128 * 120 * 8
+    assert lib.mfx_smatrix_workspace_size(t5, 1000, 64, 1) == 1000 * 120 * 16
+    # a missing workspace is reported before anything touches the (fake, aligned) pointers
+    fake = ctypes.c_void_p(1 << 20)
+    assert lib.mfx_smatrix(t5, fake, 10, vp(mw), vp(gc), None, fake, 32, 1, None, 0, None) == -8
+    assert b"workspace" in lib.mfx_last_error()
 
 
 def test_product_path_fails_loudly_without_gpu():

@@ -64,7 +64,7 @@ def test_mockup_matches_golden():
     np.testing.assert_array_equal(me, np.array(g["mockup_ref_compat"]))
 
 
-GPU_PROCS = ["gg_ttx", "gg_ttxg", "gg_ttxgg"]
+GPU_PROCS = ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"]
 
 
 @pytest.mark.gpu

@@ -132,14 +132,64 @@ def run_main(a, torch):
                 npass=sum(r["npass"] for r in res), unw=_unweighted(res, t_warm, wall, a.target))
 
 
+def train_flow(a, torch, ig):
+    """Training pairs (uniform points of the frozen VEGAS cube and their integrand values,
+    plus a held-out tenth for checkpoint selection) and the flow trained on them."""
+    from madflow_amd.flow import RQSFlow
+    from madflow_amd.phasespace_gpu import philox_uniforms
+    D = ig.ndim
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ntot = a.flow_pairs + a.flow_pairs // 10
+    y = torch.empty((ntot, D), dtype=torch.float32, device="cuda")
+    f = torch.empty(ntot, dtype=torch.float64, device="cuda")
+    done = 0
+    while done < ntot:
+        n = min(a.events_limit, ntot - done)
+        yy = philox_uniforms(n, D, 99, 0, ev0=done).float().contiguous()
+        _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=yy)
+        y[done:done + n] = yy
+        f[done:done + n] = fb
+        done += n
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    flow = RQSFlow(D, layers=8, seed=1)
+    nt = a.flow_pairs
+    flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
+               val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix)
+    del y, f
+    torch.cuda.synchronize()
+    return flow, t_pairs, time.perf_counter() - t0
+
+
+def flow_iteration(a, torch, ig, flow, k):
+    """One iteration of a.nev points from the defensive mixture of the flow and the
+    frozen VEGAS cube, each weighted by f(y)/q'(y)."""
+    from madflow_amd.phasespace_gpu import philox_uniforms
+    D = ig.ndim
+    sums, done = [], 0
+    while done < a.nev:
+        n = min(a.events_limit, a.nev - done)
+        u = philox_uniforms(n, D, 1234, 100 + k, ev0=done).float()
+        ys, winv = flow.sample_mixture(u, u, a.flow_mix)
+        r, _, _ = ig.chunk(done, n, 100 + k, y_in=ys.contiguous(), w_in=winv.contiguous())
+        sums.append(r)
+        done += n
+    s1 = math.fsum(r[0] for r in sums)
+    s2 = math.fsum(r[1] for r in sums)
+    N = float(a.nev)
+    mean = s1 / N
+    return dict(I=mean, var=max((s2 / N - mean * mean) / (N - 1), 0.0), sum_f=s1,
+                wmax=max(r[2] for r in sums), n=a.nev)
+
+
 def run_flow_compare(a, torch, main):
     """Plain VEGAS with FP64 matrix elements against VEGAS + learned proposal with FP32
     matrix elements on the same cube, each run to sigma/sigma = target.  The flow path
     reuses the FP32 grid of the main run (its warm-up time is counted in its set-up),
     then generates the training pairs, trains, and samples y ~ q with weights f(y)/q(y)."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
-    from madflow_amd.flow import RQSFlow
-    from madflow_amd.phasespace_gpu import philox_uniforms
     sync = torch.cuda.synchronize
     # ---- plain VEGAS, FP64 matrix elements
     ig64 = MEIntegrand("gg_ttxgg", precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit, **KIN)
@@ -162,60 +212,25 @@ def run_flow_compare(a, torch, main):
     del ig64, vf
     # ---- VEGAS + flow, FP32 matrix elements, on the main run's frozen grid
     ig = main["ig"]
-    D = ig.ndim
-    sync()
-    t0 = time.perf_counter()
-    ntrain = a.flow_pairs
-    y = torch.empty((ntrain, D), dtype=torch.float32, device="cuda")
-    f = torch.empty(ntrain, dtype=torch.float64, device="cuda")
-    done = 0
-    while done < ntrain:
-        n = min(a.events_limit, ntrain - done)
-        yy = philox_uniforms(n, D, 99, 0, ev0=done).float().contiguous()
-        _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=yy)
-        y[done:done + n] = yy
-        f[done:done + n] = fb
-        done += n
-    sync()
-    t_pairs = time.perf_counter() - t0
-    flow = RQSFlow(D, layers=8, seed=1)
-    flow.train(y, f, steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log)
-    del y, f
-    sync()
-    t_train = time.perf_counter() - t0 - t_pairs
+    flow, t_pairs, t_train = train_flow(a, torch, ig)
     setup_flow = main["t_warm"] + t_pairs + t_train
-
-    def itflow(k):
-        sums, done = [], 0
-        while done < a.nev:
-            n = min(a.events_limit, a.nev - done)
-            u = philox_uniforms(n, D, 1234, 100 + k, ev0=done).float()
-            ys, logq = flow.sample(u)
-            winv = torch.exp(-logq.double()).contiguous()
-            r, _, _ = ig.chunk(done, n, 100 + k, y_in=ys.contiguous(), w_in=winv)
-            sums.append(r)
-            done += n
-        s1 = math.fsum(r[0] for r in sums)
-        s2 = math.fsum(r[1] for r in sums)
-        N = float(a.nev)
-        mean = s1 / N
-        return dict(I=mean, var=max((s2 / N - mean * mean) / (N - 1), 0.0), sum_f=s1,
-                    wmax=max(r[2] for r in sums), n=a.nev)
     t0 = time.perf_counter()
-    resf = _run_to_target(itflow, a.target, a.max_iter, "flow fp32")
+    resf = _run_to_target(lambda k: flow_iteration(a, torch, ig, flow, k), a.target, a.max_iter, "flow fp32")
     sync()
     learned = _unweighted(resf, setup_flow, time.perf_counter() - t0, a.target)
-    learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"])
+    learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"],
+                   best_step=getattr(flow, "best_step", None), mixture=a.flow_mix)
 
     def gain(k):
         return learned[k] / plain[k] if plain[k] > 0 else None
     return {
         "workload": "gg->ttgg, sqrt(s)=3 TeV, pT(g)>30, dR(g,g)>0.4, %d-dim %s VEGAS cube, %d points/iteration, "
-                    "target sigma/sigma=%g" % (D, a.mapping, a.nev, a.target),
+                    "target sigma/sigma=%g" % (ig.ndim, a.mapping, a.nev, a.target),
         "plain_vegas_fp64": plain,
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
-                 "train_pairs": ntrain, "train_steps": a.flow_steps, "batch": a.flow_batch},
+                 "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
+                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
         "gain_unweighted_per_s_steady": gain("unweighted_per_s_steady"),
         "sigma_pull": (learned["sigma_pb"] - plain["sigma_pb"]) / math.hypot(learned["err_pb"], plain["err_pb"]),
@@ -239,6 +254,8 @@ def main(argv=None):
     ap.add_argument("--flow-steps", type=int, default=20000)
     ap.add_argument("--flow-batch", type=int, default=4096)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
+    ap.add_argument("--flow-mix", type=float, default=0.05)
+    ap.add_argument("--flow-eval-every", type=int, default=1000)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: the event loop shards by Philox counter range, but this build "

@@ -73,7 +73,8 @@ int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, cons
 
 /* ------------------------------------------ HELAS / ALOHA building blocks */
 /* External wavefunction for nev momenta p (nev,4): kind 0 vxxxxx, 1 oxxxxx, 2 ixxxxx
- * (wavefunctions_flow.py:54-154).  out: (nev, 6, 2) f64 device. */
+ * (wavefunctions_flow.py:54-154), 3 sxxxxx (:32-51; nhel ignored, nsign = nss).
+ * out: (nev, 6, 2) f64 device, or (nev, 3, 2) for the scalar. */
 int mfx_wavefunction(int kind, const double* p, int64_t nev, double mass, int nhel, int nsign, double* out,
                      int precision, int ref_compat, void* stream);
 
@@ -145,6 +146,17 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
 int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, double alpha, void* stream);
 
 /* ------------------------------------------------ learned proposal (flow) */
+/* Hit-or-miss unweighting of nev weighted events (replaces handing them to MadGraph's
+ * lhe_parser unweight, lhe_writer.py:319-356): keep event e with probability
+ * min(1, w[e] / wmax), the uniform taken from the Philox stream (ev0 + e, 0xffffffff,
+ * iteration) with key (seed, 0x6d66).  keep (nev,) u8 device; index_out (nev,) i32 device
+ * receives the kept event indices in order; *nkeep_host (optional, synchronises) their
+ * number.  Events with w > wmax are kept with weight w / wmax, the others with weight 1. */
+int64_t mfx_unweight_workspace_size(int64_t nev);
+int mfx_unweight(const double* w, int64_t nev, double wmax, uint32_t seed, uint32_t iteration, uint64_t ev0,
+                 uint8_t* keep, int* index_out, void* workspace, int64_t workspace_bytes, int64_t* nkeep_host,
+                 void* stream);
+
 /* Rational-quadratic spline coupling flow on [0,1]^D (no counterpart in the reference:
  * the north-star learned proposal).  masks: host int[L][D], 1 = dimension transformed
  * by that layer.  Parameters: one fp32 vector (sizes[0] floats) plus transposed bf16

@@ -56,10 +56,12 @@ __global__ void k_wavefunction(int kind, const double* __restrict__ p, int64_t n
   W<R> w;
   if (kind == 0) vxxxxx(q, R(mass), R(nhel), R(nsign), R(sqh), w);
   else if (kind == 1) oxxxxx(q, R(mass), R(nhel), R(nsign), w);
-  else ixxxxx(q, R(mass), R(nhel), R(nsign), w);
-  for (int k = 0; k < 6; ++k) {
-    out[(e * 6 + k) * 2] = double(w[k].re);
-    out[(e * 6 + k) * 2 + 1] = double(w[k].im);
+  else if (kind == 2) ixxxxx(q, R(mass), R(nhel), R(nsign), w);
+  else sxxxxx(q, R(nsign), w);
+  const int n = kind == 3 ? 3 : 6;
+  for (int k = 0; k < n; ++k) {
+    out[(e * n + k) * 2] = double(w[k].re);
+    out[(e * n + k) * 2 + 1] = double(w[k].im);
   }
 }
 
@@ -222,7 +224,7 @@ int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, cons
 
 int mfx_wavefunction(int kind, const double* p, int64_t nev, double mass, int nhel, int nsign, double* out,
                      int precision, int ref_compat, void* stream) {
-  if (kind < 0 || kind > 2) return fail(-1, "kind must be 0 (v), 1 (o) or 2 (i)%s", "");
+  if (kind < 0 || kind > 3) return fail(-1, "kind must be 0 (v), 1 (o), 2 (i) or 3 (s)%s", "");
   if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
   if (nev == 0) return 0;
   if (!p || !out) return fail(-3, "null pointer%s", "");

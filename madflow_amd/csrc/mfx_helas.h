@@ -129,6 +129,17 @@ __device__ __forceinline__ void oxxxxx(const R (&p)[4], R fmass, R nhel, R nsf, 
   }
 }
 
+// ------------------------------------------------------------------------ sxxxxx
+// Scalar external wavefunction (wavefunctions_flow.py:32-51): the two momentum slots
+// and the constant 1.  Only s[0..2] are meaningful.
+template <class R>
+__device__ __forceinline__ void sxxxxx(const R (&p)[4], R nss, W<R>& s) {
+  s[0] = cx<R>(p[0] * nss, p[3] * nss);
+  s[1] = cx<R>(p[1] * nss, p[2] * nss);
+  s[2] = cx<R>(R(1), R(0));
+  s[3] = s[4] = s[5] = cx<R>(R(0), R(0));
+}
+
 // ------------------------------------------------------------------------ vxxxxx
 // nhel = 4 gives the BRST (gauge) check polarisation eps = p/p0 (p/vmass if massive).
 template <class R>

@@ -162,6 +162,56 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
   if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
 }
 
+// ---------------------------------------------------------------- unweighting
+// Hit-or-miss unweighting (the reference hands its weighted events to MadGraph's
+// lhe_parser unweight, lhe_writer.py:319-356): event e is kept with probability
+// min(1, w_e / wmax), drawn from the Philox stream (ev0 + e, UNW_STREAM, iteration) of the
+// run's seed; kept events keep their order.  Events with w > wmax are always kept and
+// carry weight w / wmax (the caller reads it from w), all others weight 1.
+constexpr uint32_t UNW_STREAM = 0xffffffffu;
+__global__ void __launch_bounds__(GEN_BLOCK) k_unweight_flag(const double* __restrict__ w, int64_t n, double wmax,
+                                                             uint64_t ev0, uint32_t seed, uint32_t iteration,
+                                                             uint8_t* __restrict__ keep, int* __restrict__ block_count) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  bool ok = false;
+  if (e < n) {
+    double u0, u1;
+    philox_pair(ev0 + e, UNW_STREAM, seed, iteration, u0, u1);
+    const double we = w[e];
+    ok = we > 0.0 && u0 * wmax < we;
+    keep[e] = ok ? 1 : 0;
+  }
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const unsigned long long bal = __ballot(ok);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&cnt, __popcll(bal));
+  __syncthreads();
+  if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
+}
+
+__global__ void __launch_bounds__(GEN_BLOCK) k_unweight_compact(const uint8_t* __restrict__ keep, int64_t n,
+                                                                const int* __restrict__ offsets,
+                                                                int* __restrict__ index_out) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  const bool ok = e < n && keep[e];
+  __shared__ int wave_base[GEN_BLOCK / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(ok);
+  if (lane == 0) wave_base[wave] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int v = 0; v < GEN_BLOCK / 64; ++v) {
+      const int c = wave_base[v];
+      wave_base[v] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  if (ok) index_out[offsets[blockIdx.x] + wave_base[wave] + __popcll(bal & ((1ull << lane) - 1ull))] = (int)e;
+}
+
 // exclusive scan of the block counts, one block; total in offsets[nblk]
 __global__ void k_scan(const int* __restrict__ counts, int nblk, int* __restrict__ offsets) {
   __shared__ int part[1024];
@@ -506,6 +556,42 @@ int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, doub
   if (!grid || !hist) return fail(-3, "null pointer%s", "");
   hipLaunchKernelGGL(k_refine, dim3(ndim), dim3(64), 0, (hipStream_t)stream, grid, hist, nbins, alpha);
   return hip_status(hipGetLastError());
+}
+
+int64_t mfx_unweight_workspace_size(int64_t n) {
+  const int64_t nblk = (n + GEN_BLOCK - 1) / GEN_BLOCK;
+  return (2 * nblk + 1) * 4 + 256;
+}
+
+int mfx_unweight(const double* w, int64_t n, double wmax, uint32_t seed, uint32_t iteration, uint64_t ev0,
+                 uint8_t* keep, int* index_out, void* workspace, int64_t workspace_bytes, int64_t* nkeep_host,
+                 void* stream) {
+  if (n < 0 || n > 0x7fffffff) return fail(-2, "n must be in [0, 2^31)%s", "");
+  if (!(wmax > 0.0)) return fail(-2, "wmax must be > 0%s", "");
+  if (workspace_bytes < mfx_unweight_workspace_size(n)) return fail(-8, "workspace too small (see mfx_unweight_workspace_size)%s", "");
+  if (n == 0) {
+    if (nkeep_host) *nkeep_host = 0;
+    return 0;
+  }
+  if (!w || !keep || !index_out || !workspace) return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  const int nblk = (int)((n + GEN_BLOCK - 1) / GEN_BLOCK);
+  int* counts = (int*)workspace;
+  int* offsets = counts + nblk;
+  hipLaunchKernelGGL(k_unweight_flag, dim3(nblk), dim3(GEN_BLOCK), 0, s, w, n, wmax, ev0, seed, iteration, keep,
+                     counts);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
+  hipLaunchKernelGGL(k_unweight_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, keep, n, offsets, index_out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_status(e);
+  if (nkeep_host) {
+    int tot = 0;
+    e = hipMemcpyAsync(&tot, offsets + nblk, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e);
+    *nkeep_host = tot;
+  }
+  return 0;
 }
 
 }  // extern "C"

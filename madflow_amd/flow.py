@@ -171,17 +171,37 @@ class RQSFlow:
                                           betas[0], betas[1], eps, self.step, clip, _lib.stream_ptr(self.device)))
         self._cast()
 
-    def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True, resample=True):
+    def is_second_moment(self, y, f, mix=0.0, chunk=1 << 20):
+        """E_uniform[f^2 / q'] over uniform points y with integrand values f, where
+        q' = (1 - mix) q + mix: the second moment of the importance weights f/q' of
+        samples drawn from q' (the variance up to the fixed squared integral)."""
+        tot = 0.0
+        for a in range(0, y.shape[0], chunk):
+            lq = self.log_prob(y[a:a + chunk].contiguous()).double()
+            qm = (1.0 - mix) * torch.exp(lq) + mix
+            ff = f[a:a + chunk].double()
+            tot += float((ff * ff / qm).sum())
+        return tot / y.shape[0]
+
+    def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True, resample=True,
+              val=None, eval_every=1000, mix=0.0):
         """Minibatch weighted maximum likelihood on device-resident pairs (y, w).
 
         resample=True draws each minibatch with probability proportional to w (inverse-CDF
         lookup on the device) and gives every drawn point unit weight: the same expected
         loss as the w-weighted uniform minibatch, with far less gradient noise when the
-        weights are heavy-tailed (matrix elements near collinear/soft limits)."""
+        weights are heavy-tailed (matrix elements near collinear/soft limits).
+
+        val=(y_val, f_val): held-out uniform points.  Every eval_every steps the second
+        moment of the importance weights on them (is_second_moment, with the sampling
+        mixture ``mix``) is measured and the parameters with the smallest one are kept:
+        maximum likelihood on heavy-tailed weights over-fits the few largest weights
+        after enough steps, which the held-out moment sees and the training loss does not."""
         n = y.shape[0]
         g = torch.Generator(device=y.device)
         g.manual_seed(seed)
         losses = []
+        best = None
         if resample:
             w64 = w.to(torch.float64).clamp(min=0)
             cdf = torch.cumsum(w64, 0)
@@ -190,7 +210,15 @@ class RQSFlow:
                 raise ValueError("train: the weights sum to zero")
             cdf /= total
             ones = torch.ones(batch, dtype=torch.float64, device=y.device)
-        for it in range(steps):
+        for it in range(steps + 1):
+            if val is not None and (it % eval_every == 0 or it == steps):
+                m2 = self.is_second_moment(val[0], val[1], mix)
+                if best is None or m2 < best[0]:
+                    best = (m2, it, self.params.clone(), self.m1.clone(), self.m2.clone())
+                if log:
+                    log("flow step %d held-out second moment %.6g (best at step %d)" % (it, m2, best[1]))
+            if it == steps:
+                break
             if resample:
                 r = torch.rand(batch, generator=g, device=y.device, dtype=torch.float64)
                 idx = torch.searchsorted(cdf, r).clamp_(max=n - 1)
@@ -205,8 +233,33 @@ class RQSFlow:
             cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
             loss, grad = self.grad(yb, wb)
             self.adam_step(grad, lr=cur_lr)
-            if log and (it % max(1, steps // 10) == 0 or it == steps - 1):
+            if log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1):
                 log("flow step %d loss %.5f" % (it, float(loss)))
             if it % 50 == 0:
                 losses.append(loss)
+        if best is not None:
+            self.params.copy_(best[2])
+            self.m1.copy_(best[3])
+            self.m2.copy_(best[4])
+            self._cast()
+            self.best_step = best[1]
+            self.best_second_moment = best[0]
         return [float(x) for x in losses]
+
+    def sample_mixture(self, u, u_plain, mix):
+        """Stratified defensive mixture q' = (1 - mix) q + mix on the unit cube: the first
+        round(mix * n) points are u_plain as they are, the rest come from the flow applied
+        to u.  Returns (y, 1/q'(y)); the estimate sum f(y)/q'(y) / n stays unbiased."""
+        n = u.shape[0]
+        nu = int(round(mix * n))
+        y = torch.empty_like(u)
+        winv = torch.empty(n, dtype=torch.float64, device=u.device)
+        if nu > 0:
+            yu = u_plain[:nu].contiguous()
+            y[:nu] = yu
+            winv[:nu] = 1.0 / ((1.0 - mix) * torch.exp(self.log_prob(yu).double()) + mix)
+        if nu < n:
+            yf, lq = self.sample(u[nu:].contiguous())
+            y[nu:] = yf
+            winv[nu:] = 1.0 / ((1.0 - mix) * torch.exp(lq.double()) + mix)
+        return y, winv

@@ -150,6 +150,19 @@ class MEIntegrand:
             _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
         return b["result"].cpu().numpy().copy(), npass, (b["f"][:n] if keep_f else None)
 
+    def unweight(self, w, wmax, iteration, ev0=0):
+        """Hit-or-miss unweighting of the weights w (device f64) against wmax on the
+        device: returns (indices of the kept events in order, keep mask)."""
+        w = _lib.require_device_tensor(w, torch.float64, "w")
+        n = w.shape[0]
+        keep = torch.empty(n, dtype=torch.uint8, device=w.device)
+        idx = torch.empty(n, dtype=torch.int32, device=w.device)
+        ws = torch.empty(self.lib.mfx_unweight_workspace_size(n), dtype=torch.uint8, device=w.device)
+        nk = ctypes.c_int64(0)
+        _lib.check(self.lib.mfx_unweight(_ptr(w), n, float(wmax), self.p.seed, iteration, ev0, _ptr(keep), _ptr(idx),
+                                         _ptr(ws), ws.numel(), ctypes.byref(nk), _lib.stream_ptr(w.device)))
+        return idx[:nk.value], keep
+
     def iteration(self, n_events, iteration, train=True, keep_f=False):
         """One VEGAS iteration of n_events; returns a dict with I, var, max weight, counts."""
         tot = None

@@ -253,6 +253,15 @@ def philox_raw(ctr, key):
     return out
 
 
+def unweight(w, wmax, seed, iteration, ev0=0):
+    """Hit-or-miss keep mask (u8) of the device unweighting (same Philox stream)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    keep = np.zeros(w.shape[0], dtype=np.uint8)
+    lib().mfo_unweight(_p(w), ctypes.c_int64(w.shape[0]), ctypes.c_double(wmax), ctypes.c_uint32(seed),
+                       ctypes.c_uint32(iteration), ctypes.c_uint64(ev0), _p(keep))
+    return keep
+
+
 def vegas_map(edges, y):
     edges = np.ascontiguousarray(edges, dtype=np.float64)
     ndim, nb1 = edges.shape

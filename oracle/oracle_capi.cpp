@@ -206,6 +206,23 @@ int mfo_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint
   return 0;
 }
 
+// Hit-or-miss unweighting (the device's mfx_unweight): keep e iff w > 0 and
+// u * wmax < w with u the first uniform of philox (ev0 + e, 0xffffffff, iteration).
+int mfo_unweight(const double* w, int64_t n, double wmax, uint32_t seed, uint32_t iteration, uint64_t ev0,
+                 uint8_t* keep) {
+  for (int64_t e = 0; e < n; ++e) {
+    u32x4 c;
+    c.v[0] = (uint32_t)(ev0 + e);
+    c.v[1] = (uint32_t)((ev0 + e) >> 32);
+    c.v[2] = 0xffffffffu;
+    c.v[3] = iteration;
+    const u32x4 r = philox4x32_10(c, seed, 0x6d66u);
+    const double u = u53(r.v[0], r.v[1]);
+    keep[e] = (w[e] > 0.0 && u * wmax < w[e]) ? 1 : 0;
+  }
+  return 0;
+}
+
 int mfo_philox_raw(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
   u32x4 c;
   for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];

@@ -120,3 +120,22 @@ def test_ttbar_cross_section(dev, mapping):
     print("sigma", res, "+-", err, "closed", closed, flush=True)
     assert abs(res - closed) < 3 * err
     assert err / res < 2e-3
+
+
+@pytest.mark.gpu
+def test_unweighting(dev):
+    """Device hit-or-miss unweighting: the keep mask equals the oracle's bit for bit, the
+    kept indices are the mask's in order, and the kept count matches sum min(1, w/wmax)."""
+    print("case unweight", flush=True)
+    ig = MEIntegrand("gg_ttxgg", precision=32, mapping="diet", seed=13, sqrts=3000.0, pt_min=30.0, dr_min=0.4)
+    rng = np.random.default_rng(5)
+    n = 1_000_003
+    w = rng.lognormal(sigma=1.5, size=n) * (rng.random(n) > 0.2)
+    wmax = np.quantile(w, 0.999)       # some events are over-weight
+    idx, keep = ig.unweight(torch.from_numpy(w).to(dev), wmax, iteration=4, ev0=77)
+    ref = O.unweight(w, wmax, 13, 4, ev0=77)
+    assert np.array_equal(keep.cpu().numpy(), ref)
+    assert np.array_equal(idx.cpu().numpy(), np.nonzero(ref)[0])
+    expect = np.minimum(1.0, w / wmax).sum()
+    assert abs(len(idx) - expect) < 5 * np.sqrt(expect)
+    assert np.all(ref[w > wmax] == 1) and np.all(ref[w == 0] == 0)

@@ -111,3 +111,23 @@ def test_massive_diet_on_shell():
     pt = np.hypot(out[:, 2:, 1], out[:, 2:, 2])
     assert np.all(pt[flag == 1] > 30.0)
     assert 0.2 < flag.mean() < 1.0
+
+
+def test_unweight_oracle_statistics():
+    """Hit-or-miss keep mask: acceptance min(1, w/wmax) per bin of w/wmax, zero weights never kept."""
+    rng = np.random.default_rng(8)
+    n = 400_000
+    w = rng.random(n) * 2.0
+    w[:1000] = 0.0
+    keep = O.unweight(w, 1.0, seed=3, iteration=2)
+    assert keep[:1000].sum() == 0
+    r = np.minimum(1.0, w[1000:])
+    k = keep[1000:].astype(float)
+    for lo in np.arange(0, 1.0, 0.1):
+        sel = (r >= lo) & (r < lo + 0.1)
+        p = r[sel].mean()
+        assert abs(k[sel].mean() - p) < 5 * np.sqrt(p * (1 - p) / sel.sum()) + 1e-12
+    assert k[r >= 1.0].all()
+    # the stream is the Philox stream (ev, 0xffffffff, iteration) of the seed: shifting ev0 shifts it
+    a = O.unweight(w[10:20], 1.0, seed=3, iteration=2, ev0=10)
+    assert np.array_equal(a, keep[10:20])

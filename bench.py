@@ -237,6 +237,48 @@ def run_flow_compare(a, torch, main):
     }
 
 
+ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5),
+               "gg_ttxgg": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
+               "gg_ttxgg_ordered": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
+               "gg_ttxggg": dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4)}
+
+
+def me_only(a, torch):
+    """ME evaluations per second of one process's smatrix on a device-resident batch of
+    post-cut events (generated by the integrand pipeline on a flat grid): median over
+    repeats of the HIP-event time around one smatrix call."""
+    from madflow_amd.vegas import MEIntegrand
+    proc = a.me_only
+    base = "gg_ttxgg" if proc == "gg_ttxgg_ordered" else proc
+    ig = MEIntegrand(base, precision=a.precision, mapping="rambo", seed=3, events_limit=a.batch,
+                     **ME_ONLY_KIN[proc])
+    ig.chunk(0, a.batch, 0)
+    n = ig.last_chunk_npass
+    ps = ig._bufs["dense"][:n]
+    from madflow_amd.matrix import load_matrix
+    m, _ = load_matrix(proc, precision=a.precision, ref_compat=True)
+    params = ig.params if len(m.params) == len(ig.params) else ig.params[:len(m.params)]
+    m.smatrix(ps, *params)
+    times = []
+    for _ in range(a.repeats):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        m.smatrix(ps, *params)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    times.sort()
+    t = times[len(times) // 2]
+    with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
+        flops = json.load(fh)[base]["total"]
+    peak = PEAK_TFLOPS[a.precision]
+    print(json.dumps({"process": proc, "precision": a.precision, "events": n, "repeats": a.repeats,
+                      "median_s": t, "me_evals_per_s": n / t, "flops_per_me": flops,
+                      "achieved_tflops": flops * n / t / 1e12, "frac_of_peak": flops * n / t / 1e12 / peak}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +298,12 @@ def main(argv=None):
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.05)
     ap.add_argument("--flow-eval-every", type=int, default=1000)
+    ap.add_argument("--me-only", default=None, metavar="PROC",
+                    help="time only the smatrix kernel of PROC on pre-generated post-cut events and exit")
+    ap.add_argument("--batch", type=int, default=1 << 22)
+    ap.add_argument("--repeats", type=int, default=20)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 4 times (calibrates HBM byte counters) and exit")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: the event loop shards by Philox counter range, but this build "
@@ -265,6 +313,16 @@ def main(argv=None):
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if a.me_only:
+        return me_only(a, torch)
+    if a.copy_probe:
+        src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_copy": src.numel() * 4, "copies": 4}))
+        return 0
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
         flops = json.load(fh)["gg_ttxgg"]["total"]
     m = run_main(a, torch)

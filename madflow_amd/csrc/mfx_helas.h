@@ -22,9 +22,10 @@ namespace mfx {
 template <class R> __device__ __forceinline__ R sel(bool c, R a, R b) { return c ? a : b; }
 template <class R> __device__ __forceinline__ R tfsign(R y) { return y > R(0) ? R(1) : (y < R(0) ? R(-1) : R(0)); }
 
-// |p| + p_z of a massive spinor.  FP64: the reference's expression max(pp + pz, 0).
-// FP32: for pz < 0 the equivalent pT^2 / (pp - pz), which has no cancellation when the
-// particle moves close to -z (pp + pz -> 0 loses all FP32 digits there).
+// |p| + p_z of a spinor (pp = |p| for massive, p0 for massless fermions).  FP64: the
+// reference's expression max(pp + pz, 0).  FP32: for pz < 0 the equivalent
+// pT^2 / (pp - pz), which has no cancellation when the particle moves close to -z
+// (pp + pz -> 0 loses all FP32 digits there).
 template <class R> __device__ __forceinline__ R plus_component(const R (&p)[4], R pp) {
   if constexpr (StableForms<R>::value) {
     return p[3] >= R(0) ? pp + p[3] : (p[1] * p[1] + p[2] * p[2]) / (pp - p[3]);
@@ -67,7 +68,7 @@ __device__ __forceinline__ void ixxxxx(const R (&p)[4], R fmass, R nhel, R nsf, 
     fi[4] = rest ? cx<R>(z2, R(0)) : cim * sfomeg1;
     fi[5] = rest ? cx<R>(z3, R(0)) : cip * sfomeg1;
   } else {
-    const R sqp0p3 = sqrt(fmax(p[0] + p[3], R(0))) * nsf;
+    const R sqp0p3 = sqrt(plus_component(p, p[0])) * nsf;
     const bool z = sqp0p3 == R(0);
     const cx<R> chi1(sel(z, -nhel * sqrt(R(2) * p[0]), nh * p[1] / sqp0p3), sel(z, R(0), p[2] / sqp0p3));
     const cx<R> chi0(sqp0p3, R(0));
@@ -116,7 +117,7 @@ __device__ __forceinline__ void oxxxxx(const R (&p)[4], R fmass, R nhel, R nsf, 
     fo[4] = rest ? cx<R>(z2, R(0)) : cim * sfomeg0;
     fo[5] = rest ? cx<R>(z3, R(0)) : cip * sfomeg0;
   } else {
-    const R sqp0p3 = sqrt(fmax(p[0] + p[3], R(0))) * nsf;
+    const R sqp0p3 = sqrt(plus_component(p, p[0])) * nsf;
     const bool z = sqp0p3 == R(0);
     const cx<R> chi0(sel(z, -nhel * sqrt(R(2) * p[0]), nh * p[1] / sqp0p3), sel(z, R(0), -p[2] / sqp0p3));
     const cx<R> chi1(sqp0p3, R(0));

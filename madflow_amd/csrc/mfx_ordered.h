@@ -247,13 +247,17 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
 }
 
 // MFMA 16x16x4 on one wave: D += A(16x4) * B(4x16); lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] and holds D[4*(l/16) + r][l%16], r = 0..3.
+// B[l/16][l%16] and holds D[drow(l/16, r)][l%16], r = 0..3, where the row is
+// 4*(l/16) + r for f32 and (l/16) + 4*r for f64 (measured on gfx950).
 __device__ __forceinline__ mfx_f32x4 mfma16x16x4(float a, float b, mfx_f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ mfx_f64x4 mfma16x16x4(double a, double b, mfx_f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+template <class R> __device__ __forceinline__ int drow(int lq, int r);
+template <> __device__ __forceinline__ int drow<float>(int lq, int r) { return 4 * lq + r; }
+template <> __device__ __forceinline__ int drow<double>(int lq, int r) { return lq + 4 * r; }
 template <class R> struct Vec4;
 template <> struct Vec4<float> { typedef mfx_f32x4 type; };
 template <> struct Vec4<double> { typedef mfx_f64x4 type; };
@@ -304,7 +308,7 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
         if (col < NC) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int64_t ed = e0 + 4 * lq + r;
+            const int64_t ed = e0 + drow<R>(lq, r);
             if (ed < n) {
               const cx<R> x = Jh[(int64_t)col * n + ed];
               acc[r] += x.re * dre[r] + x.im * dim[r];
@@ -322,7 +326,7 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
     if (lr == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t ed = e0 + 4 * lq + r;
+        const int64_t ed = e0 + drow<R>(lq, r);
         if (ed < n) out[ed] = (accumulate ? out[ed] : 0.0) + double(acc[r]) * scale;
       }
     }

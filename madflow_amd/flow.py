@@ -184,7 +184,7 @@ class RQSFlow:
         return tot / y.shape[0]
 
     def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True, resample=True,
-              val=None, eval_every=1000, mix=0.0):
+              val=None, eval_every=1000, mix=0.0, patience=4):
         """Minibatch weighted maximum likelihood on device-resident pairs (y, w).
 
         resample=True draws each minibatch with probability proportional to w (inverse-CDF
@@ -196,7 +196,8 @@ class RQSFlow:
         moment of the importance weights on them (is_second_moment, with the sampling
         mixture ``mix``) is measured and the parameters with the smallest one are kept:
         maximum likelihood on heavy-tailed weights over-fits the few largest weights
-        after enough steps, which the held-out moment sees and the training loss does not."""
+        after enough steps, which the held-out moment sees and the training loss does not.
+        Training stops early after ``patience`` evaluations without a new best."""
         n = y.shape[0]
         g = torch.Generator(device=y.device)
         g.manual_seed(seed)
@@ -217,6 +218,8 @@ class RQSFlow:
                     best = (m2, it, self.params.clone(), self.m1.clone(), self.m2.clone())
                 if log:
                     log("flow step %d held-out second moment %.6g (best at step %d)" % (it, m2, best[1]))
+                if patience and it - best[1] >= patience * eval_every:
+                    break
             if it == steps:
                 break
             if resample:

@@ -158,6 +158,24 @@ def train_flow(a, torch, ig):
     nt = a.flow_pairs
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
                val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix)
+    if a.flow_refine_steps > 0:
+        # second stage: points drawn from the trained proposal, weighted by f/q', carry
+        # far more of the information where the integrand is large than uniform points
+        done = 0
+        yq = torch.empty((nt, D), dtype=torch.float32, device="cuda")
+        wq = torch.empty(nt, dtype=torch.float64, device="cuda")
+        while done < nt:
+            n = min(a.events_limit, nt - done)
+            u = philox_uniforms(n, D, 98, 0, ev0=done).float()
+            ys, winv = flow.sample_mixture(u, u, a.flow_mix)
+            ys = ys.contiguous()
+            _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
+            yq[done:done + n] = ys
+            wq[done:done + n] = fb
+            done += n
+        flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1, log=log,
+                   val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix)
+        del yq, wq
     del y, f
     torch.cuda.synchronize()
     return flow, t_pairs, time.perf_counter() - t0
@@ -298,6 +316,7 @@ def main(argv=None):
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.05)
     ap.add_argument("--flow-eval-every", type=int, default=1000)
+    ap.add_argument("--flow-refine-steps", type=int, default=0)
     ap.add_argument("--me-only", default=None, metavar="PROC",
                     help="time only the smatrix kernel of PROC on pre-generated post-cut events and exit")
     ap.add_argument("--batch", type=int, default=1 << 22)

@@ -316,7 +316,7 @@ def main(argv=None):
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.05)
     ap.add_argument("--flow-eval-every", type=int, default=1000)
-    ap.add_argument("--flow-refine-steps", type=int, default=0)
+    ap.add_argument("--flow-refine-steps", type=int, default=4000)
     ap.add_argument("--me-only", default=None, metavar="PROC",
                     help="time only the smatrix kernel of PROC on pre-generated post-cut events and exit")
     ap.add_argument("--batch", type=int, default=1 << 22)

@@ -102,6 +102,24 @@ def _run_to_target(iterate, target, max_iter, tag):
     return res
 
 
+def _extend_to(iterate, res, sampled_s, seconds, torch):
+    """Keep iterating (frozen proposal) until the sampling time reaches ``seconds``; returns
+    the unweighted-event figures over every sampled iteration at that common duration."""
+    t0 = time.perf_counter()
+    extra = 0.0
+    while sampled_s + extra < seconds:
+        res.append(iterate(len(res)))
+        torch.cuda.synchronize()
+        extra = time.perf_counter() - t0
+    t = sampled_s + extra
+    sum_w = math.fsum(r["sum_f"] for r in res)
+    wmax = max(r["wmax"] for r in res)
+    npts = sum(r["n"] for r in res)
+    n_unw = sum_w / wmax if wmax > 0 else 0.0
+    return dict(iterations=len(res), points=npts, sample_s=t, n_unweighted=n_unw,
+                unweighting_efficiency=sum_w / (npts * wmax) if wmax > 0 else 0.0, unweighted_per_s=n_unw / t)
+
+
 def run_main(a, torch):
     """Plain VEGAS at the main precision: W warm-up iterations, then exactly K timed ones."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
@@ -158,22 +176,22 @@ def train_flow(a, torch, ig):
     nt = a.flow_pairs
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
                val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix)
-    if a.flow_refine_steps > 0:
-        # second stage: points drawn from the trained proposal, weighted by f/q', carry
+    for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
+        # further stages: points drawn from the current proposal, weighted by f/q', carry
         # far more of the information where the integrand is large than uniform points
         done = 0
         yq = torch.empty((nt, D), dtype=torch.float32, device="cuda")
         wq = torch.empty(nt, dtype=torch.float64, device="cuda")
         while done < nt:
             n = min(a.events_limit, nt - done)
-            u = philox_uniforms(n, D, 98, 0, ev0=done).float()
+            u = philox_uniforms(n, D, 98 - rnd, 0, ev0=done).float()
             ys, winv = flow.sample_mixture(u, u, a.flow_mix)
             ys = ys.contiguous()
             _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
             yq[done:done + n] = ys
             wq[done:done + n] = fb
             done += n
-        flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1, log=log,
+        flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1 + rnd, log=log,
                    val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix)
         del yq, wq
     del y, f
@@ -206,7 +224,9 @@ def run_flow_compare(a, torch, main):
     """Plain VEGAS with FP64 matrix elements against VEGAS + learned proposal with FP32
     matrix elements on the same cube, each run to sigma/sigma = target.  The flow path
     reuses the FP32 grid of the main run (its warm-up time is counted in its set-up),
-    then generates the training pairs, trains, and samples y ~ q with weights f(y)/q(y)."""
+    then generates the training pairs, trains, and samples y ~ q with weights f(y)/q(y).
+    After the target, both keep sampling to the same sampling time (equal_time_s), which
+    gives the unweighted rates at equal wall time over samples of comparable size."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
     sync = torch.cuda.synchronize
     # ---- plain VEGAS, FP64 matrix elements
@@ -227,15 +247,18 @@ def run_flow_compare(a, torch, main):
     res64 = _run_to_target(it64, a.target, a.max_iter, "plain fp64")
     sync()
     plain = _unweighted(res64, setup64, time.perf_counter() - t0, a.target)
+    plain["equal_time"] = _extend_to(it64, list(res64), plain["sample_s"], a.equal_time_s, torch)
     del ig64, vf
     # ---- VEGAS + flow, FP32 matrix elements, on the main run's frozen grid
     ig = main["ig"]
     flow, t_pairs, t_train = train_flow(a, torch, ig)
     setup_flow = main["t_warm"] + t_pairs + t_train
     t0 = time.perf_counter()
-    resf = _run_to_target(lambda k: flow_iteration(a, torch, ig, flow, k), a.target, a.max_iter, "flow fp32")
+    itf = lambda k: flow_iteration(a, torch, ig, flow, k)
+    resf = _run_to_target(itf, a.target, a.max_iter, "flow fp32")
     sync()
     learned = _unweighted(resf, setup_flow, time.perf_counter() - t0, a.target)
+    learned["equal_time"] = _extend_to(itf, list(resf), learned["sample_s"], a.equal_time_s, torch)
     learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"],
                    best_step=getattr(flow, "best_step", None), mixture=a.flow_mix)
 
@@ -250,7 +273,9 @@ def run_flow_compare(a, torch, main):
                  "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
                  "batch": a.flow_batch, "defensive_mixture": a.flow_mix},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
-        "gain_unweighted_per_s_steady": gain("unweighted_per_s_steady"),
+        "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
+                                             plain["equal_time"]["unweighted_per_s"]
+                                             if plain["equal_time"]["unweighted_per_s"] > 0 else None),
         "sigma_pull": (learned["sigma_pb"] - plain["sigma_pb"]) / math.hypot(learned["err_pb"], plain["err_pb"]),
     }
 
@@ -314,9 +339,12 @@ def main(argv=None):
     ap.add_argument("--flow-steps", type=int, default=20000)
     ap.add_argument("--flow-batch", type=int, default=4096)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
-    ap.add_argument("--flow-mix", type=float, default=0.05)
+    ap.add_argument("--flow-mix", type=float, default=0.2)
     ap.add_argument("--flow-eval-every", type=int, default=1000)
     ap.add_argument("--flow-refine-steps", type=int, default=4000)
+    ap.add_argument("--flow-refine-rounds", type=int, default=1)
+    ap.add_argument("--equal-time-s", type=float, default=6.0,
+                    help="after reaching the target, keep sampling both methods to this much sampling time")
     ap.add_argument("--me-only", default=None, metavar="PROC",
                     help="time only the smatrix kernel of PROC on pre-generated post-cut events and exit")
     ap.add_argument("--batch", type=int, default=1 << 22)

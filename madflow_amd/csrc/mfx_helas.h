@@ -218,12 +218,31 @@ template <class R> __device__ __forceinline__ LC<R> lightcone(const W<R>& V) {
 }
 
 // ------------------------------------------------------------------------ FFV1_0
+// The fermion bilinear F1bar gamma^mu F2 in light-cone components (the coefficients of
+// V0+V3, V0-V3, V1+iV2, V1-iV2): a program that closes many diagrams on the same fermion
+// pair evaluates it once and each amplitude costs four complex products.
+template <class R> struct Bilinear { cx<R> a, b, c, d; };
+template <class R>
+__device__ __forceinline__ Bilinear<R> bilinear(const W<R>& F1, const W<R>& F2) {
+  Bilinear<R> j;
+  j.a = F1[2] * F2[4] + F1[5] * F2[3];
+  j.b = F1[3] * F2[5] + F1[4] * F2[2];
+  j.c = F1[2] * F2[5] - F1[4] * F2[3];
+  j.d = F1[3] * F2[4] - F1[5] * F2[2];
+  return j;
+}
 template <class R>
 __device__ __forceinline__ cx<R> FFV1_0(const W<R>& F1, const W<R>& F2, const W<R>& V3, cx<R> COUP) {
   const LC<R> v = lightcone(V3);
-  const cx<R> TMP0 = F1[2] * (F2[4] * v.a + F2[5] * v.c) + F1[3] * (F2[4] * v.d + F2[5] * v.b) +
-                     F1[4] * (F2[2] * v.b - F2[3] * v.c) + F1[5] * (F2[3] * v.a - F2[2] * v.d);
-  return mul_mi(COUP * TMP0);
+  if constexpr (StableForms<R>::value) {
+    const Bilinear<R> j = bilinear(F1, F2);
+    const cx<R> TMP0 = j.a * v.a + j.b * v.b + j.c * v.c + j.d * v.d;
+    return mul_mi(COUP * TMP0);
+  } else {
+    const cx<R> TMP0 = F1[2] * (F2[4] * v.a + F2[5] * v.c) + F1[3] * (F2[4] * v.d + F2[5] * v.b) +
+                       F1[4] * (F2[2] * v.b - F2[3] * v.c) + F1[5] * (F2[3] * v.a - F2[2] * v.d);
+    return mul_mi(COUP * TMP0);
+  }
 }
 
 // ------------------------------------------------------------------------ FFV1_1

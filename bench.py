@@ -37,6 +37,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 packed, FP64)
+# HBM bytes per matrix element of k_smatrix<gg_ttxgg, float>: rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md)
+HBM_BYTES_PER_ME = {32: 8369.0}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 KIN = dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4)
 
@@ -403,7 +406,10 @@ def main(argv=None):
         "unweighted": u,
         "unweighted_events_per_s": u["unweighted_per_s_to_target"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": (HBM_BYTES_PER_ME[a.precision] * m["me_n"] / max(1, m["launches"])
+                                 if a.precision in HBM_BYTES_PER_ME and a.mapping == "diet" else None),
+                     "traffic_unit": "bytes per launch (PMC)",
                      "kernel": "k_smatrix<gg_ttxgg,%s>" % ("float" if a.precision == 32 else "double"),
                      "flops_per_me": flops, "mes_per_launch": m["me_n"] / max(1, m["launches"]),
                      "kernel_ms_per_launch": m["me_ms"] / max(1, m["launches"])},

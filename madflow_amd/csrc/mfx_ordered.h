@@ -113,7 +113,7 @@ __device__ __forceinline__ cx<R> ordered_amplitude(const W<R> (&g)[NG], const W<
         mom_out(N3, P);
         den2 = den_from_slots(P, R(0));
       }
-      const R rden = R(1) / den2;
+      const R rden = recip(den2);
       const cx<R> k3 = prm.GC_10 * rden, k4 = c4 * rden;
       W<R>& o = J[i][j];
       o[0] = N3[0];

@@ -103,11 +103,115 @@ __device__ __forceinline__ void load_momenta(const double* __restrict__ all_ps, 
 // modify" the momenta keeps them inside the loop at no instruction cost.
 template <class R> __device__ __forceinline__ void opaque(R& x) { asm volatile("" : "+v"(x)); }
 
-// Hot path: all helicity rows for nev events.
+// The colour sum runs on the matrix cores for FP32 and 24 or more colour flows.
+template <class T, class R> struct MfmaColour {
+  static constexpr bool value = StableForms<R>::value && T::NCOLOR >= 24 && T::NCOLOR % 4 == 0;
+};
+
+typedef float mfx_cf32x4 __attribute__((ext_vector_type(4)));
+
+// LDS ordering between the lanes of one wave (no block barrier needed).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Re(J^dagger CF J) / CF_DEN of one helicity row for the 64 events of a wave (one event
+// per lane), as a batched dense contraction on the matrix cores: the wave stages its
+// jamps in LDS (colour-major), Y = J CF runs as (16 events x NCOLOR) x (NCOLOR x 16)
+// tiles of v_mfma_f32_16x16x4f32 for Re J and Im J (lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds D[4(l/16)+r][l%16]), each D tile is dotted with J row by row,
+// and the 16 column partial sums of each row are reduced across the quarter-wave.
+// stage: 2*NCOLOR*64 floats of this wave; cfl: the integer colour matrix [NCOLOR][NCP];
+// red: 64 floats of this wave.
+template <class T>
+__device__ __forceinline__ float colour_sum_mfma(const cx<float> (&J)[T::NCOLOR], float* stage, const float* cfl,
+                                                 float* red) {
+  constexpr int NC = T::NCOLOR, NCP = (NC + 15) / 16 * 16, KS = NC / 4;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+  wave_lds_sync();                       // the previous row's reads are done
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    stage[(2 * c) * 64 + lane] = J[c].re;
+    stage[(2 * c + 1) * 64 + lane] = J[c].im;
+  }
+  wave_lds_sync();
+  float part[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[m][r] = 0.f;
+    float are[KS], aim[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      are[k] = stage[(2 * (4 * k + lq)) * 64 + 16 * m + lr];
+      aim[k] = stage[(2 * (4 * k + lq) + 1) * 64 + 16 * m + lr];
+    }
+#pragma unroll
+    for (int nt = 0; nt < NCP / 16; ++nt) {
+      mfx_cf32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        const float b = cfl[(4 * k + lq) * NCP + 16 * nt + lr];
+        dre = __builtin_amdgcn_mfma_f32_16x16x4f32(are[k], b, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim[k], b, dim, 0, 0, 0);
+      }
+      const int col = 16 * nt + lr;
+      if (col < NC) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ev = 16 * m + 4 * lq + r;
+          part[m][r] += stage[(2 * col) * 64 + ev] * dre[r] + stage[(2 * col + 1) * 64 + ev] * dim[r];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) part[m][r] += __shfl_xor(part[m][r], off, 64);
+  if (lr == 0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[16 * m + 4 * lq + r] = part[m][r];
+  }
+  wave_lds_sync();
+  return red[lane] * float(1.0 / T::CF_DEN);
+}
+
+template <class T, bool MFMA> struct ColourLds { float stage[1], cf[1], red[1]; };
+template <class T> struct ColourLds<T, true> {
+  static constexpr int NCP = (T::NCOLOR + 15) / 16 * 16;
+  float stage[4][2 * T::NCOLOR * 64];
+  float cf[T::NCOLOR * NCP];
+  float red[4][64];
+};
+
+// Hot path: all helicity rows for nev events.  The event loop is block-uniform so that
+// every lane of a wave reaches the wave-wide colour contraction; lanes past the end
+// evaluate the last event again and write nothing.
 template <class T, class R>
 __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
+  constexpr bool MFMA = MfmaColour<T, R>::value;
+  __shared__ ColourLds<T, MFMA> lds;
+  if constexpr (MFMA) {
+    constexpr int NCP = ColourLds<T, true>::NCP;
+    for (int i = threadIdx.x; i < T::NCOLOR * NCP; i += blockDim.x) {
+      const int r = i / NCP, c = i % NCP;
+      lds.cf[i] = c < T::NCOLOR ? T::cf(float(), r, c) : 0.f;
+    }
+    __syncthreads();
+  }
+  const int wave = threadIdx.x >> 6;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nev; e += stride) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.nev; base += stride) {
+    const int64_t e0 = base + threadIdx.x;
+    const bool active = e0 < a.nev;
+    const int64_t e = active ? e0 : a.nev - 1;
     R p[T::NEXTERNAL][4];
     R inv[T::NINV];
     load_momenta<T, R>(a.all_ps, e, p);
@@ -126,9 +230,10 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
       }
       cx<R> jamp[T::NCOLOR];
       T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
-      ans += colour_sum<T, R>(jamp);
+      if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[wave], lds.cf, lds.red[wave]);
+      else ans += colour_sum<T, R>(jamp);
     }
-    a.out[e] = double(ans) / T::DENOMINATOR;
+    if (active) a.out[e] = double(ans) / T::DENOMINATOR;
   }
 }
 

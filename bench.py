@@ -194,6 +194,10 @@ def train_flow(a, torch, ig):
             yq[done:done + n] = ys
             wq[done:done + n] = fb
             done += n
+        if a.flow_refine_power != 1.0:
+            # weights (f/q')^p: p = 2 minimises the second moment of the importance weights
+            # itself (gradient -E_q'[(f/q')^2 grad log q]) and so presses on the largest weights
+            wq = (wq / wq.max()).pow(a.flow_refine_power)
         flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1 + rnd, log=log,
                    val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix)
         del yq, wq
@@ -346,6 +350,7 @@ def main(argv=None):
     ap.add_argument("--flow-eval-every", type=int, default=1000)
     ap.add_argument("--flow-refine-steps", type=int, default=4000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
+    ap.add_argument("--flow-refine-power", type=float, default=1.0)
     ap.add_argument("--equal-time-s", type=float, default=6.0,
                     help="after reaching the target, keep sampling both methods to this much sampling time")
     ap.add_argument("--me-only", default=None, metavar="PROC",

@@ -65,24 +65,22 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
   for (int k = 2; k < 6; ++k) acc[k] += t[k];
 }
 
-// One colour-ordered amplitude.  g: gluon wavefunctions in the order of the word;
-// pinv[a][b] (a < b) = 2 k_a.k_b of the gluons at word positions a, b and
-// tinv[a] = 2 k_top.k_a (FP32 forms only); prm couplings.
-template <int NG, class R>
-__device__ __forceinline__ cx<R> ordered_amplitude(const W<R> (&g)[NG], const W<R>& top, const W<R>& atop,
-                                                   const R (&pinv)[NG][NG], const R (&tinv)[NG],
-                                                   const Params<R>& prm) {
+// Colour-ordered currents of one word.  J[i][j] (i <= j) is the gluon current of the
+// word positions i..j (J[i][i] = the gluon itself), O[j] the top line carrying the top and
+// the first j gluons.  pinv[a][b] (a < b) = 2 k_a.k_b of the gluons at word positions a,
+// b and tinv[a] = 2 k_top.k_a (FP32 forms only).  All indices are compile-time constants
+// after unrolling, so every current lives in registers.
+
+// J(i..j) for every range ending at j in [J0, J1), shorter ranges first.
+template <int NG, int J0, int J1, class R>
+__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], const R (&pinv)[NG][NG], const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
-  W<R> J[NG][NG];
-#pragma unroll
-  for (int i = 0; i < NG; ++i) J[i][i] = g[i];
   // -i GC_12 multiplies the four-gluon structures (GC_12 = i gs^2: a real gs^2)
   const cx<R> c4 = mul_mi(prm.GC_12);
 #pragma unroll
-  for (int len = 2; len <= NG; ++len) {
+  for (int j = J0; j < J1; ++j) {
 #pragma unroll
-    for (int i = 0; i + len <= NG; ++i) {
-      const int j = i + len - 1;
+    for (int i = j - 1; i >= 0; --i) {
       W<R> N3;
       vvv_numerator(J[i][i], J[i + 1][j], N3);
 #pragma unroll
@@ -122,11 +120,15 @@ __device__ __forceinline__ cx<R> ordered_amplitude(const W<R> (&g)[NG], const W<
       for (int q = 2; q < 6; ++q) o[q] = k3 * N3[q] + k4 * N4[q];
     }
   }
-  // top line: O(j) carries the top and the first j gluons of the word
-  W<R> O[NG];
-  O[0] = top;
+}
+
+// O(j) for j in [O0, O1]: O(j) = sum_{i<j} FFV1_1(O(i), J(i..j-1)).
+template <int NG, int O0, int O1, class R>
+__device__ __forceinline__ void ordered_top_line(W<R> (&O)[NG], const W<R> (&J)[NG][NG], const R (&pinv)[NG][NG],
+                                                 const R (&tinv)[NG], const Params<R>& prm) {
+  constexpr bool INV = StableForms<R>::value;
 #pragma unroll
-  for (int j = 1; j < NG; ++j) {
+  for (int j = O0; j <= O1; ++j) {
     R den2 = R(0);
     if constexpr (INV) {
 #pragma unroll
@@ -145,6 +147,12 @@ __device__ __forceinline__ cx<R> ordered_amplitude(const W<R> (&g)[NG], const W<
       else add_components(O[j], t);
     }
   }
+}
+
+// jamp = -sum_{i<NG} FFV1_0(anti-top, O(i), J(i..NG-1))
+template <int NG, class R>
+__device__ __forceinline__ cx<R> ordered_close(const W<R>& atop, const W<R> (&O)[NG], const W<R> (&J)[NG][NG],
+                                               const Params<R>& prm) {
   cx<R> amp(R(0), R(0));
 #pragma unroll
   for (int i = 0; i < NG; ++i) amp += FFV1_0(atop, O[i], J[i][NG - 1], prm.GC_11);
@@ -218,29 +226,49 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         lds[(s * 12 + 2 * q + 1) * BLOCK + t] = gw[q].im;
       }
     }
+    // Orderings come in lexicographic order, so orderings 2m and 2m+1 share their first
+    // NG-2 gluons: the currents inside that prefix (J(i..j), j < NG-2, and O(1..NG-2))
+    // are computed once per pair and carried to the second ordering.
+    constexpr int PRE = NG - 2;
+    W<R> J[NG][NG], O[NG];
+    R pinv[NG][NG], tinv[NG];
+    O[0] = top;
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
       int perm[NG];
 #pragma unroll
       for (int k = 0; k < NG; ++k) perm[k] = T::perm(c, k);
-      W<R> g[NG];
+      const bool fresh = (c & 1) == 0;
 #pragma unroll
-      for (int k = 0; k < NG; ++k)
+      for (int k = 0; k < NG; ++k) {
+        if (k < PRE && !fresh) continue;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-          g[k][q].re = lds[(perm[k] * 12 + 2 * q) * BLOCK + t];
-          g[k][q].im = lds[(perm[k] * 12 + 2 * q + 1) * BLOCK + t];
+          J[k][k][q].re = lds[(perm[k] * 12 + 2 * q) * BLOCK + t];
+          J[k][k][q].im = lds[(perm[k] * 12 + 2 * q + 1) * BLOCK + t];
         }
-      R pinv[NG][NG], tinv[NG];
+      }
       if constexpr (S::NINV > 0) {
 #pragma unroll
         for (int x = 0; x < NG; ++x) {
+          if (x < PRE && !fresh) continue;
           tinv[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
-#pragma unroll
-          for (int y = x + 1; y < NG; ++y) pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
         }
+#pragma unroll
+        for (int x = 0; x < NG; ++x)
+#pragma unroll
+          for (int y = x + 1; y < NG; ++y) {
+            if (y < PRE && !fresh) continue;
+            pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
+          }
       }
-      const cx<R> amp = ordered_amplitude<NG, R>(g, top, atop, pinv, tinv, prm);
+      if (fresh) {
+        ordered_ranges<NG, 1, PRE, R>(J, pinv, prm);
+        ordered_top_line<NG, 1, PRE, R>(O, J, pinv, tinv, prm);
+      }
+      ordered_ranges<NG, PRE, NG, R>(J, pinv, prm);
+      ordered_top_line<NG, PRE + 1, NG - 1, R>(O, J, pinv, tinv, prm);
+      const cx<R> amp = ordered_close<NG, R>(atop, O, J, prm);
       jb[((int64_t)hh * NC + c) * a.n + el] = amp;
     }
   }

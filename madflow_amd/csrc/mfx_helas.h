@@ -246,12 +246,11 @@ __device__ __forceinline__ cx<R> FFV1_0(const W<R>& F1, const W<R>& F2, const W<
 }
 
 // ------------------------------------------------------------------------ FFV1_1
+// P: the off-shell fermion's momentum (minus the sum of the input slots); the slots of F1
+// are not written (callers that pass P from the external momenta never read them).
 template <class R>
-__device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& F1) {
-  F1[0] = F2[0] + V3[0];
-  F1[1] = F2[1] + V3[1];
-  R P[4];
-  mom_out(F1, P);
+__device__ __forceinline__ void FFV1_1_p(const W<R>& F2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2,
+                                         const R (&P)[4], W<R>& F1) {
   const cx<R> denom = prop_denom(COUP, den2, M1, W1);
   const LC<R> v = lightcone(V3);
   const R pp0 = P[0] + P[3], pm0 = P[0] - P[3];
@@ -262,14 +261,19 @@ __device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COU
   F1[4] = -di * (F2[4] * (v.a * pm0 - v.d * q) + F2[5] * (v.c * pm0 - v.b * q) + (F2[3] * v.c - F2[2] * v.b) * M1);
   F1[5] = di * (F2[4] * (v.a * qb - v.d * pp0) + F2[5] * (v.c * qb - v.b * pp0) + (F2[3] * v.a - F2[2] * v.d) * M1);
 }
+template <class R>
+__device__ __forceinline__ void FFV1_1(const W<R>& F2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& F1) {
+  F1[0] = F2[0] + V3[0];
+  F1[1] = F2[1] + V3[1];
+  R P[4];
+  mom_out(F1, P);
+  FFV1_1_p(F2, V3, COUP, M1, W1, den2, P, F1);
+}
 
 // ------------------------------------------------------------------------ FFV1_2
 template <class R>
-__device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COUP, R M2, R W2, R den2, W<R>& F2) {
-  F2[0] = F1[0] + V3[0];
-  F2[1] = F1[1] + V3[1];
-  R P[4];
-  mom_out(F2, P);
+__device__ __forceinline__ void FFV1_2_p(const W<R>& F1, const W<R>& V3, cx<R> COUP, R M2, R W2, R den2,
+                                         const R (&P)[4], W<R>& F2) {
   const cx<R> denom = prop_denom(COUP, den2, M2, W2);
   const LC<R> v = lightcone(V3);
   const R pp0 = P[0] + P[3], pm0 = P[0] - P[3];
@@ -280,22 +284,38 @@ __device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COU
   F2[4] = -di * (F1[4] * (v.c * qb - v.b * pp0) + F1[5] * (v.d * pp0 - v.a * qb) - (F1[2] * v.a + F1[3] * v.d) * M2);
   F2[5] = di * (F1[4] * (v.b * q - v.c * pm0) + F1[5] * (v.a * pm0 - v.d * q) + (F1[2] * v.c + F1[3] * v.b) * M2);
 }
+template <class R>
+__device__ __forceinline__ void FFV1_2(const W<R>& F1, const W<R>& V3, cx<R> COUP, R M2, R W2, R den2, W<R>& F2) {
+  F2[0] = F1[0] + V3[0];
+  F2[1] = F1[1] + V3[1];
+  R P[4];
+  mom_out(F2, P);
+  FFV1_2_p(F1, V3, COUP, M2, W2, den2, P, F2);
+}
 
 // ---------------------------------------------------------------------- VVV1P0_1
+// P2, P3: the momenta of V2, V3 (their slot momenta); V1's slots are not written.
 template <class R>
-__device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& V1) {
-  R P2[4], P3[4], P1[4];
-  mom_in(V2, P2);
-  mom_in(V3, P3);
-  V1[0] = V2[0] + V3[0];
-  V1[1] = V2[1] + V3[1];
-  mom_out(V1, P1);
+__device__ __forceinline__ void VVV1P0_1_p(const W<R>& V2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2,
+                                           const R (&P2)[4], const R (&P3)[4], W<R>& V1) {
+  R P1[4];
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) P1[mu] = -(P2[mu] + P3[mu]);
   const cx<R> TMP1 = mdotp(V3, P1), TMP2 = mdotp(V3, P2), TMP3 = mdotp(V2, P1), TMP4 = mdotp(V2, P3);
   const cx<R> TMP5 = mdot(V3, V2);
   const cx<R> di = mul_i(prop_denom(COUP, den2, M1, W1));
   const cx<R> c2 = TMP2 - TMP1, c3 = TMP3 - TMP4;
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) V1[2 + mu] = di * (TMP5 * (P3[mu] - P2[mu]) + V2[2 + mu] * c2 + V3[2 + mu] * c3);
+}
+template <class R>
+__device__ __forceinline__ void VVV1P0_1(const W<R>& V2, const W<R>& V3, cx<R> COUP, R M1, R W1, R den2, W<R>& V1) {
+  R P2[4], P3[4];
+  mom_in(V2, P2);
+  mom_in(V3, P3);
+  V1[0] = V2[0] + V3[0];
+  V1[1] = V2[1] + V3[1];
+  VVV1P0_1_p(V2, V3, COUP, M1, W1, den2, P2, P3, V1);
 }
 
 // ------------------------------------------------------------------ VVVV{1,3,4}P0_1

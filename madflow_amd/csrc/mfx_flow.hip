@@ -489,17 +489,36 @@ __global__ void __launch_bounds__(64) k_layer_bwd(BwdArgs A, const float* __rest
 }
 
 // fixed-order reduction of the per-block partials into the gradient vector
-__global__ void k_reduce_grad(const float* __restrict__ partial, int nblocks, int nlp, float* __restrict__ grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nlp) return;
+// Sum of the per-block partial gradients: 64 parameters per block, the partials of each
+// split over 4 waves (Kahan sums over blocks w, w+4, ...), combined in a fixed order in
+// LDS, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) k_reduce_grad(const float* __restrict__ partial, int nblocks, int nlp,
+                                                     float* __restrict__ grad) {
+  __shared__ float ls[4][64], lc[4][64];
+  const int pi = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + pi;
   float s = 0.f, c = 0.f;
-  for (int b = 0; b < nblocks; ++b) {
-    const float y = partial[(size_t)b * nlp + i] - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
+  if (i < nlp) {
+    for (int b = sl; b < nblocks; b += 4) {
+      const float y = partial[(size_t)b * nlp + i] - c;
+      const float t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
   }
-  grad[i] = s;
+  ls[sl][pi] = s;
+  lc[sl][pi] = c;
+  __syncthreads();
+  if (sl == 0 && i < nlp) {
+    float tot = 0.f, cc = 0.f;
+    for (int w = 0; w < 4; ++w) {
+      const float y = (ls[w][pi] - lc[w][pi]) - cc;
+      const float t = tot + y;
+      cc = (t - tot) - y;
+      tot = t;
+    }
+    grad[i] = tot;
+  }
 }
 
 // Adam step on the fp32 master parameters, then refresh the transposed bf16 copies.
@@ -682,7 +701,7 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
     const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
     hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(64), shmem, s, A, params, (const unsigned short*)params_bf16);
     const int nlp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
-    hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 255) / 256), dim3(256), 0, s, partial, nblk, nlp,
+    hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nblk, nlp,
                        grad + G.layer[l].oW1);
   }
   return hip_status(hipGetLastError());

@@ -39,7 +39,7 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 packed, FP64)
 # HBM bytes per matrix element of k_smatrix<gg_ttxgg, float>: rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md)
-HBM_BYTES_PER_ME = {32: 8369.0}
+HBM_BYTES_PER_ME = {32: 542.9}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 KIN = dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4)
 
@@ -178,13 +178,14 @@ def train_flow(a, torch, ig):
     flow = RQSFlow(D, layers=8, seed=1)
     nt = a.flow_pairs
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
-               val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix)
+               val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta)
     for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
         # further stages: points drawn from the current proposal, weighted by f/q', carry
         # far more of the information where the integrand is large than uniform points
         done = 0
         yq = torch.empty((nt, D), dtype=torch.float32, device="cuda")
         wq = torch.empty(nt, dtype=torch.float64, device="cuda")
+        fq = torch.empty(nt, dtype=torch.float64, device="cuda") if a.flow_refine_beta else None
         while done < nt:
             n = min(a.events_limit, nt - done)
             u = philox_uniforms(n, D, 98 - rnd, 0, ev0=done).float()
@@ -193,14 +194,13 @@ def train_flow(a, torch, ig):
             _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
             yq[done:done + n] = ys
             wq[done:done + n] = fb
+            if fq is not None:
+                fq[done:done + n] = fb / winv
             done += n
-        if a.flow_refine_power != 1.0:
-            # weights (f/q')^p: p = 2 minimises the second moment of the importance weights
-            # itself (gradient -E_q'[(f/q')^2 grad log q]) and so presses on the largest weights
-            wq = (wq / wq.max()).pow(a.flow_refine_power)
         flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1 + rnd, log=log,
-                   val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix)
-        del yq, wq
+                   val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix,
+                   f=fq, beta=a.flow_refine_beta)
+        del yq, wq, fq
     del y, f
     torch.cuda.synchronize()
     return flow, t_pairs, time.perf_counter() - t0
@@ -209,22 +209,7 @@ def train_flow(a, torch, ig):
 def flow_iteration(a, torch, ig, flow, k):
     """One iteration of a.nev points from the defensive mixture of the flow and the
     frozen VEGAS cube, each weighted by f(y)/q'(y)."""
-    from madflow_amd.phasespace_gpu import philox_uniforms
-    D = ig.ndim
-    sums, done = [], 0
-    while done < a.nev:
-        n = min(a.events_limit, a.nev - done)
-        u = philox_uniforms(n, D, 1234, 100 + k, ev0=done).float()
-        ys, winv = flow.sample_mixture(u, u, a.flow_mix)
-        r, _, _ = ig.chunk(done, n, 100 + k, y_in=ys.contiguous(), w_in=winv.contiguous())
-        sums.append(r)
-        done += n
-    s1 = math.fsum(r[0] for r in sums)
-    s2 = math.fsum(r[1] for r in sums)
-    N = float(a.nev)
-    mean = s1 / N
-    return dict(I=mean, var=max((s2 / N - mean * mean) / (N - 1), 0.0), sum_f=s1,
-                wmax=max(r[2] for r in sums), n=a.nev)
+    return ig.proposal_iteration(flow, a.nev, 100 + k, a.flow_mix)
 
 
 def run_flow_compare(a, torch, main):
@@ -278,7 +263,8 @@ def run_flow_compare(a, torch, main):
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
                  "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
-                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix},
+                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix,
+                 "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
         "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
                                              plain["equal_time"]["unweighted_per_s"]
@@ -350,7 +336,9 @@ def main(argv=None):
     ap.add_argument("--flow-eval-every", type=int, default=1000)
     ap.add_argument("--flow-refine-steps", type=int, default=4000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
-    ap.add_argument("--flow-refine-power", type=float, default=1.0)
+    ap.add_argument("--flow-beta", type=float, default=0.0,
+                    help="stage-1 loss: 0 = KL, beta > 0 = alpha-divergence with alpha = 1 + beta")
+    ap.add_argument("--flow-refine-beta", type=float, default=0.0, help="the same for the later stages")
     ap.add_argument("--equal-time-s", type=float, default=6.0,
                     help="after reaching the target, keep sampling both methods to this much sampling time")
     ap.add_argument("--me-only", default=None, metavar="PROC",

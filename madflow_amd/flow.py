@@ -147,13 +147,27 @@ class RQSFlow:
         return y, -ld_inv
 
     # ----------------------------------------------------------------- training
-    def grad(self, y, w):
-        """Gradient of L = -sum_i w_i log q(y_i) / sum_i w_i (returns L, grad)."""
+    def grad(self, y, w, f=None, beta=0.0):
+        """Gradient of L = -sum_i v_i log q(y_i) / sum_i v_i (returns L, grad), with
+        v_i = w_i for beta = 0 and v_i = w_i (f_i / q(y_i))^beta otherwise.
+
+        For points y_i drawn from a density p with w_i = f_i / p_i, beta = 0 is the
+        gradient of KL(f || q) and beta = a - 1 that of the alpha-divergence
+        int f^a q^(1-a) (up to a positive factor); beta = 1 minimises the second moment
+        of the importance weights f/q itself, and larger beta weighs the largest ratios
+        f/q, which set the unweighting efficiency, harder still."""
         y = _lib.require_device_tensor(y, torch.float32, "y")
         n = y.shape[0]
         save = torch.empty(self.lib.mfx_flow_save_bytes(self.D, self.L, self._m(), n) // 4 + 1,
                            dtype=torch.float32, device=y.device)
         u, ld = self._apply(y, False, save=save)
+        if beta:
+            if f is None:
+                raise ValueError("grad: beta != 0 needs the integrand values f")
+            # log of (f / q)^beta, shifted by its maximum so that exp cannot overflow
+            lv = beta * (torch.log(f.to(torch.float64)) - ld.to(torch.float64))
+            lv = lv - lv.max()
+            w = w.to(torch.float64) * torch.exp(lv)
         wsum = w.sum()
         gl = (-w / wsum).to(torch.float32).contiguous()
         gx = torch.zeros_like(y)
@@ -184,7 +198,7 @@ class RQSFlow:
         return tot / y.shape[0]
 
     def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True, resample=True,
-              val=None, eval_every=1000, mix=0.0, patience=4):
+              val=None, eval_every=1000, mix=0.0, patience=4, f=None, beta=0.0):
         """Minibatch weighted maximum likelihood on device-resident pairs (y, w).
 
         resample=True draws each minibatch with probability proportional to w (inverse-CDF
@@ -197,8 +211,14 @@ class RQSFlow:
         mixture ``mix``) is measured and the parameters with the smallest one are kept:
         maximum likelihood on heavy-tailed weights over-fits the few largest weights
         after enough steps, which the held-out moment sees and the training loss does not.
-        Training stops early after ``patience`` evaluations without a new best."""
+        Training stops early after ``patience`` evaluations without a new best.
+
+        beta > 0 trains on the alpha-divergence with a = 1 + beta instead of KL (see grad);
+        f holds the integrand values at y (w = f / p for points drawn from p; f = w for
+        uniform points, the default)."""
         n = y.shape[0]
+        if beta and f is None:
+            f = w
         g = torch.Generator(device=y.device)
         g.manual_seed(seed)
         losses = []
@@ -227,14 +247,16 @@ class RQSFlow:
                 idx = torch.searchsorted(cdf, r).clamp_(max=n - 1)
                 yb = y[idx].contiguous()
                 wb = ones
+                fb = f[idx] if beta else None
             else:
                 idx = torch.randint(0, n, (batch,), generator=g, device=y.device)
                 yb = y[idx].contiguous()
                 wb = w[idx].contiguous()
+                fb = f[idx] if beta else None
                 if float(wb.sum()) <= 0:
                     continue
             cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
-            loss, grad = self.grad(yb, wb)
+            loss, grad = self.grad(yb, wb, fb, beta)
             self.adam_step(grad, lr=cur_lr)
             if log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1):
                 log("flow step %d loss %.5f" % (it, float(loss)))

@@ -190,6 +190,32 @@ class MEIntegrand:
             out["f"] = torch.cat(fs)
         return out
 
+    def proposal_iteration(self, flow, n_events, iteration, mix=0.2, seed=1234):
+        """One iteration of n_events points drawn from the defensive mixture
+        q' = (1 - mix) q + mix of a trained flow q on this integrand's (frozen) VEGAS cube,
+        each weighted by f(y) / q'(y); the estimate is unbiased for any q, and mix = 1 is
+        plain VEGAS on the frozen grid.  Uniforms: Philox stream (seed, iteration).  The grid
+        is not refined.  Returns the same dict as iteration()."""
+        from .phasespace_gpu import philox_uniforms
+        if flow.D != self.ndim:
+            raise ValueError("proposal_iteration: the flow has %d dimensions, the cube %d" % (flow.D, self.ndim))
+        tot = None
+        npass = 0
+        ev0 = 0
+        while ev0 < n_events:
+            n = min(self.events_limit, n_events - ev0)
+            u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0).float()
+            ys, winv = flow.sample_mixture(u, u, mix)
+            r, k, _ = self.chunk(ev0, n, iteration, y_in=ys.contiguous(), w_in=winv.contiguous())
+            tot = r if tot is None else _merge(tot, r)
+            npass += k
+            ev0 += n
+        N = float(n_events)
+        mean = tot[0] / N
+        var = max((tot[1] / N - mean * mean) / (N - 1.0), 0.0)
+        return dict(I=mean, var=var, sigma=math.sqrt(var), wmax=tot[2], nonzero=tot[3], npass=npass, n=n_events,
+                    sum_f=tot[0])
+
 
 def _merge(a, b):
     c = a.copy()

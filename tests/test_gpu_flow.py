@@ -72,6 +72,34 @@ def test_flow_gradients_match_autograd(dev):
     assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 3e-2
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("beta", [1.0, 2.0])
+def test_flow_alpha_divergence_gradient(dev, beta):
+    """beta > 0: the device gradient is that of the alpha-divergence sum_i w_i (f_i / q_i)^beta
+    of the restated flow (autograd through log q), divided by the positive factor
+    beta * sum_i w_i (f_i / q_i)^beta."""
+    print("case flow alpha-divergence beta=%g" % beta, flush=True)
+    D = 8
+    flow = RQSFlow(D, layers=8, seed=7, device=dev)
+    p = _perturbed(flow, seed=12)
+    n = 512
+    x = torch.rand((n, D), generator=torch.Generator().manual_seed(13), dtype=torch.float32)
+    w = torch.rand(n, generator=torch.Generator().manual_seed(14), dtype=torch.float64) ** 2
+    f = torch.rand(n, generator=torch.Generator().manual_seed(15), dtype=torch.float64) ** 4 + 1e-3
+    _, grad = flow.grad(x.to(dev), w.to(dev), f.to(dev), beta)
+    pr = p.clone().double().requires_grad_(True)
+    _, ldr = R.apply(flow, pr, x.double())
+    terms = w * (f * torch.exp(-ldr)) ** beta
+    terms.sum().backward()
+    g_ref = pr.grad.numpy() / (beta * float(terms.sum()))
+    g = grad.cpu().numpy()
+    cos = float(np.dot(g, g_ref) / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
+    print("alpha-divergence grad cosine", cos, "rel norm err", np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref),
+          flush=True)
+    assert cos > 0.999
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 3e-2
+
+
 def _target(y):
     """Peaked, correlated test density on [0,1]^D (unnormalised)."""
     z = (y - 0.3) / 0.08

@@ -333,12 +333,12 @@ def main(argv=None):
     ap.add_argument("--flow-batch", type=int, default=4096)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.2)
-    ap.add_argument("--flow-eval-every", type=int, default=1000)
-    ap.add_argument("--flow-refine-steps", type=int, default=4000)
+    ap.add_argument("--flow-eval-every", type=int, default=500)
+    ap.add_argument("--flow-refine-steps", type=int, default=3000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
     ap.add_argument("--flow-beta", type=float, default=0.0,
                     help="stage-1 loss: 0 = KL, beta > 0 = alpha-divergence with alpha = 1 + beta")
-    ap.add_argument("--flow-refine-beta", type=float, default=0.0, help="the same for the later stages")
+    ap.add_argument("--flow-refine-beta", type=float, default=3.0, help="the same for the later stages")
     ap.add_argument("--equal-time-s", type=float, default=6.0,
                     help="after reaching the target, keep sampling both methods to this much sampling time")
     ap.add_argument("--me-only", default=None, metavar="PROC",

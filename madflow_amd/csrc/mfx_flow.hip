@@ -10,7 +10,9 @@
 //   forward  x -> u  with log|det du/dx|   (density: log q(x) = sum of log-dets)
 //   inverse  u -> x  (sampling; analytic root of the RQ quadratic)
 //   backward gradients of a per-sample weighted objective through every layer
-// Kernels work on one wave per 16 samples: the 128x128 and 128xNP GEMMs of the forward
+// Kernels work on one block of 4 waves per 16 samples (the waves split the MFMA output
+// tiles, so that a small training batch still puts a wave on every SIMD): the 128x128
+// and 128xNP GEMMs of the forward
 // pass run on bf16 MFMA (v_mfma_f32_16x16x16bf16, fp32 accumulation, bf16 weights),
 // the backward GEMMs on fp32 MFMA (v_mfma_f32_16x16x4f32); the spline math is per-lane
 // VALU.  Weight-gradient partials are written per block and reduced in a fixed order.
@@ -27,7 +29,9 @@ constexpr int H = 128;          // hidden width
 constexpr int K = 16;           // spline bins
 constexpr int NPD = 3 * K + 1;  // raw spline parameters per transformed dimension
 constexpr int DMAX = 16;        // maximum flow dimension
-constexpr int TILE = 16;        // samples per wave
+constexpr int TILE = 16;        // samples per block
+constexpr int WPB = 4;          // waves per block
+constexpr int NT = 64 * WPB;    // threads per block
 constexpr float MINW = 1e-3f, MINH = 1e-3f, MIND = 1e-3f;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -215,24 +219,23 @@ __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
 }
 
 // ------------------------------------------------------------------ MLP forward
-// One wave, 16 samples.  xa: LDS [16][DMAX] conditioning inputs (only nA used);
-// h1s/h2s: LDS bf16 [16][H]; raws: LDS fp32 [16][NP].
+// One block, 16 samples; wave w computes the output tiles n0 = 16 (w + 4 i).
+// xa: LDS [16][DMAX] conditioning inputs (only nA used); h1s/h2s: LDS bf16 [16][H];
+// raws: LDS fp32 [16][NP].
 __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, const unsigned short* __restrict__ Pb,
                             const float* xa, unsigned short* h1s, unsigned short* h2s, float* raws) {
-  const int lane = threadIdx.x & 63;
-  // layer 1 on VALU: 16 x 128 outputs, 32 per lane
-  for (int q = lane; q < TILE * H; q += 64) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // layer 1 on VALU: 16 x 128 outputs, 8 per thread
+  for (int q = tid; q < TILE * H; q += NT) {
     const int s = q / H, j = q % H;
     float v = P[g.ob1 + j];
     for (int i = 0; i < g.nA; ++i) v += xa[s * DMAX + i] * P[g.oW1 + i * H + j];
     h1s[s * H + j] = f2bf(fmaxf(v, 0.f));
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   const int r = lane & 15, kq = (lane >> 4) * 4;
   // layer 2: h2 = relu(h1 W2 + b2), B operand from W2t [out][in] (bf16)
-  for (int n0 = 0; n0 < H; n0 += 16) {
+  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < H; k0 += 16) {
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1s + r * H + k0 + kq);
@@ -244,11 +247,9 @@ __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, con
       h2s[row * H + col] = f2bf(fmaxf(acc[i] + P[g.ob2 + col], 0.f));
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   // layer 3: raw = h2 W3 + b3, B operand from W3t [NP][H] (bf16)
-  for (int n0 = 0; n0 < g.NP; n0 += 16) {
+  for (int n0 = 16 * wid; n0 < g.NP; n0 += 16 * WPB) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < H; k0 += 16) {
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(h2s + r * H + k0 + kq);
@@ -260,9 +261,7 @@ __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, con
       raws[row * g.NP + col] = acc[i] + P[g.ob3 + col];
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
 }
 
 struct FwdArgs {
@@ -272,13 +271,14 @@ struct FwdArgs {
   const float* x_in;       // (n, D)
   float* x_out;            // (n, D)
   float* logdet;           // (n,) accumulated
+  float* save_x;           // optional (n, D): copy of the layer input
   unsigned short* save_h1; // optional (n, H) bf16
   unsigned short* save_h2; // optional (n, H) bf16
   float* save_raw;         // optional (n, NPmax)
   int inverse;
 };
 
-__global__ void __launch_bounds__(64) k_layer(FwdArgs A, const float* __restrict__ P,
+__global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict__ P,
                                               const unsigned short* __restrict__ Pb) {
   const LayerGeom& g = A.geo.layer[A.layer];
   const int D = A.geo.D;
@@ -287,29 +287,26 @@ __global__ void __launch_bounds__(64) k_layer(FwdArgs A, const float* __restrict
   __shared__ __attribute__((aligned(16))) unsigned short h1s[TILE * H];
   __shared__ __attribute__((aligned(16))) unsigned short h2s[TILE * H];
   extern __shared__ float raws[];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int64_t s0 = (int64_t)blockIdx.x * TILE;
-  for (int q = lane; q < TILE * DMAX; q += 64) {
+  for (int q = tid; q < TILE * DMAX; q += NT) {
     const int s = q / DMAX, d = q % DMAX;
     const int64_t e = s0 + s;
     const float v = (e < A.n && d < D) ? A.x_in[e * D + d] : 0.5f;
     xs[q] = v;
+    if (A.save_x && e < A.n && d < D) A.save_x[e * D + d] = v;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  for (int q = lane; q < TILE * DMAX; q += 64) {
+  __syncthreads();
+  for (int q = tid; q < TILE * DMAX; q += NT) {
     const int s = q / DMAX, i = q % DMAX;
     xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   mlp_forward(g, P, Pb, xa, h1s, h2s, raws);
   // spline on every (sample, transformed dimension); the log-dets of one sample are
-  // summed by one lane afterwards in a fixed order
+  // summed by one thread afterwards in a fixed order
   __shared__ float lds_ld[TILE * DMAX];
-  for (int q = lane; q < TILE * g.nB; q += 64) {
+  for (int q = tid; q < TILE * g.nB; q += NT) {
     const int s = q / g.nB, b = q % g.nB;
     const int d = g.B[b];
     const float* raw = raws + s * g.NP + b * NPD;
@@ -317,31 +314,29 @@ __global__ void __launch_bounds__(64) k_layer(FwdArgs A, const float* __restrict
     xs[s * DMAX + d] = r.z;
     lds_ld[s * DMAX + b] = r.ld;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  if (lane < TILE) {
-    const int64_t e = s0 + lane;
+  __syncthreads();
+  if (tid < TILE) {
+    const int64_t e = s0 + tid;
     if (e < A.n) {
       float ld = 0.f;
-      for (int b = 0; b < g.nB; ++b) ld += lds_ld[lane * DMAX + b];
+      for (int b = 0; b < g.nB; ++b) ld += lds_ld[tid * DMAX + b];
       A.logdet[e] += ld;
     }
   }
-  for (int q = lane; q < TILE * D; q += 64) {
+  for (int q = tid; q < TILE * D; q += NT) {
     const int s = q / D, d = q % D;
     const int64_t e = s0 + s;
     if (e < A.n) A.x_out[e * D + d] = xs[s * DMAX + d];
   }
   if (A.save_h1) {
-    for (int q = lane; q < TILE * H; q += 64) {
+    for (int q = tid; q < TILE * H; q += NT) {
       const int64_t e = s0 + q / H;
       if (e < A.n) {
         A.save_h1[e * H + q % H] = h1s[q];
         A.save_h2[e * H + q % H] = h2s[q];
       }
     }
-    for (int q = lane; q < TILE * g.NP; q += 64) {
+    for (int q = tid; q < TILE * g.NP; q += NT) {
       const int64_t e = s0 + q / g.NP;
       if (e < A.n) A.save_raw[e * A.geo.NPmax + q % g.NP] = raws[q];
     }
@@ -362,37 +357,33 @@ struct BwdArgs {
   float* partial;                  // (nblocks, layer nparams) weight-gradient partials
 };
 
-__global__ void __launch_bounds__(64) k_layer_bwd(BwdArgs A, const float* __restrict__ P,
+__global__ void __launch_bounds__(NT) k_layer_bwd(BwdArgs A, const float* __restrict__ P,
                                                   const unsigned short* __restrict__ Pb) {
   const LayerGeom& g = A.geo.layer[A.layer];
   const int D = A.geo.D;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int r = lane & 15, kq = lane >> 4;  // fp32 MFMA: k index = lane/16
   const int64_t s0 = (int64_t)blockIdx.x * TILE;
   __shared__ float xs[TILE * DMAX], gxs[TILE * DMAX];
   __shared__ float h1f[TILE * H], h2f[TILE * H], gh[TILE * H];
   extern __shared__ float go[];   // [TILE][NP]
-  for (int q = lane; q < TILE * DMAX; q += 64) {
+  for (int q = tid; q < TILE * DMAX; q += NT) {
     const int s = q / DMAX, d = q % DMAX;
     const int64_t e = s0 + s;
     const bool ok = e < A.n && d < D;
     xs[q] = ok ? A.x_in[e * D + d] : 0.5f;
     gxs[q] = ok ? A.gx[e * D + d] : 0.f;
   }
-  for (int q = lane; q < TILE * H; q += 64) {
+  for (int q = tid; q < TILE * H; q += NT) {
     const int64_t e = s0 + q / H;
     h1f[q] = e < A.n ? bf2f(A.h1[e * H + q % H]) : 0.f;
     h2f[q] = e < A.n ? bf2f(A.h2[e * H + q % H]) : 0.f;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   // spline backward: d raw (into go) and d x_B (replaces the incoming gradient of x_B)
-  for (int q = lane; q < TILE * g.NP; q += 64) go[q] = 0.f;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  for (int q = lane; q < TILE * g.nB; q += 64) {
+  for (int q = tid; q < TILE * g.NP; q += NT) go[q] = 0.f;
+  __syncthreads();
+  for (int q = tid; q < TILE * g.nB; q += NT) {
     const int s = q / g.nB, b = q % g.nB;
     const int64_t e = s0 + s;
     if (e >= A.n) continue;
@@ -403,26 +394,25 @@ __global__ void __launch_bounds__(64) k_layer_bwd(BwdArgs A, const float* __rest
     for (int j = 0; j < NPD; ++j) go[s * g.NP + b * NPD + j] = graw[j];
     gxs[s * DMAX + d] = gxb;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   const int nlp = g.ob3 + g.NP - g.oW1;   // parameters of this layer
   float* part = A.partial + (size_t)blockIdx.x * nlp;
   // db3 and dW3 = h2^T go  (M = H, N = NP, K = 16 samples)
-  for (int q = lane; q < g.NP; q += 64) {
+  for (int q = tid; q < g.NP; q += NT) {
     float v = 0.f;
     for (int s = 0; s < TILE; ++s) v += go[s * g.NP + q];
     part[g.ob3 - g.oW1 + q] = v;
   }
-  for (int m0 = 0; m0 < H; m0 += 16)
-    for (int n0 = 0; n0 < g.NP; n0 += 16) {
+  const int nt3 = g.NP / 16;
+  for (int t = wid; t < (H / 16) * nt3; t += WPB) {
+      const int m0 = 16 * (t / nt3), n0 = 16 * (t % nt3);
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       for (int k0 = 0; k0 < TILE; k0 += 4) acc = mfma_f32(h2f[(k0 + kq) * H + m0 + r], go[(k0 + kq) * g.NP + n0 + r], acc);
       for (int i = 0; i < 4; ++i) part[g.oW3 - g.oW1 + (m0 + 4 * kq + i) * g.NP + n0 + r] = acc[i];
     }
   // gh2 = (go W3^T) * relu'(h2)  (M = 16, N = H, K = NP), with the bf16 weights the
   // forward pass used: W3^T[np][j] is the transposed bf16 copy W3t
-  for (int n0 = 0; n0 < H; n0 += 16) {
+  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < g.NP; k0 += 4)
       acc = mfma_f32(go[r * g.NP + k0 + kq], bf2f(Pb[g.oW3t + (k0 + kq) * H + n0 + r]), acc);
@@ -431,24 +421,22 @@ __global__ void __launch_bounds__(64) k_layer_bwd(BwdArgs A, const float* __rest
       gh[row * H + col] = h2f[row * H + col] > 0.f ? acc[i] : 0.f;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   // db2, dW2 = h1^T gh2
-  for (int q = lane; q < H; q += 64) {
+  for (int q = tid; q < H; q += NT) {
     float v = 0.f;
     for (int s = 0; s < TILE; ++s) v += gh[s * H + q];
     part[g.ob2 - g.oW1 + q] = v;
   }
-  for (int m0 = 0; m0 < H; m0 += 16)
-    for (int n0 = 0; n0 < H; n0 += 16) {
+  for (int t = wid; t < (H / 16) * (H / 16); t += WPB) {
+      const int m0 = 16 * (t / (H / 16)), n0 = 16 * (t % (H / 16));
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       for (int k0 = 0; k0 < TILE; k0 += 4) acc = mfma_f32(h1f[(k0 + kq) * H + m0 + r], gh[(k0 + kq) * H + n0 + r], acc);
       for (int i = 0; i < 4; ++i) part[g.oW2 - g.oW1 + (m0 + 4 * kq + i) * H + n0 + r] = acc[i];
     }
   // gh1 = (gh2 W2^T) * relu'(h1); W2^T[o][i] is the bf16 copy W2t
   __shared__ float gh1[TILE * H];
-  for (int n0 = 0; n0 < H; n0 += 16) {
+  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < H; k0 += 4) acc = mfma_f32(gh[r * H + k0 + kq], bf2f(Pb[g.oW2t + (k0 + kq) * H + n0 + r]), acc);
     for (int i = 0; i < 4; ++i) {
@@ -456,32 +444,28 @@ __global__ void __launch_bounds__(64) k_layer_bwd(BwdArgs A, const float* __rest
       gh1[row * H + col] = h1f[row * H + col] > 0.f ? acc[i] : 0.f;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
   // db1, dW1 = x_A^T gh1, and d x_A += gh1 W1^T
-  for (int q = lane; q < H; q += 64) {
+  for (int q = tid; q < H; q += NT) {
     float v = 0.f;
     for (int s = 0; s < TILE; ++s) v += gh1[s * H + q];
     part[g.ob1 - g.oW1 + q] = v;
   }
-  for (int q = lane; q < DMAX * H; q += 64) {
+  for (int q = tid; q < DMAX * H; q += NT) {
     const int i = q / H, j = q % H;
     float v = 0.f;
     if (i < g.nA)
       for (int s = 0; s < TILE; ++s) v += xs[s * DMAX + g.A[i]] * gh1[s * H + j];
     part[q] = v;   // W1 block is the first DMAX*H entries of the layer
   }
-  for (int q = lane; q < TILE * g.nA; q += 64) {
+  for (int q = tid; q < TILE * g.nA; q += NT) {
     const int s = q / g.nA, i = q % g.nA;
     float v = 0.f;
     for (int j = 0; j < H; ++j) v += gh1[s * H + j] * P[g.oW1 + i * H + j];
     gxs[s * DMAX + g.A[i]] += v;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  for (int q = lane; q < TILE * D; q += 64) {
+  __syncthreads();
+  for (int q = tid; q < TILE * D; q += NT) {
     const int s = q / D, d = q % D;
     const int64_t e = s0 + s;
     if (e < A.n) A.gx[e * D + d] = gxs[s * DMAX + d];
@@ -639,20 +623,19 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
     A.x_out = x_out;
     A.logdet = logdet;
     A.inverse = inverse;
+    A.save_x = nullptr;
     A.save_h1 = A.save_h2 = nullptr;
     A.save_raw = nullptr;
     if (save && !inverse) {
       char* base = (char*)save + per_layer * l;
       float* xsave = (float*)base;
-      // keep the layer input for the backward pass
-      hipError_t e = hipMemcpyAsync(xsave, cur, (size_t)n * D * 4, hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) return hip_status(e);
+      A.save_x = xsave;   // the kernel keeps the layer input for the backward pass
       A.save_raw = xsave + (size_t)n * D;
       A.save_h1 = (unsigned short*)(A.save_raw + (size_t)n * G.NPmax);
       A.save_h2 = A.save_h1 + (size_t)n * H;
     }
     const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-    hipLaunchKernelGGL(k_layer, dim3(nblk), dim3(64), shmem, s, A, params, (const unsigned short*)params_bf16);
+    hipLaunchKernelGGL(k_layer, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
     cur = x_out;
   }
   return hip_status(hipGetLastError());
@@ -699,7 +682,7 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
     A.gx = gx;
     A.partial = partial;
     const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-    hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(64), shmem, s, A, params, (const unsigned short*)params_bf16);
+    hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
     const int nlp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
     hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nblk, nlp,
                        grad + G.layer[l].oW1);

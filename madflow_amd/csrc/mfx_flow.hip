@@ -10,8 +10,8 @@
 //   forward  x -> u  with log|det du/dx|   (density: log q(x) = sum of log-dets)
 //   inverse  u -> x  (sampling; analytic root of the RQ quadratic)
 //   backward gradients of a per-sample weighted objective through every layer
-// Kernels work on one block of 4 waves per 16 samples (the waves split the MFMA output
-// tiles, so that a small training batch still puts a wave on every SIMD): the 128x128
+// Kernels work on one block of 8 waves per 16 samples (the waves split the MFMA output
+// tiles, so that a small training batch still puts two waves on every SIMD): the 128x128
 // and 128xNP GEMMs of the forward
 // pass run on bf16 MFMA (v_mfma_f32_16x16x16bf16, fp32 accumulation, bf16 weights),
 // the backward GEMMs on fp32 MFMA (v_mfma_f32_16x16x4f32); the spline math is per-lane
@@ -30,7 +30,7 @@ constexpr int K = 16;           // spline bins
 constexpr int NPD = 3 * K + 1;  // raw spline parameters per transformed dimension
 constexpr int DMAX = 16;        // maximum flow dimension
 constexpr int TILE = 16;        // samples per block
-constexpr int WPB = 4;          // waves per block
+constexpr int WPB = 8;          // waves per block
 constexpr int NT = 64 * WPB;    // threads per block
 constexpr float MINW = 1e-3f, MINH = 1e-3f, MIND = 1e-3f;
 
@@ -483,11 +483,24 @@ __global__ void __launch_bounds__(256) k_reduce_grad(const float* __restrict__ p
   const int i = blockIdx.x * 64 + pi;
   float s = 0.f, c = 0.f;
   if (i < nlp) {
-    for (int b = sl; b < nblocks; b += 4) {
-      const float y = partial[(size_t)b * nlp + i] - c;
-      const float t = s + y;
-      c = (t - s) - y;
-      s = t;
+    // the loads of 8 blocks are issued before their compensated adds, which keep the
+    // order b = sl, sl + 4, ...
+    for (int b0 = sl; b0 < nblocks; b0 += 32) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int b = b0 + 4 * u;
+        v[u] = b < nblocks ? partial[(size_t)b * nlp + i] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (b0 + 4 * u < nblocks) {
+          const float y = v[u] - c;
+          const float t = s + y;
+          c = (t - s) - y;
+          s = t;
+        }
+      }
     }
   }
   ls[sl][pi] = s;

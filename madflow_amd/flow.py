@@ -147,7 +147,7 @@ class RQSFlow:
         return y, -ld_inv
 
     # ----------------------------------------------------------------- training
-    def grad(self, y, w, f=None, beta=0.0):
+    def grad(self, y, w, f=None, beta=0.0, need_loss=True):
         """Gradient of L = -sum_i v_i log q(y_i) / sum_i v_i (returns L, grad), with
         v_i = w_i for beta = 0 and v_i = w_i (f_i / q(y_i))^beta otherwise.
 
@@ -173,10 +173,10 @@ class RQSFlow:
         gx = torch.zeros_like(y)
         partial = torch.empty(self.lib.mfx_flow_partial_bytes(self.D, self.L, self._m(), n) // 4 + 1,
                               dtype=torch.float32, device=y.device)
-        grad = torch.zeros_like(self.params)
+        grad = torch.empty_like(self.params)     # every entry is written by the backward pass
         _lib.check(self.lib.mfx_flow_backward(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), n, _p(save), _p(gl),
                                               _p(gx), _p(partial), _p(grad), _lib.stream_ptr(y.device)))
-        loss = -(w.to(torch.float64) * ld.to(torch.float64)).sum() / wsum.to(torch.float64)
+        loss = -(w.to(torch.float64) * ld.to(torch.float64)).sum() / wsum.to(torch.float64) if need_loss else None
         return loss, grad
 
     def adam_step(self, grad, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=10.0):
@@ -256,9 +256,10 @@ class RQSFlow:
                 if float(wb.sum()) <= 0:
                     continue
             cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
-            loss, grad = self.grad(yb, wb, fb, beta)
+            show = log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1)
+            loss, grad = self.grad(yb, wb, fb, beta, need_loss=show or it % 50 == 0)
             self.adam_step(grad, lr=cur_lr)
-            if log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1):
+            if show:
                 log("flow step %d loss %.5f" % (it, float(loss)))
             if it % 50 == 0:
                 losses.append(loss)

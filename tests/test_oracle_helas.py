@@ -106,3 +106,19 @@ def test_sqh_reference_rounding():
     b = O.wavefunction("v", p, 0.0, 1, 1, ref_compat=True)[0]
     ratio = np.abs(b[3:6]) / np.abs(a[3:6])
     np.testing.assert_allclose(ratio, 0.70710676908493042 / np.sqrt(0.5), rtol=1e-15)
+
+
+@pytest.mark.parametrize("nss", [1, -1])
+def test_scalar_wavefunction_layout(nss):
+    """sxxxxx: the momentum slots follow the same HELAS convention as the vector and fermion
+    wavefunctions of the same particle direction (slots [0], [1] = sign * (E + i pz, px + i py)),
+    and the scalar component is 1."""
+    p = momenta(125.0, "s")
+    s = O.wavefunction("s", p, 125.0, 0, nss)
+    v = O.wavefunction("v", p, 125.0, 1, nss)
+    assert s.shape == (len(p), 3)
+    np.testing.assert_array_equal(s[:, :2], v[:, :2])
+    np.testing.assert_array_equal(s[:, 2], np.ones(len(p)))
+    # the slots give back the momentum: P = (Re s0, Re s1, Im s1, Im s0) * sign
+    P = nss * np.stack([s[:, 0].real, s[:, 1].real, s[:, 1].imag, s[:, 0].imag], axis=1)
+    np.testing.assert_array_equal(P, p)

@@ -210,16 +210,28 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
       }
   }
   cx<R>* jb = reinterpret_cast<cx<R>*>(a.jbuf);
+  // Helicity rows are enumerated here as r = 4 g + 2 ht + ha: g = the gluon helicities
+  // (bit s = gluon slot s), ht / ha = the top / anti-top helicity.  The gluon currents of
+  // an ordering depend on g alone, so they are computed once for the four rows of g; the
+  // top line is linear in the top spinor and is computed once per ht; the closing
+  // vertex is linear in the anti-top spinor.  Rows r0 .. r1-1 are this launch's, each
+  // stored at slot r - r0.  With an explicit helicity vector there is one row (r = 0),
+  // and every helicity comes from the vector.
+  static_assert(T::NCOMB == (4 << NG), "every helicity combination is a row");
+  const int r0 = a.h0, r1 = a.h0 + a.nhel;
+  constexpr int PRE = NG - 2;
+  W<R> J[NG][NG], O[NG];
+  R pinv[NG][NG], tinv[NG];
 #pragma unroll 1
-  for (int hh = 0; hh < a.nhel; ++hh) {
-    const unsigned hb = a.s.use_helv ? 0u : T::hel_bits(a.h0 + hh);
+  for (int g = r0 >> 2; g <= (r1 - 1) >> 2; ++g) {
+    unsigned hbg = 0u;
+#pragma unroll
+    for (int s = 0; s < NG; ++s) hbg |= ((unsigned)(g >> s) & 1u) << T::gluon_leg(s);
     const int* helv = a.s.use_helv ? a.s.helv : nullptr;
-    W<R> top, atop;
-    T::template fermions<R>(p, hb, helv, prm, top, atop);
 #pragma unroll
     for (int s = 0; s < NG; ++s) {
       W<R> gw;
-      T::template gluon<R>(p, s, hb, helv, prm, gw);
+      T::template gluon<R>(p, s, hbg, helv, prm, gw);
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         lds[(s * 12 + 2 * q) * BLOCK + t] = gw[q].re;
@@ -227,12 +239,8 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
       }
     }
     // Orderings come in lexicographic order, so orderings 2m and 2m+1 share their first
-    // NG-2 gluons: the currents inside that prefix (J(i..j), j < NG-2, and O(1..NG-2))
-    // are computed once per pair and carried to the second ordering.
-    constexpr int PRE = NG - 2;
-    W<R> J[NG][NG], O[NG];
-    R pinv[NG][NG], tinv[NG];
-    O[0] = top;
+    // NG-2 gluons: the currents inside that prefix (J(i..j), j < NG-2) are computed once
+    // per pair and carried to the second ordering.
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
       int perm[NG];
@@ -250,10 +258,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
       }
       if constexpr (S::NINV > 0) {
 #pragma unroll
-        for (int x = 0; x < NG; ++x) {
-          if (x < PRE && !fresh) continue;
-          tinv[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
-        }
+        for (int x = 0; x < NG; ++x) tinv[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
 #pragma unroll
         for (int x = 0; x < NG; ++x)
 #pragma unroll
@@ -262,14 +267,26 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
             pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
           }
       }
-      if (fresh) {
-        ordered_ranges<NG, 1, PRE, R>(J, pinv, prm);
-        ordered_top_line<NG, 1, PRE, R>(O, J, pinv, tinv, prm);
-      }
+      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, pinv, prm);
       ordered_ranges<NG, PRE, NG, R>(J, pinv, prm);
-      ordered_top_line<NG, PRE + 1, NG - 1, R>(O, J, pinv, tinv, prm);
-      const cx<R> amp = ordered_close<NG, R>(atop, O, J, prm);
-      jb[((int64_t)hh * NC + c) * a.n + el] = amp;
+#pragma unroll 1
+      for (int ht = 0; ht < 2; ++ht) {
+        const int rb = 4 * g + 2 * ht;
+        if (rb + 1 < r0 || rb >= r1) continue;
+        W<R> top, atop;
+        T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
+        O[0] = top;
+        ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
+#pragma unroll 1
+        for (int ha = 0; ha < 2; ++ha) {
+          const int r = rb + ha;
+          if (r < r0 || r >= r1) continue;
+          if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv, prm,
+                                          top, atop);
+          const cx<R> amp = ordered_close<NG, R>(atop, O, J, prm);
+          jb[((int64_t)(r - r0) * NC + c) * a.n + el] = amp;
+        }
+      }
     }
   }
 }

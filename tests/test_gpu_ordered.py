@@ -90,11 +90,13 @@ def test_ordered_matches_fused_kernel(dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [32, 64])
-def test_ordered_small_workspace_splits(dev, precision):
+@pytest.mark.parametrize("precision,rows", [(32, 40), (64, 40), (32, 200), (32, 400)])
+def test_ordered_small_workspace_splits(dev, precision, rows):
     """A workspace of a few tiles forces event sub-chunks and helicity groups (with the
-    colour kernel accumulating across groups): the result is unchanged."""
-    print("case ordered workspace split", precision, flush=True)
+    colour kernel accumulating across groups): the result is unchanged.  40, 200 and 400
+    event-rows for 100 events give groups of 1, 2 and 4 helicity rows, i.e. part of, one
+    half of, and all four top / anti-top rows that share one gluon helicity set."""
+    print("case ordered workspace split", precision, rows, flush=True)
     m, _ = load_matrix("gg_ttxggg", precision=precision, ref_compat=True)
     ps, _ = phase_space_points("gg_ttxggg", 100, seed=25)
     tps = torch.from_numpy(ps).to(dev)
@@ -102,6 +104,6 @@ def test_ordered_small_workspace_splits(dev, precision):
     full = m.smatrix(tps, *vals).cpu().numpy()
     m2, _ = load_matrix("gg_ttxggg", precision=precision, ref_compat=True)
     row = 120 * 2 * (4 if precision == 32 else 8)
-    m2.workspace_limit = 40 * row                # 40 event-rows: 3 sub-chunks x 128 helicity groups
+    m2.workspace_limit = rows * row
     small = m2.smatrix(tps, *vals).cpu().numpy()
     np.testing.assert_allclose(small, full, rtol=1e-6 if precision == 32 else 1e-13, atol=0)

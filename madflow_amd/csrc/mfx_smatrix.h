@@ -126,16 +126,41 @@ __device__ __forceinline__ void wave_lds_sync() {
 // stage: 2*NCOLOR*64 floats of this wave; cfl: the integer colour matrix [NCOLOR][NCP];
 // red: 64 floats of this wave.
 template <class T>
+__device__ __forceinline__ float colour_sum_mfma_staged(const float* stage, const float* cfl, float* red);
+
+template <class T>
 __device__ __forceinline__ float colour_sum_mfma(const cx<float> (&J)[T::NCOLOR], float* stage, const float* cfl,
                                                  float* red) {
-  constexpr int NC = T::NCOLOR, NCP = (NC + 15) / 16 * 16, KS = NC / 4;
-  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+  constexpr int NC = T::NCOLOR;
+  const int lane = threadIdx.x & 63;
   wave_lds_sync();                       // the previous row's reads are done
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     stage[(2 * c) * 64 + lane] = J[c].re;
     stage[(2 * c + 1) * 64 + lane] = J[c].im;
   }
+  return colour_sum_mfma_staged<T>(stage, cfl, red);
+}
+
+// Jamps accumulated in place in a wave's LDS stage (colour-major, one column per lane):
+// jamp[c] += amp reads and writes this lane's two words of colour c.
+struct LdsCx {
+  float* re;
+  float* im;
+  __device__ __forceinline__ void operator=(const cx<float>& a) { *re = a.re; *im = a.im; }
+  __device__ __forceinline__ void operator+=(const cx<float>& a) { *re += a.re; *im += a.im; }
+  __device__ __forceinline__ void operator-=(const cx<float>& a) { *re -= a.re; *im -= a.im; }
+};
+struct LdsJamps {
+  float* base;   // stage + lane
+  __device__ __forceinline__ LdsCx operator[](int c) const { return LdsCx{base + 2 * c * 64, base + (2 * c + 1) * 64}; }
+};
+
+// The contraction of jamps already staged in LDS (see colour_sum_mfma).
+template <class T>
+__device__ __forceinline__ float colour_sum_mfma_staged(const float* stage, const float* cfl, float* red) {
+  constexpr int NC = T::NCOLOR, NCP = (NC + 15) / 16 * 16, KS = NC / 4;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   wave_lds_sync();
   float part[4][4];
 #pragma unroll
@@ -187,6 +212,7 @@ template <class T, bool MFMA> struct ColourLds { float stage[1], cf[1], red[1]; 
 template <class T> struct ColourLds<T, true> {
   static constexpr int NCP = (T::NCOLOR + 15) / 16 * 16;
   float stage[4][2 * T::NCOLOR * 64];
+  float stage2[4][2 * T::NCOLOR * 64];   // the second row of a pair
   float cf[T::NCOLOR * NCP];
   float red[4][64];
 };
@@ -218,20 +244,52 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
     if constexpr (StableForms<R>::value) load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     R ans = R(0);
+    if constexpr (T::DUAL && StableForms<R>::value) {
+      // every helicity combination is a row: walk them in pairs that differ only in the
+      // helicity of leg DUAL_LEG, whose shared currents amplitudes2 computes once
+      static_assert(T::NCOMB == (1 << T::NEXTERNAL), "pairs of rows need every combination");
+      constexpr int X = T::DUAL_LEG;
 #pragma unroll 1
-    for (int h = 0; h < T::NCOMB; ++h) {
+      for (int q = 0; q < T::NCOMB / 2; ++q) {
 #pragma unroll
-      for (int l = 0; l < T::NEXTERNAL; ++l)
+        for (int l = 0; l < T::NEXTERNAL; ++l)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) opaque(p[l][k]);
-      if constexpr (StableForms<R>::value) {
+          for (int k = 0; k < 4; ++k) opaque(p[l][k]);
 #pragma unroll
         for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
+        const unsigned hb = ((unsigned)q & ((1u << X) - 1u)) | (((unsigned)q >> X) << (X + 1));
+        if constexpr (MFMA) {
+          // the two rows' jamps accumulate in place in the wave's LDS stages, which frees
+          // their registers for the currents the two rows share
+          const int lane = threadIdx.x & 63;
+          LdsJamps ja{lds.stage[wave] + lane}, jb{lds.stage2[wave] + lane};
+          wave_lds_sync();                 // the previous pair's contractions are done
+          T::template amplitudes2<R>(p, inv, hb, prm, ja, jb);
+          ans += colour_sum_mfma_staged<T>(lds.stage[wave], lds.cf, lds.red[wave]);
+          ans += colour_sum_mfma_staged<T>(lds.stage2[wave], lds.cf, lds.red[wave]);
+        } else {
+          cx<R> jamp[T::NCOLOR], jamp2[T::NCOLOR];
+          T::template amplitudes2<R>(p, inv, hb, prm, jamp, jamp2);
+          ans += colour_sum<T, R>(jamp);
+          ans += colour_sum<T, R>(jamp2);
+        }
       }
-      cx<R> jamp[T::NCOLOR];
-      T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
-      if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[wave], lds.cf, lds.red[wave]);
-      else ans += colour_sum<T, R>(jamp);
+    } else {
+#pragma unroll 1
+      for (int h = 0; h < T::NCOMB; ++h) {
+#pragma unroll
+        for (int l = 0; l < T::NEXTERNAL; ++l)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) opaque(p[l][k]);
+        if constexpr (StableForms<R>::value) {
+#pragma unroll
+          for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
+        }
+        cx<R> jamp[T::NCOLOR];
+        T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
+        if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[wave], lds.cf, lds.red[wave]);
+        else ans += colour_sum<T, R>(jamp);
+      }
     }
     if (active) a.out[e] = double(ans) / T::DENOMINATOR;
   }

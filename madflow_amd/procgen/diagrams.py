@@ -247,7 +247,7 @@ def build_program(gluons, top_leg, atop_leg, externals, nexternal):
     return prog, len(diags)
 
 
-def schedule_ops(ops, nexternal):
+def schedule_ops(ops, nexternal, inputs=None):
     """Reorder a HELAS program to keep few wavefunctions live at once.
 
     Greedy list scheduling over amplitudes: next evaluate the amplitude whose missing
@@ -256,6 +256,7 @@ def schedule_ops(ops, nexternal):
     still computed once and before its first use, so any evaluator sees the same
     values; only the order (and hence the register footprint of the fused kernel)
     changes.  For g g > t t~ g g the peak live set drops from 43 to 18 wavefunctions.
+    ``inputs``: the ids available before the program (default: the external legs).
     """
     wf_def = {op["out"]: op for op in ops if op["kind"] == "wf"}
     amps = [op for op in ops if op["kind"] == "amp"]
@@ -263,7 +264,7 @@ def schedule_ops(ops, nexternal):
     for op in ops:
         for a in op["args"]:
             cons[a] = cons.get(a, 0) + 1
-    computed = set(range(nexternal))
+    computed = set(range(nexternal)) if inputs is None else set(inputs)
     remaining = dict(cons)
 
     def deps(w, acc):

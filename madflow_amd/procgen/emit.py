@@ -61,7 +61,7 @@ def den_expr(op, desc):
     return "(" + " + ".join(terms) + ")"
 
 
-def device_program(desc):
+def merged_ops(desc):
     """The amplitude program of the fused kernel.  Amplitudes that close on the same
     fermion pair with proportional jamp-coefficient vectors are evaluated once, on the
     sum of their vector currents (FFV1_0 is linear in V, and momentum conservation gives
@@ -95,7 +95,52 @@ def device_program(desc):
         ops.append(dict(kind="amp", routine=routine, out=members[0][0]["out"],
                         args=[f1, f2] + [m[0]["args"][2] for m in members],
                         factors=[m[1] for m in members], coup=coup, norm=list(norm)))
-    return schedule_ops(ops, desc["nexternal"])
+    return ops
+
+
+def device_program(desc):
+    """The merged amplitude program of one helicity row, scheduled for few live currents."""
+    return schedule_ops(merged_ops(desc), desc["nexternal"])
+
+
+DUAL_OFF = 100000
+
+
+def dual_program(desc):
+    """One program for two helicity rows that differ only in the helicity of external leg
+    X: every current that does not depend on leg X (the gluon currents and the currents of
+    the other fermion line, for X a fermion) is computed once for both rows.  The currents
+    that depend on X, and every amplitude, are evaluated twice; the copies carry ids
+    offset by DUAL_OFF and the amplitude copies (``jset`` 1) accumulate into the second
+    row's jamps.  X is the leg with the fewest dependent currents.  The single-row schedule
+    is kept, each twin right after its original.  Returns (X, ops)."""
+    ops = device_program(desc)
+    n = desc["nexternal"]
+    best = None
+    for X in range(n):
+        dep = {X}
+        for op in ops:
+            if op["kind"] == "wf" and any(a in dep for a in op["args"]):
+                dep.add(op["out"])
+        if best is None or len(dep) < best[0]:
+            best = (len(dep), X)
+    X = best[1]
+    dep = {X}
+    out = []
+    for op in ops:
+        if op["kind"] == "wf" and not any(a in dep for a in op["args"]):
+            out.append(op)
+            continue
+        if op["kind"] == "wf":
+            dep.add(op["out"])
+        twin = dict(op)
+        twin["args"] = [a + DUAL_OFF if a in dep else a for a in op["args"]]
+        if op["kind"] == "wf":
+            twin["out"] = op["out"] + DUAL_OFF
+        else:
+            twin["jset"] = 1
+        out += [op, twin]
+    return X, out
 
 
 def _mom_expr(legs, overall, nin):
@@ -114,6 +159,70 @@ def _scaled(expr, re, im):
     if im == 0:
         return expr if re > 0 else "-" + expr
     return ("mul_i(%s)" if im > 0 else "mul_mi(%s)") % expr
+
+
+def _emit_ops(w, prog, desc, legs_of):
+    """Emit the statements of an amplitude program (wavefunction ops and amplitudes)."""
+    n = desc["nexternal"]
+    for op in prog:
+        args = ", ".join("w%d" % a for a in op["args"])
+        coup = "prm.%s" % op["coup"]
+        if op["kind"] == "wf":
+            mass = "ZERO" if op["mass"] == "ZERO" else "prm.%s" % op["mass"]
+            width = "ZERO" if op["width"] == "ZERO" else "prm.%s" % op["width"]
+            # FP32: P^2 - M^2 from the FP64 invariants; FP64: from the current's slot momentum,
+            # the reference's own arithmetic (Lorentz square of the summed momenta)
+            r = op["routine"]
+            m = _re.match(r"VVVV(\d)P0_1$", r)
+            slot_r = "VVVVP0_1s<%s>" % m.group(1) if m else r
+            dexpr = den_expr(op, desc)
+            nin = desc["ninitial"]
+            w("  W<R> w%d;" % op["out"])
+            # FP32: momenta from the external momenta of each current's legs, so that no
+            # wavefunction has to keep its momentum slots in registers
+            if r in ("FFV1_1", "FFV1_2"):
+                w("  if constexpr (StableForms<R>::value) {")
+                w("    R P[4];")
+                w("    #pragma unroll")
+                w("    for (int c = 0; c < 4; ++c) P[c] = %s;" % _mom_expr(op["legs"], -1, nin))
+                w("    %s_p(%s, %s, %s, %s, %s, P, w%d);" % (r, args, coup, mass, width, dexpr, op["out"]))
+                w("  } else %s(%s, %s, %s, %s, w%d);" % (r, args, coup, mass, width, op["out"]))
+            elif r == "VVV1P0_1":
+                a2, a3 = op["args"]
+                w("  if constexpr (StableForms<R>::value) {")
+                w("    R P2[4], P3[4];")
+                w("    #pragma unroll")
+                w("    for (int c = 0; c < 4; ++c) {")
+                w("      P2[c] = %s;" % _mom_expr(legs_of[a2], 1, nin))
+                w("      P3[c] = %s;" % _mom_expr(legs_of[a3], 1, nin))
+                w("    }")
+                w("    VVV1P0_1_p(%s, %s, %s, %s, %s, P2, P3, w%d);" % (args, coup, mass, width, dexpr, op["out"]))
+                w("  } else VVV1P0_1(%s, %s, %s, %s, w%d);" % (args, coup, mass, width, op["out"]))
+            else:
+                w("  if constexpr (StableForms<R>::value) %s(%s, %s, %s, %s, %s, w%d);" % (
+                    r, args, coup, mass, width, dexpr, op["out"]))
+                w("  else %s(%s, %s, %s, %s, w%d);" % (slot_r, args, coup, mass, width, op["out"]))
+        else:
+            f1, f2, vs = op["args"][0], op["args"][1], op["args"][2:]
+            jn = "jamp2" if op.get("jset") else "jamp"
+            if len(vs) == 1:
+                (r0, i0), = op["factors"]
+                w("  { const cx<R> amp0 = %s(w%d, w%d, w%d, %s);" % (op["routine"], f1, f2, vs[0], coup))
+                w("    const cx<R> amp = %s;" % _scaled("amp0", r0, i0))
+            else:
+                w("  { W<R> vs;")
+                w("    #pragma unroll")
+                w("    for (int q = 2; q < 6; ++q) vs[q] = %s;" % " + ".join(
+                    _scaled("w%d[q]" % v, r, i) for v, (r, i) in zip(vs, op["factors"])))
+                w("    const cx<R> amp = %s(w%d, w%d, vs, %s);" % (op["routine"], f1, f2, coup))
+            for (c, re, im) in op["norm"]:
+                if im == 0:
+                    w("    %s[%d] %s= amp;" % (jn, c, "+" if re > 0 else "-"))
+                else:
+                    w("    %s[%d] %s= mul_i(amp);" % (jn, c, "+" if im > 0 else "-"))
+            w("  }")
+            # keep the machine scheduler from hoisting later currents above this point
+            w("  __builtin_amdgcn_sched_barrier(0);")
 
 
 def emit_hip(desc):
@@ -184,66 +293,38 @@ def emit_hip(desc):
     for op in desc["ops"]:
         if op["kind"] == "wf":
             legs_of[op["out"]] = op["legs"]
-    for op in device_program(desc):
-        args = ", ".join("w%d" % a for a in op["args"])
-        coup = "prm.%s" % op["coup"]
-        if op["kind"] == "wf":
-            mass = "ZERO" if op["mass"] == "ZERO" else "prm.%s" % op["mass"]
-            width = "ZERO" if op["width"] == "ZERO" else "prm.%s" % op["width"]
-            # FP32: P^2 - M^2 from the FP64 invariants; FP64: from the current's slot momentum,
-            # the reference's own arithmetic (Lorentz square of the summed momenta)
-            r = op["routine"]
-            m = _re.match(r"VVVV(\d)P0_1$", r)
-            slot_r = "VVVVP0_1s<%s>" % m.group(1) if m else r
-            dexpr = den_expr(op, desc)
-            nin = desc["ninitial"]
-            w("  W<R> w%d;" % op["out"])
-            # FP32: momenta from the external momenta of each current's legs, so that no
-            # wavefunction has to keep its momentum slots in registers
-            if r in ("FFV1_1", "FFV1_2"):
-                w("  if constexpr (StableForms<R>::value) {")
-                w("    R P[4];")
-                w("    #pragma unroll")
-                w("    for (int c = 0; c < 4; ++c) P[c] = %s;" % _mom_expr(op["legs"], -1, nin))
-                w("    %s_p(%s, %s, %s, %s, %s, P, w%d);" % (r, args, coup, mass, width, dexpr, op["out"]))
-                w("  } else %s(%s, %s, %s, %s, w%d);" % (r, args, coup, mass, width, op["out"]))
-            elif r == "VVV1P0_1":
-                a2, a3 = op["args"]
-                w("  if constexpr (StableForms<R>::value) {")
-                w("    R P2[4], P3[4];")
-                w("    #pragma unroll")
-                w("    for (int c = 0; c < 4; ++c) {")
-                w("      P2[c] = %s;" % _mom_expr(legs_of[a2], 1, nin))
-                w("      P3[c] = %s;" % _mom_expr(legs_of[a3], 1, nin))
-                w("    }")
-                w("    VVV1P0_1_p(%s, %s, %s, %s, %s, P2, P3, w%d);" % (args, coup, mass, width, dexpr, op["out"]))
-                w("  } else VVV1P0_1(%s, %s, %s, %s, w%d);" % (args, coup, mass, width, op["out"]))
-            else:
-                w("  if constexpr (StableForms<R>::value) %s(%s, %s, %s, %s, %s, w%d);" % (
-                    r, args, coup, mass, width, dexpr, op["out"]))
-                w("  else %s(%s, %s, %s, %s, w%d);" % (slot_r, args, coup, mass, width, op["out"]))
-        else:
-            f1, f2, vs = op["args"][0], op["args"][1], op["args"][2:]
-            if len(vs) == 1:
-                (r0, i0), = op["factors"]
-                w("  { const cx<R> amp0 = %s(w%d, w%d, w%d, %s);" % (op["routine"], f1, f2, vs[0], coup))
-                w("    const cx<R> amp = %s;" % _scaled("amp0", r0, i0))
-            else:
-                w("  { W<R> vs;")
-                w("    #pragma unroll")
-                w("    for (int q = 2; q < 6; ++q) vs[q] = %s;" % " + ".join(
-                    _scaled("w%d[q]" % v, r, i) for v, (r, i) in zip(vs, op["factors"])))
-                w("    const cx<R> amp = %s(w%d, w%d, vs, %s);" % (op["routine"], f1, f2, coup))
-            for (c, re, im) in op["norm"]:
-                if im == 0:
-                    w("    jamp[%d] %s= amp;" % (c, "+" if re > 0 else "-"))
-                else:
-                    w("    jamp[%d] %s= mul_i(amp);" % (c, "+" if im > 0 else "-"))
-            w("  }")
-            # keep the machine scheduler from hoisting later currents above this point
-            w("  __builtin_amdgcn_sched_barrier(0);")
+    _emit_ops(w, device_program(desc), desc, legs_of)
     w("}")
     w("")
+    dual = desc["ncomb"] == 1 << n
+    if dual:
+        X, prog2 = dual_program(desc)
+        for k in list(legs_of):
+            legs_of[k + DUAL_OFF] = legs_of[k]
+        w("// Two helicity rows at once: hb with leg %d at helicity -1 (into jamp) and +1 (into" % X)
+        w("// jamp2).  Currents that do not depend on leg %d are computed once for both rows." % X)
+        w("// JA: an array cx<R>[NCOLOR] or an accessor whose elements support =, += and -=.")
+        w("template <class R, class JA>")
+        w("__device__ __forceinline__ void amplitudes2(const R (&p)[NEXTERNAL][4], const R (&inv)[NINV], unsigned hb,")
+        w("                                            const Params<R>& prm, JA& jamp, JA& jamp2) {")
+        w("  const R ZERO = R(0);")
+        for l, e in enumerate(desc["externals"]):
+            mass = "ZERO" if e["mass"] == "ZERO" else "prm.mdl_MT"
+            for ident, hel in ([(l, "R(-1)"), (l + DUAL_OFF, "R(1)")] if l == X else
+                               [(l, "(((hb >> %d) & 1u) ? R(1) : R(-1))" % l)]):
+                if e["routine"] == "vxxxxx":
+                    w("  W<R> w%d; vxxxxx(p[%d], %s, %s, R(%d), prm.sqh, w%d);" % (ident, l, mass, hel, e["sign"], ident))
+                else:
+                    w("  W<R> w%d; %s(p[%d], %s, %s, R(%d), w%d);" % (ident, e["routine"], l, mass, hel, e["sign"],
+                                                                     ident))
+        w("  #pragma unroll")
+        w("  for (int c = 0; c < NCOLOR; ++c) {")
+        w("    jamp[c] = cx<R>(R(0), R(0));")
+        w("    jamp2[c] = cx<R>(R(0), R(0));")
+        w("  }")
+        _emit_ops(w, prog2, desc, legs_of)
+        w("}")
+        w("")
     w("struct Traits {")
     w("  static constexpr int NEXTERNAL = %d, NCOMB = %d, NCOLOR = %d, NDIAGS = %d, NGRAPHS = %d, NINV = %d;" % (
         n, desc["ncomb"], ncol, desc["ndiags"], desc["ngraphs"], n * (n - 1) // 2))
@@ -257,6 +338,14 @@ def emit_hip(desc):
     w("      cx<R> (&jamp)[NCOLOR]) {")
     w("    %s::amplitudes(p, inv, hb, helv, prm, jamp);" % name)
     w("  }")
+    w("  static constexpr bool DUAL = %s;" % ("true" if dual else "false"))
+    if dual:
+        w("  static constexpr int DUAL_LEG = %d;" % X)
+        w("  template <class R, class JA> static __device__ __forceinline__ void amplitudes2(")
+        w("      const R (&p)[NEXTERNAL][4], const R (&inv)[NINV], unsigned hb, const Params<R>& prm,")
+        w("      JA& jamp, JA& jamp2) {")
+        w("    %s::amplitudes2(p, inv, hb, prm, jamp, jamp2);" % name)
+        w("  }")
     w("  static __device__ __forceinline__ unsigned hel_bits(int h) { return HEL_BITS[h]; }")
     w("  static __device__ __forceinline__ float cf(float, int i, int j) { return CF_NUM_F[i][j]; }")
     w("  static __device__ __forceinline__ double cf(double, int i, int j) { return CF_NUM_D[i][j]; }")

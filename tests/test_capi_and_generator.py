@@ -115,3 +115,35 @@ def test_jamp_coefficients_units():
     for terms in d["jamp"]:
         for _, re, im in terms:
             assert (abs(re), abs(im)) in ((1, 0), (0, 1))
+
+
+@pytest.mark.parametrize("ng", [0, 1, 2])
+def test_dual_row_program(ng):
+    """The two-row program of the fused kernels: every single-row operation appears once,
+    exactly the currents that depend on the paired leg X get a twin, every amplitude is
+    evaluated for both rows, and each operation only reads values defined before it."""
+    from madflow_amd.procgen.emit import device_program, dual_program, DUAL_OFF
+    d = gg_ttx_ng(ng)
+    n = d["nexternal"]
+    single = device_program(d)
+    X, dual = dual_program(d)
+    dep = {X}
+    for op in single:
+        if op["kind"] == "wf" and any(a in dep for a in op["args"]):
+            dep.add(op["out"])
+    orig = [op for op in dual if not op.get("jset") and (op["kind"] == "amp" or op["out"] < DUAL_OFF)]
+    key = lambda op: (op["kind"], op["out"], tuple(op["args"]))
+    assert [key(op) for op in orig] == [key(op) for op in single]
+    twins = [op for op in dual if op["kind"] == "wf" and op["out"] >= DUAL_OFF]
+    assert sorted(op["out"] - DUAL_OFF for op in twins) == sorted(dep - {X})
+    amps = [op for op in dual if op["kind"] == "amp"]
+    assert len(amps) == 2 * sum(op["kind"] == "amp" for op in single)
+    assert sum(1 for op in amps if op.get("jset")) == len(amps) // 2
+    defined = set(range(n)) | {X + DUAL_OFF}
+    for op in dual:
+        assert all(a in defined for a in op["args"])
+        if op.get("jset"):
+            # the second row reads the twin of every X-dependent value and nothing else
+            assert all((a - DUAL_OFF in dep) if a >= DUAL_OFF else (a not in dep) for a in op["args"])
+        if op["kind"] == "wf":
+            defined.add(op["out"])

@@ -176,6 +176,15 @@ int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n);
  * mfx_flow_partial_bytes; the reduction over samples has a fixed order. */
 int mfx_flow_backward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
                       const float* save, const float* gl, float* gx, float* partial, float* grad, void* stream);
+/* Minibatch of a weighted training set: for each of the batch uniforms r, the first
+ * index k with cdf[k] >= r (cdf: normalised running sum of the n weights, f64), whose
+ * point y[k] (D floats) goes to yb and, if f is not NULL, f[k] to fb.  No reference
+ * counterpart (the learned proposal is new). */
+int mfx_flow_minibatch(const double* cdf, int64_t n, const double* r, int64_t batch, const float* y, int D,
+                       const double* f, float* yb, double* fb, void* stream);
+/* Backward seed gl_i = -v_i / sum_j v_j with v_i = w_i (f_i / q_i)^beta, q_i = exp(logq_i)
+ * (w NULL: w_i = 1; beta = 0: KL, f unused), reduced in a fixed order; n <= 2^24. */
+int mfx_flow_seed(const double* w, const double* f, const float* logq, int64_t n, float beta, float* gl, void* stream);
 /* Adam update of the fp32 parameters (gradient clipped to +-clip, non-finite -> 0). */
 int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
                   float beta2, float eps, int step, float clip, void* stream);

@@ -518,6 +518,64 @@ __global__ void __launch_bounds__(256) k_reduce_grad(const float* __restrict__ p
   }
 }
 
+// Minibatch draw with probability proportional to the training weights: sample i takes
+// the first index k with cdf[k] >= r[i] (cdf = normalised running sum of the weights),
+// and copies that point's coordinates and integrand value.
+__global__ void k_minibatch(const double* __restrict__ cdf, int64_t n, const double* __restrict__ r, int64_t batch,
+                            const float* __restrict__ y, int D, const double* __restrict__ f, float* __restrict__ yb,
+                            double* __restrict__ fb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch) return;
+  const double u = r[i];
+  int64_t lo = 0, hi = n - 1;          // invariant: the answer lies in [lo, hi]
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] >= u) hi = mid;
+    else lo = mid + 1;
+  }
+  for (int d = 0; d < D; ++d) yb[i * D + d] = y[lo * D + d];
+  if (f) fb[i] = f[lo];
+}
+
+// Per-sample seed of the backward pass for the self-normalised objective
+// -sum_i v_i log q_i / sum_i v_i with v_i = w_i (f_i / q_i)^beta (w_i = 1 if w is null):
+// gl_i = -v_i / sum v.  One block; the maximum (for a safe exp) and the sum are reduced
+// in LDS in a fixed order.
+constexpr int SEED_T = 1024;
+__global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, const double* __restrict__ f,
+                                                 const float* __restrict__ ld, int n, float beta, float* __restrict__ gl) {
+  __shared__ double red[SEED_T];
+  const int t = threadIdx.x;
+  double mx = -HUGE_VAL;
+  if (beta != 0.f)
+    for (int i = t; i < n; i += SEED_T) mx = fmax(mx, (double)beta * (log(f[i]) - (double)ld[i]));
+  red[t] = mx;
+  __syncthreads();
+  for (int s = SEED_T / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  mx = beta != 0.f ? red[0] : 0.0;
+  if (!(mx > -HUGE_VAL)) mx = 0.0;     // every f_i = 0: all v_i = 0, the seed is zero
+  __syncthreads();
+  double sum = 0.0;
+  for (int i = t; i < n; i += SEED_T) {
+    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp((double)beta * (log(f[i]) - (double)ld[i]) - mx) : 1.0);
+    sum += v;
+  }
+  red[t] = sum;
+  __syncthreads();
+  for (int s = SEED_T / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double tot = red[0];
+  for (int i = t; i < n; i += SEED_T) {
+    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp((double)beta * (log(f[i]) - (double)ld[i]) - mx) : 1.0);
+    gl[i] = (float)(tot > 0.0 ? -v / tot : 0.0);
+  }
+}
+
 // Adam step on the fp32 master parameters, then refresh the transposed bf16 copies.
 __global__ void k_adam(float* __restrict__ P, const float* __restrict__ grad, float* __restrict__ m1,
                        float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float bc1, float bc2,
@@ -700,6 +758,23 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
     hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nblk, nlp,
                        grad + G.layer[l].oW1);
   }
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_minibatch(const double* cdf, int64_t n, const double* r, int64_t batch, const float* y, int D,
+                       const double* f, float* yb, double* fb, void* stream) {
+  if (n <= 0 || batch < 0 || D < 1 || D > DMAX) return fail(-2, "n > 0, batch >= 0 and 1 <= D <= 16 required%s", "");
+  if (!cdf || !r || !y || !yb || (f && !fb)) return fail(-3, "null pointer%s", "");
+  if (batch == 0) return 0;
+  hipLaunchKernelGGL(k_minibatch, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, (hipStream_t)stream, cdf, n, r,
+                     batch, y, D, f, yb, fb);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_seed(const double* w, const double* f, const float* logq, int64_t n, float beta, float* gl, void* stream) {
+  if (n <= 0 || n > (1 << 24)) return fail(-2, "0 < n <= 2^24 required%s", "");
+  if (!logq || !gl || (beta != 0.f && !f)) return fail(-3, "null pointer (f is needed for beta != 0)%s", "");
+  hipLaunchKernelGGL(k_seed, dim3(1), dim3(SEED_T), 0, (hipStream_t)stream, w, f, logq, (int)n, beta, gl);
   return hip_status(hipGetLastError());
 }
 

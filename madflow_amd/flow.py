@@ -58,6 +58,10 @@ def _sig(lib):
     lib.mfx_flow_backward.restype = i
     lib.mfx_flow_adam.argtypes = [vp, vp, vp, vp, i64, f, f, f, f, i, f, vp]
     lib.mfx_flow_adam.restype = i
+    lib.mfx_flow_minibatch.argtypes = [vp, i64, vp, i64, vp, i, vp, vp, vp, vp]
+    lib.mfx_flow_minibatch.restype = i
+    lib.mfx_flow_seed.argtypes = [vp, vp, vp, i64, f, vp, vp]
+    lib.mfx_flow_seed.restype = i
 
 
 def _p(t):
@@ -149,7 +153,7 @@ class RQSFlow:
     # ----------------------------------------------------------------- training
     def grad(self, y, w, f=None, beta=0.0, need_loss=True):
         """Gradient of L = -sum_i v_i log q(y_i) / sum_i v_i (returns L, grad), with
-        v_i = w_i for beta = 0 and v_i = w_i (f_i / q(y_i))^beta otherwise.
+        v_i = w_i for beta = 0 and v_i = w_i (f_i / q(y_i))^beta otherwise (w = None: w_i = 1).
 
         For points y_i drawn from a density p with w_i = f_i / p_i, beta = 0 is the
         gradient of KL(f || q) and beta = a - 1 that of the alpha-divergence
@@ -161,22 +165,19 @@ class RQSFlow:
         save = torch.empty(self.lib.mfx_flow_save_bytes(self.D, self.L, self._m(), n) // 4 + 1,
                            dtype=torch.float32, device=y.device)
         u, ld = self._apply(y, False, save=save)
-        if beta:
-            if f is None:
-                raise ValueError("grad: beta != 0 needs the integrand values f")
-            # log of (f / q)^beta, shifted by its maximum so that exp cannot overflow
-            lv = beta * (torch.log(f.to(torch.float64)) - ld.to(torch.float64))
-            lv = lv - lv.max()
-            w = w.to(torch.float64) * torch.exp(lv)
-        wsum = w.sum()
-        gl = (-w / wsum).to(torch.float32).contiguous()
+        if beta and f is None:
+            raise ValueError("grad: beta != 0 needs the integrand values f")
+        w64 = None if w is None else _lib.require_device_tensor(w, None, "w").to(torch.float64)
+        f64 = None if not beta else _lib.require_device_tensor(f, None, "f").to(torch.float64)
+        gl = torch.empty(n, dtype=torch.float32, device=y.device)
+        _lib.check(self.lib.mfx_flow_seed(_p(w64), _p(f64), _p(ld), n, float(beta), _p(gl), _lib.stream_ptr(y.device)))
         gx = torch.zeros_like(y)
         partial = torch.empty(self.lib.mfx_flow_partial_bytes(self.D, self.L, self._m(), n) // 4 + 1,
                               dtype=torch.float32, device=y.device)
         grad = torch.empty_like(self.params)     # every entry is written by the backward pass
         _lib.check(self.lib.mfx_flow_backward(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), n, _p(save), _p(gl),
                                               _p(gx), _p(partial), _p(grad), _lib.stream_ptr(y.device)))
-        loss = -(w.to(torch.float64) * ld.to(torch.float64)).sum() / wsum.to(torch.float64) if need_loss else None
+        loss = (gl.to(torch.float64) * ld.to(torch.float64)).sum() if need_loss else None
         return loss, grad
 
     def adam_step(self, grad, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=10.0):
@@ -230,7 +231,10 @@ class RQSFlow:
             if not total > 0:
                 raise ValueError("train: the weights sum to zero")
             cdf /= total
-            ones = torch.ones(batch, dtype=torch.float64, device=y.device)
+            y = _lib.require_device_tensor(y, torch.float32, "y")
+            fc = _lib.require_device_tensor(f, None, "f").to(torch.float64) if beta else None
+            yb = torch.empty((batch, self.D), dtype=torch.float32, device=y.device)
+            fb = torch.empty(batch, dtype=torch.float64, device=y.device) if beta else None
         for it in range(steps + 1):
             if val is not None and (it % eval_every == 0 or it == steps):
                 m2 = self.is_second_moment(val[0], val[1], mix)
@@ -244,15 +248,14 @@ class RQSFlow:
                 break
             if resample:
                 r = torch.rand(batch, generator=g, device=y.device, dtype=torch.float64)
-                idx = torch.searchsorted(cdf, r).clamp_(max=n - 1)
-                yb = y[idx].contiguous()
-                wb = ones
-                fb = f[idx] if beta else None
+                _lib.check(self.lib.mfx_flow_minibatch(_p(cdf), n, _p(r), batch, _p(y), self.D, _p(fc), _p(yb),
+                                                       _p(fb), _lib.stream_ptr(y.device)))
+                wb = None
             else:
                 idx = torch.randint(0, n, (batch,), generator=g, device=y.device)
                 yb = y[idx].contiguous()
                 wb = w[idx].contiguous()
-                fb = f[idx] if beta else None
+                fb = f[idx].contiguous() if beta else None
                 if float(wb.sum()) <= 0:
                     continue
             cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)

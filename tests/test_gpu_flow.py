@@ -128,3 +128,45 @@ def test_flow_training_reduces_variance(dev):
     exact = f.mean().item()
     err = math.hypot(wgt.std().item() / math.sqrt(len(wgt)), plain.std().item() / math.sqrt(len(plain)))
     assert abs(wgt.mean().item() - exact) < 4 * err
+
+
+@pytest.mark.gpu
+def test_minibatch_draw_and_seed(dev):
+    """mfx_flow_minibatch picks the first index whose normalised running weight reaches
+    the uniform (numpy searchsorted, side='left'), zero-weight points are never drawn,
+    and mfx_flow_seed gives -v_i / sum v with v_i = w_i (f_i / q_i)^beta."""
+    import ctypes
+    from madflow_amd import _lib
+    print("case flow minibatch / seed", flush=True)
+    flow = RQSFlow(3, layers=2, seed=1, device=dev)
+    lib = flow.lib
+    rng = np.random.default_rng(21)
+    n, batch = 5000, 4096
+    w = rng.random(n) ** 3 * (rng.random(n) > 0.3)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    r = rng.random(batch)
+    r[:16] = cdf[rng.integers(0, n, 16)]           # uniforms that hit a running sum exactly
+    y = rng.random((n, 3)).astype(np.float32)
+    f = rng.random(n) + 0.1
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    cdf_t, r_t, y_t, f_t = t(cdf), t(r), t(y), t(f)
+    yb = torch.empty((batch, 3), dtype=torch.float32, device=dev)
+    fb = torch.empty(batch, dtype=torch.float64, device=dev)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.check(lib.mfx_flow_minibatch(vp(cdf_t), n, vp(r_t), batch, vp(y_t), 3, vp(f_t), vp(yb), vp(fb),
+                                      _lib.stream_ptr(dev)))
+    idx = np.minimum(np.searchsorted(cdf, r, side="left"), n - 1)
+    assert np.all(w[idx] > 0)
+    np.testing.assert_array_equal(yb.cpu().numpy(), y[idx])
+    np.testing.assert_array_equal(fb.cpu().numpy(), f[idx])
+    ld = rng.normal(size=batch).astype(np.float32)
+    wb = rng.random(batch)
+    fi_t, ld_t = t(f[idx]), t(ld)              # kept referenced until the kernels ran
+    for beta, wt in ((0.0, None), (0.0, wb), (2.0, wb)):
+        gl = torch.empty(batch, dtype=torch.float32, device=dev)
+        w_t = t(wt) if wt is not None else None
+        _lib.check(lib.mfx_flow_seed(vp(w_t) if w_t is not None else None, vp(fi_t), vp(ld_t), batch,
+                                     beta, vp(gl), _lib.stream_ptr(dev)))
+        v = (wt if wt is not None else np.ones(batch)) * (f[idx] * np.exp(-ld.astype(np.float64))) ** beta
+        np.testing.assert_allclose(gl.cpu().numpy(), -v / v.sum(), rtol=1e-6, atol=0)

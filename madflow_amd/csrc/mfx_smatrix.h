@@ -211,8 +211,11 @@ __device__ __forceinline__ float colour_sum_mfma_staged(const float* stage, cons
 template <class T, bool MFMA> struct ColourLds { float stage[1], cf[1], red[1]; };
 template <class T> struct ColourLds<T, true> {
   static constexpr int NCP = (T::NCOLOR + 15) / 16 * 16;
-  float stage[4][2 * T::NCOLOR * 64];
-  float stage2[4][2 * T::NCOLOR * 64];   // the second row of a pair
+  // per wave: the staging of the register jamps, plus one in-place jamp array for each
+  // further row of a pass (with four rows, the first row is staged in the second row's
+  // array once that row has been contracted)
+  static constexpr int NST = T::MULTI_K >= 2 ? 3 : 2;
+  float stage[NST][4][2 * T::NCOLOR * 64];
   float cf[T::NCOLOR * NCP];
   float red[4][64];
 };
@@ -244,29 +247,49 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
     if constexpr (StableForms<R>::value) load_invariants<T, R>(a.all_ps, e, inv);
     const Params<R> prm = load_params<R>(a, e);
     R ans = R(0);
-    if constexpr (T::DUAL && StableForms<R>::value) {
-      // every helicity combination is a row: walk them in pairs that differ only in the
-      // helicity of leg DUAL_LEG, whose shared currents amplitudes2 computes once
-      static_assert(T::NCOMB == (1 << T::NEXTERNAL), "pairs of rows need every combination");
-      constexpr int X = T::DUAL_LEG;
+    if constexpr (T::MULTI_K >= 1 && StableForms<R>::value) {
+      // every helicity combination is a row: walk them in groups of 2^MULTI_K rows that
+      // differ only in the helicities of the legs MULTI_LEG, whose shared currents the
+      // multi-row program computes once per group
+      constexpr int K = T::MULTI_K;
+      static_assert(T::NCOMB == (1 << T::NEXTERNAL), "groups of rows need every combination");
+      static_assert(K <= 2, "at most four rows per pass");
+      constexpr int XA = K == 2 && T::MULTI_LEG[K - 1] < T::MULTI_LEG[0] ? T::MULTI_LEG[K - 1] : T::MULTI_LEG[0];
+      constexpr int XB = K == 2 && T::MULTI_LEG[K - 1] < T::MULTI_LEG[0] ? T::MULTI_LEG[0] : T::MULTI_LEG[K - 1];
 #pragma unroll 1
-      for (int q = 0; q < T::NCOMB / 2; ++q) {
+      for (int q = 0; q < (T::NCOMB >> K); ++q) {
 #pragma unroll
         for (int l = 0; l < T::NEXTERNAL; ++l)
 #pragma unroll
           for (int k = 0; k < 4; ++k) opaque(p[l][k]);
 #pragma unroll
         for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
-        const unsigned hb = ((unsigned)q & ((1u << X) - 1u)) | (((unsigned)q >> X) << (X + 1));
-        if constexpr (MFMA) {
-          // the two rows' jamps accumulate in place in the wave's LDS stages, which frees
-          // their registers for the currents the two rows share
+        // zero bits inserted at the paired legs' positions, the lower position first
+        unsigned hb = ((unsigned)q & ((1u << XA) - 1u)) | (((unsigned)q >> XA) << (XA + 1));
+        if constexpr (K == 2) hb = (hb & ((1u << XB) - 1u)) | ((hb >> XB) << (XB + 1));
+        if constexpr (K == 2) {
+          static_assert(MFMA, "four rows per pass keep three rows' jamps in the colour-sum LDS");
+          // rows 1..3 accumulate in place in the wave's LDS arrays; row 0 stays in
+          // registers and is staged in row 1's array after row 1 has been contracted
           const int lane = threadIdx.x & 63;
-          LdsJamps ja{lds.stage[wave] + lane}, jb{lds.stage2[wave] + lane};
+          cx<R> jamp[T::NCOLOR];
+          LdsJamps j1{lds.stage[0][wave] + lane}, j2{lds.stage[1][wave] + lane}, j3{lds.stage[2][wave] + lane};
+          wave_lds_sync();                 // the previous group's contractions are done
+          T::template amplitudes4<R>(p, inv, hb, prm, jamp, j1, j2, j3);
+          ans += colour_sum_mfma_staged<T>(lds.stage[0][wave], lds.cf, lds.red[wave]);
+          ans += colour_sum_mfma<T>(jamp, lds.stage[0][wave], lds.cf, lds.red[wave]);
+          ans += colour_sum_mfma_staged<T>(lds.stage[1][wave], lds.cf, lds.red[wave]);
+          ans += colour_sum_mfma_staged<T>(lds.stage[2][wave], lds.cf, lds.red[wave]);
+        } else if constexpr (MFMA) {
+          // the second row's jamps accumulate in place in the wave's second LDS stage,
+          // which frees their registers for the currents the two rows share
+          const int lane = threadIdx.x & 63;
+          cx<R> jamp[T::NCOLOR];
+          LdsJamps jb{lds.stage[1][wave] + lane};
           wave_lds_sync();                 // the previous pair's contractions are done
-          T::template amplitudes2<R>(p, inv, hb, prm, ja, jb);
-          ans += colour_sum_mfma_staged<T>(lds.stage[wave], lds.cf, lds.red[wave]);
-          ans += colour_sum_mfma_staged<T>(lds.stage2[wave], lds.cf, lds.red[wave]);
+          T::template amplitudes2<R>(p, inv, hb, prm, jamp, jb);
+          ans += colour_sum_mfma<T>(jamp, lds.stage[0][wave], lds.cf, lds.red[wave]);
+          ans += colour_sum_mfma_staged<T>(lds.stage[1][wave], lds.cf, lds.red[wave]);
         } else {
           cx<R> jamp[T::NCOLOR], jamp2[T::NCOLOR];
           T::template amplitudes2<R>(p, inv, hb, prm, jamp, jamp2);
@@ -287,7 +310,7 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
         }
         cx<R> jamp[T::NCOLOR];
         T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
-        if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[wave], lds.cf, lds.red[wave]);
+        if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[0][wave], lds.cf, lds.red[wave]);
         else ans += colour_sum<T, R>(jamp);
       }
     }

@@ -106,41 +106,56 @@ def device_program(desc):
 DUAL_OFF = 100000
 
 
-def dual_program(desc):
-    """One program for two helicity rows that differ only in the helicity of external leg
-    X: every current that does not depend on leg X (the gluon currents and the currents of
-    the other fermion line, for X a fermion) is computed once for both rows.  The currents
-    that depend on X, and every amplitude, are evaluated twice; the copies carry ids
-    offset by DUAL_OFF and the amplitude copies (``jset`` 1) accumulate into the second
-    row's jamps.  X is the leg with the fewest dependent currents.  The single-row schedule
-    is kept, each twin right after its original.  Returns (X, ops)."""
+def multi_program(desc, k):
+    """One program for the 2^k helicity rows that differ only in the helicities of k
+    external legs X_0 .. X_{k-1} (the k legs with the fewest dependent currents).  A
+    current that depends on the legs in the set m is evaluated once per helicity
+    assignment of m, 2^|m| times; a current that depends on none of them is computed once
+    for all rows.  Every amplitude is evaluated once per row.  Copy ids: id + DUAL_OFF * r,
+    r = the bits (bit b = leg X_b at helicity +1) of the assignment; amplitude copies
+    carry ``jset`` = the row c, whose bit b is the helicity of X_b.  The single-row
+    schedule is kept, the copies right after their original.  Returns (legs, ops)."""
     ops = device_program(desc)
     n = desc["nexternal"]
-    best = None
+    size = {}
     for X in range(n):
         dep = {X}
         for op in ops:
             if op["kind"] == "wf" and any(a in dep for a in op["args"]):
                 dep.add(op["out"])
-        if best is None or len(dep) < best[0]:
-            best = (len(dep), X)
-    X = best[1]
-    dep = {X}
+        size[X] = len(dep)
+    legs = sorted(range(n), key=lambda X: (size[X], X))[:k]
+    mask = {X: 1 << b for b, X in enumerate(legs)}
     out = []
+
+    def name(a, r):
+        r &= mask.get(a, 0)
+        return a + DUAL_OFF * r
+
     for op in ops:
-        if op["kind"] == "wf" and not any(a in dep for a in op["args"]):
-            out.append(op)
-            continue
+        m = 0
+        for a in op["args"]:
+            m |= mask.get(a, 0)
         if op["kind"] == "wf":
-            dep.add(op["out"])
-        twin = dict(op)
-        twin["args"] = [a + DUAL_OFF if a in dep else a for a in op["args"]]
-        if op["kind"] == "wf":
-            twin["out"] = op["out"] + DUAL_OFF
+            mask[op["out"]] = m
+            subs = [r for r in range(1 << k) if r & ~m == 0]
         else:
-            twin["jset"] = 1
-        out += [op, twin]
-    return X, out
+            subs = list(range(1 << k))
+        for r in subs:
+            c = dict(op)
+            c["args"] = [name(a, r) for a in op["args"]]
+            if op["kind"] == "wf":
+                c["out"] = op["out"] + DUAL_OFF * r
+            elif r:
+                c["jset"] = r
+            out.append(c)
+    return legs, out
+
+
+def dual_program(desc):
+    """multi_program with one paired leg: two helicity rows per pass.  Returns (X, ops)."""
+    legs, ops = multi_program(desc, 1)
+    return legs[0], ops
 
 
 def _mom_expr(legs, overall, nin):
@@ -204,7 +219,7 @@ def _emit_ops(w, prog, desc, legs_of):
                 w("  else %s(%s, %s, %s, %s, w%d);" % (slot_r, args, coup, mass, width, op["out"]))
         else:
             f1, f2, vs = op["args"][0], op["args"][1], op["args"][2:]
-            jn = "jamp2" if op.get("jset") else "jamp"
+            jn = "jamp%d" % (op["jset"] + 1) if op.get("jset") else "jamp"
             if len(vs) == 1:
                 (r0, i0), = op["factors"]
                 w("  { const cx<R> amp0 = %s(w%d, w%d, w%d, %s);" % (op["routine"], f1, f2, vs[0], coup))
@@ -297,21 +312,32 @@ def emit_hip(desc):
     w("}")
     w("")
     dual = desc["ncomb"] == 1 << n
+    k = MULTI_K.get(name, 1) if dual else 0
     if dual:
-        X, prog2 = dual_program(desc)
-        for k in list(legs_of):
-            legs_of[k + DUAL_OFF] = legs_of[k]
-        w("// Two helicity rows at once: hb with leg %d at helicity -1 (into jamp) and +1 (into" % X)
-        w("// jamp2).  Currents that do not depend on leg %d are computed once for both rows." % X)
-        w("// JA: an array cx<R>[NCOLOR] or an accessor whose elements support =, += and -=.")
-        w("template <class R, class JA>")
-        w("__device__ __forceinline__ void amplitudes2(const R (&p)[NEXTERNAL][4], const R (&inv)[NINV], unsigned hb,")
-        w("                                            const Params<R>& prm, JA& jamp, JA& jamp2) {")
+        legs, progk = multi_program(desc, k)
+        for key in list(legs_of):
+            for r in range(1, 1 << k):
+                legs_of[key + DUAL_OFF * r] = legs_of[key]
+        nrow = 1 << k
+        fname = "amplitudes%d" % nrow
+        jnames = ["jamp"] + ["jamp%d" % (c + 1) for c in range(1, nrow)]
+        w("// %d helicity rows at once: hb with the legs %s at helicity -1 / +1 (row c: bit b of c" % (
+            nrow, ", ".join(str(X) for X in legs)))
+        w("// is the helicity of the b-th of them).  Currents that do not depend on a leg are")
+        w("// shared by the rows that differ only in that leg's helicity.")
+        w("// J0 .. J%d: arrays cx<R>[NCOLOR] or accessors whose elements support =, += and -=." % (nrow - 1))
+        w("template <class R, %s>" % ", ".join("class J%d" % c for c in range(nrow)))
+        w("__device__ __forceinline__ void %s(const R (&p)[NEXTERNAL][4], const R (&inv)[NINV], unsigned hb," % fname)
+        w("    const Params<R>& prm, %s) {" % ", ".join("J%d& %s" % (c, j) for c, j in enumerate(jnames)))
         w("  const R ZERO = R(0);")
         for l, e in enumerate(desc["externals"]):
             mass = "ZERO" if e["mass"] == "ZERO" else "prm.mdl_MT"
-            for ident, hel in ([(l, "R(-1)"), (l + DUAL_OFF, "R(1)")] if l == X else
-                               [(l, "(((hb >> %d) & 1u) ? R(1) : R(-1))" % l)]):
+            if l in legs:
+                b = legs.index(l)
+                idents = [(l, "R(-1)"), (l + DUAL_OFF * (1 << b), "R(1)")]
+            else:
+                idents = [(l, "(((hb >> %d) & 1u) ? R(1) : R(-1))" % l)]
+            for ident, hel in idents:
                 if e["routine"] == "vxxxxx":
                     w("  W<R> w%d; vxxxxx(p[%d], %s, %s, R(%d), prm.sqh, w%d);" % (ident, l, mass, hel, e["sign"], ident))
                 else:
@@ -319,10 +345,10 @@ def emit_hip(desc):
                                                                      ident))
         w("  #pragma unroll")
         w("  for (int c = 0; c < NCOLOR; ++c) {")
-        w("    jamp[c] = cx<R>(R(0), R(0));")
-        w("    jamp2[c] = cx<R>(R(0), R(0));")
+        for j in jnames:
+            w("    %s[c] = cx<R>(R(0), R(0));" % j)
         w("  }")
-        _emit_ops(w, prog2, desc, legs_of)
+        _emit_ops(w, progk, desc, legs_of)
         w("}")
         w("")
     w("struct Traits {")
@@ -338,13 +364,14 @@ def emit_hip(desc):
     w("      cx<R> (&jamp)[NCOLOR]) {")
     w("    %s::amplitudes(p, inv, hb, helv, prm, jamp);" % name)
     w("  }")
-    w("  static constexpr bool DUAL = %s;" % ("true" if dual else "false"))
+    w("  static constexpr int MULTI_K = %d;   // rows per pass: 2^MULTI_K (0: one row)" % k)
     if dual:
-        w("  static constexpr int DUAL_LEG = %d;" % X)
-        w("  template <class R, class JA> static __device__ __forceinline__ void amplitudes2(")
+        w("  static constexpr int MULTI_LEG[%d] = {%s};" % (k, ", ".join(str(X) for X in legs)))
+        w("  template <class R, %s> static __device__ __forceinline__ void %s(" % (
+            ", ".join("class J%d" % c for c in range(nrow)), fname))
         w("      const R (&p)[NEXTERNAL][4], const R (&inv)[NINV], unsigned hb, const Params<R>& prm,")
-        w("      JA& jamp, JA& jamp2) {")
-        w("    %s::amplitudes2(p, inv, hb, prm, jamp, jamp2);" % name)
+        w("      %s) {" % ", ".join("J%d& %s" % (c, j) for c, j in enumerate(jnames)))
+        w("    %s::%s(p, inv, hb, prm, %s);" % (name, fname, ", ".join(jnames)))
         w("  }")
     w("  static __device__ __forceinline__ unsigned hel_bits(int h) { return HEL_BITS[h]; }")
     w("  static __device__ __forceinline__ float cf(float, int i, int j) { return CF_NUM_F[i][j]; }")
@@ -368,6 +395,8 @@ def emit_hip_unit(desc):
 # processes evaluated by the colour-ordered kernels (mfx_ordered.h): their n! jamps do not
 # fit the register file next to a Feynman-diagram program.  "gg_ttxgg_ordered" is the
 # same physics as gg_ttxgg through the ordered kernels (cross-check and comparison).
+# helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
+MULTI_K = {}
 ORDERED = {"gg_ttxggg": "gg_ttxggg", "gg_ttxgg_ordered": "gg_ttxgg"}
 
 

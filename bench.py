@@ -41,14 +41,13 @@ PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 pack
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md)
 HBM_BYTES_PER_ME = {32: 6184.3}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
-KIN = dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4)
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dense, npass, params, budget_s=12.0):
+def cpu_baseline(proc, dense, npass, params, budget_s=12.0):
     """Time the CPU oracle (FP64 C++, OpenMP over events) on the first passing events of
     the last chunk: doubles the sample until one timed call takes budget_s / 3."""
     import numpy as np
@@ -62,14 +61,14 @@ def cpu_baseline(dense, npass, params, budget_s=12.0):
     n = 256
     while True:
         t0 = time.perf_counter()
-        O.smatrix("gg_ttxgg", ps[:n], rp, coup, ref_compat=True, nthreads=cores)
+        O.smatrix(proc, ps[:n], rp, coup, ref_compat=True, nthreads=cores)
         t = time.perf_counter() - t0
         if t > budget_s / 3 or n >= nmax:
             break
         n = min(nmax, n * 2)
     return {"value": n / t, "unit": "ME evals/s", "cores": cores, "kind": "port",
-            "sample": "%d post-cut gg->ttgg events of the benchmark stream, FP64 oracle smatrix "
-                      "(C++, OpenMP over events), %.1f s" % (n, t)}
+            "sample": "%d post-cut %s events of the benchmark stream, FP64 oracle smatrix "
+                      "(C++, OpenMP over events), %.1f s" % (n, LABEL[proc], t)}
 
 
 def _unweighted(res, setup_s, sample_s, target):
@@ -126,8 +125,8 @@ def _extend_to(iterate, res, sampled_s, seconds, torch):
 def run_main(a, torch):
     """Plain VEGAS at the main precision: W warm-up iterations, then exactly K timed ones."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
-    ig = MEIntegrand("gg_ttxgg", precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
-                     **KIN)
+    ig = MEIntegrand(a.process, precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
+                     **ME_ONLY_KIN[a.process])
     vf = VegasFlow(ig.ndim, a.nev)
     vf.compile(ig)
     log("warm-up: %d VEGAS iterations of %d points (grid adaptation)" % (a.warmup, a.nev))
@@ -222,7 +221,8 @@ def run_flow_compare(a, torch, main):
     from madflow_amd.vegas import MEIntegrand, VegasFlow
     sync = torch.cuda.synchronize
     # ---- plain VEGAS, FP64 matrix elements
-    ig64 = MEIntegrand("gg_ttxgg", precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit, **KIN)
+    ig64 = MEIntegrand(a.process, precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit,
+                       **ME_ONLY_KIN[a.process])
     vf = VegasFlow(ig64.ndim, a.nev)
     vf.compile(ig64)
     sync()
@@ -257,8 +257,8 @@ def run_flow_compare(a, torch, main):
     def gain(k):
         return learned[k] / plain[k] if plain[k] > 0 else None
     return {
-        "workload": "gg->ttgg, sqrt(s)=3 TeV, pT(g)>30, dR(g,g)>0.4, %d-dim %s VEGAS cube, %d points/iteration, "
-                    "target sigma/sigma=%g" % (ig.ndim, a.mapping, a.nev, a.target),
+        "workload": "%s, %s, %d-dim %s VEGAS cube, %d points/iteration, target sigma/sigma=%g" % (
+            LABEL[a.process], _kin_text(a.process), ig.ndim, a.mapping, a.nev, a.target),
         "plain_vegas_fp64": plain,
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
@@ -277,6 +277,22 @@ ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_mi
                "gg_ttxgg": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
                "gg_ttxgg_ordered": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
                "gg_ttxggg": dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4)}
+LABEL = {"gg_ttx": "gg->tt", "gg_ttxg": "gg->ttg", "gg_ttxgg": "gg->ttgg", "gg_ttxgg_ordered": "gg->ttgg",
+         "gg_ttxggg": "gg->ttggg"}
+KERNEL = {"gg_ttxggg": "k_ordered_jamps+k_colour_mfma<gg_ttxggg,%s>",
+          "gg_ttxgg_ordered": "k_ordered_jamps+k_colour_mfma<gg_ttxgg_ordered,%s>"}
+
+
+def _kin_text(proc):
+    k = ME_ONLY_KIN[proc]
+    t = ["sqrt(s)=%g TeV" % (k["sqrts"] / 1000.0)]
+    if k.get("pt_min"):
+        t.append("pT(g)>%g GeV" % k["pt_min"])
+    if k.get("abs_eta_max"):
+        t.append("|eta(g)|<%g" % k["abs_eta_max"])
+    if k.get("dr_min"):
+        t.append("dR(g,g)>%g" % k["dr_min"])
+    return ", ".join(t)
 
 
 def me_only(a, torch):
@@ -322,6 +338,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nev", type=int, default=16_000_000)
     ap.add_argument("--events-limit", type=int, default=1 << 22)
+    ap.add_argument("--process", default="gg_ttxgg", choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"],
+                    help="process of the VEGAS / flow workload (the benchmark metric is quoted on gg_ttxgg)")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--mapping", default="diet", choices=["diet", "rambo"])
     ap.add_argument("--target", type=float, default=1e-3)
@@ -367,7 +385,7 @@ def main(argv=None):
         print(json.dumps({"copy_probe_bytes_per_copy": src.numel() * 4, "copies": 4}))
         return 0
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
-        flops = json.load(fh)["gg_ttxgg"]["total"]
+        flops = json.load(fh)[a.process]["total"]
     m = run_main(a, torch)
     kernel_s = m["me_ms"] * 1e-3
     achieved = flops * m["me_n"] / kernel_s / 1e12
@@ -390,9 +408,10 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f32" if a.precision == 32 else "f64",
         "data": "synthetic (Philox4x32-10 points, seed 3)",
-        "config": {"workload": "gg->ttgg LO, sqrt(s)=3 TeV, pT(g)>30 GeV, dR(g,g)>0.4, %s phase space on a "
+        "config": {"workload": "%s LO, %s, %s phase space on a "
                                "%d-dim 50-bin VEGAS grid, %d points/iteration, %s amplitudes and colour sum, FP64 "
-                               "phase space and VEGAS sums" % (a.mapping, m["ig"].ndim, a.nev,
+                               "phase space and VEGAS sums" % (LABEL[a.process], _kin_text(a.process), a.mapping,
+                                                               m["ig"].ndim, a.nev,
                                                                "FP32" if a.precision == 32 else "FP64"),
                    "points_per_step": a.nev, "passing_per_step": m["npass"] // a.steps},
         "me_kernel_evals_per_s": m["me_n"] / kernel_s,
@@ -401,15 +420,17 @@ def main(argv=None):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": (HBM_BYTES_PER_ME[a.precision] * m["me_n"] / max(1, m["launches"])
-                                 if a.precision in HBM_BYTES_PER_ME and a.mapping == "diet" else None),
+                                 if a.process == "gg_ttxgg" and a.precision in HBM_BYTES_PER_ME
+                                 and a.mapping == "diet" else None),
                      "traffic_unit": "bytes per launch (PMC)",
-                     "kernel": "k_smatrix<gg_ttxgg,%s>" % ("float" if a.precision == 32 else "double"),
+                     "kernel": (KERNEL.get(a.process, "k_smatrix<%s,%%s>" % a.process)
+                                % ("float" if a.precision == 32 else "double")),
                      "flops_per_me": flops, "mes_per_launch": m["me_n"] / max(1, m["launches"]),
                      "kernel_ms_per_launch": m["me_ms"] / max(1, m["launches"])},
     }
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(b["dense"], m["ig"].last_chunk_npass, m["ig"].params)
+        res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass, m["ig"].params)
     if not a.no_flow:
         res["flow_comparison"] = run_flow_compare(a, torch, m)
     print(json.dumps(res))

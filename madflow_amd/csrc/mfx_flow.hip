@@ -546,9 +546,11 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
                                                  const float* __restrict__ ld, int n, float beta, float* __restrict__ gl) {
   __shared__ double red[SEED_T];
   const int t = threadIdx.x;
+  // f_i <= 0 (or not a number) gives v_i = 0
+  auto lv = [&](int i) { return f[i] > 0.0 ? (double)beta * (log(f[i]) - (double)ld[i]) : -HUGE_VAL; };
   double mx = -HUGE_VAL;
   if (beta != 0.f)
-    for (int i = t; i < n; i += SEED_T) mx = fmax(mx, (double)beta * (log(f[i]) - (double)ld[i]));
+    for (int i = t; i < n; i += SEED_T) mx = fmax(mx, lv(i));
   red[t] = mx;
   __syncthreads();
   for (int s = SEED_T / 2; s > 0; s >>= 1) {
@@ -559,10 +561,7 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
   if (!(mx > -HUGE_VAL)) mx = 0.0;     // every f_i = 0: all v_i = 0, the seed is zero
   __syncthreads();
   double sum = 0.0;
-  for (int i = t; i < n; i += SEED_T) {
-    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp((double)beta * (log(f[i]) - (double)ld[i]) - mx) : 1.0);
-    sum += v;
-  }
+  for (int i = t; i < n; i += SEED_T) sum += (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
   red[t] = sum;
   __syncthreads();
   for (int s = SEED_T / 2; s > 0; s >>= 1) {
@@ -571,7 +570,7 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
   }
   const double tot = red[0];
   for (int i = t; i < n; i += SEED_T) {
-    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp((double)beta * (log(f[i]) - (double)ld[i]) - mx) : 1.0);
+    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
     gl[i] = (float)(tot > 0.0 ? -v / tot : 0.0);
   }
 }

@@ -1,0 +1,93 @@
+"""Les Houches event files of unweighted events.
+
+madflow writes the events of an integration through MadGraph's ``lhe_parser``
+(lhe_writer.py:94-356): weighted events go to ``weighted_events.lhe.gz`` and MadGraph's
+hit-or-miss unweighting writes ``unweighted_events.lhe.gz``, where every kept event
+carries the cross section as its weight (lhe_writer.py:341-350).  Here the unweighting
+runs on the device (``MEIntegrand.unweighted_events``) and this module writes the kept
+events directly, gzip-compressed, in the Les Houches Accord v3 layout
+(<init> block, then one <event> block per event: NUP IDPRUP XWGTUP SCALUP AQEDUP AQCDUP,
+then IDUP ISTUP MOTHUP1 MOTHUP2 ICOLUP1 ICOLUP2 PUP1..5 VTIMUP SPINUP per particle).
+
+As in the reference's writer (lhe_writer.py:151-200), mothers and colour lines are 0
+and the scale and couplings are written as given (0 by default); the spin entry is 9
+(unpolarised, the LHA code for "no helicity information").  The incoming partons are
+written as gluons (the reference's writer labels them as protons), tops as 6 / -6.
+"""
+
+import gzip
+import math
+
+import numpy as np
+
+# PDG codes of the external legs of each process: g g > t t~ g ...
+PDG = {
+    "gg_ttx": [21, 21, 6, -6],
+    "gg_ttxg": [21, 21, 6, -6, 21],
+    "gg_ttxgg": [21, 21, 6, -6, 21, 21],
+    "gg_ttxggg": [21, 21, 6, -6, 21, 21, 21],
+}
+
+
+def _mass(p):
+    m2 = p[0] * p[0] - p[1] * p[1] - p[2] * p[2] - p[3] * p[3]
+    return math.sqrt(m2) if m2 > 0 else 0.0
+
+
+def write_lhe(path, process, momenta, sigma_pb, err_pb, sqrts, scale=0.0, aqed=0.0, aqcd=0.0, process_id=1):
+    """Write unweighted events to ``path`` (gzip if it ends in .gz).
+
+    momenta: (nev, nexternal, 4) array (E, px, py, pz) in GeV, legs in the process
+    order (two incoming first).  Every event gets the weight sigma_pb (unweighted events,
+    IDWTUP = 3).  Returns the number of events written."""
+    pdg = PDG[process]
+    ps = np.asarray(momenta, dtype=np.float64)
+    if ps.ndim != 3 or ps.shape[1] != len(pdg) or ps.shape[2] != 4:
+        raise ValueError("momenta must have shape (nev, %d, 4)" % len(pdg))
+    opener = gzip.open if str(path).endswith(".gz") else open
+    ebeam = 0.5 * float(sqrts)
+    with opener(path, "wt") as f:
+        f.write('<LesHouchesEvents version="3.0">\n')
+        f.write("<header>\n<madflow_amd process=\"%s\" events=\"%d\"/>\n</header>\n" % (process, len(ps)))
+        f.write("<init>\n")
+        f.write(" %d %d %.10e %.10e 0 0 0 0 3 1\n" % (pdg[0], pdg[1], ebeam, ebeam))
+        f.write(" %.10e %.10e %.10e %d\n" % (sigma_pb, err_pb, sigma_pb, process_id))
+        f.write("</init>\n")
+        for ev in ps:
+            f.write("<event>\n")
+            f.write(" %d %d %+.10e %.10e %.10e %.10e\n" % (len(pdg), process_id, sigma_pb, scale, aqed, aqcd))
+            for leg, p in enumerate(ev):
+                status = -1 if leg < 2 else 1
+                f.write(" %8d %2d 0 0 0 0 %+.12e %+.12e %+.12e %.12e %.12e 0.0 9.0\n" % (
+                    pdg[leg], status, p[1], p[2], p[3], p[0], _mass(p)))
+            f.write("</event>\n")
+        f.write("</LesHouchesEvents>\n")
+    return len(ps)
+
+
+def read_lhe(path):
+    """Read back (process_line, events) from a file written by write_lhe: events is a list
+    of (event_line, [(pid, status, px, py, pz, E, m), ...])."""
+    opener = gzip.open if str(path).endswith(".gz") else open
+    init, events, cur, block = None, [], None, None
+    with opener(path, "rt") as f:
+        for line in f:
+            s = line.strip()
+            if s == "<init>":
+                block, init = "init", []
+            elif s == "</init>":
+                block = None
+            elif s == "<event>":
+                block, cur = "event", None
+            elif s == "</event>":
+                events.append(cur)
+                block = None
+            elif block == "init":
+                init.append([float(x) for x in s.split()])
+            elif block == "event":
+                v = s.split()
+                if cur is None:
+                    cur = ([float(x) for x in v], [])
+                else:
+                    cur[1].append((int(v[0]), int(v[1])) + tuple(float(x) for x in v[6:11]))
+    return init, events

@@ -163,6 +163,33 @@ class MEIntegrand:
                                          _ptr(ws), ws.numel(), ctypes.byref(nk), _lib.stream_ptr(w.device)))
         return idx[:nk.value], keep
 
+    def unweighted_events(self, n_events, iteration, wmax, flow=None, mix=0.2, seed=1234):
+        """Generate n_events points on the frozen grid (from the flow's defensive mixture
+        if ``flow`` is given, as in proposal_iteration), weight them, and keep each with
+        probability min(1, w / wmax) on the device.  Returns (momenta of the kept events,
+        (K, nexternal, 4) host array, their weights, the number of over-weight events).
+        Points, weights and keep decisions are reproducible: they depend only on the
+        seeds, the iteration and the event index."""
+        from .phasespace_gpu import philox_uniforms
+        kept, wts, over = [], [], 0
+        ev0 = 0
+        while ev0 < n_events:
+            n = min(self.events_limit, n_events - ev0)
+            if flow is not None:
+                u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0).float()
+                ys, winv = flow.sample_mixture(u, u, mix)
+                _, _, f = self.chunk(ev0, n, iteration, keep_f=True, y_in=ys.contiguous(), w_in=winv.contiguous())
+            else:
+                _, _, f = self.chunk(ev0, n, iteration, keep_f=True)
+            idx, _ = self.unweight(f, wmax, iteration, ev0)
+            il = idx.long()
+            kept.append(self._bufs["all_ps"][:n].index_select(0, il).cpu().numpy())
+            fw = f.index_select(0, il)
+            wts.append(fw.cpu().numpy())
+            over += int((fw > wmax).sum())
+            ev0 += n
+        return np.concatenate(kept), np.concatenate(wts), over
+
     def iteration(self, n_events, iteration, train=True, keep_f=False):
         """One VEGAS iteration of n_events; returns a dict with I, var, max weight, counts."""
         tot = None

@@ -176,3 +176,39 @@ def test_ttggg_vegas_and_flow_proposal(dev):
           % (plain, plain_err, learned, learned_err, rel_plain, rel_flow), flush=True)
     assert plain > 0 and learned > 0 and learned_err > 0
     assert abs(learned - plain) < 3 * math.hypot(plain_err, learned_err)
+
+
+@pytest.mark.gpu
+def test_unweighted_events_to_lhe(dev, tmp_path):
+    """Unweighted events on the frozen grid: the kept count matches sum min(1, w/wmax)
+    over the same (reproducible) points within its binomial spread, the kept events are
+    physical (momentum conservation, on-shell tops), a second call returns the same
+    events, and the Les Houches file holds exactly them."""
+    from madflow_amd.lhe import write_lhe, read_lhe
+    print("case unweighted events -> LHE", flush=True)
+    ig = MEIntegrand("gg_ttxgg", precision=32, mapping="diet", seed=17, sqrts=3000.0, pt_min=30.0, dr_min=0.4)
+    vf = VegasFlow(ig.ndim, 1 << 18)
+    vf.compile(ig)
+    vf.run_integration(3)
+    vf.freeze_grid()
+    sigma, err = vf.run_integration(1)
+    wmax = 0.1 * vf.history[-1]["wmax"]                  # a tenth: thousands of kept events
+    n = 1 << 18
+    ps, w, over = ig.unweighted_events(n, 7, wmax)
+    _, _, f = ig.chunk(0, n, 7, keep_f=True)
+    expect = float(torch.clamp(f / wmax, max=1.0).sum())
+    assert abs(len(ps) - expect) < 5 * np.sqrt(expect)
+    assert over == int((f > wmax).sum())
+    assert np.all(w > 0)
+    pin = ps[:, 0] + ps[:, 1]
+    pout = ps[:, 2:].sum(axis=1)
+    np.testing.assert_allclose(pout, pin, atol=1e-9 * 3000.0)
+    for leg in (2, 3):
+        m2 = ps[:, leg, 0] ** 2 - np.sum(ps[:, leg, 1:] ** 2, axis=1)
+        np.testing.assert_allclose(m2, 173.0 ** 2, rtol=1e-9)
+    ps2, w2, _ = ig.unweighted_events(n, 7, wmax)
+    np.testing.assert_array_equal(ps2, ps)
+    path = tmp_path / "unweighted_events.lhe.gz"
+    assert write_lhe(path, "gg_ttxgg", ps, sigma, err, 3000.0) == len(ps)
+    _, events = read_lhe(path)
+    assert len(events) == len(ps)

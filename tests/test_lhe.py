@@ -1,0 +1,41 @@
+"""Les Houches output (madflow_amd/lhe.py): the written file reads back with the same
+events, momenta and process information; the layout follows the Les Houches Accord."""
+
+import gzip
+
+import numpy as np
+import pytest
+
+from madflow_amd.lhe import write_lhe, read_lhe, PDG
+from tests.helpers import phase_space_points
+
+
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxgg"])
+def test_lhe_round_trip(tmp_path, name):
+    ps, _ = phase_space_points(name, 50, seed=3)
+    path = tmp_path / ("events_%s.lhe.gz" % name)
+    n = write_lhe(path, name, ps, sigma_pb=12.5, err_pb=0.25, sqrts=ps[0, 0, 0] * 2)
+    assert n == 50
+    with gzip.open(path, "rt") as f:
+        text = f.read()
+    assert text.startswith('<LesHouchesEvents version="3.0">') and text.rstrip().endswith("</LesHouchesEvents>")
+    assert text.count("<event>") == 50 and text.count("</event>") == 50
+    init, events = read_lhe(path)
+    beam, proc = init
+    assert beam[:2] == [21.0, 21.0] and beam[2] == beam[3] == ps[0, 0, 0]
+    assert beam[8] == 3.0                                   # IDWTUP: unweighted events
+    assert proc == [12.5, 0.25, 12.5, 1.0]
+    assert len(events) == 50
+    for ev, (head, parts) in zip(ps, events):
+        assert head[0] == len(PDG[name]) and head[2] == 12.5
+        assert [p[0] for p in parts] == PDG[name]
+        assert [p[1] for p in parts] == [-1, -1] + [1] * (len(PDG[name]) - 2)
+        got = np.array([[p[5], p[2], p[3], p[4]] for p in parts])
+        np.testing.assert_allclose(got, ev, rtol=1e-11, atol=1e-9)
+        masses = [p[6] for p in parts]
+        assert abs(masses[2] - 173.0) < 1e-6 and abs(masses[3] - 173.0) < 1e-6
+
+
+def test_lhe_rejects_wrong_shape(tmp_path):
+    with pytest.raises(ValueError):
+        write_lhe(tmp_path / "x.lhe", "gg_ttxg", np.zeros((3, 4, 4)), 1.0, 0.1, 1000.0)

@@ -102,10 +102,11 @@ int mfx_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint
 /* Phase-space / integrand configuration (plain C struct). */
 typedef struct mfx_ps_params {
   int n_out;            /* outgoing particles (t, t~, gluons) */
-  int ndim;             /* VEGAS dimensions: 4*n_out (RAMBO) or 3*n_out-4 (RAMBO on diet) */
+  int ndim;             /* VEGAS dimensions: 4*n_out (RAMBO), 3*n_out-4 (RAMBO on diet), 4*n_out+2 (hadronic) */
   int nbins;            /* VEGAS bins per dimension (<= 64) */
-  int mapping;          /* 0 RAMBO (phasespace.py:145), 1 RAMBO on diet (minimal dimensions) */
-  double sqrts;         /* fixed partonic sqrt(s) in GeV; incoming partons along +-z */
+  int mapping;          /* 0 RAMBO (phasespace.py:145), 1 RAMBO on diet (minimal dimensions),
+                           2 hadronic: x1, x2 then RAMBO at sqrt(x1 x2 s) (ramboflow, phasespace.py:255-317) */
+  double sqrts;         /* partonic sqrt(s) in GeV (mappings 0, 1) or the hadronic sqrt(s) (mapping 2) */
   double masses[8];     /* outgoing masses */
   double gev2pb;        /* unit conversion applied with the flux 1/(2 s) (phasespace.py:315-317) */
   double pt_min;        /* cuts on outgoing legs >= cut_first_leg: pT > pt_min */
@@ -113,6 +114,10 @@ typedef struct mfx_ps_params {
   double dr_min;        /*   Delta R(i,j) > dr_min between those legs (0 disables) */
   int cut_first_leg;    /* first external leg index the cuts apply to (n_out + 2: no cuts) */
   uint32_t seed;        /* Philox key */
+  double shat_min;      /* mapping 2: lower end of shat (phasespace.py:270-273 uses (sum of masses)^2) */
+  int lab_frame;        /* mapping 2: 1 = momenta boosted to the lab frame (phasespace.py:322-357), 0 = the
+                           partonic c.m. frame (the reference's default, com_output = True) */
+  int reserved;
 } mfx_ps_params;
 
 /* Device workspace needed by mfx_ps_generate / mfx_vegas_accumulate for nev events. */
@@ -120,16 +125,18 @@ int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins);
 
 /* One VEGAS iteration's events: Philox uniforms (counter ev0+e, iteration), or the
  * points y_in (nev, ndim) f32 of an external proposal, -> VEGAS map (grid: device
- * [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520).  w_in (optional, nev)
- * multiplies the weights (e.g. 1/q of the proposal).  Writes the
- * full momenta all_ps (nev, n_out+2, 4), wps = phase-space weight x VEGAS jacobian x
- * gev2pb/(2s) (0 for cut events), flag (1 = pass), slot (dense index or -1) and the
- * passing events' momenta densely into dense (the input of mfx_smatrix, the reference's
- * boolean_mask compaction).  *npass_host receives the number of passing events (this
- * synchronises the stream). */
+ * [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520; in the partonic c.m. frame,
+ * as the reference applies them).  w_in (optional, nev) multiplies the weights (e.g. 1/q
+ * of the proposal).  Writes the full momenta all_ps (nev, n_out+2, 4), wps = phase-space
+ * weight x VEGAS jacobian x gev2pb/(2 shat) (0 for cut events), flag (1 = pass), slot
+ * (dense index or -1) and the passing events' momenta densely into dense (the input of
+ * mfx_smatrix, the reference's boolean_mask compaction).  x12 (optional, nev x 2): the
+ * momentum fractions x1, x2 (1, 1 for the fixed-energy mappings).  *npass_host receives
+ * the number of passing events (this synchronises the stream). */
 int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
                     const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
-                    double* dense, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
+                    double* dense, double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
+                    void* stream);
 
 /* Accumulate one iteration: f = me[slot] * wps (or f_in if given; the reference's
  * scatter_nd back to full length, madflow_exec.py:466-468).  result (device, 6 + ndim*nbins

@@ -31,6 +31,8 @@ struct GenConst {
   RamboConst rc;
   int ndim, nbins, mapping, nexternal;
   double sqrts, norm;              // norm = GeV^-2 -> pb conversion / (2 s)
+  double gev2pb, shat_min;         // hadronic mapping: per-event flux, lower end of shat
+  int lab_frame;
   double pt_min, abs_eta_max, dr_min;
   int cut_first, cut_last;         // outgoing-leg range the gluon cuts apply to
   uint32_t seed;
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
                                                         int64_t nev, uint32_t iteration, const float* __restrict__ y_in,
                                                         const double* __restrict__ w_in, double* __restrict__ all_ps,
                                                         double* __restrict__ wps, uint8_t* __restrict__ flag,
-                                                        int* __restrict__ block_count) {
+                                                        double* __restrict__ x12, int* __restrict__ block_count) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
   bool ok = false;
   if (e < nev) {
@@ -139,18 +141,45 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
       uniforms(g, ev0 + e, iteration, y);
     }
     const double jac = vmap(g, grid, y, x, nullptr);
-    double p[RAMBO_MAXN][4];
-    const double w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p) : diet_event(g.rc, x, g.sqrts, p);
-    ok = y_ok && pass_cuts(g, p) && isfinite(w);
+    double p[RAMBO_MAXN + 2][4];           // incoming partons, then the outgoing legs
+    double w, norm = g.norm, x1 = 1.0, x2 = 1.0, eh = 0.5 * g.sqrts;
+    if (g.mapping == 2) {
+      // ramboflow (phasespace.py:213-232,255-317): tau = x0 (1 - tau_min) + tau_min,
+      // x1 = tau^x1', x2 = tau / x1, weight (1 - tau_min) (-ln tau); RAMBO at
+      // sqrt(shat) = sqrt(x1 x2 s) on the other 4 n_out dimensions; flux 1 / (2 shat)
+      const double s = g.sqrts * g.sqrts, taumin = g.shat_min / s, dtau = 1.0 - taumin;
+      const double tau = x[0] * dtau + taumin;
+      x1 = pow(tau, x[1]);
+      x2 = tau / x1;
+      const double shat = x1 * x2 * s;
+      w = dtau * -log(tau) * rambo_event(g.rc, x + 2, sqrt(shat), p + 2);
+      norm = g.gev2pb / (2.0 * shat);
+      eh = 0.5 * sqrt(shat);
+    } else {
+      w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p + 2) : diet_event(g.rc, x, g.sqrts, p + 2);
+    }
+    p[0][0] = eh; p[0][1] = 0.0; p[0][2] = 0.0; p[0][3] = eh;
+    p[1][0] = eh; p[1][1] = 0.0; p[1][2] = 0.0; p[1][3] = -eh;
+    ok = y_ok && pass_cuts(g, p + 2) && isfinite(w);
+    if (g.mapping == 2 && g.lab_frame) {
+      // boost along z to the lab frame, rapidity -eta = (1/2) ln(x1 / x2) (phasespace.py:322-357)
+      const double eta = -0.5 * log(x1 / x2), ch = cosh(eta), sh = sinh(eta);
+      for (int i = 0; i < g.nexternal; ++i) {
+        const double E = p[i][0], pz = p[i][3];
+        p[i][0] = ch * E - sh * pz;
+        p[i][3] = ch * pz - sh * E;
+      }
+    }
     double* out = all_ps + e * (g.nexternal * 4);
-    const double eh = 0.5 * g.sqrts;
-    out[0] = eh; out[1] = 0.0; out[2] = 0.0; out[3] = eh;
-    out[4] = eh; out[5] = 0.0; out[6] = 0.0; out[7] = -eh;
-    for (int i = 0; i < g.rc.n; ++i)
-      for (int k = 0; k < 4; ++k) out[8 + 4 * i + k] = p[i][k];
+    for (int i = 0; i < g.nexternal; ++i)
+      for (int k = 0; k < 4; ++k) out[4 * i + k] = p[i][k];
+    if (x12) {
+      x12[2 * e] = x1;
+      x12[2 * e + 1] = x2;
+    }
     const double wext = w_in ? w_in[e] : 1.0;
     ok = ok && isfinite(wext);
-    wps[e] = ok ? w * jac * g.norm * wext : 0.0;
+    wps[e] = ok ? w * jac * norm * wext : 0.0;
     flag[e] = ok ? 1 : 0;
   }
   __shared__ int cnt;
@@ -443,8 +472,9 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
   if (!p) return fail(-3, "null parameters%s", "");
   const int n = p->n_out;
   if (n < 2 || n > RAMBO_MAXN) return fail(-1, "n_out must be in [2, 8]%s", "");
-  const int need = p->mapping == 0 ? 4 * n : 3 * n - 4;
-  if (p->ndim != need) return fail(-2, "ndim must be 4*n_out (RAMBO) or 3*n_out-4 (diet)%s", "");
+  if (p->mapping < 0 || p->mapping > 2) return fail(-2, "mapping must be 0 (RAMBO), 1 (diet) or 2 (hadronic)%s", "");
+  const int need = p->mapping == 0 ? 4 * n : p->mapping == 1 ? 3 * n - 4 : 4 * n + 2;
+  if (p->ndim != need) return fail(-2, "ndim must be 4*n_out (RAMBO), 3*n_out-4 (diet) or 4*n_out+2 (hadronic)%s", "");
   if (p->ndim > MAXDIM || p->nbins < 2 || p->nbins > MAXBINS) return fail(-2, "ndim/nbins out of range%s", "");
   memset(&g, 0, sizeof(g));
   g.rc.n = n;
@@ -456,6 +486,8 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
     if (p->masses[i] != 0.0) g.rc.massive = 1;
   }
   if (g.rc.total_mass >= p->sqrts) return fail(-2, "not enough energy for the outgoing masses%s", "");
+  if (p->mapping == 2 && !(p->shat_min >= g.rc.total_mass * g.rc.total_mass && p->shat_min < p->sqrts * p->sqrts))
+    return fail(-2, "hadronic mapping: shat_min must lie in [(sum of masses)^2, s)%s", "");
   g.rc.logw0 = log(MFX_PI / 2.0) * (n - 1) - 2.0 * lgamma((double)(n - 1)) - log((double)(n - 1));
   g.rc.inorm = 1.0 / pow(2.0 * MFX_PI, 3 * n - 4);
   g.ndim = p->ndim;
@@ -464,6 +496,9 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
   g.nexternal = n + 2;
   g.sqrts = p->sqrts;
   g.norm = p->gev2pb / (2.0 * p->sqrts * p->sqrts);
+  g.gev2pb = p->gev2pb;
+  g.shat_min = p->shat_min;
+  g.lab_frame = p->lab_frame;
   g.pt_min = p->pt_min;
   g.abs_eta_max = p->abs_eta_max;
   g.dr_min = p->dr_min;
@@ -491,7 +526,8 @@ int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins) {
 
 int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
                     const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
-                    double* dense, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+                    double* dense, double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
+                    void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -509,7 +545,7 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, y_in, w_in, all_ps,
-                     wps, flag, counts);
+                     wps, flag, x12, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);

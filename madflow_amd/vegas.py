@@ -32,6 +32,7 @@ class PsParams(ctypes.Structure):
         ("sqrts", ctypes.c_double), ("masses", ctypes.c_double * 8), ("gev2pb", ctypes.c_double),
         ("pt_min", ctypes.c_double), ("abs_eta_max", ctypes.c_double), ("dr_min", ctypes.c_double),
         ("cut_first_leg", ctypes.c_int), ("seed", ctypes.c_uint32),
+        ("shat_min", ctypes.c_double), ("lab_frame", ctypes.c_int), ("reserved", ctypes.c_int),
     ]
 
 
@@ -40,8 +41,8 @@ def _sig(lib):
     P = ctypes.POINTER(PsParams)
     lib.mfx_integrand_workspace_size.argtypes = [i64, i, i]
     lib.mfx_integrand_workspace_size.restype = i64
-    lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64),
-                                    vp]
+    lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64,
+                                    ctypes.POINTER(i64), vp]
     lib.mfx_ps_generate.restype = i
     lib.mfx_vegas_accumulate.argtypes = [P, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_vegas_accumulate.restype = i
@@ -63,24 +64,47 @@ def _ptr(t):
 
 
 class MEIntegrand:
-    """sigma = int dPhi |M|^2 / (2 s) * GeV^-2->pb for one process at fixed sqrt(s)."""
+    """sigma = int dPhi |M|^2 / (2 s) * GeV^-2->pb for one process at fixed sqrt(s)
+    (mapping "rambo" or "diet"), or the hadronic
+    sigma = int dx1 dx2 (x1 f(x1) x2 f(x2) / (x1 x2)) dPhi |M|^2 / (2 shat) * GeV^-2->pb
+    at the hadronic sqrt(s) (mapping "hadronic", ramboflow: phasespace.py:255-317; the
+    integrand of madflow_exec.py:422-470).
+
+    Hadronic mode: ``pdf(x, q2)`` returns x f_g(x, Q^2) for device tensors (the gluon of
+    both beams; pdfflow's xfxQ2 in the reference); ``scale`` None is the reference's dynamic
+    scale Q = sum of the outgoing transverse masses / 2 (madflow_exec.py:427-430), a number
+    a fixed Q; ``alphas(q2)`` (optional) gives alpha_s(Q^2) per event, which then sets the
+    couplings of every event, else alpha_s stays frozen.  ``lab_frame`` boosts the
+    momenta to the lab frame (the reference's default is the partonic frame); the cuts
+    act in the partonic frame either way."""
 
     def __init__(self, process, sqrts, precision=64, mapping="rambo", pt_min=0.0, abs_eta_max=0.0, dr_min=0.0,
                  seed=1, nbins=BINS_MAX, alpha_s=0.118, device="cuda", events_limit=1 << 22, ref_compat=True,
-                 width=None):
+                 width=None, pdf=None, scale=None, alphas=None, lab_frame=False, shat_min=None):
         n_out, masses = PROCESS_SETUP[process]
         self.process = process
         self.device = torch.device(device)
         self.matrix, mod = load_matrix(process, precision=precision, ref_compat=ref_compat)
         model = mod.get_model_param()
-        model.freeze_alpha_s(alpha_s)
-        self.params = list(model.evaluate())
+        self.model = model
+        self.hadronic = mapping == "hadronic"
+        if self.hadronic and pdf is None:
+            raise ValueError("the hadronic mapping needs a pdf(x, q2) -> x f(x, Q^2)")
+        self.pdf, self.scale, self.alphas = pdf, scale, alphas
+        if alphas is None:
+            model.freeze_alpha_s(alpha_s)
+            self.params = list(model.evaluate())
+        else:
+            self.params = list(model.evaluate(torch.tensor([float(alpha_s)], dtype=torch.float64)))
         if width is not None:
             self.params[1] = float(width)
+        self.width = width
         p = PsParams()
         p.n_out = n_out
-        p.mapping = 0 if mapping == "rambo" else 1
-        p.ndim = 4 * n_out if p.mapping == 0 else 3 * n_out - 4
+        p.mapping = {"rambo": 0, "diet": 1, "hadronic": 2}[mapping]
+        p.ndim = (4 * n_out, 3 * n_out - 4, 4 * n_out + 2)[p.mapping]
+        p.shat_min = float(sum(masses) ** 2 if shat_min is None else shat_min)
+        p.lab_frame = int(bool(lab_frame))
         p.nbins = nbins
         p.sqrts = float(sqrts)
         for k, m in enumerate(masses):
@@ -119,6 +143,7 @@ class MEIntegrand:
             ws=torch.empty(ws, dtype=torch.uint8, device=d),
             result=torch.empty(6 + self.ndim * self.nbins, dtype=torch.float64, device=d),
             f=torch.empty(n, dtype=torch.float64, device=d),
+            x12=torch.empty((n, 2), dtype=torch.float64, device=d),
         )
         return self._bufs
 
@@ -132,22 +157,30 @@ class MEIntegrand:
         npass = ctypes.c_int64(0)
         _lib.check(self.lib.mfx_ps_generate(
             ctypes.byref(self.p), _ptr(self.grid), ev0, n, iteration, _ptr(y_in), _ptr(w_in), _ptr(b["all_ps"]),
-            _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["ws"]), b["ws"].numel(),
-            ctypes.byref(npass), s))
+            _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(b["x12"]), _ptr(b["ws"]),
+            b["ws"].numel(), ctypes.byref(npass), s))
         npass = npass.value
         self.last_chunk_npass = npass
+        params, f_in = self.params, None
+        if self.hadronic and npass:
+            params, lumi = self._hadronic_factors(b, n, npass)
         if self.me_timer is not None and npass:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
+            me = self.matrix.smatrix(b["dense"][:npass], *params)
             e1.record()
             self.me_timer.append((e0, e1, npass))
         else:
-            me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+            me = self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:0]
+        if self.hadronic:
+            # f = luminosity x |M|^2 x weight (madflow_exec.py:455-459), scattered to full length
+            f_in = torch.zeros(n, dtype=torch.float64, device=self.device)
+            if npass:
+                f_in[b["flag"][:n].bool()] = me * lumi * b["wps"][:n][b["flag"][:n].bool()]
         _lib.check(self.lib.mfx_vegas_accumulate(
-            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
-            _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]),
+            _ptr(f_in), _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
         return b["result"].cpu().numpy().copy(), npass, (b["f"][:n] if keep_f else None)
 
     def unweight(self, w, wmax, iteration, ev0=0):
@@ -216,6 +249,27 @@ class MEIntegrand:
         if keep_f:
             out["f"] = torch.cat(fs)
         return out
+
+    def _hadronic_factors(self, b, n, npass):
+        """Couplings and parton luminosity of the passing events of a chunk: the scale Q
+        (fixed, or the dynamic sum of outgoing transverse masses / 2), alpha_s(Q^2) if a
+        running alpha_s is given, and x1 f(x1) x2 f(x2) / (x1 x2) at Q^2."""
+        d = b["dense"][:npass]
+        x = b["x12"][:n][b["flag"][:n].bool()]
+        if self.scale is None:
+            out = d[:, 2:, :]
+            mt = torch.sqrt(torch.clamp(out[..., 0] ** 2 - out[..., 3] ** 2, min=0.0)).sum(dim=1)
+            q2 = (0.5 * mt) ** 2
+        else:
+            q2 = torch.full((npass,), float(self.scale) ** 2, dtype=torch.float64, device=self.device)
+        params = self.params
+        if self.alphas is not None:
+            params = list(self.model.evaluate(self.alphas(q2).to(torch.float64).reshape(-1)))
+            if self.width is not None:
+                params[1] = float(self.width)
+        x1, x2 = x[:, 0], x[:, 1]
+        lumi = self.pdf(x1, q2) * self.pdf(x2, q2) / (x1 * x2)
+        return params, lumi
 
     def proposal_iteration(self, flow, n_events, iteration, mix=0.2, seed=1234):
         """One iteration of n_events points drawn from the defensive mixture

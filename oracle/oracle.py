@@ -292,13 +292,17 @@ def generate(cfg, edges, ev0, nev, iteration):
     all_ps = np.zeros((nev, n + 2, 4))
     wps = np.zeros(nev)
     flag = np.zeros(nev, dtype=np.uint8)
+    x12 = np.zeros((nev, 2))
     lib().mfo_generate(ctypes.c_int(n), ctypes.c_int(cfg["ndim"]), ctypes.c_int(cfg["nbins"]),
                        ctypes.c_int(cfg["mapping"]), ctypes.c_double(cfg["sqrts"]), _p(masses),
                        ctypes.c_double(cfg["gev2pb"]), ctypes.c_double(cfg["pt_min"]),
                        ctypes.c_double(cfg["abs_eta_max"]), ctypes.c_double(cfg["dr_min"]),
                        ctypes.c_int(cfg["cut_first_leg"]), ctypes.c_uint32(cfg["seed"]), _p(edges),
                        ctypes.c_uint64(ev0), ctypes.c_int64(nev), ctypes.c_uint32(iteration), _p(all_ps), _p(wps),
-                       _p(flag))
+                       _p(flag), ctypes.c_double(cfg.get("shat_min", 0.0)), ctypes.c_int(cfg.get("lab_frame", 0)),
+                       _p(x12))
+    if cfg["mapping"] == 2:
+        return all_ps, wps, flag, x12
     return all_ps, wps, flag
 
 

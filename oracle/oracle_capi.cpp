@@ -254,10 +254,13 @@ int mfo_vegas_refine(double* edges, const double* hist, int ndim, int nbins, dou
 // Oracle of one VEGAS iteration's event generation: same Philox stream, VEGAS map,
 // RAMBO (mapping 0) or RAMBO on diet (mapping 1), cuts on legs cut_first_leg..n_out+1
 // (pT > pt_min, |eta| < abs_eta_max, Delta R > dr_min), wps = w * jac * gev2pb / (2 s).
+// mapping 2 (hadronic) follows ramboflow (phasespace.py:255-317): _get_x1x2 (:213-232) on
+// the first two dimensions, RAMBO at sqrt(shat) on the rest, the flux 1/(2 shat), the cuts
+// in the partonic frame (:503-512) and, with lab_frame, _boost_to_lab (:322-357).
 int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, const double* masses, double gev2pb,
                  double pt_min, double abs_eta_max, double dr_min, int cut_first_leg, uint32_t seed,
                  const double* edges, uint64_t ev0, int64_t nev, uint32_t iteration, double* all_ps, double* wps,
-                 uint8_t* flag) {
+                 uint8_t* flag, double shat_min, int lab_frame, double* x12) {
   VegasGrid g;
   g.ndim = ndim;
   g.nbins = nbins;
@@ -270,14 +273,22 @@ int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, cons
   for (int64_t e = 0; e < nev; ++e) {
     philox_uniforms(ev0 + e, ndim, seed, iteration, y.data());
     double jac = vegas_map(g, y.data(), x.data(), bins.data());
-    double lw;
-    if (mapping == 0) {
+    double lw, roots = sqrts, x1 = 1, x2 = 1, wtau = 1;
+    if (mapping == 2) {
+      const double s = sqrts * sqrts, taumin = shat_min / s, delta_tau = 1 - taumin;
+      const double tau = x[0] * delta_tau + taumin;
+      x1 = std::pow(tau, x[1]);
+      x2 = tau / x1;
+      wtau = delta_tau * -std::log(tau);
+      roots = std::sqrt(x1 * x2 * s);
+      lw = rambo_massless_event(x.data() + 2, n_out, roots, M_PI, p.data());
+    } else if (mapping == 0) {
       lw = rambo_massless_event(x.data(), n_out, sqrts, M_PI, p.data());
     } else {
       lw = diet_massless(x.data(), n_out, sqrts, M_PI, p.data());
     }
-    if (massive) lw += massive_factor(p.data(), n_out, masses, sqrts);
-    double w = std::exp(lw) / std::pow(2 * M_PI, 3 * n_out - 4);
+    if (massive) lw += massive_factor(p.data(), n_out, masses, roots);
+    double w = wtau * std::exp(lw) / std::pow(2 * M_PI, 3 * n_out - 4);
     bool ok = std::isfinite(w);
     for (int i = cut_first_leg - 2; i < n_out; ++i) {
       const double* q = &p[4 * i];
@@ -288,10 +299,23 @@ int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, cons
           if (!(delta_r(q, &p[4 * j]) > dr_min)) ok = false;
     }
     double* o = all_ps + e * next * 4;
-    o[0] = sqrts / 2; o[1] = 0; o[2] = 0; o[3] = sqrts / 2;
-    o[4] = sqrts / 2; o[5] = 0; o[6] = 0; o[7] = -sqrts / 2;
+    o[0] = roots / 2; o[1] = 0; o[2] = 0; o[3] = roots / 2;
+    o[4] = roots / 2; o[5] = 0; o[6] = 0; o[7] = -roots / 2;
     for (int k = 0; k < 4 * n_out; ++k) o[8 + k] = p[k];
-    wps[e] = ok ? w * jac * gev2pb / (2 * sqrts * sqrts) : 0.0;
+    if (mapping == 2 && lab_frame) {
+      // p_lab = p_com . B with B = [[cosh, 0, 0, -sinh], [0,1,0,0], [0,0,1,0], [-sinh, 0, 0, cosh]]
+      const double eta = -0.5 * std::log(x1 / x2), ch = std::cosh(eta), sh = std::sinh(eta);
+      for (int i = 0; i < next; ++i) {
+        const double E = o[4 * i], pz = o[4 * i + 3];
+        o[4 * i] = E * ch + pz * -sh;
+        o[4 * i + 3] = E * -sh + pz * ch;
+      }
+    }
+    if (x12) {
+      x12[2 * e] = x1;
+      x12[2 * e + 1] = x2;
+    }
+    wps[e] = ok ? w * jac * gev2pb / (2 * roots * roots) : 0.0;
     flag[e] = ok ? 1 : 0;
   }
   return 0;

@@ -11,6 +11,12 @@ import torch
 from oracle import oracle as O
 from madflow_amd.vegas import MEIntegrand, VegasFlow, GEV2PB
 
+
+def _toy_pdf(x, q2):
+    """A gluon-like x g(x) for tests (no PDF grid is available): 1.7 x^-0.2 (1 - x)^5."""
+    return 1.7 * x ** -0.2 * (1.0 - x) ** 5
+
+
 CASES = [
     ("gg_ttxg", dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5), "rambo"),
     ("gg_ttxgg", dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4), "rambo"),
@@ -18,6 +24,8 @@ CASES = [
     ("gg_ttx", dict(sqrts=1000.0), "diet"),
     ("gg_ttxggg", dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4), "diet"),
     ("gg_ttxggg", dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4), "rambo"),
+    ("gg_ttx", dict(sqrts=13000.0, pdf=_toy_pdf), "hadronic"),
+    ("gg_ttxgg", dict(sqrts=13000.0, pt_min=30.0, dr_min=0.4, pdf=_toy_pdf, lab_frame=True), "hadronic"),
 ]
 
 
@@ -25,7 +33,9 @@ def _cfg(ig):
     p = ig.p
     return dict(n_out=p.n_out, ndim=p.ndim, nbins=p.nbins, mapping=p.mapping, sqrts=p.sqrts,
                 masses=list(p.masses), gev2pb=p.gev2pb, pt_min=p.pt_min, abs_eta_max=p.abs_eta_max,
-                dr_min=p.dr_min, cut_first_leg=p.cut_first_leg, seed=p.seed)
+                dr_min=p.dr_min, cut_first_leg=p.cut_first_leg, seed=p.seed, shat_min=p.shat_min,
+                lab_frame=p.lab_frame)
+
 
 
 def _warp_grid(ig, rng):
@@ -46,7 +56,10 @@ def test_generate_parity(dev, name, kw, mapping):
     ig._alloc(n)
     ig.chunk(ev0, n, it)
     b = ig._bufs
-    ps_ref, wps_ref, flag_ref = O.generate(_cfg(ig), edges, ev0, n, it)
+    ref = O.generate(_cfg(ig), edges, ev0, n, it)
+    ps_ref, wps_ref, flag_ref = ref[:3]
+    if mapping == "hadronic":
+        np.testing.assert_allclose(b["x12"][:n].cpu().numpy(), ref[3], rtol=1e-13)
     flag = b["flag"][:n].cpu().numpy()
     assert np.array_equal(flag, flag_ref)
     assert np.max(np.abs(b["all_ps"][:n].cpu().numpy() - ps_ref)) < 1e-9 * kw["sqrts"]
@@ -212,3 +225,38 @@ def test_unweighted_events_to_lhe(dev, tmp_path):
     assert write_lhe(path, "gg_ttxgg", ps, sigma, err, 3000.0) == len(ps)
     _, events = read_lhe(path)
     assert len(events) == len(ps)
+
+
+@pytest.mark.gpu
+def test_hadronic_ttbar_cross_section(dev):
+    """Hadronic mode (ramboflow x1, x2 sampling, per-event flux, parton luminosity
+    x1 g(x1) x2 g(x2) / (x1 x2)): pp -> gg -> tt~ at sqrt(s) = 13 TeV with a toy gluon
+    density and Gamma_t = 0 must reproduce the direct 2D integral of the closed-form
+    partonic cross section, int dx1 dx2 g(x1) g(x2) sigma_hat(x1 x2 s), within 3 sigma."""
+    from scipy import integrate
+    print("case hadronic sigma ttbar", flush=True)
+    ig = MEIntegrand("gg_ttx", precision=64, mapping="hadronic", seed=2, sqrts=13000.0, ref_compat=False,
+                     width=0.0, pdf=_toy_pdf, scale=173.0)
+    assert ig.ndim == 10
+    vf = VegasFlow(ig.ndim, 1000000)
+    vf.compile(ig)
+    vf.run_integration(3)
+    vf.freeze_grid()
+    res, err = vf.run_integration(5)
+    m, a, s = 173.0, 0.118, 13000.0 ** 2
+
+    def sig_hat(sh):
+        rho = 4 * m * m / sh
+        b = math.sqrt(max(1 - rho, 0.0))
+        return math.pi * a * a / (3 * sh) * ((1 + rho + rho * rho / 16) * math.log((1 + b) / (1 - b))
+                                            - b * (7 / 4 + 31 * rho / 16)) * GEV2PB
+
+    t = 4 * m * m / s
+    g = lambda x: _toy_pdf(x, 0.0) / x
+    # substitute x = exp(u) to tame the small-x growth of g
+    direct, derr = integrate.dblquad(
+        lambda u2, u1: g(math.exp(u1)) * g(math.exp(u2)) * math.exp(u1 + u2) * sig_hat(math.exp(u1 + u2) * s),
+        math.log(t), 0.0, lambda u1: math.log(t) - u1, lambda u1: 0.0, epsabs=0, epsrel=1e-7)
+    print("hadronic sigma", res, "+-", err, "direct", direct, flush=True)
+    assert abs(res - direct) < 3 * err + 1e-6 * direct
+    assert err / res < 1e-2

@@ -100,3 +100,56 @@ def test_cut_variables():
     q = np.array([10.0, -3.0, 4.0, -5.0])
     dphi = abs(math.atan2(4, 3) - math.atan2(4, -3))
     assert abs(O.delta_r(p, q) - math.hypot(2 * O.eta(p), dphi)) < 1e-12
+
+
+def _hadronic_cfg(lab_frame, shat_min=346.0 ** 2, n_out=2, masses=(173.0, 173.0)):
+    return dict(n_out=n_out, ndim=4 * n_out + 2, nbins=50, mapping=2, sqrts=13000.0,
+                masses=list(masses), gev2pb=389379365.6, pt_min=0.0, abs_eta_max=0.0, dr_min=0.0,
+                cut_first_leg=n_out + 2, seed=5, shat_min=shat_min, lab_frame=lab_frame)
+
+
+def test_hadronic_tau_jacobian_closed_form():
+    """ramboflow's (x1, x2) sampling (phasespace.py:213-232): the weight (1 - tau_min)
+    (-ln tau) times a function of (x1, x2) averages to its integral over x1 x2 > tau_min.
+    For massless 2-body RAMBO the phase-space weight is the constant 1/(8 pi), so
+    wps 2 shat 8 pi / gev2pb isolates the (x1, x2) weight; its mean must be the area
+    1 - tau_min + tau_min ln tau_min, and its x1-weighted mean the first moment."""
+    cfg = _hadronic_cfg(0, shat_min=0.1 * 13000.0 ** 2, masses=(0.0, 0.0))
+    n = 400000
+    edges = np.tile(np.linspace(0, 1, 51), (cfg["ndim"], 1))
+    ps, wps, flag, x12 = O.generate(cfg, edges, 0, n, 1)
+    s = cfg["sqrts"] ** 2
+    shat = x12[:, 0] * x12[:, 1] * s
+    wt = wps * 2 * shat * 8 * math.pi / cfg["gev2pb"]
+    t = cfg["shat_min"] / s
+    assert np.all(x12[:, 0] * x12[:, 1] >= t * (1 - 1e-12)) and np.all(x12 <= 1)
+    area = 1 - t + t * math.log(t)
+    assert abs(wt.mean() - area) < 4 * wt.std() / math.sqrt(n)
+    first = 0.5 * (1 - t) ** 2                       # = int_t^1 x1 (1 - t / x1) dx1
+    m1 = (wt * x12[:, 0]).mean()
+    assert abs(m1 - first) < 4 * (wt * x12[:, 0]).std() / math.sqrt(n)
+
+
+@pytest.mark.parametrize("n_out,masses", [(2, (173.0, 173.0)), (3, (173.0, 173.0, 0.0))])
+def test_hadronic_lab_frame(n_out, masses):
+    """_boost_to_lab (phasespace.py:322-357): the lab-frame partons carry x1 sqrt(s)/2 and
+    x2 sqrt(s)/2 along +-z, momentum is conserved, invariant masses do not change, and the
+    partonic-frame output is the same event before the boost."""
+    edges = np.tile(np.linspace(0, 1, 51), (4 * n_out + 2, 1))
+    com = O.generate(_hadronic_cfg(0, n_out=n_out, masses=masses), edges, 0, 2000, 3)
+    lab = O.generate(_hadronic_cfg(1, n_out=n_out, masses=masses), edges, 0, 2000, 3)
+    np.testing.assert_array_equal(com[3], lab[3])
+    np.testing.assert_array_equal(com[1], lab[1])
+    x1, x2 = lab[3][:, 0], lab[3][:, 1]
+    e = 6500.0
+    np.testing.assert_allclose(lab[0][:, 0, 0], x1 * e, rtol=1e-12)
+    np.testing.assert_allclose(lab[0][:, 0, 3], x1 * e, rtol=1e-12)
+    np.testing.assert_allclose(lab[0][:, 1, 0], x2 * e, rtol=1e-12)
+    np.testing.assert_allclose(lab[0][:, 1, 3], -x2 * e, rtol=1e-12)
+    p_in = lab[0][:, 0] + lab[0][:, 1]
+    np.testing.assert_allclose(lab[0][:, 2:].sum(axis=1), p_in, atol=1e-8 * 13000)
+    for i in range(n_out + 2):
+        np.testing.assert_allclose(minkowski(lab[0][:, i], lab[0][:, i]), minkowski(com[0][:, i], com[0][:, i]),
+                                   atol=1e-7 * 13000 ** 2 / 1e4)
+    # partonic frame: symmetric partons of energy sqrt(shat)/2
+    np.testing.assert_allclose(com[0][:, 0, 0], 0.5 * np.sqrt(x1 * x2) * 13000.0, rtol=1e-12)

@@ -71,6 +71,56 @@ def cpu_baseline(proc, dense, npass, params, budget_s=12.0):
                       "(C++, OpenMP over events), %.1f s" % (n, LABEL[proc], t)}
 
 
+def cpu_c1_end_to_end(cores, n=1_000_000, warmup=2, iters=5, target=1e-3):
+    """The survey's C1 run on the CPU oracle: gg -> tt~ at sqrt(s) = 1 TeV, RAMBO on the
+    8-dim VEGAS cube (50 bins), FP64, n points x (warmup + iters) iterations: oracle
+    Philox -> map -> RAMBO (one thread), oracle smatrix (OpenMP over events), numpy sums
+    and f^2 histograms, oracle grid refinement.  Returns sigma, ME evals/s and unweighted
+    events/s to the target with the warm-up counted (N_unw = sum w / max w)."""
+    import numpy as np
+    from oracle import oracle as O
+    cfg = dict(n_out=2, ndim=8, nbins=50, mapping=0, sqrts=1000.0, masses=[173.0, 173.0], gev2pb=389379365.6,
+               pt_min=0.0, abs_eta_max=0.0, dr_min=0.0, cut_first_leg=4, seed=1)
+    edges = np.tile(np.linspace(0.0, 1.0, 51), (8, 1))
+    rp, coup = O.default_params(ref_gs=True)
+    res, n_me, t_me = [], 0, 0.0
+    t0 = time.perf_counter()
+    t_setup = None
+    for it in range(warmup + iters):
+        ps, wps, flag = O.generate(cfg, edges, 0, n, it)
+        me = np.zeros(n)
+        ok = flag == 1
+        t1 = time.perf_counter()
+        me[ok] = O.smatrix("gg_ttx", ps[ok], rp, coup, ref_compat=True, nthreads=cores)
+        t_me += time.perf_counter() - t1
+        n_me += int(ok.sum())
+        f = me * wps
+        mean = f.mean()
+        var = max((np.mean(f * f) - mean * mean) / (n - 1), 0.0)
+        if it < warmup:
+            y = O.philox_uniforms(n, 8, cfg["seed"], it)
+            bins = np.minimum((y * 50).astype(int), 49)
+            hist = np.stack([np.bincount(bins[:, d], weights=f * f, minlength=50) for d in range(8)])
+            edges = O.vegas_refine(edges, hist, alpha=1.5)
+            if it == warmup - 1:
+                t_setup = time.perf_counter() - t0
+        else:
+            res.append(dict(I=mean, var=var, sum_f=float(f.sum()), wmax=float(f.max()), n=n))
+            from madflow_amd.vegas import combine
+            sig, err = combine(res)
+            if err / sig <= target:
+                break
+    wall = time.perf_counter() - t0
+    from madflow_amd.vegas import combine
+    sig, err = combine(res)
+    n_unw = sum(r["sum_f"] for r in res) / max(r["wmax"] for r in res)
+    return {"workload": "gg->tt, sqrt(s)=1 TeV, RAMBO 8-dim VEGAS cube, 50 bins, FP64, %d points/iteration, "
+                        "%d warm-up + %d iterations" % (n, warmup, len(res)),
+            "sigma_pb": sig, "err_pb": err, "rel_err": err / sig, "reached_target": err / sig <= target,
+            "seconds": wall, "setup_s": t_setup, "me_evals_per_s": n_me / t_me,
+            "unweighted_events": n_unw, "unweighted_per_s_to_target": n_unw / wall}
+
+
 def _unweighted(res, setup_s, sample_s, target):
     """sigma, error, N_unw = sum w / max w, and unweighted events per second: steady (the
     sample alone) and to the target (set-up counted; if the sample stopped short of the
@@ -432,6 +482,7 @@ def main(argv=None):
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
         res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass, m["ig"].params)
+        res["cpu_baseline"]["c1_end_to_end"] = cpu_c1_end_to_end(res["cpu_baseline"]["cores"])
     if not a.no_flow:
         fc = run_flow_compare(a, torch, m)
         res["flow_comparison"] = fc

@@ -260,3 +260,27 @@ def test_hadronic_ttbar_cross_section(dev):
     print("hadronic sigma", res, "+-", err, "direct", direct, flush=True)
     assert abs(res - direct) < 3 * err + 1e-6 * direct
     assert err / res < 1e-2
+
+
+@pytest.mark.gpu
+def test_hadronic_dynamic_scale_and_running_alphas(dev):
+    """The dynamic scale Q = sum of outgoing m_T / 2 (madflow_exec.py:427-430) reaches the
+    couplings of each event: with alpha_s(Q^2) = 0.118 (Q / 173 GeV)^-0.2 every event's
+    result equals the frozen-coupling result scaled by (alpha_s(Q^2) / 0.118)^2, the power
+    of alpha_s in gg -> tt~, with Q computed independently from the momenta."""
+    print("case hadronic running alpha_s", flush=True)
+    kw = dict(precision=64, mapping="hadronic", seed=4, sqrts=13000.0, ref_compat=False, pdf=_toy_pdf)
+    run = MEIntegrand("gg_ttx", alphas=lambda q2: 0.118 * (q2 / 173.0 ** 2) ** -0.1, **kw)
+    frozen = MEIntegrand("gg_ttx", **kw)
+    n = 20000
+    _, _, f_run = run.chunk(0, n, 0, keep_f=True)
+    f_run = f_run.cpu().numpy().copy()
+    _, _, f_fix = frozen.chunk(0, n, 0, keep_f=True)
+    f_fix = f_fix.cpu().numpy()
+    ps = frozen._bufs["all_ps"][:n].cpu().numpy()
+    mt = np.sqrt(np.maximum(ps[:, 2:, 0] ** 2 - ps[:, 2:, 3] ** 2, 0.0)).sum(axis=1)
+    q2 = (0.5 * mt) ** 2
+    ratio = (0.118 * (q2 / 173.0 ** 2) ** -0.1 / 0.118) ** 2
+    nz = f_fix > 0
+    assert nz.sum() > 0.9 * n
+    np.testing.assert_allclose(f_run[nz], f_fix[nz] * ratio[nz], rtol=1e-12)

@@ -90,3 +90,22 @@ def test_helicity_sum():
     rp, coup = O.default_params()
     tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in d["helicities"]) / d["denominator"]
     np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), tot, rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg"])
+def test_no_helicity_row_vanishes(name):
+    """Helicity filtering (dropping rows whose |M|^2 is identically zero, as MadGraph does)
+    would gain nothing here: with massive tops every helicity row contributes at random
+    phase-space points."""
+    import json
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "madflow_amd", "processes", name + ".json")) as fh:
+        hels = json.load(fh)["helicities"]
+    ps, _ = phase_space_points(name, 8, seed=5)
+    rp, coup = O.default_params(ref_gs=True)
+    tot = O.smatrix(name, ps, rp, coup, ref_compat=True)
+    for h in hels:
+        v = O.matrix_hel(name, ps, np.array(h, dtype=np.int32), rp, coup, ref_compat=True)
+        v = v[0] if isinstance(v, tuple) else v
+        assert np.max(np.abs(v) / tot) > 1e-6

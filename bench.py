@@ -47,7 +47,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(proc, dense, npass, params, budget_s=12.0):
+def cpu_baseline(proc, dense, npass, params, names, budget_s=12.0):
     """Time the CPU oracle (FP64 C++, OpenMP over events) on the first passing events of
     the last chunk: doubles the sample until one timed call takes budget_s / 3."""
     import numpy as np
@@ -56,8 +56,7 @@ def cpu_baseline(proc, dense, npass, params, budget_s=12.0):
     nmax = min(npass, 1 << 20)
     ps = dense[:nmax].cpu().numpy()
     rp = np.array([float(params[0]), float(params[1])], dtype=np.float64)
-    c = [complex(x.item()) if hasattr(x, "item") else complex(x) for x in params[2:5]]
-    coup = {"GC_10": c[0], "GC_11": c[1], "GC_12": c[2]}
+    coup = {nm: complex(x.item()) if hasattr(x, "item") else complex(x) for nm, x in zip(names[2:], params[2:])}
     n = 256
     while True:
         t0 = time.perf_counter()
@@ -481,7 +480,8 @@ def main(argv=None):
     }
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass, m["ig"].params)
+        res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass, m["ig"].params,
+                                           list(m["ig"].matrix.params))
         res["cpu_baseline"]["c1_end_to_end"] = cpu_c1_end_to_end(res["cpu_baseline"]["cores"])
     if not a.no_flow:
         fc = run_flow_compare(a, torch, m)

@@ -15,7 +15,11 @@ Parity status of each oracle function (how it is pinned):
 * g g > t t~ + 1..3 g: colour-dressed brute force over explicit colour indices,
   gauge (BRST) invariance, Lorentz invariance and Bose symmetry — tests/test_oracle_multijet.py
 * RAMBO: the reference's closed-form massless volume and on-shell checks — tests/test_oracle_phasespace.py
+* hadronic mapping (ramboflow x1/x2, _boost_to_lab): closed forms of the (x1, x2) weight's
+  moments and the lab-frame parton momenta — tests/test_oracle_phasespace.py
 * VEGAS: closed-form integrals and grid invariants — tests/test_oracle_vegas.py
+* unweighting: the kept count against sum min(1, w/wmax), over-weight and zero-weight
+  events — tests/test_oracle_vegas.py
 """
 
 import ctypes

@@ -78,6 +78,7 @@ def cpu_c1_end_to_end(cores, n=1_000_000, warmup=2, iters=5, target=1e-3):
     events/s to the target with the warm-up counted (N_unw = sum w / max w)."""
     import numpy as np
     from oracle import oracle as O
+    from madflow_amd.vegas import combine
     cfg = dict(n_out=2, ndim=8, nbins=50, mapping=0, sqrts=1000.0, masses=[173.0, 173.0], gev2pb=389379365.6,
                pt_min=0.0, abs_eta_max=0.0, dr_min=0.0, cut_first_leg=4, seed=1)
     edges = np.tile(np.linspace(0.0, 1.0, 51), (8, 1))
@@ -105,12 +106,10 @@ def cpu_c1_end_to_end(cores, n=1_000_000, warmup=2, iters=5, target=1e-3):
                 t_setup = time.perf_counter() - t0
         else:
             res.append(dict(I=mean, var=var, sum_f=float(f.sum()), wmax=float(f.max()), n=n))
-            from madflow_amd.vegas import combine
             sig, err = combine(res)
             if err / sig <= target:
                 break
     wall = time.perf_counter() - t0
-    from madflow_amd.vegas import combine
     sig, err = combine(res)
     n_unw = sum(r["sum_f"] for r in res) / max(r["wmax"] for r in res)
     return {"workload": "gg->tt, sqrt(s)=1 TeV, RAMBO 8-dim VEGAS cube, 50 bins, FP64, %d points/iteration, "

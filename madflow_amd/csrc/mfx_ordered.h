@@ -175,7 +175,12 @@ struct OrderedStage {
   static constexpr int NG = T::NG;
   static constexpr int NWF = NG * 12;                  // 5 gluons x 6 complex
   static constexpr int NINV = StableForms<R>::value ? (NG + 1) * NG / 2 : 0;   // gluon-gluon + top-gluon
-  static constexpr int NSLOT = NWF + NINV;
+  // FP32: the top and anti-top spinors of both helicities (momentum slots once, spinor
+  // components per helicity), staged once per event; FP64 recomputes them (its LDS
+  // budget is spent)
+  static constexpr int FB = NWF + NINV;
+  static constexpr int NFS = StableForms<R>::value ? 40 : 0;
+  static constexpr int NSLOT = NWF + NINV + NFS;
 };
 
 template <class T, class R, int BLOCK>
@@ -209,6 +214,34 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         lds[(S::NWF + T::pair_index(x, y)) * BLOCK + t] = R(v);
       }
   }
+  if constexpr (S::NFS > 0) {
+    const int* hv = a.s.use_helv ? a.s.helv : nullptr;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      W<R> top, atop;
+      T::template fermions<R>(p, ((unsigned)h << T::TOP_LEG) | ((unsigned)h << T::ATOP_LEG), hv, prm, top, atop);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int ot = q < 2 ? 2 * q : 8 + 8 * h + 2 * (q - 2);       // top
+        const int oa = q < 2 ? 4 + 2 * q : 24 + 8 * h + 2 * (q - 2);  // anti-top
+        if (q >= 2 || h == 0) {
+          lds[(S::FB + ot) * BLOCK + t] = top[q].re;
+          lds[(S::FB + ot + 1) * BLOCK + t] = top[q].im;
+          lds[(S::FB + oa) * BLOCK + t] = atop[q].re;
+          lds[(S::FB + oa + 1) * BLOCK + t] = atop[q].im;
+        }
+      }
+    }
+  }
+  // spinor of the top (anti = 0) or anti-top (anti = 1) at helicity bit h, from LDS
+  auto staged_fermion = [&](int anti, int h, W<R>& w) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int o = q < 2 ? 4 * anti + 2 * q : 8 + 16 * anti + 8 * h + 2 * (q - 2);
+      w[q].re = lds[(S::FB + o) * BLOCK + t];
+      w[q].im = lds[(S::FB + o + 1) * BLOCK + t];
+    }
+  };
   cx<R>* jb = reinterpret_cast<cx<R>*>(a.jbuf);
   // Helicity rows are enumerated here as r = 4 g + 2 ht + ha: g = the gluon helicities
   // (bit s = gluon slot s), ht / ha = the top / anti-top helicity.  The gluon currents of
@@ -274,15 +307,17 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         const int rb = 4 * g + 2 * ht;
         if (rb + 1 < r0 || rb >= r1) continue;
         W<R> top, atop;
-        T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
+        if constexpr (S::NFS > 0) staged_fermion(0, ht, top);
+        else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
         O[0] = top;
         ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
 #pragma unroll 1
         for (int ha = 0; ha < 2; ++ha) {
           const int r = rb + ha;
           if (r < r0 || r >= r1) continue;
-          if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv, prm,
-                                          top, atop);
+          if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
+          else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
+                                               prm, top, atop);
           const cx<R> amp = ordered_close<NG, R>(atop, O, J, prm);
           jb[((int64_t)(r - r0) * NC + c) * a.n + el] = amp;
         }

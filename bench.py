@@ -39,7 +39,7 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 packed, FP64)
 # HBM bytes per matrix element of k_smatrix<gg_ttxgg, float>: rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md)
-HBM_BYTES_PER_ME = {32: 6184.3}
+HBM_BYTES_PER_ME = {32: 6058.9}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 
 

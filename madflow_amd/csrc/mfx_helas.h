@@ -245,6 +245,39 @@ __device__ __forceinline__ cx<R> FFV1_0(const W<R>& F1, const W<R>& F2, const W<
   }
 }
 
+// FFV1_0 of two amplitudes that differ only in F2 (two helicity rows of one pass): the
+// vertex is linear in F2, so the row vector L = F1 gamma.V is formed once and dotted with
+// both spinors, TMP0 = sum_k L_k F2[k] (the same sum as FFV1_0, regrouped).
+template <class R>
+__device__ __forceinline__ void FFV1_0_pair2(const W<R>& F1, const W<R>& F2a, const W<R>& F2b, const W<R>& V3,
+                                             cx<R> COUP, cx<R>& ampa, cx<R>& ampb) {
+  const LC<R> v = lightcone(V3);
+  const cx<R> L2 = F1[4] * v.b - F1[5] * v.d, L3 = F1[5] * v.a - F1[4] * v.c;
+  const cx<R> L4 = F1[2] * v.a + F1[3] * v.d, L5 = F1[2] * v.c + F1[3] * v.b;
+  ampa = mul_mi(COUP * (L2 * F2a[2] + L3 * F2a[3] + L4 * F2a[4] + L5 * F2a[5]));
+  ampb = mul_mi(COUP * (L2 * F2b[2] + L3 * F2b[3] + L4 * F2b[4] + L5 * F2b[5]));
+}
+// The same for two amplitudes that differ only in F1: the column vector gamma.V F2 is shared.
+template <class R>
+__device__ __forceinline__ void FFV1_0_pair1(const W<R>& F1a, const W<R>& F1b, const W<R>& F2, const W<R>& V3,
+                                             cx<R> COUP, cx<R>& ampa, cx<R>& ampb) {
+  const LC<R> v = lightcone(V3);
+  const cx<R> R2 = F2[4] * v.a + F2[5] * v.c, R3 = F2[4] * v.d + F2[5] * v.b;
+  const cx<R> R4 = F2[2] * v.b - F2[3] * v.c, R5 = F2[3] * v.a - F2[2] * v.d;
+  ampa = mul_mi(COUP * (F1a[2] * R2 + F1a[3] * R3 + F1a[4] * R4 + F1a[5] * R5));
+  ampb = mul_mi(COUP * (F1b[2] * R2 + F1b[3] * R3 + F1b[4] * R4 + F1b[5] * R5));
+}
+
+// The same for two amplitudes that differ only in V: the fermion bilinear is shared.
+template <class R>
+__device__ __forceinline__ void FFV1_0_pairV(const W<R>& F1, const W<R>& F2, const W<R>& Va, const W<R>& Vb,
+                                             cx<R> COUP, cx<R>& ampa, cx<R>& ampb) {
+  const Bilinear<R> j = bilinear(F1, F2);
+  const LC<R> a = lightcone(Va), b = lightcone(Vb);
+  ampa = mul_mi(COUP * (j.a * a.a + j.b * a.b + j.c * a.c + j.d * a.d));
+  ampb = mul_mi(COUP * (j.a * b.a + j.b * b.b + j.c * b.c + j.d * b.d));
+}
+
 // ------------------------------------------------------------------------ FFV1_1
 // P: the off-shell fermion's momentum (minus the sum of the input slots); the slots of F1
 // are not written (callers that pass P from the external momenta never read them).

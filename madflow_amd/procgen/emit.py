@@ -176,10 +176,67 @@ def _scaled(expr, re, im):
     return ("mul_i(%s)" if im > 0 else "mul_mi(%s)") % expr
 
 
+def _pair_side(a, b):
+    """For an amplitude and its next-row twin: 1, 2 or 3 if they differ only in F1, only in
+    F2 or only in the vector current(s) (then one FFV1_0_pair call evaluates both), else 0."""
+    if a["kind"] != "amp" or b["kind"] != "amp" or a.get("jset") or b.get("jset") != 1:
+        return 0
+    if a["routine"] != "FFV1_0" or b["routine"] != "FFV1_0":
+        return 0
+    if a["factors"] != b["factors"] or a["norm"] != b["norm"] or a["coup"] != b["coup"]:
+        return 0
+    d1, d2, dv = a["args"][0] != b["args"][0], a["args"][1] != b["args"][1], a["args"][2:] != b["args"][2:]
+    if d1 + d2 + dv != 1:
+        return 0
+    return 1 if d1 else 2 if d2 else 3
+
+
 def _emit_ops(w, prog, desc, legs_of):
-    """Emit the statements of an amplitude program (wavefunction ops and amplitudes)."""
+    """Emit the statements of an amplitude program (wavefunction ops and amplitudes).  An
+    amplitude followed by its twin for the next helicity row of a pass, differing only in
+    one fermion, is emitted as one FFV1_0_pair call."""
     n = desc["nexternal"]
-    for op in prog:
+    skip = set()
+    for i, op in enumerate(prog):
+        if i in skip:
+            continue
+        side = _pair_side(op, prog[i + 1]) if i + 1 < len(prog) else 0
+        if side:
+            skip.add(i + 1)
+            tw = prog[i + 1]
+            coup = "prm.%s" % op["coup"]
+            f1, f2, vs = op["args"][0], op["args"][1], op["args"][2:]
+            w("  { cx<R> amp0, amp1;")
+            if len(vs) == 1:
+                (r0, i0), = op["factors"]
+                va, vb = "w%d" % vs[0], "w%d" % tw["args"][2]
+                post = "    const cx<R> amp = %s, ampt = %s;" % (_scaled("amp0", r0, i0), _scaled("amp1", r0, i0))
+            else:
+                for nm, vv in (("vs", vs), ("vst", tw["args"][2:])):
+                    if nm == "vst" and side != 3:
+                        continue
+                    w("    W<R> %s;" % nm)
+                    w("    #pragma unroll")
+                    w("    for (int q = 2; q < 6; ++q) %s[q] = %s;" % (nm, " + ".join(
+                        _scaled("w%d[q]" % v, r, i_) for v, (r, i_) in zip(vv, op["factors"]))))
+                va, vb = "vs", "vst"
+                post = "    const cx<R> amp = amp0, ampt = amp1;"
+            if side == 2:
+                w("    FFV1_0_pair2(w%d, w%d, w%d, %s, %s, amp0, amp1);" % (f1, f2, tw["args"][1], va, coup))
+            elif side == 1:
+                w("    FFV1_0_pair1(w%d, w%d, w%d, %s, %s, amp0, amp1);" % (f1, tw["args"][0], f2, va, coup))
+            else:
+                w("    FFV1_0_pairV(w%d, w%d, %s, %s, %s, amp0, amp1);" % (f1, f2, va, vb, coup))
+            w(post)
+            for jn, a in (("jamp", "amp"), ("jamp2", "ampt")):
+                for (c, re, im) in op["norm"]:
+                    if im == 0:
+                        w("    %s[%d] %s= %s;" % (jn, c, "+" if re > 0 else "-", a))
+                    else:
+                        w("    %s[%d] %s= mul_i(%s);" % (jn, c, "+" if im > 0 else "-", a))
+            w("  }")
+            w("  __builtin_amdgcn_sched_barrier(0);")
+            continue
         args = ", ".join("w%d" % a for a in op["args"])
         coup = "prm.%s" % op["coup"]
         if op["kind"] == "wf":

@@ -159,6 +159,25 @@ __device__ __forceinline__ cx<R> ordered_close(const W<R>& atop, const W<R> (&O)
   return cx<R>(-amp.re, -amp.im);
 }
 
+// The closing vertex for both anti-top helicities at once: it is linear in the anti-top
+// spinor, so the column vectors gamma.J(i..NG-1) O(i) are shared (FFV1_0_pair1).
+template <int NG, class R>
+__device__ __forceinline__ void ordered_close_pair(const W<R>& atop0, const W<R>& atop1, const W<R> (&O)[NG],
+                                                   const W<R> (&J)[NG][NG], const Params<R>& prm, cx<R>& amp0,
+                                                   cx<R>& amp1) {
+  amp0 = cx<R>(R(0), R(0));
+  amp1 = cx<R>(R(0), R(0));
+#pragma unroll
+  for (int i = 0; i < NG; ++i) {
+    cx<R> a, b;
+    FFV1_0_pair1(atop0, atop1, O[i], J[i][NG - 1], prm.GC_11, a, b);
+    amp0 += a;
+    amp1 += b;
+  }
+  amp0 = cx<R>(-amp0.re, -amp0.im);
+  amp1 = cx<R>(-amp1.re, -amp1.im);
+}
+
 struct OrderedArgs {
   SmatrixArgs s;
   int64_t ev0;          // first event of this sub-chunk
@@ -311,6 +330,19 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
         O[0] = top;
         ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
+        if constexpr (S::NFS > 0) {
+          if (rb >= r0 && rb + 1 < r1) {
+            // both anti-top rows of this (g, ht) in the launch: one paired closing vertex
+            W<R> atop1;
+            staged_fermion(1, 0, atop);
+            staged_fermion(1, 1, atop1);
+            cx<R> amp0, amp1;
+            ordered_close_pair<NG, R>(atop, atop1, O, J, prm, amp0, amp1);
+            jb[((int64_t)(rb - r0) * NC + c) * a.n + el] = amp0;
+            jb[((int64_t)(rb + 1 - r0) * NC + c) * a.n + el] = amp1;
+            continue;
+          }
+        }
 #pragma unroll 1
         for (int ha = 0; ha < 2; ++ha) {
           const int r = rb + ha;

@@ -278,6 +278,47 @@ __device__ __forceinline__ void FFV1_0_pairV(const W<R>& F1, const W<R>& F2, con
   ampb = mul_mi(COUP * (j.a * b.a + j.b * b.b + j.c * b.c + j.d * b.d));
 }
 
+// gamma.V F2 as the column (c2, c3, c4, c5) that FFV1_0 dots with F1 (the shared vector of
+// FFV1_0_pair1) and that FFV1_1 multiplies by its propagator numerator.  A sum of
+// FFV1_0 or FFV1_1 terms over different (F2, V) pairs with the same F1, or with the same
+// off-shell momentum, is linear in this column: summing the columns first makes the
+// dot product, or the propagator, a single step.
+template <class R> struct VslashF { cx<R> c2, c3, c4, c5; };
+template <class R>
+__device__ __forceinline__ VslashF<R> vslash_f(const W<R>& F2, const W<R>& V3) {
+  const LC<R> v = lightcone(V3);
+  VslashF<R> x;
+  x.c2 = F2[4] * v.a + F2[5] * v.c;
+  x.c3 = F2[4] * v.d + F2[5] * v.b;
+  x.c4 = F2[2] * v.b - F2[3] * v.c;
+  x.c5 = F2[3] * v.a - F2[2] * v.d;
+  return x;
+}
+template <class R> __device__ __forceinline__ void add_col(VslashF<R>& acc, const VslashF<R>& x) {
+  acc.c2 += x.c2;
+  acc.c3 += x.c3;
+  acc.c4 += x.c4;
+  acc.c5 += x.c5;
+}
+// FFV1_0(F1, F2, V3) summed over the (F2, V3) pairs whose columns make up x
+template <class R> __device__ __forceinline__ cx<R> ffv1_0_col(const W<R>& F1, const VslashF<R>& x, cx<R> COUP) {
+  return mul_mi(COUP * (F1[2] * x.c2 + F1[3] * x.c3 + F1[4] * x.c4 + F1[5] * x.c5));
+}
+// FFV1_1 spinor components (F1[2..5]) summed over the (F2, V3) pairs whose columns make
+// up x, all with off-shell momentum P (slot convention) and den2 = P^2 - M1^2: the
+// propagator (P.gamma + M1) / (den2 + i M1 W1) applied to the column
+template <class R>
+__device__ __forceinline__ void ffv1_1_col(const VslashF<R>& x, cx<R> COUP, R M1, R W1, R den2, const R (&P)[4],
+                                           W<R>& F1) {
+  const cx<R> di = mul_i(prop_denom(COUP, den2, M1, W1));
+  const R pp0 = P[0] + P[3], pm0 = P[0] - P[3];
+  const cx<R> q(P[1], P[2]), qb(P[1], -P[2]);
+  F1[2] = di * (x.c2 * M1 - q * x.c5 - x.c4 * pp0);
+  F1[3] = -di * (qb * x.c4 + x.c5 * pm0 - x.c3 * M1);
+  F1[4] = -di * (x.c2 * pm0 - q * x.c3 - x.c4 * M1);
+  F1[5] = di * (qb * x.c2 - x.c3 * pp0 + x.c5 * M1);
+}
+
 // ------------------------------------------------------------------------ FFV1_1
 // P: the off-shell fermion's momentum (minus the sum of the input slots); the slots of F1
 // are not written (callers that pass P from the external momenta never read them).

@@ -122,60 +122,51 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], const R (&pinv
   }
 }
 
-// O(j) for j in [O0, O1]: O(j) = sum_{i<j} FFV1_1(O(i), J(i..j-1)).
+// O(j) for j in [O0, O1]: O(j) = sum_{i<j} FFV1_1(O(i), J(i..j-1)).  Every term has the
+// off-shell momentum of O(j) (top + gluons 0..j-1), so the gamma.J O columns are summed
+// and the top propagator is applied once per j.
 template <int NG, int O0, int O1, class R>
 __device__ __forceinline__ void ordered_top_line(W<R> (&O)[NG], const W<R> (&J)[NG][NG], const R (&pinv)[NG][NG],
                                                  const R (&tinv)[NG], const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
 #pragma unroll
   for (int j = O0; j <= O1; ++j) {
-    R den2 = R(0);
+    VslashF<R> x = vslash_f(O[0], J[0][j - 1]);
+#pragma unroll
+    for (int i = 1; i < j; ++i) add_col(x, vslash_f(O[i], J[i][j - 1]));
+    O[j][0] = O[0][0] + J[0][j - 1][0];
+    O[j][1] = O[0][1] + J[0][j - 1][1];
+    R P[4];
+    mom_out(O[j], P);
+    R den2;
     if constexpr (INV) {
+      den2 = R(0);
 #pragma unroll
       for (int a = 0; a < j; ++a) {
         den2 += tinv[a];
 #pragma unroll
         for (int b = a + 1; b < j; ++b) den2 += pinv[a][b];
       }
+    } else {
+      den2 = den_from_slots(P, prm.mdl_MT);
     }
-#pragma unroll
-    for (int i = 0; i < j; ++i) {
-      W<R> t;
-      if constexpr (INV) FFV1_1(O[i], J[i][j - 1], prm.GC_11, prm.mdl_MT, prm.mdl_WT, den2, t);
-      else FFV1_1(O[i], J[i][j - 1], prm.GC_11, prm.mdl_MT, prm.mdl_WT, t);
-      if (i == 0) O[j] = t;
-      else add_components(O[j], t);
-    }
+    ffv1_1_col(x, prm.GC_11, prm.mdl_MT, prm.mdl_WT, den2, P, O[j]);
   }
 }
 
-// jamp = -sum_{i<NG} FFV1_0(anti-top, O(i), J(i..NG-1))
+// The closing vertex: jamp = -sum_{i<NG} FFV1_0(anti-top, O(i), J(i..NG-1)).  It is linear
+// in the sum of the gamma.J(i..NG-1) O(i) columns, formed once for both anti-top
+// helicities (ordered_jamp dots it with each anti-top spinor).
 template <int NG, class R>
-__device__ __forceinline__ cx<R> ordered_close(const W<R>& atop, const W<R> (&O)[NG], const W<R> (&J)[NG][NG],
-                                               const Params<R>& prm) {
-  cx<R> amp(R(0), R(0));
+__device__ __forceinline__ VslashF<R> ordered_close_col(const W<R> (&O)[NG], const W<R> (&J)[NG][NG]) {
+  VslashF<R> x = vslash_f(O[0], J[0][NG - 1]);
 #pragma unroll
-  for (int i = 0; i < NG; ++i) amp += FFV1_0(atop, O[i], J[i][NG - 1], prm.GC_11);
-  return cx<R>(-amp.re, -amp.im);
+  for (int i = 1; i < NG; ++i) add_col(x, vslash_f(O[i], J[i][NG - 1]));
+  return x;
 }
-
-// The closing vertex for both anti-top helicities at once: it is linear in the anti-top
-// spinor, so the column vectors gamma.J(i..NG-1) O(i) are shared (FFV1_0_pair1).
-template <int NG, class R>
-__device__ __forceinline__ void ordered_close_pair(const W<R>& atop0, const W<R>& atop1, const W<R> (&O)[NG],
-                                                   const W<R> (&J)[NG][NG], const Params<R>& prm, cx<R>& amp0,
-                                                   cx<R>& amp1) {
-  amp0 = cx<R>(R(0), R(0));
-  amp1 = cx<R>(R(0), R(0));
-#pragma unroll
-  for (int i = 0; i < NG; ++i) {
-    cx<R> a, b;
-    FFV1_0_pair1(atop0, atop1, O[i], J[i][NG - 1], prm.GC_11, a, b);
-    amp0 += a;
-    amp1 += b;
-  }
-  amp0 = cx<R>(-amp0.re, -amp0.im);
-  amp1 = cx<R>(-amp1.re, -amp1.im);
+template <class R>
+__device__ __forceinline__ cx<R> ordered_jamp(const W<R>& atop, const VslashF<R>& x, const Params<R>& prm) {
+  return -ffv1_0_col(atop, x, prm.GC_11);
 }
 
 struct OrderedArgs {
@@ -330,19 +321,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
         O[0] = top;
         ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
-        if constexpr (S::NFS > 0) {
-          if (rb >= r0 && rb + 1 < r1) {
-            // both anti-top rows of this (g, ht) in the launch: one paired closing vertex
-            W<R> atop1;
-            staged_fermion(1, 0, atop);
-            staged_fermion(1, 1, atop1);
-            cx<R> amp0, amp1;
-            ordered_close_pair<NG, R>(atop, atop1, O, J, prm, amp0, amp1);
-            jb[((int64_t)(rb - r0) * NC + c) * a.n + el] = amp0;
-            jb[((int64_t)(rb + 1 - r0) * NC + c) * a.n + el] = amp1;
-            continue;
-          }
-        }
+        const VslashF<R> x = ordered_close_col<NG, R>(O, J);
 #pragma unroll 1
         for (int ha = 0; ha < 2; ++ha) {
           const int r = rb + ha;
@@ -350,8 +329,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
           if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
           else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
                                                prm, top, atop);
-          const cx<R> amp = ordered_close<NG, R>(atop, O, J, prm);
-          jb[((int64_t)(r - r0) * NC + c) * a.n + el] = amp;
+          jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
         }
       }
     }

@@ -327,6 +327,9 @@ ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_mi
                "gg_ttxggg": dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4)}
 LABEL = {"gg_ttx": "gg->tt", "gg_ttxg": "gg->ttg", "gg_ttxgg": "gg->ttgg", "gg_ttxgg_ordered": "gg->ttgg",
          "gg_ttxggg": "gg->ttggg"}
+# processes that are another process through other kernels: the oracle, the flop count
+# and the kinematics of the base process apply
+BASE = {"gg_ttxgg_ordered": "gg_ttxgg"}
 KERNEL = {"gg_ttxggg": "k_ordered_jamps+k_colour_mfma<gg_ttxggg,%s>",
           "gg_ttxgg_ordered": "k_ordered_jamps+k_colour_mfma<gg_ttxgg_ordered,%s>"}
 
@@ -349,7 +352,7 @@ def me_only(a, torch):
     repeats of the HIP-event time around one smatrix call."""
     from madflow_amd.vegas import MEIntegrand
     proc = a.me_only
-    base = "gg_ttxgg" if proc == "gg_ttxgg_ordered" else proc
+    base = BASE.get(proc, proc)
     ig = MEIntegrand(base, precision=a.precision, mapping="rambo", seed=3, events_limit=a.batch,
                      **ME_ONLY_KIN[proc])
     ig.chunk(0, a.batch, 0)
@@ -386,7 +389,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nev", type=int, default=16_000_000)
     ap.add_argument("--events-limit", type=int, default=1 << 22)
-    ap.add_argument("--process", default="gg_ttxgg", choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"],
+    ap.add_argument("--process", default="gg_ttxgg",
+                    choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"],
                     help="process of the VEGAS / flow workload (the benchmark metric is quoted on gg_ttxgg)")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--mapping", default="diet", choices=["diet", "rambo"])
@@ -433,7 +437,7 @@ def main(argv=None):
         print(json.dumps({"copy_probe_bytes_per_copy": src.numel() * 4, "copies": 4}))
         return 0
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
-        flops = json.load(fh)[a.process]["total"]
+        flops = json.load(fh)[BASE.get(a.process, a.process)]["total"]
     m = run_main(a, torch)
     kernel_s = m["me_ms"] * 1e-3
     achieved = flops * m["me_n"] / kernel_s / 1e12
@@ -479,7 +483,8 @@ def main(argv=None):
     }
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass, m["ig"].params,
+        res["cpu_baseline"] = cpu_baseline(BASE.get(a.process, a.process), b["dense"], m["ig"].last_chunk_npass,
+                                           m["ig"].params,
                                            list(m["ig"].matrix.params))
         res["cpu_baseline"]["c1_end_to_end"] = cpu_c1_end_to_end(res["cpu_baseline"]["cores"])
     if not a.no_flow:

@@ -35,8 +35,9 @@ typedef float mfx_f32x4 __attribute__((ext_vector_type(4)));
 typedef double mfx_f64x4 __attribute__((ext_vector_type(4)));
 
 // Lorentz structure of VVV1P0_1 without coupling or propagator; V1 slots = V2 + V3.
+// d = V2.V3, the dot product the structure contains (the four-gluon sums reuse it).
 template <class R>
-__device__ __forceinline__ void vvv_numerator(const W<R>& V2, const W<R>& V3, W<R>& N) {
+__device__ __forceinline__ void vvv_numerator(const W<R>& V2, const W<R>& V3, W<R>& N, cx<R>& d) {
   R P2[4], P3[4], P1[4];
   mom_in(V2, P2);
   mom_in(V3, P3);
@@ -48,15 +49,14 @@ __device__ __forceinline__ void vvv_numerator(const W<R>& V2, const W<R>& V3, W<
   const cx<R> c2 = TMP2 - TMP1, c3 = TMP3 - TMP4;
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) N[2 + mu] = TMP5 * (P3[mu] - P2[mu]) + V2[2 + mu] * c2 + V3[2 + mu] * c3;
+  d = TMP5;
 }
 
-// 2 V3 (V2.V4) - V2 (V3.V4) - V4 (V2.V3), added into N[2..5].
+// N[2..5] += c V
 template <class R>
-__device__ __forceinline__ void vvvv_ordered_add(const W<R>& V2, const W<R>& V3, const W<R>& V4, W<R>& N) {
-  const cx<R> d24 = mdot(V2, V4), d34 = mdot(V3, V4), d23 = mdot(V2, V3);
-  const cx<R> t = d24 + d24;
+__device__ __forceinline__ void add_scaled(W<R>& N, const W<R>& V, cx<R> c) {
 #pragma unroll
-  for (int mu = 0; mu < 4; ++mu) N[2 + mu] += V3[2 + mu] * t - V2[2 + mu] * d34 - V4[2 + mu] * d23;
+  for (int mu = 2; mu < 6; ++mu) N[mu] += V[mu] * c;
 }
 
 template <class R>
@@ -70,10 +70,20 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
 // the first j gluons.  pinv[a][b] (a < b) = 2 k_a.k_b of the gluons at word positions a,
 // b and tinv[a] = 2 k_top.k_a (FP32 forms only).  All indices are compile-time constants
 // after unrolling, so every current lives in registers.
+//
+// The four-gluon sum of J(i..j) runs over the splits i <= k < l < j into
+// V2 = J(i..k), V3 = J(k+1..l), V4 = J(l+1..j).  Two of its three terms only involve dot
+// products of adjacent ranges, and summed over one split index those are
+// S(a..b) = sum_{a<=m<b} J(a..m).J(m+1..b), the dot products of the three-gluon sum of
+// range a..b (kept in S[a][b]):
+//   sum_{k<l} V2 (V3.V4) = sum_{i<=k<j-1} J(i..k) S(k+1..j)
+//   sum_{k<l} V4 (V2.V3) = sum_{i<l<j}   J(l+1..j) S(i..l)
+// so only the 2 V3 (V2.V4) term needs a dot product per split pair.
 
-// J(i..j) for every range ending at j in [J0, J1), shorter ranges first.
+// J(i..j) (and S(i..j)) for every range ending at j in [J0, J1), shorter ranges first.
 template <int NG, int J0, int J1, class R>
-__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], const R (&pinv)[NG][NG], const Params<R>& prm) {
+__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG][NG], const R (&pinv)[NG][NG],
+                                               const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
   // -i GC_12 multiplies the four-gluon structures (GC_12 = i gs^2: a real gs^2)
   const cx<R> c4 = mul_mi(prm.GC_12);
@@ -82,13 +92,17 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], const R (&pinv
 #pragma unroll
     for (int i = j - 1; i >= 0; --i) {
       W<R> N3;
-      vvv_numerator(J[i][i], J[i + 1][j], N3);
+      cx<R> sd;
+      vvv_numerator(J[i][i], J[i + 1][j], N3, sd);
 #pragma unroll
       for (int k = i + 1; k < j; ++k) {
         W<R> t;
-        vvv_numerator(J[i][k], J[k + 1][j], t);
+        cx<R> d;
+        vvv_numerator(J[i][k], J[k + 1][j], t, d);
         add_components(N3, t);
+        sd += d;
       }
+      S[i][j] = sd;
       W<R> N4;
       N4[0] = N3[0];
       N4[1] = N3[1];
@@ -97,7 +111,14 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], const R (&pinv
 #pragma unroll
       for (int k = i; k < j; ++k)
 #pragma unroll
-        for (int l = k + 1; l < j; ++l) vvvv_ordered_add(J[i][k], J[k + 1][l], J[l + 1][j], N4);
+        for (int l = k + 1; l < j; ++l) {
+          const cx<R> d24 = mdot(J[i][k], J[l + 1][j]);
+          add_scaled(N4, J[k + 1][l], d24 + d24);
+        }
+#pragma unroll
+      for (int k = i; k < j - 1; ++k) add_scaled(N4, J[i][k], -S[k + 1][j]);
+#pragma unroll
+      for (int l = i + 1; l < j; ++l) add_scaled(N4, J[l + 1][j], -S[i][l]);
       // propagator 1/P^2 of the massless gluon
       R den2;
       if constexpr (INV) {
@@ -264,6 +285,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
   const int r0 = a.h0, r1 = a.h0 + a.nhel;
   constexpr int PRE = NG - 2;
   W<R> J[NG][NG], O[NG];
+  cx<R> SD[NG][NG];   // adjacent-range dot sums S(a..b) of the four-gluon terms
   R pinv[NG][NG], tinv[NG];
 #pragma unroll 1
   for (int g = r0 >> 2; g <= (r1 - 1) >> 2; ++g) {
@@ -310,8 +332,8 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
             pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
           }
       }
-      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, pinv, prm);
-      ordered_ranges<NG, PRE, NG, R>(J, pinv, prm);
+      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, pinv, prm);
+      ordered_ranges<NG, PRE, NG, R>(J, SD, pinv, prm);
 #pragma unroll 1
       for (int ht = 0; ht < 2; ++ht) {
         const int rb = 4 * g + 2 * ht;

@@ -55,6 +55,7 @@ PROCESS_SETUP = {
     "gg_ttx": (2, [173.0, 173.0]),
     "gg_ttxg": (3, [173.0, 173.0, 0.0]),
     "gg_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
+    "gg_ttxgg_ordered": (4, [173.0, 173.0, 0.0, 0.0]),
     "gg_ttxggg": (5, [173.0, 173.0, 0.0, 0.0, 0.0]),
 }
 

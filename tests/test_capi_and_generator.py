@@ -147,3 +147,16 @@ def test_dual_row_program(ng):
             assert all((a - DUAL_OFF in dep) if a >= DUAL_OFF else (a not in dep) for a in op["args"])
         if op["kind"] == "wf":
             defined.add(op["out"])
+
+
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"])
+def test_process_module_model_params(name):
+    """Every drop-in module's get_model_param() evaluates exactly the parameters its
+    smatrix takes, in that order."""
+    import importlib
+    mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % name)
+    cls = getattr(mod, "Matrix_1_%s" % name)
+    model = mod.get_model_param()
+    model.freeze_alpha_s(0.118)
+    vals = list(model.evaluate())
+    assert len(vals) == len(cls.params)

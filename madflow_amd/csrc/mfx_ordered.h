@@ -35,18 +35,21 @@ typedef float mfx_f32x4 __attribute__((ext_vector_type(4)));
 typedef double mfx_f64x4 __attribute__((ext_vector_type(4)));
 
 // Lorentz structure of VVV1P0_1 without coupling or propagator; V1 slots = V2 + V3.
-// d = V2.V3, the dot product the structure contains (the four-gluon sums reuse it).
+// With P1 = -(P2 + P3) it reads (V2.V3)(P3 - P2) + V2 (2 V3.P2 + V3.P3) - V3 (V2.P2 + 2 V2.P3).
+// Colour-ordered currents of gluons alone are conserved, P.J(i..j) = 0 (external gluons
+// on shell and transverse; Berends and Giele, Nucl. Phys. B306 (1988) 759), so the
+// V2.P2 and V3.P3 terms vanish and are left out.  d = V2.V3, the dot product the
+// structure contains (the four-gluon sums reuse it).
 template <class R>
 __device__ __forceinline__ void vvv_numerator(const W<R>& V2, const W<R>& V3, W<R>& N, cx<R>& d) {
-  R P2[4], P3[4], P1[4];
+  R P2[4], P3[4];
   mom_in(V2, P2);
   mom_in(V3, P3);
   N[0] = V2[0] + V3[0];
   N[1] = V2[1] + V3[1];
-  mom_out(N, P1);
-  const cx<R> TMP1 = mdotp(V3, P1), TMP2 = mdotp(V3, P2), TMP3 = mdotp(V2, P1), TMP4 = mdotp(V2, P3);
+  const cx<R> TMP2 = mdotp(V3, P2), TMP4 = mdotp(V2, P3);
   const cx<R> TMP5 = mdot(V3, V2);
-  const cx<R> c2 = TMP2 - TMP1, c3 = TMP3 - TMP4;
+  const cx<R> c2 = TMP2 + TMP2, c3 = -(TMP4 + TMP4);
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) N[2 + mu] = TMP5 * (P3[mu] - P2[mu]) + V2[2 + mu] * c2 + V3[2 + mu] * c3;
   d = TMP5;

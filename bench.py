@@ -4,11 +4,13 @@ Workload (default): LO g g > t t~ g g at a fixed partonic sqrt(s) = 3 TeV. SM co
 m_t = 173, Gamma_t = 1.4915, alpha_s = 0.118. Cuts: pT(g) > 30 GeV and Delta R(g,g) > 0.4.
 Phase space is RAMBO on diet on the 8-dimensional VEGAS cube (50 bins per dimension),
 with 1.6e7 points per VEGAS iteration. Wavefunctions, amplitudes and the colour sum are
-FP32; phase space and VEGAS accumulation are FP64.
+FP32; phase space and VEGAS accumulation are FP64. |M|^2 runs through the colour-ordered
+kernels (k_ordered_jamps + MFMA k_colour_mfma, process gg_ttxgg_ordered); --process
+gg_ttxgg runs the same physics through the fused Feynman-diagram kernel.
 
 One step is one VEGAS iteration on the frozen grid, after the warm-up iterations have
 adapted it: Philox uniforms -> VEGAS map -> phase space -> cuts -> compaction -> fused
-smatrix kernel -> compensated accumulation. Everything is device resident.
+smatrix kernels -> compensated accumulation. Everything is device resident.
 
 Reported (one JSON line):
   value                      ME evaluations per second over the K timed steps (whole pipeline)
@@ -37,9 +39,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 packed, FP64)
-# HBM bytes per matrix element of k_smatrix<gg_ttxgg, float>: rocprofv3 FETCH_SIZE and WRITE_SIZE
-# passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md)
-HBM_BYTES_PER_ME = {32: 6058.9}
+# HBM bytes per matrix element, (process, precision): rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md for the fused
+# Feynman-diagram kernel, profiles/pmc_k_ordered_gg_ttxgg_f32.md for the colour-ordered
+# kernels, jamps staged through HBM included)
+HBM_BYTES_PER_ME = {("gg_ttxgg", 32): 6058.9}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 
 
@@ -225,7 +229,8 @@ def train_flow(a, torch, ig):
     flow = RQSFlow(D, layers=8, seed=1)
     nt = a.flow_pairs
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
-               val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta)
+               val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta,
+               patience=a.flow_patience)
     for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
         # further stages: points drawn from the current proposal, weighted by f/q', carry
         # far more of the information where the integrand is large than uniform points
@@ -311,7 +316,7 @@ def run_flow_compare(a, torch, main):
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
                  "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
-                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix,
+                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience,
                  "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
         "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
@@ -389,7 +394,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nev", type=int, default=16_000_000)
     ap.add_argument("--events-limit", type=int, default=1 << 22)
-    ap.add_argument("--process", default="gg_ttxgg",
+    # gg->ttgg: the colour-ordered kernels are the faster form in FP32 and FP64; gg_ttxgg is
+    # the same physics through the fused Feynman-diagram kernel
+    ap.add_argument("--process", default="gg_ttxgg_ordered",
                     choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"],
                     help="process of the VEGAS / flow workload (the benchmark metric is quoted on gg_ttxgg)")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
@@ -404,6 +411,8 @@ def main(argv=None):
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.2)
     ap.add_argument("--flow-eval-every", type=int, default=500)
+    ap.add_argument("--flow-patience", type=int, default=2,
+                    help="stage 1 stops after this many held-out evaluations without a new best")
     ap.add_argument("--flow-refine-steps", type=int, default=3000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
     ap.add_argument("--flow-beta", type=float, default=0.0,
@@ -465,15 +474,17 @@ def main(argv=None):
                                "phase space and VEGAS sums" % (LABEL[a.process], _kin_text(a.process), a.mapping,
                                                                m["ig"].ndim, a.nev,
                                                                "FP32" if a.precision == 32 else "FP64"),
-                   "points_per_step": a.nev, "passing_per_step": m["npass"] // a.steps},
+                   "points_per_step": a.nev, "passing_per_step": m["npass"] // a.steps,
+                   "me_kernels": ("colour-ordered (Berends-Giele currents per ordering, MFMA colour sum)"
+                                  if a.process in KERNEL else "fused Feynman-diagram program")},
         "me_kernel_evals_per_s": m["me_n"] / kernel_s,
         "unweighted": u,
         "unweighted_events_per_s": u["unweighted_per_s_to_target"],
         "unweighted_events_per_s_method": "plain VEGAS, %s matrix elements" % ("FP32" if a.precision == 32 else "FP64"),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": (HBM_BYTES_PER_ME[a.precision] * m["me_n"] / max(1, m["launches"])
-                                 if a.process == "gg_ttxgg" and a.precision in HBM_BYTES_PER_ME
+                     "traffic": (HBM_BYTES_PER_ME[(a.process, a.precision)] * m["me_n"] / max(1, m["launches"])
+                                 if (a.process, a.precision) in HBM_BYTES_PER_ME
                                  and a.mapping == "diet" else None),
                      "traffic_unit": "bytes per launch (PMC)",
                      "kernel": (KERNEL.get(a.process, "k_smatrix<%s,%%s>" % a.process)

@@ -193,6 +193,10 @@ __device__ __forceinline__ cx<R> ordered_jamp(const W<R>& atop, const VslashF<R>
   return -ffv1_0_col(atop, x, prm.GC_11);
 }
 
+// the top-line currents O(1..N-1) of both top helicities (empty when unused)
+template <class R, int N, bool ON> struct TopPrefix { W<R> o[2][N]; };
+template <class R, int N> struct TopPrefix<R, N, false> {};
+
 struct OrderedArgs {
   SmatrixArgs s;
   int64_t ev0;          // first event of this sub-chunk
@@ -289,6 +293,11 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
   constexpr int PRE = NG - 2;
   W<R> J[NG][NG], O[NG];
   cx<R> SD[NG][NG];   // adjacent-range dot sums S(a..b) of the four-gluon terms
+  // FP32, five or more gluons: the top-line currents O(1..PRE) of both top helicities,
+  // kept from the first ordering of a pair for the second.  Not for FP64 (no registers
+  // to spare) nor for four gluons (the kernel fits two waves per SIMD without them).
+  constexpr bool SHARE_TOP = S::NFS > 0 && NG >= 5;
+  TopPrefix<R, PRE + 1, SHARE_TOP> OP;
   R pinv[NG][NG], tinv[NG];
 #pragma unroll 1
   for (int g = r0 >> 2; g <= (r1 - 1) >> 2; ++g) {
@@ -337,24 +346,57 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
       }
       if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, pinv, prm);
       ordered_ranges<NG, PRE, NG, R>(J, SD, pinv, prm);
+      if constexpr (SHARE_TOP) {
+        // unrolled: OP is indexed by ht
+#pragma unroll
+        for (int ht = 0; ht < 2; ++ht) {
+          const int rb = 4 * g + 2 * ht;
+          if (rb + 1 < r0 || rb >= r1) continue;
+          W<R> top, atop;
+          if constexpr (S::NFS > 0) staged_fermion(0, ht, top);
+          else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
+          O[0] = top;
+          // O(1..PRE) only involve the prefix gluons: computed once per pair of orderings
+          if (fresh) {
+            ordered_top_line<NG, 1, PRE, R>(O, J, pinv, tinv, prm);
+#pragma unroll
+            for (int j = 1; j <= PRE; ++j) OP.o[ht][j] = O[j];
+          } else {
+#pragma unroll
+            for (int j = 1; j <= PRE; ++j) O[j] = OP.o[ht][j];
+          }
+          ordered_top_line<NG, PRE + 1, NG - 1, R>(O, J, pinv, tinv, prm);
+          const VslashF<R> x = ordered_close_col<NG, R>(O, J);
 #pragma unroll 1
-      for (int ht = 0; ht < 2; ++ht) {
-        const int rb = 4 * g + 2 * ht;
-        if (rb + 1 < r0 || rb >= r1) continue;
-        W<R> top, atop;
-        if constexpr (S::NFS > 0) staged_fermion(0, ht, top);
-        else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
-        O[0] = top;
-        ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
-        const VslashF<R> x = ordered_close_col<NG, R>(O, J);
+          for (int ha = 0; ha < 2; ++ha) {
+            const int r = rb + ha;
+            if (r < r0 || r >= r1) continue;
+            if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
+            else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
+                                                 prm, top, atop);
+            jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
+          }
+        }
+      } else {
 #pragma unroll 1
-        for (int ha = 0; ha < 2; ++ha) {
-          const int r = rb + ha;
-          if (r < r0 || r >= r1) continue;
-          if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
-          else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
-                                               prm, top, atop);
-          jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
+        for (int ht = 0; ht < 2; ++ht) {
+          const int rb = 4 * g + 2 * ht;
+          if (rb + 1 < r0 || rb >= r1) continue;
+          W<R> top, atop;
+          if constexpr (S::NFS > 0) staged_fermion(0, ht, top);
+          else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
+          O[0] = top;
+          ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
+          const VslashF<R> x = ordered_close_col<NG, R>(O, J);
+#pragma unroll 1
+          for (int ha = 0; ha < 2; ++ha) {
+            const int r = rb + ha;
+            if (r < r0 || r >= r1) continue;
+            if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
+            else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
+                                                 prm, top, atop);
+            jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
+          }
         }
       }
     }

@@ -23,9 +23,9 @@
 // each finished jamp is stored to a workspace [helicity][colour][event] (coalesced),
 // so no lane ever holds more than one ordering's currents.
 // Kernel 2 (k_colour_mfma): |M|^2 = sum_h Re(J_h^dagger CF J_h) / (CF_DEN * DENOMINATOR)
-// as a batched dense contraction on the matrix cores: per helicity, a 16-event tile of
-// Re J and Im J (16 x NCOLOR) times the integer colour matrix (NCOLOR x NCOLOR)
-// with v_mfma_{f32,f64}_16x16x4, then the row-wise dot product with J.
+// as a batched dense contraction on the matrix cores: per helicity, the integer colour
+// matrix (NCOLOR x NCOLOR) times a 16-event tile of Re J and Im J (NCOLOR x 16) with
+// v_mfma_{f32,f64}_16x16x4, then the column-wise dot product with J.
 #pragma once
 #include "mfx_smatrix.h"
 
@@ -420,19 +420,29 @@ template <> struct Vec4<float> { typedef mfx_f32x4 type; };
 template <> struct Vec4<double> { typedef mfx_f64x4 type; };
 
 // |M|^2 of the events of one sub-chunk from their jamps: one wave per 16-event tile,
-// four tiles per block; the integer colour matrix (padded to NCP columns) sits in LDS.
+// four tiles per block.  Per helicity and tile the matrix cores form D = CF J^T
+// (NCP x 16: colour rows, event columns) from the integer colour matrix (A operand, in
+// LDS) and the tile's jamps (B operand: lane (i, q) holds J[event i][colour 4 s + q] for
+// every k-step s), then each lane dots its D entries with jamps it already holds.  For
+// that, the colour of A row drow(q, r) of row tile mt is 4 (4 mt + r) + q: the output the
+// lane holds in register r is then the row of the colour of its k-step 4 mt + r, and the
+// jamps are read once.  The four lane quarters of each event are summed at the end.
 template <class T, class R>
 __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n, int nhel, double scale,
                                                      int accumulate, double* out) {
   constexpr int NC = T::NCOLOR;
-  constexpr int NCP = (NC + 15) / 16 * 16;    // padded output columns
+  constexpr int NCP = (NC + 15) / 16 * 16;    // padded colour rows of D
   constexpr int KS = NC / 4;                  // k-steps (NCOLOR is a multiple of 4)
   static_assert(NC % 4 == 0, "NCOLOR must be a multiple of 4");
   typedef typename Vec4<R>::type V4;
-  __shared__ float cf[NC * NCP];   // integer numerators: exact in float
-  for (int i = threadIdx.x; i < NC * NCP; i += 256) {
-    const int r = i / NCP, c = i % NCP;
-    cf[i] = c < NC ? T::cf(float(), r, c) : 0.0f;
+  __shared__ float cf[NCP * NC];   // integer numerators, rows permuted as above: exact in float
+  for (int i = threadIdx.x; i < NCP * NC; i += 256) {
+    const int row = i / NC, k = i % NC;
+    const int mt = row / 16, x = row % 16;
+    // x = drow(q, r): 4 q + r for f32, q + 4 r for f64
+    const int q = drow<R>(1, 0) == 4 ? x / 4 : x % 4, r = drow<R>(1, 0) == 4 ? x % 4 : x / 4;
+    const int colour = 4 * (4 * mt + r) + q;
+    cf[i] = colour < NC ? T::cf(float(), colour, k) : 0.0f;
   }
   __syncthreads();
   const cx<R>* J = reinterpret_cast<const cx<R>*>(jbuf);
@@ -440,9 +450,10 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
   const int lr = lane & 15, lq = lane >> 4;
   const int64_t ntiles = (n + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
-    const int64_t e0 = tile * 16;
-    const int64_t ea = e0 + lr;                 // event of this lane's A row
-    R acc[4] = {R(0), R(0), R(0), R(0)};
+    const int64_t ea = tile * 16 + lr;          // the event of this lane's B column
+    // each helicity's colour terms are summed in R, the helicities in FP64 (an FP32 sum
+    // over every helicity and colour would lose digits with the number of rows)
+    double acc = 0.0;
     for (int h = 0; h < nhel; ++h) {
       const cx<R>* Jh = J + (int64_t)h * NC * n;
       R are[KS], aim[KS];
@@ -452,41 +463,29 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
         are[s] = v.re;
         aim[s] = v.im;
       }
-#pragma unroll 1
-      for (int nt = 0; nt < NCP / 16; ++nt) {
+      R part = R(0);
+#pragma unroll
+      for (int mt = 0; mt < NCP / 16; ++mt) {
         V4 dre = {R(0), R(0), R(0), R(0)}, dim = {R(0), R(0), R(0), R(0)};
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-          const R b = R(cf[(4 * s + lq) * NCP + nt * 16 + lr]);
-          dre = mfma16x16x4(are[s], b, dre);
-          dim = mfma16x16x4(aim[s], b, dim);
+          const R c = R(cf[(16 * mt + lr) * NC + 4 * s + lq]);
+          dre = mfma16x16x4(c, are[s], dre);
+          dim = mfma16x16x4(c, aim[s], dim);
         }
-        const int col = nt * 16 + lr;
-        if (col < NC) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int64_t ed = e0 + drow<R>(lq, r);
-            if (ed < n) {
-              const cx<R> x = Jh[(int64_t)col * n + ed];
-              acc[r] += x.re * dre[r] + x.im * dim[r];
-            }
-          }
-        }
+        for (int r = 0; r < 4; ++r)
+          if (4 * mt + r < KS) part += are[4 * mt + r] * dre[r] + aim[4 * mt + r] * dim[r];
+        // one row tile at a time: interleaving the tiles would hold every tile's
+        // accumulators at once
+        __builtin_amdgcn_sched_barrier(0);
       }
+      acc += double(part);
     }
-    // sum over the 16 columns held by the lanes of each quarter-wave
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) acc[r] += __shfl_xor(acc[r], off, 64);
-    }
-    if (lr == 0) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t ed = e0 + drow<R>(lq, r);
-        if (ed < n) out[ed] = (accumulate ? out[ed] : 0.0) + double(acc[r]) * scale;
-      }
-    }
+    // sum the four lane quarters that hold the colours of each event
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (lq == 0 && ea < n) out[ea] = (accumulate ? out[ea] : 0.0) + acc * scale;
   }
 }
 

@@ -226,9 +226,10 @@ def train_flow(a, torch, ig):
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
     t0 = time.perf_counter()
-    flow = RQSFlow(D, layers=8, seed=1)
+    flow = RQSFlow(D, layers=8, seed=a.flow_seed)
     nt = a.flow_pairs
-    flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, log=log,
+    s0 = 1000 * (a.flow_seed - 1)   # minibatch-draw seeds of the stages
+    flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, seed=s0, log=log,
                val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta,
                patience=a.flow_patience)
     for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
@@ -249,7 +250,8 @@ def train_flow(a, torch, ig):
             if fq is not None:
                 fq[done:done + n] = fb / winv
             done += n
-        flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=1 + rnd, log=log,
+        flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=s0 + 1 + rnd,
+                   log=log,
                    val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix,
                    f=fq, beta=a.flow_refine_beta)
         del yq, wq, fq
@@ -316,7 +318,7 @@ def run_flow_compare(a, torch, main):
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
                  "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
-                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience,
+                 "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience, "seed": a.flow_seed,
                  "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
         "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
@@ -407,13 +409,14 @@ def main(argv=None):
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
     ap.add_argument("--flow-steps", type=int, default=20000)
-    ap.add_argument("--flow-batch", type=int, default=4096)
+    ap.add_argument("--flow-batch", type=int, default=8192)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-mix", type=float, default=0.2)
-    ap.add_argument("--flow-eval-every", type=int, default=500)
+    ap.add_argument("--flow-eval-every", type=int, default=250)
+    ap.add_argument("--flow-seed", type=int, default=1, help="initial weights and minibatch draws of the flow")
     ap.add_argument("--flow-patience", type=int, default=2,
                     help="stage 1 stops after this many held-out evaluations without a new best")
-    ap.add_argument("--flow-refine-steps", type=int, default=3000)
+    ap.add_argument("--flow-refine-steps", type=int, default=1500)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
     ap.add_argument("--flow-beta", type=float, default=0.0,
                     help="stage-1 loss: 0 = KL, beta > 0 = alpha-divergence with alpha = 1 + beta")

@@ -58,8 +58,8 @@ int mfx_smatrix(int proc, const double* all_ps, int64_t nev, const double* mass_
                 const double* gc_ev, double* out, int precision, int ref_compat, void* workspace,
                 int64_t workspace_bytes, void* stream);
 
-/* Bytes of workspace for one pass over nev events (per_helicity = 1: for mfx_matrix);
- * 0 when the process needs none.  No counterpart in the reference: its TF op allocated
+/* Bytes of workspace for one pass over nev events (per_helicity = 1: for mfx_matrix),
+ * nev rounded up to a 16-event tile; 0 when the process needs none.  No counterpart in the reference: its TF op allocated
  * its intermediates itself (custom_op/generation.py:206-211). */
 int64_t mfx_smatrix_workspace_size(int proc, int64_t nev, int precision, int per_helicity);
 

@@ -173,7 +173,8 @@ int64_t mfx_smatrix_workspace_size(int proc, int64_t nev, int precision, int per
   if (precision != 32 && precision != 64) return fail(-5, "precision must be 32 or 64%s", "");
   const ProcessEntry& p = process(proc);
   if (!p.ws_per_row || nev <= 0) return 0;
-  return nev * (per_helicity ? 1 : p.ncomb) * p.ws_per_row(precision);
+  // passes are whole 16-event tiles (the jamp rows are tile-aligned)
+  return (nev + 15) / 16 * 16 * (per_helicity ? 1 : p.ncomb) * p.ws_per_row(precision);
 }
 
 static int check_workspace(const ProcessEntry& p, int precision, int per_helicity, int64_t nev, const void* ws,

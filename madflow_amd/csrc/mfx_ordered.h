@@ -201,6 +201,8 @@ struct OrderedArgs {
   SmatrixArgs s;
   int64_t ev0;          // first event of this sub-chunk
   int64_t n;            // events in this sub-chunk
+  int64_t ld;           // jbuf row stride in events: n rounded up to a 16-event tile, so that
+                        // every tile of a colour row is whole 128-byte lines
   int h0, nhel;         // helicity rows h0 .. h0+nhel-1 (or 1 row for an explicit vector)
   void* jbuf;           // [nhel][NCOLOR][n] complex R
 };
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
             if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
             else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
                                                  prm, top, atop);
-            jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
+            jb[((int64_t)(r - r0) * NC + c) * a.ld + el] = ordered_jamp(atop, x, prm);
           }
         }
       } else {
@@ -395,7 +397,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
             if constexpr (S::NFS > 0) staged_fermion(1, ha, atop);
             else if (ha) T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG) | (1u << T::ATOP_LEG), helv,
                                                  prm, top, atop);
-            jb[((int64_t)(r - r0) * NC + c) * a.n + el] = ordered_jamp(atop, x, prm);
+            jb[((int64_t)(r - r0) * NC + c) * a.ld + el] = ordered_jamp(atop, x, prm);
           }
         }
       }
@@ -428,8 +430,8 @@ template <> struct Vec4<double> { typedef mfx_f64x4 type; };
 // lane holds in register r is then the row of the colour of its k-step 4 mt + r, and the
 // jamps are read once.  The four lane quarters of each event are summed at the end.
 template <class T, class R>
-__global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n, int nhel, double scale,
-                                                     int accumulate, double* out) {
+__global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n, int64_t ld, int nhel,
+                                                     double scale, int accumulate, double* out) {
   constexpr int NC = T::NCOLOR;
   constexpr int NCP = (NC + 15) / 16 * 16;    // padded colour rows of D
   constexpr int KS = NC / 4;                  // k-steps (NCOLOR is a multiple of 4)
@@ -455,11 +457,11 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
     // over every helicity and colour would lose digits with the number of rows)
     double acc = 0.0;
     for (int h = 0; h < nhel; ++h) {
-      const cx<R>* Jh = J + (int64_t)h * NC * n;
+      const cx<R>* Jh = J + (int64_t)h * NC * ld;
       R are[KS], aim[KS];
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        const cx<R> v = ea < n ? Jh[(int64_t)(4 * s + lq) * n + ea] : cx<R>(R(0), R(0));
+        const cx<R> v = ea < n ? Jh[(int64_t)(4 * s + lq) * ld + ea] : cx<R>(R(0), R(0));
         are[s] = v.re;
         aim[s] = v.im;
       }
@@ -489,15 +491,15 @@ __global__ void __launch_bounds__(256) k_colour_mfma(const void* jbuf, int64_t n
   }
 }
 
-// jamps of a single helicity row [NCOLOR][n] -> (n, NCOLOR, 2) f64
+// jamps of a single helicity row [NCOLOR][ld] -> (n, NCOLOR, 2) f64
 template <class R>
-__global__ void k_jamp_transpose(const void* jbuf, int64_t n, int ncolor, double* jamp_out) {
+__global__ void k_jamp_transpose(const void* jbuf, int64_t n, int64_t ld, int ncolor, double* jamp_out) {
   const cx<R>* J = reinterpret_cast<const cx<R>*>(jbuf);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * ncolor) return;
   const int64_t e = i / ncolor;
   const int c = (int)(i % ncolor);
-  const cx<R> v = J[(int64_t)c * n + e];
+  const cx<R> v = J[(int64_t)c * ld + e];
   jamp_out[2 * i] = double(v.re);
   jamp_out[2 * i + 1] = double(v.im);
 }
@@ -520,7 +522,8 @@ hipError_t launch_ordered(const SmatrixArgs& s, int precision, int per_helicity,
   const int64_t want = s.nev < 65536 ? s.nev : 65536;
   int hg = nrows;
   while (hg > 1 && ws_bytes / (row * hg) < want) hg /= 2;
-  const int64_t nsub = ws ? ws_bytes / (row * hg) : 0;
+  // whole 16-event tiles per pass: the row stride (n rounded up to 16) then fits too
+  const int64_t nsub = ws ? ws_bytes / (row * hg) / 16 * 16 : 0;
   if (nsub < 16) return hipErrorInvalidValue;   // the entry point checks the size first
   const double scale = 1.0 / (T::CF_DEN * (per_helicity ? 1.0 : T::DENOMINATOR));
   for (int64_t off = 0; off < s.nev; off += nsub) {
@@ -533,27 +536,29 @@ hipError_t launch_ordered(const SmatrixArgs& s, int precision, int per_helicity,
       a.s = s;
       a.ev0 = off;
       a.n = n;
+      a.ld = (n + 15) / 16 * 16;
       a.h0 = h0;
       a.nhel = hg;
       a.jbuf = ws;
       const int acc = h0 > 0;
       if (precision == 32) {
         hipLaunchKernelGGL((k_ordered_jamps<T, float, 64>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
-        hipLaunchKernelGGL((k_colour_mfma<T, float>), dim3((unsigned)cblk), dim3(256), 0, st, (const void*)ws, n, hg,
-                           scale, acc, s.out + off);
+        hipLaunchKernelGGL((k_colour_mfma<T, float>), dim3((unsigned)cblk), dim3(256), 0, st, (const void*)ws, n, a.ld,
+                           hg, scale, acc, s.out + off);
       } else {
         hipLaunchKernelGGL((k_ordered_jamps<T, double, 64>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
         hipLaunchKernelGGL((k_colour_mfma<T, double>), dim3((unsigned)cblk), dim3(256), 0, st, (const void*)ws, n,
-                           hg, scale, acc, s.out + off);
+                           a.ld, hg, scale, acc, s.out + off);
       }
     }
     if (per_helicity && s.jamp_out) {
       const unsigned nb = (unsigned)((n * T::NCOLOR + 255) / 256);
+      const int64_t ld = (n + 15) / 16 * 16;
       if (precision == 32)
-        hipLaunchKernelGGL((k_jamp_transpose<float>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, T::NCOLOR,
+        hipLaunchKernelGGL((k_jamp_transpose<float>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, ld, T::NCOLOR,
                            s.jamp_out + off * T::NCOLOR * 2);
       else
-        hipLaunchKernelGGL((k_jamp_transpose<double>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, T::NCOLOR,
+        hipLaunchKernelGGL((k_jamp_transpose<double>), dim3(nb), dim3(256), 0, st, (const void*)ws, n, ld, T::NCOLOR,
                            s.jamp_out + off * T::NCOLOR * 2);
     }
     const hipError_t err = hipGetLastError();

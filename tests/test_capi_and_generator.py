@@ -46,8 +46,9 @@ def test_registry_and_argument_checks():
     # the fused processes need no workspace; the colour-ordered ones say how much
     assert lib.mfx_smatrix_workspace_size(ttgg, 1000, 32, 0) == 0
     t5 = lib.mfx_process_index(b"gg_ttxggg")
-    assert lib.mfx_smatrix_workspace_size(t5, 1000, 32, 0) == 1000 * 128 * 120 * 8
-    assert lib.mfx_smatrix_workspace_size(t5, 1000, 64, 1) == 1000 * 120 * 16
+    assert lib.mfx_smatrix_workspace_size(t5, 1000, 32, 0) == 1008 * 128 * 120 * 8   # whole 16-event tiles
+    assert lib.mfx_smatrix_workspace_size(t5, 1000, 64, 1) == 1008 * 120 * 16
+    assert lib.mfx_smatrix_workspace_size(t5, 1024, 64, 1) == 1024 * 120 * 16
     # a missing workspace is reported before anything touches the (fake, aligned) pointers
     fake = ctypes.c_void_p(1 << 20)
     assert lib.mfx_smatrix(t5, fake, 10, vp(mw), vp(gc), None, fake, 32, 1, None, 0, None) == -8

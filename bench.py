@@ -43,7 +43,7 @@ PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 pack
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md for the fused
 # Feynman-diagram kernel, profiles/pmc_k_ordered_gg_ttxgg_f32.md for the colour-ordered
 # kernels, jamps staged through HBM included)
-HBM_BYTES_PER_ME = {("gg_ttxgg", 32): 6058.9, ("gg_ttxgg_ordered", 32): 24776.7}
+HBM_BYTES_PER_ME = {("gg_ttxgg", 32): 28105.2, ("gg_ttxgg_ordered", 32): 24776.7}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 
 

@@ -523,8 +523,18 @@ hipError_t launch_ordered(const SmatrixArgs& s, int precision, int per_helicity,
   int hg = nrows;
   while (hg > 1 && ws_bytes / (row * hg) < want) hg /= 2;
   // whole 16-event tiles per pass: the row stride (n rounded up to 16) then fits too
-  const int64_t nsub = ws ? ws_bytes / (row * hg) / 16 * 16 : 0;
+  int64_t nsub = ws ? ws_bytes / (row * hg) / 16 * 16 : 0;
   if (nsub < 16) return hipErrorInvalidValue;   // the entry point checks the size first
+  // Passes of a whole number of waves per SIMD (4 SIMDs per CU, one event per lane) when a
+  // pass holds few: at one wave per SIMD a pass of 1.07 waves per SIMD takes two wave-times,
+  // with most SIMDs idle in the second.  From 8 waves per SIMD on, the partial last round
+  // costs less than the extra pass that rounding down would add.
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int64_t fill = (int64_t)cus * 4 * 64;
+  if (nsub > fill && nsub < 8 * fill) nsub = nsub / fill * fill;
   const double scale = 1.0 / (T::CF_DEN * (per_helicity ? 1.0 : T::DENOMINATOR));
   for (int64_t off = 0; off < s.nev; off += nsub) {
     const int64_t n = s.nev - off < nsub ? s.nev - off : nsub;

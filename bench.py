@@ -416,7 +416,7 @@ def main(argv=None):
     ap.add_argument("--flow-seed", type=int, default=1, help="initial weights and minibatch draws of the flow")
     ap.add_argument("--flow-patience", type=int, default=2,
                     help="stage 1 stops after this many held-out evaluations without a new best")
-    ap.add_argument("--flow-refine-steps", type=int, default=1500)
+    ap.add_argument("--flow-refine-steps", type=int, default=1000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)
     ap.add_argument("--flow-beta", type=float, default=0.0,
                     help="stage-1 loss: 0 = KL, beta > 0 = alpha-divergence with alpha = 1 + beta")

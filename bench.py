@@ -4,9 +4,10 @@ Workload (default): LO g g > t t~ g g at a fixed partonic sqrt(s) = 3 TeV. SM co
 m_t = 173, Gamma_t = 1.4915, alpha_s = 0.118. Cuts: pT(g) > 30 GeV and Delta R(g,g) > 0.4.
 Phase space is RAMBO on diet on the 8-dimensional VEGAS cube (50 bins per dimension),
 with 1.6e7 points per VEGAS iteration. Wavefunctions, amplitudes and the colour sum are
-FP32; phase space and VEGAS accumulation are FP64. |M|^2 runs through the colour-ordered
-kernels (k_ordered_jamps + MFMA k_colour_mfma, process gg_ttxgg_ordered); --process
-gg_ttxgg runs the same physics through the fused Feynman-diagram kernel.
+FP32; phase space and VEGAS accumulation are FP64. |M|^2 runs through the drop-in class
+Matrix_1_gg_ttxgg, whose default kernels are the colour-ordered ones (k_ordered_jamps +
+MFMA k_colour_mfma); --kernels fused runs the same physics through the fused
+Feynman-diagram kernel.
 
 One step is one VEGAS iteration on the frozen grid, after the warm-up iterations have
 adapted it: Philox uniforms -> VEGAS map -> phase space -> cuts -> compaction -> fused
@@ -43,7 +44,7 @@ PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 pack
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md for the fused
 # Feynman-diagram kernel, profiles/pmc_k_ordered_gg_ttxgg_f32.md for the colour-ordered
 # kernels, jamps staged through HBM included)
-HBM_BYTES_PER_ME = {("gg_ttxgg", 32): 28105.2, ("gg_ttxgg_ordered", 32): 24776.7}
+HBM_BYTES_PER_ME = {("gg_ttxgg_fused", 32): 28105.2, ("gg_ttxgg", 32): 24776.7}
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 
 
@@ -51,12 +52,24 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (16 on a one-GPU box, the
+    share of the host's cores that box gives one GPU), else the cores this process may run
+    on; and the host's core count beside it."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    return min(threads, avail), {"host_cores": os.cpu_count(), "cores_available": avail,
+                                 "threads_from": "OMP_NUM_THREADS" if env.isdigit() and int(env) > 0 else "affinity"}
+
+
 def cpu_baseline(proc, dense, npass, params, names, budget_s=12.0):
-    """Time the CPU oracle (FP64 C++, OpenMP over events) on the first passing events of
-    the last chunk: doubles the sample until one timed call takes budget_s / 3."""
+    """Time the CPU oracle (FP64 C++ built -O3 -march=x86-64-v3, OpenMP over events) on
+    the first passing events of the last chunk: doubles the sample until one timed call
+    takes budget_s / 3."""
     import numpy as np
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    cores, host = cpu_threads()
     nmax = min(npass, 1 << 20)
     ps = dense[:nmax].cpu().numpy()
     rp = np.array([float(params[0]), float(params[1])], dtype=np.float64)
@@ -69,9 +82,11 @@ def cpu_baseline(proc, dense, npass, params, names, budget_s=12.0):
         if t > budget_s / 3 or n >= nmax:
             break
         n = min(nmax, n * 2)
-    return {"value": n / t, "unit": "ME evals/s", "cores": cores, "kind": "port",
-            "sample": "%d post-cut %s events of the benchmark stream, FP64 oracle smatrix "
-                      "(C++, OpenMP over events), %.1f s" % (n, LABEL[proc], t)}
+    out = {"value": n / t, "unit": "ME evals/s", "cores": cores, "kind": "port",
+           "sample": "%d post-cut %s events of the benchmark stream, FP64 oracle smatrix "
+                     "(C++ -O3 -march=x86-64-v3, OpenMP over events, %d threads), %.1f s" % (n, LABEL[proc], cores, t)}
+    out.update(host)
+    return out
 
 
 def cpu_c1_end_to_end(cores, n=1_000_000, warmup=2, iters=5, target=1e-3):
@@ -158,27 +173,39 @@ def _run_to_target(iterate, target, max_iter, tag):
 
 def _extend_to(iterate, res, sampled_s, seconds, torch):
     """Keep iterating (frozen proposal) until the sampling time reaches ``seconds``; returns
-    the unweighted-event figures over every sampled iteration at that common duration."""
+    the sampled iterations with the sampling time at which each ended (the first
+    len(res) iterations end together at sampled_s, when the target was reached)."""
+    out = [(sampled_s, r) for r in res]
     t0 = time.perf_counter()
     extra = 0.0
     while sampled_s + extra < seconds:
-        res.append(iterate(len(res)))
+        r = iterate(len(out))
         torch.cuda.synchronize()
         extra = time.perf_counter() - t0
-    t = sampled_s + extra
+        out.append((sampled_s + extra, r))
+    return out
+
+
+def _rate_at(timed, t_cut, wall):
+    """Unweighted-event figures of the iterations finished by sampling time t_cut, per
+    second of ``wall`` (the sampling time, or set-up plus sampling time)."""
+    res = [r for t, r in timed if t <= t_cut + 1e-9]
+    if not res:
+        return dict(iterations=0, points=0, sample_s=t_cut, wall_s=wall, n_unweighted=0.0,
+                    unweighting_efficiency=0.0, unweighted_per_s=0.0)
     sum_w = math.fsum(r["sum_f"] for r in res)
     wmax = max(r["wmax"] for r in res)
     npts = sum(r["n"] for r in res)
     n_unw = sum_w / wmax if wmax > 0 else 0.0
-    return dict(iterations=len(res), points=npts, sample_s=t, n_unweighted=n_unw,
-                unweighting_efficiency=sum_w / (npts * wmax) if wmax > 0 else 0.0, unweighted_per_s=n_unw / t)
+    return dict(iterations=len(res), points=npts, sample_s=t_cut, wall_s=wall, n_unweighted=n_unw,
+                unweighting_efficiency=sum_w / (npts * wmax) if wmax > 0 else 0.0, unweighted_per_s=n_unw / wall)
 
 
 def run_main(a, torch):
     """Plain VEGAS at the main precision: W warm-up iterations, then exactly K timed ones."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
     ig = MEIntegrand(a.process, precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
-                     **ME_ONLY_KIN[a.process])
+                     kernels=a.kernels, **ME_ONLY_KIN[a.process])
     vf = VegasFlow(ig.ndim, a.nev)
     vf.compile(ig)
     log("warm-up: %d VEGAS iterations of %d points (grid adaptation)" % (a.warmup, a.nev))
@@ -232,6 +259,7 @@ def train_flow(a, torch, ig):
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, seed=s0, log=log,
                val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta,
                patience=a.flow_patience)
+    flow.stage_steps = [flow.steps_run]
     for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
         # further stages: points drawn from the current proposal, weighted by f/q', carry
         # far more of the information where the integrand is large than uniform points
@@ -254,6 +282,7 @@ def train_flow(a, torch, ig):
                    log=log,
                    val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix,
                    f=fq, beta=a.flow_refine_beta)
+        flow.stage_steps.append(flow.steps_run)
         del yq, wq, fq
     del y, f
     torch.cuda.synchronize()
@@ -277,7 +306,7 @@ def run_flow_compare(a, torch, main):
     sync = torch.cuda.synchronize
     # ---- plain VEGAS, FP64 matrix elements
     ig64 = MEIntegrand(a.process, precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit,
-                       **ME_ONLY_KIN[a.process])
+                       kernels=a.kernels, **ME_ONLY_KIN[a.process])
     vf = VegasFlow(ig64.ndim, a.nev)
     vf.compile(ig64)
     sync()
@@ -294,7 +323,9 @@ def run_flow_compare(a, torch, main):
     res64 = _run_to_target(it64, a.target, a.max_iter, "plain fp64")
     sync()
     plain = _unweighted(res64, setup64, time.perf_counter() - t0, a.target)
-    plain["equal_time"] = _extend_to(it64, list(res64), plain["sample_s"], a.equal_time_s, torch)
+    timed64 = _extend_to(it64, list(res64), plain["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup64), torch)
+    plain["equal_time"] = _rate_at(timed64, a.equal_time_s, a.equal_time_s)
+    plain["equal_wall"] = _rate_at(timed64, a.equal_wall_s - setup64, a.equal_wall_s)
     del ig64, vf
     # ---- VEGAS + flow, FP32 matrix elements, on the main run's frozen grid
     ig = main["ig"]
@@ -305,7 +336,9 @@ def run_flow_compare(a, torch, main):
     resf = _run_to_target(itf, a.target, a.max_iter, "flow fp32")
     sync()
     learned = _unweighted(resf, setup_flow, time.perf_counter() - t0, a.target)
-    learned["equal_time"] = _extend_to(itf, list(resf), learned["sample_s"], a.equal_time_s, torch)
+    timedf = _extend_to(itf, list(resf), learned["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup_flow), torch)
+    learned["equal_time"] = _rate_at(timedf, a.equal_time_s, a.equal_time_s)
+    learned["equal_wall"] = _rate_at(timedf, a.equal_wall_s - setup_flow, a.equal_wall_s)
     learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"],
                    best_step=getattr(flow, "best_step", None), mixture=a.flow_mix)
 
@@ -317,28 +350,37 @@ def run_flow_compare(a, torch, main):
         "plain_vegas_fp64": plain,
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
-                 "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10, "train_steps": a.flow_steps,
+                 "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10,
+                 "stage_steps_max": [a.flow_steps] + [a.flow_refine_steps] * (a.flow_refine_rounds
+                                                                              if a.flow_refine_steps else 0),
+                 "stage_steps_run": list(getattr(flow, "stage_steps", [])),
                  "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience, "seed": a.flow_seed,
                  "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
         "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
                                              plain["equal_time"]["unweighted_per_s"]
                                              if plain["equal_time"]["unweighted_per_s"] > 0 else None),
+        "gain_unweighted_per_s_equal_wall": (learned["equal_wall"]["unweighted_per_s"] /
+                                             plain["equal_wall"]["unweighted_per_s"]
+                                             if plain["equal_wall"]["unweighted_per_s"] > 0 else None),
         "sigma_pull": (learned["sigma_pb"] - plain["sigma_pb"]) / math.hypot(learned["err_pb"], plain["err_pb"]),
     }
 
 
 ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5),
                "gg_ttxgg": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
-               "gg_ttxgg_ordered": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
                "gg_ttxggg": dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4)}
-LABEL = {"gg_ttx": "gg->tt", "gg_ttxg": "gg->ttg", "gg_ttxgg": "gg->ttgg", "gg_ttxgg_ordered": "gg->ttgg",
-         "gg_ttxggg": "gg->ttggg"}
-# processes that are another process through other kernels: the oracle, the flop count
-# and the kinematics of the base process apply
-BASE = {"gg_ttxgg_ordered": "gg_ttxgg"}
-KERNEL = {"gg_ttxggg": "k_ordered_jamps+k_colour_mfma<gg_ttxggg,%s>",
-          "gg_ttxgg_ordered": "k_ordered_jamps+k_colour_mfma<gg_ttxgg_ordered,%s>"}
+LABEL = {"gg_ttx": "gg->tt", "gg_ttxg": "gg->ttg", "gg_ttxgg": "gg->ttgg", "gg_ttxggg": "gg->ttggg"}
+
+
+def kernel_info(proc, kernels, precision):
+    """(registry name, kernel label, family) of a process's smatrix kernels."""
+    from madflow_amd.matrix import load_matrix
+    m, _ = load_matrix(proc, kernels=kernels)
+    real = "float" if precision == 32 else "double"
+    if m.kernel_family == "ordered":
+        return m.registry_name, "k_ordered_jamps+k_colour_mfma<%s,%s>" % (m.registry_name, real), "ordered"
+    return m.registry_name, "k_smatrix<%s,%s>" % (m.registry_name, real), "fused"
 
 
 def _kin_text(proc):
@@ -359,14 +401,14 @@ def me_only(a, torch):
     repeats of the HIP-event time around one smatrix call."""
     from madflow_amd.vegas import MEIntegrand
     proc = a.me_only
-    base = BASE.get(proc, proc)
+    base = proc
     ig = MEIntegrand(base, precision=a.precision, mapping="rambo", seed=3, events_limit=a.batch,
                      **ME_ONLY_KIN[proc])
     ig.chunk(0, a.batch, 0)
     n = ig.last_chunk_npass
     ps = ig._bufs["dense"][:n]
     from madflow_amd.matrix import load_matrix
-    m, _ = load_matrix(proc, precision=a.precision, ref_compat=True)
+    m, _ = load_matrix(proc, precision=a.precision, ref_compat=True, kernels=a.kernels)
     params = ig.params if len(m.params) == len(ig.params) else ig.params[:len(m.params)]
     m.smatrix(ps, *params)
     times = []
@@ -383,7 +425,8 @@ def me_only(a, torch):
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
         flops = json.load(fh)[base]["total"]
     peak = PEAK_TFLOPS[a.precision]
-    print(json.dumps({"process": proc, "precision": a.precision, "events": n, "repeats": a.repeats,
+    print(json.dumps({"process": proc, "kernels": m.registry_name, "precision": a.precision, "events": n,
+                      "repeats": a.repeats,
                       "median_s": t, "me_evals_per_s": n / t, "flops_per_me": flops,
                       "achieved_tflops": flops * n / t / 1e12, "frac_of_peak": flops * n / t / 1e12 / peak}))
     return 0
@@ -396,11 +439,11 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nev", type=int, default=16_000_000)
     ap.add_argument("--events-limit", type=int, default=1 << 22)
-    # gg->ttgg: the colour-ordered kernels are the faster form in FP32 and FP64; gg_ttxgg is
-    # the same physics through the fused Feynman-diagram kernel
-    ap.add_argument("--process", default="gg_ttxgg_ordered",
-                    choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"],
+    ap.add_argument("--process", default="gg_ttxgg", choices=["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"],
                     help="process of the VEGAS / flow workload (the benchmark metric is quoted on gg_ttxgg)")
+    ap.add_argument("--kernels", default=None, choices=["fused", "ordered"],
+                    help="kernel family of the matrix element (default: the drop-in class's, the faster form; "
+                         "colour-ordered for gg_ttxgg)")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--mapping", default="diet", choices=["diet", "rambo"])
     ap.add_argument("--target", type=float, default=1e-3)
@@ -423,6 +466,8 @@ def main(argv=None):
     ap.add_argument("--flow-refine-beta", type=float, default=3.0, help="the same for the later stages")
     ap.add_argument("--equal-time-s", type=float, default=6.0,
                     help="after reaching the target, keep sampling both methods to this much sampling time")
+    ap.add_argument("--equal-wall-s", type=float, default=14.0,
+                    help="... and to this much wall time with each method's set-up counted")
     ap.add_argument("--me-only", default=None, metavar="PROC",
                     help="time only the smatrix kernel of PROC on pre-generated post-cut events and exit")
     ap.add_argument("--batch", type=int, default=1 << 22)
@@ -449,7 +494,8 @@ def main(argv=None):
         print(json.dumps({"copy_probe_bytes_per_copy": src.numel() * 4, "copies": 4}))
         return 0
     with open(os.path.join(ROOT, "madflow_amd", "processes", "flops_per_me.json")) as fh:
-        flops = json.load(fh)[BASE.get(a.process, a.process)]["total"]
+        flops = json.load(fh)[a.process]["total"]
+    reg_name, kernel_label, family = kernel_info(a.process, a.kernels, a.precision)
     m = run_main(a, torch)
     kernel_s = m["me_ms"] * 1e-3
     achieved = flops * m["me_n"] / kernel_s / 1e12
@@ -478,26 +524,26 @@ def main(argv=None):
                                                                m["ig"].ndim, a.nev,
                                                                "FP32" if a.precision == 32 else "FP64"),
                    "points_per_step": a.nev, "passing_per_step": m["npass"] // a.steps,
+                   "matrix_element": "Matrix_1_%s (library kernels %s)" % (a.process, reg_name),
                    "me_kernels": ("colour-ordered (Berends-Giele currents per ordering, MFMA colour sum)"
-                                  if a.process in KERNEL else "fused Feynman-diagram program")},
+                                  if family == "ordered" else "fused Feynman-diagram program")},
         "me_kernel_evals_per_s": m["me_n"] / kernel_s,
         "unweighted": u,
         "unweighted_events_per_s": u["unweighted_per_s_to_target"],
         "unweighted_events_per_s_method": "plain VEGAS, %s matrix elements" % ("FP32" if a.precision == 32 else "FP64"),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": (HBM_BYTES_PER_ME[(a.process, a.precision)] * m["me_n"] / max(1, m["launches"])
-                                 if (a.process, a.precision) in HBM_BYTES_PER_ME
+                     "traffic": (HBM_BYTES_PER_ME[(reg_name, a.precision)] * m["me_n"] / max(1, m["launches"])
+                                 if (reg_name, a.precision) in HBM_BYTES_PER_ME
                                  and a.mapping == "diet" else None),
                      "traffic_unit": "bytes per launch (PMC)",
-                     "kernel": (KERNEL.get(a.process, "k_smatrix<%s,%%s>" % a.process)
-                                % ("float" if a.precision == 32 else "double")),
+                     "kernel": kernel_label,
                      "flops_per_me": flops, "mes_per_launch": m["me_n"] / max(1, m["launches"]),
                      "kernel_ms_per_launch": m["me_ms"] / max(1, m["launches"])},
     }
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
-        res["cpu_baseline"] = cpu_baseline(BASE.get(a.process, a.process), b["dense"], m["ig"].last_chunk_npass,
+        res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass,
                                            m["ig"].params,
                                            list(m["ig"].matrix.params))
         res["cpu_baseline"]["c1_end_to_end"] = cpu_c1_end_to_end(res["cpu_baseline"]["cores"])

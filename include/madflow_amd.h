@@ -30,7 +30,10 @@ int mfx_version(void);
 const char* mfx_last_error(void);
 
 /* ------------------------------------------------------ process registry */
-/* Compiled-in processes (g g > t t~ + 0..3 g); names as in matrix_1_<name>.py. */
+/* Compiled-in processes (g g > t t~ + 0..3 g).  A process name as in matrix_1_<name>.py
+ * selects its default kernels (the faster form: colour-ordered for g g > t t~ g g and
+ * g g > t t~ g g g, fused Feynman-diagram programs for the others); <name>_fused selects
+ * the fused kernel of a process whose default is colour-ordered (gg_ttxgg_fused). */
 int mfx_process_count(void);
 const char* mfx_process_name(int proc);
 int mfx_process_index(const char* name);
@@ -87,12 +90,16 @@ int mfx_aloha(int routine, const double* ins, int n_in, int64_t nev, const doubl
 
 /* --------------------------------------------------------- phase space */
 /* RAMBO (phasespace.py:145-212) for nev events: xrand (nev, xstride) f64 device with
- * 4 uniforms per outgoing particle; sqrts per event (sqrts_stride 1) or one value
- * (sqrts_stride 0); masses host [n] (NULL = massless).  Massive events run their own
- * Newton iteration to |sum E - sqrts| <= 1e-14 sqrts.
- * p_out (nev, n, 4) f64 device, wt_out (nev,) f64 device. */
-int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double* sqrts, int sqrts_stride,
-              const double* masses, double* p_out, double* wt_out, void* stream);
+ * 4 uniforms per outgoing particle; masses host [n] (NULL = massless).  sqrts_ev NULL:
+ * every event at the host value sqrts, rejected with -9 ("not enough energy") when the
+ * masses sum to sqrts or more, like rambo(check_physical=True) (phasespace.py:152-159);
+ * else per-event device values (nev,).  Massive events run their own Newton iteration
+ * to |sum E - sqrts| <= 1e-14 sqrts (at most 50 steps).
+ * p_out (nev, n, 4) f64 device, wt_out (nev,) f64 device.  status: NULL or device int
+ * (nev,): 0 physical; 1 not enough energy for that event's sqrts (zero momenta and
+ * weight); 2 the Newton iteration did not converge (weight 0). */
+int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double* sqrts_ev, double sqrts,
+              const double* masses, double* p_out, double* wt_out, int* status, void* stream);
 
 /* Philox4x32-10 uniforms in (0,1): out (nev, ndim) f64, counter (ev0+e, dim/2, iteration), key (seed). */
 int mfx_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration, double* out,
@@ -115,8 +122,9 @@ typedef struct mfx_ps_params {
   int cut_first_leg;    /* first external leg index the cuts apply to (n_out + 2: no cuts) */
   uint32_t seed;        /* Philox key */
   double shat_min;      /* mapping 2: lower end of shat (phasespace.py:270-273 uses (sum of masses)^2) */
-  int lab_frame;        /* mapping 2: 1 = momenta boosted to the lab frame (phasespace.py:322-357), 0 = the
-                           partonic c.m. frame (the reference's default, com_output = True) */
+  int lab_frame;        /* mapping 2: 1 = momenta boosted to the lab frame (phasespace.py:322-357; what the
+                           reference's integration script writes, com_output = False at madflow_exec.py:389),
+                           0 = the partonic c.m. frame (PhaseSpaceGenerator's own default, phasespace.py:377) */
   int reserved;
 } mfx_ps_params;
 
@@ -126,9 +134,12 @@ int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins);
 /* One VEGAS iteration's events: Philox uniforms (counter ev0+e, iteration), or the
  * points y_in (nev, ndim) f32 of an external proposal, -> VEGAS map (grid: device
  * [ndim][nbins+1]) -> RAMBO -> cuts (phasespace.py:424-520; in the partonic c.m. frame,
- * as the reference applies them).  w_in (optional, nev) multiplies the weights (e.g. 1/q
- * of the proposal).  Writes the full momenta all_ps (nev, n_out+2, 4), wps = phase-space
- * weight x VEGAS jacobian x gev2pb/(2 shat) (0 for cut events), flag (1 = pass), slot
+ * as the reference applies them).  Rejected with -2 before any launch when the outgoing
+ * masses sum to sqrts or more (phasespace.py:152-159) or, for the hadronic mapping, when
+ * shat_min lies outside [(sum of masses)^2, s).  w_in (optional, nev) multiplies the
+ * weights (e.g. 1/q of the proposal).  Writes the full momenta all_ps (nev, n_out+2, 4),
+ * wps = phase-space weight x VEGAS jacobian x gev2pb/(2 shat) (0 for cut events), flag
+ * (1 = pass, 0 = cut, 2 = the massive Newton iteration did not converge), slot
  * (dense index or -1) and the passing events' momenta densely into dense (the input of
  * mfx_smatrix, the reference's boolean_mask compaction).  x12 (optional, nev x 2): the
  * momentum fractions x1, x2 (1, 1 for the fixed-energy mappings).  *npass_host receives

@@ -36,7 +36,7 @@ UNITS = [
     "procs/gg_ttx.hip",
     "procs/gg_ttxg.hip",
     "procs/gg_ttxgg.hip",
-    "procs/gg_ttxgg_ordered.hip",
+    "procs/gg_ttxgg_fused.hip",
     "procs/gg_ttxggg.hip",
 ]
 

@@ -17,12 +17,15 @@ static Registry& registry() {
   return r;
 }
 // Called once per process translation unit while the library loads; the table stays
-// sorted by number of external legs so process indices do not depend on link order.
+// sorted by (number of external legs, name) so process indices do not depend on link order.
+static bool registry_after(const ProcessEntry& a, const ProcessEntry& b) {
+  return a.nexternal != b.nexternal ? a.nexternal > b.nexternal : strcmp(a.name, b.name) > 0;
+}
 int register_process(const ProcessEntry& e) {
   Registry& r = registry();
   if (r.n >= MAXPROC) return -1;
   int pos = r.n;
-  while (pos > 0 && r.table[pos - 1].nexternal > e.nexternal) {
+  while (pos > 0 && registry_after(r.table[pos - 1], e)) {
     r.table[pos] = r.table[pos - 1];
     --pos;
   }

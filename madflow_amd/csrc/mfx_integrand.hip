@@ -143,6 +143,7 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
     const double jac = vmap(g, grid, y, x, nullptr);
     double p[RAMBO_MAXN + 2][4];           // incoming partons, then the outgoing legs
     double w, norm = g.norm, x1 = 1.0, x2 = 1.0, eh = 0.5 * g.sqrts;
+    bool converged = true;
     if (g.mapping == 2) {
       // ramboflow (phasespace.py:213-232,255-317): tau = x0 (1 - tau_min) + tau_min,
       // x1 = tau^x1', x2 = tau / x1, weight (1 - tau_min) (-ln tau); RAMBO at
@@ -152,15 +153,16 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
       x1 = pow(tau, x[1]);
       x2 = tau / x1;
       const double shat = x1 * x2 * s;
-      w = dtau * -log(tau) * rambo_event(g.rc, x + 2, sqrt(shat), p + 2);
+      w = dtau * -log(tau) * rambo_event(g.rc, x + 2, sqrt(shat), p + 2, converged);
       norm = g.gev2pb / (2.0 * shat);
       eh = 0.5 * sqrt(shat);
     } else {
-      w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p + 2) : diet_event(g.rc, x, g.sqrts, p + 2);
+      w = g.mapping == 0 ? rambo_event(g.rc, x, g.sqrts, p + 2, converged)
+                         : diet_event(g.rc, x, g.sqrts, p + 2, converged);
     }
     p[0][0] = eh; p[0][1] = 0.0; p[0][2] = 0.0; p[0][3] = eh;
     p[1][0] = eh; p[1][1] = 0.0; p[1][2] = 0.0; p[1][3] = -eh;
-    ok = y_ok && pass_cuts(g, p + 2) && isfinite(w);
+    ok = y_ok && converged && pass_cuts(g, p + 2) && isfinite(w);
     if (g.mapping == 2 && g.lab_frame) {
       // boost along z to the lab frame, rapidity -eta = (1/2) ln(x1 / x2) (phasespace.py:322-357)
       const double eta = -0.5 * log(x1 / x2), ch = cosh(eta), sh = sinh(eta);
@@ -180,7 +182,8 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
     const double wext = w_in ? w_in[e] : 1.0;
     ok = ok && isfinite(wext);
     wps[e] = ok ? w * jac * norm * wext : 0.0;
-    flag[e] = ok ? 1 : 0;
+    // 1 passes, 0 cut, 2 the massive Newton iteration did not converge (off shell)
+    flag[e] = ok ? 1 : (converged ? 0 : 2);
   }
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
@@ -272,7 +275,7 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_compact(int nexternal, int64_t ne
                                                        const uint8_t* __restrict__ flag, const int* __restrict__ offsets,
                                                        double* __restrict__ dense, int* __restrict__ slot) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
-  const bool ok = e < nev && flag[e];
+  const bool ok = e < nev && flag[e] == 1;
   __shared__ int wave_base[GEN_BLOCK / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long bal = __ballot(ok);

@@ -24,11 +24,16 @@ struct RamboConst {
   double inorm;   // 1 / (2 pi)^(3n-4)
 };
 
-__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4]);
+// Newton iterations allowed for the massive rescaling factor xi
+constexpr int NEWTON_MAX = 50;
+
+__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4], bool& converged);
 
 // x: 4n uniforms (cos theta, phi, and two for the energy) per particle; returns weight.
+// converged: false when the massive rescaling's Newton iteration did not reach
+// |sum E - sqrts| <= 1e-14 sqrts in NEWTON_MAX steps (the event is then off shell).
 __device__ __forceinline__ double rambo_event(const RamboConst& rc, const double* x, double sqrts,
-                                              double (*p)[4]) {
+                                              double (*p)[4], bool& converged) {
   const int n = rc.n;
   double Q[4] = {0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
@@ -58,20 +63,23 @@ __device__ __forceinline__ double rambo_event(const RamboConst& rc, const double
     p[i][3] = (p[i][3] + b3 * t) * xs;
   }
   double logw = rc.logw0 + (2 * n - 4) * log(sqrts);
-  if (rc.massive) logw += massive_rescale(rc, sqrts, p);
+  converged = true;
+  if (rc.massive) logw += massive_rescale(rc, sqrts, p, converged);
   return exp(logw) * rc.inorm;
 }
 
 // Massive momenta from massless ones (phasespace.py:195-212): rescale the 3-momenta by
 // xi with sum_i sqrt(m_i^2 + xi^2 E_i^2) = sqrts; returns the log of the weight factor
-// xi^(2n-3) prod(|p_i|/E_i) / sum(|p_i|^2/E_i) * sqrts.
-__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4]) {
+// xi^(2n-3) prod(|p_i|/E_i) / sum(|p_i|^2/E_i) * sqrts.  Needs sum m_i < sqrts (the
+// callers check it); converged reports whether Newton met its tolerance.
+__device__ __forceinline__ double massive_rescale(const RamboConst& rc, double sqrts, double (*p)[4], bool& converged) {
   const int n = rc.n;
   double logw = 0.0;
+  converged = false;
   {
     // Newton for xi, starting above the root (f(xi0) >= 0 by Minkowski's inequality)
     double xi = sqrt(1.0 - (rc.total_mass / sqrts) * (rc.total_mass / sqrts));
-    for (int it = 0; it < 50; ++it) {
+    for (int it = 0; it < NEWTON_MAX; ++it) {
       const double x2 = xi * xi;
       double f0 = -sqrts, g0 = 0.0;
       for (int i = 0; i < n; ++i) {
@@ -80,7 +88,10 @@ __device__ __forceinline__ double massive_rescale(const RamboConst& rc, double s
         f0 += nE;
         g0 += E2 / nE;
       }
-      if (fabs(f0) <= 1e-14 * sqrts) break;
+      if (fabs(f0) <= 1e-14 * sqrts) {
+        converged = true;
+        break;
+      }
       xi -= f0 / (xi * g0);
     }
     const double x2 = xi * xi;
@@ -123,7 +134,8 @@ __device__ __host__ inline double diet_solve(double r, int k) {
 // whose invariant masses are drawn from their exact marginal densities; massive
 // particles are then obtained with RAMBO's xi rescaling.  x[0..n-3] fix the masses,
 // x[n-2+2i], x[n-1+2i] the decay angles of step i.
-__device__ __forceinline__ double diet_event(const RamboConst& rc, const double* x, double sqrts, double (*p)[4]) {
+__device__ __forceinline__ double diet_event(const RamboConst& rc, const double* x, double sqrts, double (*p)[4],
+                                             bool& converged) {
   const int n = rc.n;
   double Q[4] = {sqrts, 0.0, 0.0, 0.0};
   double M = sqrts;
@@ -149,7 +161,8 @@ __device__ __forceinline__ double diet_event(const RamboConst& rc, const double*
   }
   for (int j = 0; j < 4; ++j) p[n - 1][j] = Q[j];
   double logw = rc.logw0 + (2 * n - 4) * log(sqrts);
-  if (rc.massive) logw += massive_rescale(rc, sqrts, p);
+  converged = true;
+  if (rc.massive) logw += massive_rescale(rc, sqrts, p, converged);
   return exp(logw) * rc.inorm;
 }
 

@@ -4,6 +4,7 @@
 #include "mfx_api_common.h"
 #include "../../include/madflow_amd.h"
 #include <math.h>
+#include <stdio.h>
 
 namespace mfx {
 
@@ -24,17 +25,34 @@ RamboConst make_rambo_const(int n, const double* masses) {
   return rc;
 }
 
+// status per event: 0 physical, 1 not enough energy (sum of masses >= sqrts: zero momenta
+// and weight, the per-event form of phasespace.py:152-159), 2 the massive Newton
+// iteration did not converge (weight 0).
 __global__ void k_rambo(const double* __restrict__ xrand, int64_t nev, int xstride, RamboConst rc,
-                        const double* __restrict__ sqrts, int sqrts_stride, double* __restrict__ p_out,
-                        double* __restrict__ wt_out) {
+                        const double* __restrict__ sqrts_ev, double sqrts, double* __restrict__ p_out,
+                        double* __restrict__ wt_out, int* __restrict__ status) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nev) return;
   double p[RAMBO_MAXN][4];
-  const double s = sqrts[sqrts_stride ? e : 0];
-  const double w = rambo_event(rc, xrand + e * xstride, s, p);
+  const double s = sqrts_ev ? sqrts_ev[e] : sqrts;
+  double w = 0.0;
+  int st = 0;
+  if (!(s > rc.total_mass)) {
+    st = 1;
+    for (int i = 0; i < rc.n; ++i)
+      for (int k = 0; k < 4; ++k) p[i][k] = 0.0;
+  } else {
+    bool converged;
+    w = rambo_event(rc, xrand + e * xstride, s, p, converged);
+    if (!converged) {
+      st = 2;
+      w = 0.0;
+    }
+  }
   for (int i = 0; i < rc.n; ++i)
     for (int k = 0; k < 4; ++k) p_out[(e * rc.n + i) * 4 + k] = p[i][k];
   wt_out[e] = w;
+  if (status) status[e] = st;
 }
 
 __global__ void k_philox(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration,
@@ -55,17 +73,26 @@ using namespace mfx;
 
 extern "C" {
 
-int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double* sqrts, int sqrts_stride,
-              const double* masses, double* p_out, double* wt_out, void* stream) {
+int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double* sqrts_ev, double sqrts,
+              const double* masses, double* p_out, double* wt_out, int* status, void* stream) {
   if (n < 2 || n > RAMBO_MAXN) return fail(-1, "RAMBO needs 2 <= n <= 8 outgoing particles%s", "");
   if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
   if (xstride < 4 * n) return fail(-2, "xstride must be >= 4 n%s", "");
-  if (nev == 0) return 0;
-  if (!xrand || !sqrts || !p_out || !wt_out) return fail(-3, "null pointer%s", "");
+  if (masses)
+    for (int i = 0; i < n; ++i)
+      if (!(masses[i] >= 0.0)) return fail(-2, "masses must be >= 0%s", "");
   RamboConst rc = make_rambo_const(n, masses);
+  if (!sqrts_ev && !(sqrts > rc.total_mass)) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "not enough energy (sqrt(s) = %g) for outgoing masses summing to %g", sqrts,
+             rc.total_mass);
+    return fail(-9, "%s", msg);
+  }
+  if (nev == 0) return 0;
+  if (!xrand || !p_out || !wt_out) return fail(-3, "null pointer%s", "");
   dim3 grid((unsigned)((nev + 255) / 256)), blk(256);
-  hipLaunchKernelGGL(k_rambo, grid, blk, 0, (hipStream_t)stream, xrand, nev, xstride, rc, sqrts, sqrts_stride, p_out,
-                     wt_out);
+  hipLaunchKernelGGL(k_rambo, grid, blk, 0, (hipStream_t)stream, xrand, nev, xstride, rc, sqrts_ev, sqrts, p_out,
+                     wt_out, status);
   return hip_status(hipGetLastError());
 }
 

@@ -266,6 +266,7 @@ class RQSFlow:
                 log("flow step %d loss %.5f" % (it, float(loss)))
             if it % 50 == 0:
                 losses.append(loss)
+        self.steps_run = it          # optimiser steps taken (early stopping may end before steps)
         if best is not None:
             self.params.copy_(best[2])
             self.m1.copy_(best[3])

@@ -34,28 +34,44 @@ def _mass(p):
     return math.sqrt(m2) if m2 > 0 else 0.0
 
 
-def write_lhe(path, process, momenta, sigma_pb, err_pb, sqrts, scale=0.0, aqed=0.0, aqcd=0.0, process_id=1):
+def write_lhe(path, process, momenta, sigma_pb, err_pb, sqrts, scale=0.0, aqed=0.0, aqcd=0.0, process_id=1,
+              weights=None, wmax=None):
     """Write unweighted events to ``path`` (gzip if it ends in .gz).
 
     momenta: (nev, nexternal, 4) array (E, px, py, pz) in GeV, legs in the process
     order (two incoming first).  Every event gets the weight sigma_pb (unweighted events,
-    IDWTUP = 3).  Returns the number of events written."""
+    IDWTUP = 3), except that with ``weights`` (the kept events' weights w, as returned by
+    MEIntegrand.unweighted_events) and ``wmax`` an event with w > wmax, which hit-or-miss
+    keeps with probability 1 < w / wmax, carries sigma_pb * w / wmax so the sample stays
+    unbiased; the file then declares weighted events (IDWTUP = 4) and its header gives the
+    number of over-weight events.  Returns the number of events written."""
     pdg = PDG[process]
     ps = np.asarray(momenta, dtype=np.float64)
     if ps.ndim != 3 or ps.shape[1] != len(pdg) or ps.shape[2] != 4:
         raise ValueError("momenta must have shape (nev, %d, 4)" % len(pdg))
+    xw = np.full(len(ps), float(sigma_pb))
+    if weights is not None:
+        if wmax is None or not wmax > 0:
+            raise ValueError("weights need the positive maximum weight wmax they were unweighted against")
+        w = np.asarray(weights, dtype=np.float64).reshape(-1)
+        if w.shape != (len(ps),):
+            raise ValueError("weights must have one entry per event")
+        xw = float(sigma_pb) * np.maximum(1.0, w / float(wmax))
+    over = int(np.count_nonzero(xw != float(sigma_pb)))
+    idwtup = 4 if over else 3
     opener = gzip.open if str(path).endswith(".gz") else open
     ebeam = 0.5 * float(sqrts)
     with opener(path, "wt") as f:
         f.write('<LesHouchesEvents version="3.0">\n')
-        f.write("<header>\n<madflow_amd process=\"%s\" events=\"%d\"/>\n</header>\n" % (process, len(ps)))
+        f.write("<header>\n<madflow_amd process=\"%s\" events=\"%d\" overweight_events=\"%d\"/>\n</header>\n"
+                % (process, len(ps), over))
         f.write("<init>\n")
-        f.write(" %d %d %.10e %.10e 0 0 0 0 3 1\n" % (pdg[0], pdg[1], ebeam, ebeam))
+        f.write(" %d %d %.10e %.10e 0 0 0 0 %d 1\n" % (pdg[0], pdg[1], ebeam, ebeam, idwtup))
         f.write(" %.10e %.10e %.10e %d\n" % (sigma_pb, err_pb, sigma_pb, process_id))
         f.write("</init>\n")
-        for ev in ps:
+        for ev, wt in zip(ps, xw):
             f.write("<event>\n")
-            f.write(" %d %d %+.10e %.10e %.10e %.10e\n" % (len(pdg), process_id, sigma_pb, scale, aqed, aqcd))
+            f.write(" %d %d %+.10e %.10e %.10e %.10e\n" % (len(pdg), process_id, wt, scale, aqed, aqcd))
             for leg, p in enumerate(ev):
                 status = -1 if leg < 2 else 1
                 f.write(" %8d %2d 0 0 0 0 %+.12e %+.12e %+.12e %.12e %.12e 0.0 9.0\n" % (

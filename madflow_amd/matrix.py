@@ -10,6 +10,11 @@ device buffers to the fused HIP kernel (``mfx_smatrix`` / ``mfx_matrix``):
   Returns a float64 ``(nev,)`` tensor on the same device.
 * ``matrix(all_ps, hel, *params)``: one helicity row, colour-summed, not averaged.
 
+Each process has one or two kernel families in the library (``kernels`` of the class, the
+first the default): ``"fused"``, one straight-line Feynman-diagram program per helicity
+row, and ``"ordered"``, colour-ordered currents with an MFMA colour sum.  The default is
+the faster form; ``Matrix_1_<proc>(kernels=...)`` selects another.
+
 Precision is a runtime choice (``precision=64`` FP64 build, ``32`` FP32 build) of the
 same kernels; ``ref_compat=True`` uses the reference's float32-rounded sqrt(1/2)
 (``python_package/madflow/wavefunctions_flow.py:10``) so FP64 results match it to
@@ -28,7 +33,6 @@ _PROC_PARAMS = {
     "gg_ttx": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
     "gg_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "gg_ttxgg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
-    "gg_ttxgg_ordered": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "gg_ttxggg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
 }
 _GC_SLOT = {"GC_10": 0, "GC_11": 1, "GC_12": 2}
@@ -59,18 +63,30 @@ class MatrixBase:
     params = ()
     precision = 64
     ref_compat = True
+    kernels = ("fused",)
 
-    def __init__(self, precision=None, ref_compat=None):
+    def __init__(self, precision=None, ref_compat=None, kernels=None):
         if precision is not None:
             self.precision = int(precision)
         if ref_compat is not None:
             self.ref_compat = bool(ref_compat)
+        if kernels is not None and kernels not in self.kernels:
+            raise ValueError("%s has the kernel families %s, not %r" % (type(self).__name__, self.kernels, kernels))
+        self.kernel_family = kernels or self.kernels[0]
         self._idx = None
+        self._ws = {}
+
+    @property
+    def registry_name(self):
+        """Name of the kernels in the library's process registry."""
+        if self.kernel_family == self.kernels[0]:
+            return self.process_name
+        return "%s_%s" % (self.process_name, self.kernel_family)
 
     @property
     def index(self):
         if self._idx is None:
-            self._idx = process_index(self.process_name)
+            self._idx = process_index(self.registry_name)
         return self._idx
 
     # ---------------------------------------------------------------- helpers
@@ -116,7 +132,9 @@ class MatrixBase:
 
     def _workspace(self, nev, per_helicity, device):
         """(pointer, bytes) of a cached device workspace, grown on demand; (None, 0) when the
-        process needs none."""
+        process needs none.  One workspace is kept per (device, current stream) and is
+        allocated while that stream is current, so the caching allocator orders its reuse
+        after that stream's work, and calls on different streams never share one."""
         lib = _lib.load()
         need = int(lib.mfx_smatrix_workspace_size(self.index, nev, self.precision, int(per_helicity)))
         if need < 0:
@@ -124,10 +142,11 @@ class MatrixBase:
         if need == 0:
             return None, 0
         size = min(need, self.workspace_limit)
-        ws = getattr(self, "_ws", None)
-        if ws is None or ws.numel() < size or ws.device != device:
-            self._ws = None
-            ws = self._ws = torch.empty(size, dtype=torch.uint8, device=device)
+        key = (device.index, torch.cuda.current_stream(device).cuda_stream)
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < size:
+            self._ws.pop(key, None)
+            ws = self._ws[key] = torch.empty(size, dtype=torch.uint8, device=device)
         return ctypes.c_void_p(ws.data_ptr()), ws.numel()
 
     def _smatrix(self, all_ps, params):
@@ -169,7 +188,8 @@ class MatrixBase:
 
 
 def load_matrix(process_name, **kw):
-    """Instantiate the drop-in class of a generated process module by name."""
+    """Instantiate the drop-in class of a generated process module by name (keywords:
+    precision, ref_compat, kernels)."""
     import importlib
     mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % process_name)
     cls = getattr(mod, "Matrix_1_%s" % process_name)

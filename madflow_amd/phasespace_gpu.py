@@ -30,31 +30,45 @@ def philox_uniforms(nev, ndim, seed, iteration, ev0=0, device="cuda"):
     return out
 
 
-def rambo(xrand, n_particles, sqrts, masses=None):
-    """RAMBO on device: xrand (nev, >= 4n) float64 GPU tensor -> (p (nev,n,4), wt (nev,))."""
-    xrand = _lib.require_device_tensor(xrand, torch.float64, "xrand")
-    nev = xrand.shape[0]
-    dev = xrand.device
-    if isinstance(sqrts, torch.Tensor) and sqrts.numel() > 1:
-        s = _lib.require_device_tensor(sqrts.reshape(-1), torch.float64, "sqrts")
-        sstride = 1
-    else:
-        s = torch.full((1,), float(sqrts), dtype=torch.float64, device=dev)
-        sstride = 0
+def rambo(xrand, n_particles, sqrts, masses=None, check_physical=False, return_status=False):
+    """RAMBO on device: xrand (nev, >= 4n) float64 GPU tensor -> (p (nev,n,4), wt (nev,)).
+
+    ``sqrts`` is a number (every event) or a device tensor of per-event values.  A number
+    at or below the sum of the masses raises ValueError, as ``rambo(check_physical=True)``
+    does (phasespace.py:152-159; the library rejects it whatever check_physical says).  With
+    per-event values, ``check_physical`` raises when any event lacks the energy, and
+    ``return_status`` adds the per-event status (0 physical, 1 not enough energy, 2 the
+    massive Newton iteration did not converge; weight 0 for both)."""
     m = None
     if masses is not None and np.sum(masses) != 0:
         m = np.ascontiguousarray(masses, dtype=np.float64)
         if m.shape != (n_particles,):
             raise ValueError("masses must have one entry per outgoing particle")
-        if isinstance(sqrts, (int, float)) and m.sum() > sqrts:
-            raise ValueError(f"Not enough energy ({sqrts}) to generate particles of mass: {list(masses)}")
+    s_ev, s_val = None, 0.0
+    if isinstance(sqrts, torch.Tensor) and sqrts.numel() > 1:
+        s_ev = _lib.require_device_tensor(sqrts.reshape(-1), torch.float64, "sqrts")
+    else:
+        s_val = float(sqrts.reshape(()).item() if isinstance(sqrts, torch.Tensor) else sqrts)
+        if not s_val > (m.sum() if m is not None else 0.0):
+            raise ValueError("Not enough energy (%s) to generate particles of mass: %s" % (
+                s_val, list(masses) if masses is not None else []))
+    xrand = _lib.require_device_tensor(xrand, torch.float64, "xrand")
+    nev = xrand.shape[0]
+    dev = xrand.device
     p = torch.empty((nev, n_particles, 4), dtype=torch.float64, device=dev)
     wt = torch.empty(nev, dtype=torch.float64, device=dev)
+    status = torch.empty(nev, dtype=torch.int32, device=dev)
     if nev:
         _lib.check(_lib.load().mfx_rambo(
-            ctypes.c_void_p(xrand.data_ptr()), nev, xrand.shape[1], n_particles, ctypes.c_void_p(s.data_ptr()),
-            sstride, m.ctypes.data_as(ctypes.c_void_p) if m is not None else None, ctypes.c_void_p(p.data_ptr()),
-            ctypes.c_void_p(wt.data_ptr()), _lib.stream_ptr(dev)))
+            ctypes.c_void_p(xrand.data_ptr()), nev, xrand.shape[1], n_particles,
+            ctypes.c_void_p(s_ev.data_ptr()) if s_ev is not None else None, s_val,
+            m.ctypes.data_as(ctypes.c_void_p) if m is not None else None, ctypes.c_void_p(p.data_ptr()),
+            ctypes.c_void_p(wt.data_ptr()), ctypes.c_void_p(status.data_ptr()), _lib.stream_ptr(dev)))
+    if check_physical and nev and bool((status == 1).any()):
+        raise ValueError("Not enough energy to generate particles of mass %s for %d events" % (
+            (list(masses) if masses is not None else []), int((status == 1).sum())))
+    if return_status:
+        return p, wt, status
     return p, wt
 
 

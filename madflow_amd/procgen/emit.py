@@ -297,14 +297,14 @@ def _emit_ops(w, prog, desc, legs_of):
             w("  __builtin_amdgcn_sched_barrier(0);")
 
 
-def emit_hip(desc):
+def emit_hip(desc, uname=None):
     """HIP device code: constants + a templated amplitude program for one helicity.
 
     Every wavefunction is an SSA value (``W<R> wN``) so the compiler can keep the live
-    set in VGPRs; the jamp accumulators are an array indexed by constants.
+    set in VGPRs; the jamp accumulators are an array indexed by constants.  ``uname`` is
+    the name the kernels register under (the process name when they are its default).
     """
-    name = desc["name"]
-    C = _cname(name)
+    name = uname or desc["name"]
     n = desc["nexternal"]
     ncol = desc["ncolor"]
     lines = []
@@ -369,7 +369,7 @@ def emit_hip(desc):
     w("}")
     w("")
     dual = desc["ncomb"] == 1 << n
-    k = MULTI_K.get(name, 1) if dual else 0
+    k = MULTI_K.get(desc["name"], 1) if dual else 0
     if dual:
         legs, progk = multi_program(desc, k)
         for key in list(legs_of):
@@ -439,22 +439,32 @@ def emit_hip(desc):
     return "\n".join(lines) + "\n"
 
 
-def emit_hip_unit(desc):
+def emit_hip_unit(uname):
     """Per-process translation unit: instantiates the fused kernels for one process."""
     return ("// Generated by madflow_amd/procgen/emit.py — do not edit.\n"
             "#include \"../mfx_smatrix.h\"\n"
             "#include \"%s.inc\"\n"
-            "MFX_INSTANTIATE_PROCESS(%s)\n") % (desc["name"], desc["name"])
+            "MFX_INSTANTIATE_PROCESS(%s)\n") % (uname, uname)
 
 
 # ------------------------------------------------------ colour-ordered process emission
 
-# processes evaluated by the colour-ordered kernels (mfx_ordered.h): their n! jamps do not
-# fit the register file next to a Feynman-diagram program.  "gg_ttxgg_ordered" is the
-# same physics as gg_ttxgg through the ordered kernels (cross-check and comparison).
+# Kernel families of each process; the first is the default, registered under the process
+# name (what the drop-in module Matrix_1_<name> runs), the others under <name>_<family>.
+# "fused": one straight-line Feynman-diagram program per helicity row (mfx_smatrix.h);
+# "ordered": colour-ordered Berends-Giele currents per ordering and an MFMA colour sum
+# (mfx_ordered.h).  For g g > t t~ g g the colour-ordered kernels are the faster form in
+# both precisions; for g g > t t~ g g g the n! jamps of the fused form do not fit the
+# register file next to a 1,890-amplitude program.
+KERNELS = {"gg_ttx": ("fused",), "gg_ttxg": ("fused",), "gg_ttxgg": ("ordered", "fused"),
+           "gg_ttxggg": ("ordered",)}
 # helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
 MULTI_K = {}
-ORDERED = {"gg_ttxggg": "gg_ttxggg", "gg_ttxgg_ordered": "gg_ttxgg"}
+
+
+def unit_name(name, family):
+    """Registry name of a process's kernels of one family."""
+    return name if KERNELS[name][0] == family else "%s_%s" % (name, family)
 
 
 def emit_ordered_inc(desc, proc_name):
@@ -548,8 +558,9 @@ Keeps the interface of the PyOut exporter's ``matrix_<proc>.py``
 (``madgraph_plugin/template_files/matrix_method_python.inc:36-138``): the class
 ``{class_name}`` with ``nexternal, ndiags, ncomb, helicities, denominator,
 initial_states, mirror_initial_states`` and ``smatrix(all_ps, {params})``, plus
-``get_model_param``.  The per-event work runs in the fused HIP kernel behind the
-``mfx_smatrix`` C entry point; this file only validates and forwards buffers.
+``get_model_param``.  The per-event work runs in HIP kernels behind the ``mfx_smatrix``
+C entry point; this file only validates and forwards buffers.
+{kernel_text}
 """
 
 from madflow_amd.matrix import MatrixBase, model_for
@@ -575,6 +586,7 @@ class {class_name}(MatrixBase):
     helicities = {helicities}
     denominator = {denominator}
     params = {param_list}
+    kernels = {kernels}
 
     def smatrix(self, all_ps, {params}):
         """Sum over all helicities of matrix(), divided by the denominator: f64[nev]."""
@@ -599,7 +611,35 @@ def emit_module(desc):
         ngraphs=desc["ngraphs"], ncomb=desc["ncomb"], ncolor=desc["ncolor"],
         initial_states=",".join("(%d, %d)" % tuple(s) for s in desc["initial_states"]),
         mirror=desc["mirror_initial_states"], helicities=json.dumps(desc["helicities"]),
-        denominator=desc["denominator"], param_list=json.dumps(desc["params"]))
+        denominator=desc["denominator"], param_list=json.dumps(desc["params"]),
+        kernels=repr(tuple(KERNELS[desc["name"]])), kernel_text=_kernel_text(desc["name"]))
+
+
+_FAMILY_TEXT = {"fused": "the fused Feynman-diagram kernel", "ordered": "the colour-ordered kernels"}
+
+
+def _kernel_text(name):
+    fam = KERNELS[name]
+    t = "\nKernels: %s." % _FAMILY_TEXT[fam[0]]
+    for f in fam[1:]:
+        t += "\n``%s(kernels=%r)`` runs %s instead." % ("Matrix_1_" + name, f, _FAMILY_TEXT[f])
+    return t
+
+
+def emitted_files(desc):
+    """{relative path: text} of every file generated for one process description."""
+    name = desc["name"]
+    out = {os.path.join("processes", name + ".json"): json.dumps(desc, separators=(",", ":")),
+           os.path.join("processes", "matrix_1_%s.py" % name): emit_module(desc)}
+    for fam in KERNELS[name]:
+        u = unit_name(name, fam)
+        if fam == "fused":
+            out[os.path.join("csrc", "procs", u + ".inc")] = emit_hip(desc, u)
+            out[os.path.join("csrc", "procs", u + ".hip")] = emit_hip_unit(u)
+        else:
+            out[os.path.join("csrc", "procs", u + ".ordered.inc")] = emit_ordered_inc(desc, u)
+            out[os.path.join("csrc", "procs", u + ".hip")] = emit_ordered_unit(u)
+    return out
 
 
 def main(argv=None):
@@ -607,33 +647,11 @@ def main(argv=None):
     os.makedirs(INC_DIR, exist_ok=True)
     for ng in NGLUONS:
         desc = gg_ttx_ng(ng)
-        name = desc["name"]
-        with open(os.path.join(PROC_DIR, name + ".json"), "w") as f:
-            json.dump(desc, f, separators=(",", ":"))
-        with open(os.path.join(PROC_DIR, "matrix_1_%s.py" % name), "w") as f:
-            f.write(emit_module(desc))
-        if name in ORDERED:
-            stale = os.path.join(INC_DIR, name + ".inc")
-            if os.path.exists(stale):
-                os.remove(stale)
-        else:
-            with open(os.path.join(INC_DIR, name + ".inc"), "w") as f:
-                f.write(emit_hip(desc))
-            with open(os.path.join(INC_DIR, name + ".hip"), "w") as f:
-                f.write(emit_hip_unit(desc))
-        for oname, base in ORDERED.items():
-            if base != name:
-                continue
-            with open(os.path.join(INC_DIR, oname + ".ordered.inc"), "w") as f:
-                f.write(emit_ordered_inc(desc, oname))
-            with open(os.path.join(INC_DIR, oname + ".hip"), "w") as f:
-                f.write(emit_ordered_unit(oname))
-            if oname != name:
-                od = dict(desc, name=oname, class_name="Matrix_1_" + oname)
-                with open(os.path.join(PROC_DIR, "matrix_1_%s.py" % oname), "w") as f:
-                    f.write(emit_module(od))
-            print("emitted", oname, "(colour-ordered kernels)", len(desc["orderings"]), "orderings")
-        print("emitted", name, desc["ndiags"], "diagrams", desc["ngraphs"], "amplitudes")
+        for rel, text in emitted_files(desc).items():
+            with open(os.path.join(PKG, rel), "w") as f:
+                f.write(text)
+        print("emitted", desc["name"], desc["ndiags"], "diagrams", desc["ngraphs"], "amplitudes, kernels",
+              ", ".join(unit_name(desc["name"], f) + " (" + f + ")" for f in KERNELS[desc["name"]]))
 
 
 if __name__ == "__main__":

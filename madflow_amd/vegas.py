@@ -55,7 +55,6 @@ PROCESS_SETUP = {
     "gg_ttx": (2, [173.0, 173.0]),
     "gg_ttxg": (3, [173.0, 173.0, 0.0]),
     "gg_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
-    "gg_ttxgg_ordered": (4, [173.0, 173.0, 0.0, 0.0]),
     "gg_ttxggg": (5, [173.0, 173.0, 0.0, 0.0, 0.0]),
 }
 
@@ -76,16 +75,20 @@ class MEIntegrand:
     scale Q = sum of the outgoing transverse masses / 2 (madflow_exec.py:427-430), a number
     a fixed Q; ``alphas(q2)`` (optional) gives alpha_s(Q^2) per event, which then sets the
     couplings of every event, else alpha_s stays frozen.  ``lab_frame`` boosts the
-    momenta to the lab frame (the reference's default is the partonic frame); the cuts
-    act in the partonic frame either way."""
+    momenta to the lab frame; None (the default) does so in hadronic mode, as the
+    reference's integration script does (``PhaseSpaceGenerator(..., com_output=False)``,
+    madflow_exec.py:389, so its LHE momenta are lab-frame), while the phase-space class
+    alone defaults to the partonic frame (phasespace.py:377).  The cuts act in the partonic
+    frame either way.  ``kernels`` selects the matrix element's kernel family (the
+    drop-in class's default when None)."""
 
     def __init__(self, process, sqrts, precision=64, mapping="rambo", pt_min=0.0, abs_eta_max=0.0, dr_min=0.0,
                  seed=1, nbins=BINS_MAX, alpha_s=0.118, device="cuda", events_limit=1 << 22, ref_compat=True,
-                 width=None, pdf=None, scale=None, alphas=None, lab_frame=False, shat_min=None):
+                 width=None, pdf=None, scale=None, alphas=None, lab_frame=None, shat_min=None, kernels=None):
         n_out, masses = PROCESS_SETUP[process]
         self.process = process
         self.device = torch.device(device)
-        self.matrix, mod = load_matrix(process, precision=precision, ref_compat=ref_compat)
+        self.matrix, mod = load_matrix(process, precision=precision, ref_compat=ref_compat, kernels=kernels)
         model = mod.get_model_param()
         self.model = model
         self.hadronic = mapping == "hadronic"
@@ -105,7 +108,7 @@ class MEIntegrand:
         p.mapping = {"rambo": 0, "diet": 1, "hadronic": 2}[mapping]
         p.ndim = (4 * n_out, 3 * n_out - 4, 4 * n_out + 2)[p.mapping]
         p.shat_min = float(sum(masses) ** 2 if shat_min is None else shat_min)
-        p.lab_frame = int(bool(lab_frame))
+        p.lab_frame = int(bool(self.hadronic if lab_frame is None else lab_frame))
         p.nbins = nbins
         p.sqrts = float(sqrts)
         for k, m in enumerate(masses):
@@ -178,7 +181,8 @@ class MEIntegrand:
             # f = luminosity x |M|^2 x weight (madflow_exec.py:455-459), scattered to full length
             f_in = torch.zeros(n, dtype=torch.float64, device=self.device)
             if npass:
-                f_in[b["flag"][:n].bool()] = me * lumi * b["wps"][:n][b["flag"][:n].bool()]
+                ok = b["flag"][:n] == 1
+                f_in[ok] = me * lumi * b["wps"][:n][ok]
         _lib.check(self.lib.mfx_vegas_accumulate(
             ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]),
             _ptr(f_in), _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
@@ -256,7 +260,7 @@ class MEIntegrand:
         (fixed, or the dynamic sum of outgoing transverse masses / 2), alpha_s(Q^2) if a
         running alpha_s is given, and x1 f(x1) x2 f(x2) / (x1 x2) at Q^2."""
         d = b["dense"][:npass]
-        x = b["x12"][:n][b["flag"][:n].bool()]
+        x = b["x12"][:n][b["flag"][:n] == 1]
         if self.scale is None:
             out = d[:, 2:, :]
             mt = torch.sqrt(torch.clamp(out[..., 0] ** 2 - out[..., 3] ** 2, min=0.0)).sum(dim=1)

@@ -287,9 +287,10 @@ int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, cons
     } else {
       lw = diet_massless(x.data(), n_out, sqrts, M_PI, p.data());
     }
-    if (massive) lw += massive_factor(p.data(), n_out, masses, roots);
+    bool converged = true;
+    if (massive) lw += massive_factor(p.data(), n_out, masses, roots, &converged);
     double w = wtau * std::exp(lw) / std::pow(2 * M_PI, 3 * n_out - 4);
-    bool ok = std::isfinite(w);
+    bool ok = std::isfinite(w) && converged;
     for (int i = cut_first_leg - 2; i < n_out; ++i) {
       const double* q = &p[4 * i];
       if (!(pt_of(q) > pt_min)) ok = false;
@@ -316,7 +317,7 @@ int mfo_generate(int n_out, int ndim, int nbins, int mapping, double sqrts, cons
       x12[2 * e + 1] = x2;
     }
     wps[e] = ok ? w * jac * gev2pb / (2 * roots * roots) : 0.0;
-    flag[e] = ok ? 1 : 0;
+    flag[e] = ok ? 1 : (converged ? 0 : 2);   // 2: the massive Newton iteration did not converge
   }
   return 0;
 }

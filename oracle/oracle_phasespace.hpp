@@ -210,11 +210,13 @@ inline void rambo_batch(const double* xrand, int64_t nev, int xstride, int n, co
 }
 
 // Massive rescaling of massless momenta p (n x 4) in place (phasespace.py:195-212),
-// per-event Newton; returns the log weight factor.
-inline double massive_factor(double* p, int n, const double* masses, double sqrts) {
+// per-event Newton; returns the log weight factor.  *converged (if given) reports whether
+// |sum E - sqrts| <= 1e-14 sqrts was reached within 50 steps.
+inline double massive_factor(double* p, int n, const double* masses, double sqrts, bool* converged = nullptr) {
   double tm = 0;
   for (int i = 0; i < n; ++i) tm += masses[i];
   double xf = std::sqrt(1 - (tm / sqrts) * (tm / sqrts));
+  if (converged) *converged = false;
   for (int it = 0; it < 50; ++it) {
     double x2 = xf * xf, f0 = 0, g0 = 0;
     for (int i = 0; i < n; ++i) {
@@ -224,7 +226,10 @@ inline double massive_factor(double* p, int n, const double* masses, double sqrt
       g0 += E * E / nE;
     }
     f0 -= sqrts;
-    if (std::fabs(f0) <= 1e-14 * sqrts) break;
+    if (std::fabs(f0) <= 1e-14 * sqrts) {
+      if (converged) *converged = true;
+      break;
+    }
     xf = xf - f0 / (xf * g0);
   }
   double wt2 = 1.0, wt3 = 0.0, x2 = xf * xf;

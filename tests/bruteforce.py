@@ -240,3 +240,132 @@ def matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compa
     sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
     M = amplitude_tensor(p, hel, mt, wt, gs, sqh)
     return float(np.sum(np.abs(M) ** 2))
+
+
+# ------------------------------------------------ explicit colour assignments
+# The tensor recursion above keeps every external colour as an open axis, so its largest
+# current grows as 8**n_gluons (32,768 complex per Lorentz component at five gluons, and
+# its four-gluon vertex materialises 8**8).  The form below fixes every external gluon
+# colour instead and batches the assignments: a current is (B, 4, 8) with B the number of
+# assignments, the quark line (B, 4, 3, 3).  It shares the vertex algebra and the external
+# wavefunctions with the recursion above, not its tensor bookkeeping, and the two are
+# compared with each other at three and four gluons.
+#
+# Schur's lemma gives the colour sum with the first gluon's colour held at one value: the
+# amplitude is an invariant tensor, so rho(a, a') = sum over every other colour of
+# M(a, ...) conj(M(a', ...)) commutes with the (irreducible) adjoint representation and is
+# c * delta(a, a'); the full sum is 8 * rho(a0, a0) for any a0.  fix_first=False sums all
+# 8**n assignments instead (the tests use both and compare them).
+
+# colour tensors of the three four-gluon Lorentz structures, grouped by structure:
+# VVVV1 = t_c_d23 - t_b_d24, VVVV3 = t_c_d23 - t_a_d34, VVVV4 = t_b_d24 - t_a_d34
+_K_CD23 = C4[0] + C4[1]
+_K_BD24 = -C4[0] + C4[2]
+_K_AD34 = -C4[1] - C4[2]
+_GM = np.array([G[m] * METRIC[m, m] for m in range(4)])    # gamma_mu with a lowered index
+
+
+def _colour_assignments(ng, fix_first):
+    free = ng - 1 if fix_first else ng
+    idx = np.indices((8,) * free).reshape(free, -1).T
+    if fix_first:
+        idx = np.concatenate([np.zeros((idx.shape[0], 1), dtype=idx.dtype), idx], axis=1)
+    return idx                                             # (B, ng)
+
+
+def amplitude_batched(p, hel, mt, wt, gs, sqh, fix_first=True):
+    """Amplitudes M[B, j_tbar, i_top] for every assignment of external gluon colours.
+
+    Returns (M, colours, weight): sum_B |M|^2 * weight is the colour-summed |M|^2."""
+    n = p.shape[0]
+    gl = [0, 1] + list(range(4, n))
+    ng = len(gl)
+    GC10, GC11, GC12 = -gs, 1j * gs, 1j * gs * gs
+    ref = sqh != np.sqrt(0.5)
+    cols = _colour_assignments(ng, fix_first)
+    B = cols.shape[0]
+    K = {}
+    J = {}
+    for i, leg in enumerate(gl):
+        w = O.wavefunction("v", p[leg:leg + 1], 0.0, hel[leg], -1 if leg < 2 else 1, ref_compat=ref)[0]
+        K[i] = (-1 if leg < 2 else 1) * p[leg]
+        cur = np.zeros((B, 4, 8), complex)
+        cur[np.arange(B), :, cols[:, i]] = w[2:6]
+        J[(i,)] = cur
+    ubar = O.wavefunction("o", p[2:3], mt, hel[2], 1)[0][2:6]
+    vcol = O.wavefunction("i", p[3:4], mt, hel[3], -1)[0][2:6]
+
+    def KS(S):
+        return sum(K[i] for i in S)
+
+    def lower(X):                                        # metric on the Lorentz axis 1
+        return X * np.diag(METRIC)[None, :, None]
+
+    for size in range(2, ng + 1):
+        for S in itertools.combinations(range(ng), size):
+            KSv = KS(S)
+            acc = np.zeros((B, 4, 8), complex)
+            first, rest = S[0], S[1:]
+            for r in range(0, len(rest)):
+                for comb in itertools.combinations(rest, r):
+                    S1 = (first,) + comb
+                    S2 = tuple(x for x in rest if x not in comb)
+                    A, Bc = J[S1], J[S2]
+                    P1, P2, P3 = -KSv, KS(S1), KS(S2)
+                    Am, Bm = lower(A), lower(Bc)
+                    e3P = np.einsum("m,Bmc->Bc", P1 - P2, Bm)
+                    e2P = np.einsum("m,Bmb->Bb", P3 - P1, Am)
+                    d23 = np.einsum("Bmb,Bmc->Bbc", A, Bm)
+                    t1 = np.einsum("Bmb,Bab->Bma", A, np.einsum("abc,Bc->Bab", F, e3P))
+                    t2 = np.einsum("Bmc,Bac->Bma", Bc, np.einsum("abc,Bb->Bac", F, e2P))
+                    t3 = np.einsum("m,Ba->Bma", P2 - P3, np.einsum("abc,Bbc->Ba", F, d23))
+                    acc += GC10 * (t1 + t2 + t3)
+            if size >= 3:
+                for S1, S2, S3 in _partitions3(S):
+                    A, Bc, Cc = J[S1], J[S2], J[S3]
+                    d12 = np.einsum("Bmb,Bmc->Bbc", A, lower(Bc)).reshape(B, 64)
+                    d13 = np.einsum("Bmb,Bmd->Bbd", A, lower(Cc)).reshape(B, 64)
+                    d23 = np.einsum("Bmc,Bmd->Bcd", Bc, lower(Cc)).reshape(B, 64)
+                    z1 = (d12 @ _K_CD23.transpose(1, 2, 0, 3).reshape(64, 64)).reshape(B, 8, 8)   # (a, d)
+                    z2 = (d13 @ _K_BD24.transpose(1, 3, 0, 2).reshape(64, 64)).reshape(B, 8, 8)   # (a, c)
+                    z3 = (d23 @ _K_AD34.transpose(2, 3, 0, 1).reshape(64, 64)).reshape(B, 8, 8)   # (a, b)
+                    acc += GC12 * (np.einsum("Bad,Bmd->Bma", z1, Cc) + np.einsum("Bac,Bmc->Bma", z2, Bc)
+                                   + np.einsum("Bab,Bmb->Bma", z3, A))
+            J[S] = acc * (-1j / mdot(KSv, KSv))
+
+    GMt = _GM.reshape(4, 16)                             # (mu, b y)
+
+    def vertex_right(Urow, Jc):
+        """Urow (B, 4, 3, 3) x GC11 T^a gamma.J^a -> (B, 4, 3, 3):
+        out[y, m, l] = sum_{b, k} U[b, k, l] X[b, y, k, m] with
+        X[b, y, k, m] = sum_{mu, a} gamma_mu[b, y] J^{mu a} T^a[k, m]."""
+        JT = (Jc.reshape(B * 4, 8) @ T.reshape(8, 9)).reshape(B, 4, 9)              # (B, mu, k m)
+        X = (JT.transpose(0, 2, 1).reshape(B * 9, 4) @ GMt).reshape(B, 3, 3, 4, 4)  # (B, k, m, b, y)
+        return GC11 * np.einsum("Bbkl,Bkmby->Byml", Urow, X)
+
+    U = {(): np.broadcast_to(np.einsum("b,kl->bkl", ubar, np.eye(3)), (B, 4, 3, 3)).copy()}
+    Sall = tuple(range(ng))
+    M = np.zeros((B, 3, 3), complex)
+    for size in range(1, ng + 1):
+        for S in itertools.combinations(range(ng), size):
+            acc = np.zeros((B, 4, 3, 3), complex)
+            for r in range(1, size + 1):
+                for Sl in itertools.combinations(S, r):
+                    S0 = tuple(x for x in S if x not in Sl)
+                    acc += vertex_right(U[S0], J[Sl])
+            if S == Sall:
+                M = np.einsum("Byml,y->Bml", acc, vcol)
+            else:
+                P = pt_plus = p[2] + KS(S)
+                prop = 1j * (slash(P.astype(complex)) + mt * np.eye(4)) / (mdot(P, P) - mt * mt + 1j * mt * wt)
+                U[S] = np.einsum("Bbkl,bc->Bckl", acc, prop)
+    return M, cols, (8.0 if fix_first else 1.0)
+
+
+def matrix_hel_batched(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False,
+                       fix_first=True):
+    """Colour-summed |M|^2 of one helicity row from explicit colour assignments."""
+    gs = 2 * np.sqrt(np.pi * alpha_s)
+    sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
+    M, _, weight = amplitude_batched(p, hel, mt, wt, gs, sqh, fix_first=fix_first)
+    return float(np.sum(np.abs(M) ** 2) * weight)

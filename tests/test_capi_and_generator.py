@@ -24,8 +24,9 @@ def test_library_exports_header_symbols():
 def test_registry_and_argument_checks():
     lib = _lib.load()
     names = [lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())]
-    assert names[:2] == ["gg_ttx", "gg_ttxg"]
-    assert set(names) >= {"gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"}
+    # sorted by (external legs, name): the process names select the default kernels,
+    # gg_ttxgg_fused the fused kernel of g g > t t~ g g
+    assert names == ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_fused", "gg_ttxggg"]
     info = np.zeros(5, dtype=np.int32)
     ttgg = lib.mfx_process_index(b"gg_ttxgg")
     assert lib.mfx_process_info(ttgg, info.ctypes.data_as(ctypes.c_void_p)) == 0
@@ -43,8 +44,11 @@ def test_registry_and_argument_checks():
     assert lib.mfx_smatrix(99, None, 0, vp(mw), vp(gc), None, None, 64, 1, None, 0, None) < 0
     # empty batches are valid no-ops
     assert lib.mfx_smatrix(0, None, 0, vp(mw), vp(gc), None, None, 64, 1, None, 0, None) == 0
-    # the fused processes need no workspace; the colour-ordered ones say how much
-    assert lib.mfx_smatrix_workspace_size(ttgg, 1000, 32, 0) == 0
+    assert lib.mfx_process_info(lib.mfx_process_index(b"gg_ttxgg_fused"), info.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert list(info) == [6, 64, 24, 123, 159]
+    # the fused kernels need no workspace; the colour-ordered ones say how much
+    assert lib.mfx_smatrix_workspace_size(lib.mfx_process_index(b"gg_ttxgg_fused"), 1000, 32, 0) == 0
+    assert lib.mfx_smatrix_workspace_size(ttgg, 1000, 32, 0) == 1008 * 64 * 24 * 8
     t5 = lib.mfx_process_index(b"gg_ttxggg")
     assert lib.mfx_smatrix_workspace_size(t5, 1000, 32, 0) == 1008 * 128 * 120 * 8   # whole 16-event tiles
     assert lib.mfx_smatrix_workspace_size(t5, 1000, 64, 1) == 1008 * 120 * 16
@@ -53,6 +57,51 @@ def test_registry_and_argument_checks():
     fake = ctypes.c_void_p(1 << 20)
     assert lib.mfx_smatrix(t5, fake, 10, vp(mw), vp(gc), None, fake, 32, 1, None, 0, None) == -8
     assert b"workspace" in lib.mfx_last_error()
+
+
+def test_physical_input_guards():
+    """Not enough energy for the outgoing masses is rejected on the host before any launch,
+    as rambo(check_physical=True) raises (phasespace.py:152-159): mfx_rambo with one
+    sqrt(s), mfx_ps_generate for every mapping, and a hadronic shat_min below the mass
+    threshold.  The pointers are never dereferenced (fake, aligned addresses)."""
+    from madflow_amd.vegas import PsParams, _sig
+    lib = _lib.load()
+    _sig(lib)
+    fake = ctypes.c_void_p(1 << 20)
+    m = np.array([173.0, 173.0, 10.0])
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.mfx_rambo(fake, 10, 12, 3, None, ctypes.c_double(356.0), vp(m), fake, fake, None, None) == -9
+    assert b"not enough energy" in lib.mfx_last_error()
+    assert lib.mfx_rambo(fake, 10, 12, 3, None, ctypes.c_double(300.0), vp(m), fake, fake, None, None) == -9
+    assert lib.mfx_rambo(fake, 10, 12, 3, None, ctypes.c_double(500.0), vp(-m), fake, fake, None, None) == -2
+    # massless particles need only a positive energy
+    assert lib.mfx_rambo(fake, 10, 12, 3, None, ctypes.c_double(0.0), None, fake, fake, None, None) == -9
+    for mapping, ndim in ((0, 12), (1, 5), (2, 14)):
+        p = PsParams()
+        p.n_out, p.ndim, p.nbins, p.mapping = 3, ndim, 50, mapping
+        p.sqrts = 300.0
+        for k, v in enumerate(m):
+            p.masses[k] = v
+        p.gev2pb, p.cut_first_leg, p.shat_min = 1.0, 5, 356.0 ** 2
+        npass = ctypes.c_int64(0)
+        rc = lib.mfx_ps_generate(ctypes.byref(p), fake, 0, 10, 0, None, None, fake, fake, fake, fake, fake, None,
+                                 fake, 1 << 30, ctypes.byref(npass), None)
+        assert rc == -2 and b"energy" in lib.mfx_last_error(), (mapping, rc)
+    p.sqrts = 13000.0
+    p.shat_min = 300.0 ** 2          # below the (sum of masses)^2 threshold
+    rc = lib.mfx_ps_generate(ctypes.byref(p), fake, 0, 10, 0, None, None, fake, fake, fake, fake, fake, None,
+                             fake, 1 << 30, ctypes.byref(npass), None)
+    assert rc == -2 and b"shat_min" in lib.mfx_last_error()
+
+
+def test_rambo_wrapper_rejects_unphysical():
+    """The Python wrapper raises ValueError for one sqrt(s) below the masses before it
+    touches a device (the reference's message, phasespace.py:156-158)."""
+    import torch
+    from madflow_amd.phasespace_gpu import rambo
+    x = torch.zeros((4, 12), dtype=torch.float64)
+    with pytest.raises(ValueError, match="Not enough energy"):
+        rambo(x, 3, 300.0, [173.0, 173.0, 10.0])
 
 
 def test_product_path_fails_loudly_without_gpu():
@@ -82,7 +131,7 @@ def test_colour_matrices():
     cf = colour.colour_matrix(colour.all_orderings([0, 1]))
     den, num = colour.line_numerators(cf)
     assert den == [3, 3] and num == [[16, -2], [-2, 16]]      # mockup_debug_me.py:506-507
-    for gl, d0 in (([0, 1, 4], 9), ([0, 1, 4, 5], 54)):
+    for gl, d0 in (([0, 1, 4], 9), ([0, 1, 4, 5], 54), ([0, 1, 4, 5, 6], 324)):
         cf = colour.colour_matrix(colour.all_orderings(gl))
         den, num = colour.line_numerators(cf)
         n = len(cf)
@@ -93,6 +142,69 @@ def test_colour_matrices():
         assert float(v @ np.array(cf, dtype=float) @ v) > 0
         # trace of CF = N_orderings * C_F^k N (sum over colours of |T^s|^2)
         assert sum(cf[i][i] for i in range(n)) == n * colour.trace_sum(list(gl) + list(reversed(gl)))
+        # ... and every diagonal element is N C_F^k with C_F = 4/3 (Casimir of the fundamental)
+        from fractions import Fraction
+        assert all(cf[i][i] == 3 * Fraction(4, 3) ** len(gl) for i in range(n))
+
+
+# Gell-Mann generators T^a = lambda^a / 2 (explicit 3x3 matrices, independent of the
+# generator's trace algebra)
+def _gell_mann():
+    L = np.zeros((8, 3, 3), dtype=complex)
+    L[0][0, 1] = L[0][1, 0] = 1
+    L[1][0, 1], L[1][1, 0] = -1j, 1j
+    L[2][0, 0], L[2][1, 1] = 1, -1
+    L[3][0, 2] = L[3][2, 0] = 1
+    L[4][0, 2], L[4][2, 0] = -1j, 1j
+    L[5][1, 2] = L[5][2, 1] = 1
+    L[6][1, 2], L[6][2, 1] = -1j, 1j
+    L[7][0, 0] = L[7][1, 1] = 1 / np.sqrt(3)
+    L[7][2, 2] = -2 / np.sqrt(3)
+    return L / 2
+
+
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
+def test_colour_matrix_gell_mann_brute_force(name):
+    """CF[s][t] = sum over every gluon colour and the quark colours i, j of
+    (T^{a_s1} ... T^{a_sk})_ij conj((T^{a_t1} ... T^{a_tk})_ij), by explicit products of
+    Gell-Mann matrices over all 8^k colour assignments (chunked over the first gluon's
+    colour), against the integer numerators and row denominator of the committed process
+    description, which the colour-ordered and fused kernels are emitted from."""
+    with open(os.path.join(ROOT, "madflow_amd", "processes", name + ".json")) as f:
+        d = json.load(f)
+    T = _gell_mann()
+    gl = sorted({leg for o in d["orderings"] for leg in o})
+    k = len(gl)
+    pos = {leg: i for i, leg in enumerate(gl)}
+    nord = len(d["orderings"])
+    cf = np.zeros((nord, nord), dtype=complex)
+    for a0 in range(8):
+        free = np.indices((8,) * (k - 1)).reshape(k - 1, -1).T
+        cols = np.concatenate([np.full((len(free), 1), a0), free], axis=1)     # (8^(k-1), k)
+        C = np.empty((nord, len(cols) * 9), dtype=complex)
+        for s, word in enumerate(d["orderings"]):
+            M = T[cols[:, pos[word[0]]]]
+            for leg in word[1:]:
+                M = M @ T[cols[:, pos[leg]]]
+            C[s] = M.reshape(-1)
+        cf += C @ C.conj().T
+    want = np.array(d["color_num"], dtype=float) / np.array(d["color_denom"], dtype=float)[:, None]
+    assert np.max(np.abs(cf.imag)) < 1e-12
+    np.testing.assert_allclose(cf.real, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("name,k_final", [("gg_ttx", 0), ("gg_ttxg", 1), ("gg_ttxgg", 2), ("gg_ttxggg", 3)])
+def test_denominators(name, k_final):
+    """smatrix averages over the initial state and divides by the symmetry factor of the
+    final state: 2 x 2 gluon helicities, 8 x 8 gluon colours, k! for k identical final
+    gluons (256, 256, 512, 1536)."""
+    import math
+    with open(os.path.join(ROOT, "madflow_amd", "processes", name + ".json")) as f:
+        d = json.load(f)
+    assert d["denominator"] == 4 * 64 * math.factorial(k_final)
+    import importlib
+    cls = getattr(importlib.import_module("madflow_amd.processes.matrix_1_%s" % name), "Matrix_1_%s" % name)
+    assert cls.denominator == d["denominator"]
 
 
 def test_trace_sum_closed_forms():
@@ -102,13 +214,39 @@ def test_trace_sum_closed_forms():
     assert colour.trace_sum([0, 1, 1, 0]) == Fraction(16, 3)
 
 
-def test_emitted_descriptions_are_current():
-    """The committed process descriptions equal what the generator produces now."""
-    for ng in (0, 1, 2):
-        d = gg_ttx_ng(ng)
-        with open(os.path.join(ROOT, "madflow_amd", "processes", d["name"] + ".json")) as f:
-            committed = json.load(f)
-        assert json.loads(json.dumps(d)) == committed
+@pytest.mark.parametrize("ng", [0, 1, 2, 3])
+def test_emitted_files_are_current(ng):
+    """Every committed generated file (process description, drop-in module, HIP kernels of
+    each kernel family) equals what the generator produces now."""
+    from madflow_amd.procgen.emit import emitted_files
+    d = gg_ttx_ng(ng)
+    files = emitted_files(d)
+    assert os.path.join("processes", d["name"] + ".json") in files
+    for rel, text in files.items():
+        with open(os.path.join(ROOT, "madflow_amd", rel)) as f:
+            committed = f.read()
+        if rel.endswith(".json"):
+            assert json.loads(committed) == json.loads(text), rel
+        else:
+            assert committed == text, rel
+
+
+def test_kernel_families_are_registered():
+    """Each drop-in class's kernel families are compiled into the library under the names
+    the class resolves them to."""
+    import importlib
+    lib = _lib.load()
+    names = {lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())}
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"):
+        mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % name)
+        cls = getattr(mod, "Matrix_1_%s" % name)
+        for fam in cls.kernels:
+            assert cls(kernels=fam).registry_name in names, (name, fam)
+        with pytest.raises(ValueError):
+            cls(kernels="nope")
+    from madflow_amd.processes.matrix_1_gg_ttxgg import Matrix_1_gg_ttxgg
+    assert Matrix_1_gg_ttxgg().registry_name == "gg_ttxgg"
+    assert Matrix_1_gg_ttxgg.kernels[0] == "ordered"
 
 
 def test_jamp_coefficients_units():
@@ -150,7 +288,7 @@ def test_dual_row_program(ng):
             defined.add(op["out"])
 
 
-@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_ordered", "gg_ttxggg"])
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
 def test_process_module_model_params(name):
     """Every drop-in module's get_model_param() evaluates exactly the parameters its
     smatrix takes, in that order."""

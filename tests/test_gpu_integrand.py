@@ -114,6 +114,27 @@ def test_accumulate_and_refine(dev):
 
 
 @pytest.mark.gpu
+def test_device_refine_hand_worked(dev):
+    """The device refinement on the two cases worked by hand from Lepage's steps (the
+    derivations are in tests/test_oracle_vegas.py): 5 bins with histogram (0, 0, 1, 0, 0)
+    and 3 bins with histogram (0, 0, 6) at alpha 1.5."""
+    import ctypes
+    from madflow_amd import _lib
+    lib = _lib.load()
+    m1 = (0.6 / math.log(2.5)) ** 1.5
+    m2 = (0.4 / math.log(5.0 / 3.0)) ** 1.5
+    cases = [(np.linspace(0.0, 1.0, 6), [0.0, 0.0, 1.0, 0.0, 0.0], [0.0, 0.32, 0.44, 0.56, 0.68, 1.0]),
+             (np.array([0.0, 1 / 3, 2 / 3, 1.0]), [0.0, 0.0, 6.0],
+              [0.0, 1 / 3 + (1 + m2 / m1) / 9, 2 / 3 + (2 * (m1 + m2) / 3 - m1) / m2 / 3, 1.0])]
+    for edges, hist, want in cases:
+        g = torch.from_numpy(np.array([edges])).to(dev)
+        h = torch.tensor([hist], dtype=torch.float64, device=dev)
+        _lib.check(lib.mfx_vegas_refine(ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(h.data_ptr()), 1, len(hist),
+                                        ctypes.c_double(1.5), _lib.stream_ptr(dev)))
+        np.testing.assert_allclose(g.cpu().numpy()[0], want, rtol=0, atol=1e-14)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mapping", ["rambo", "diet"])
 def test_ttbar_cross_section(dev, mapping):
     """gg -> tt~ at sqrt(s) = 1 TeV, alpha_s = 0.118, m_t = 173, Gamma_t = 0: the closed
@@ -222,9 +243,13 @@ def test_unweighted_events_to_lhe(dev, tmp_path):
     ps2, w2, _ = ig.unweighted_events(n, 7, wmax)
     np.testing.assert_array_equal(ps2, ps)
     path = tmp_path / "unweighted_events.lhe.gz"
-    assert write_lhe(path, "gg_ttxgg", ps, sigma, err, 3000.0) == len(ps)
-    _, events = read_lhe(path)
+    assert write_lhe(path, "gg_ttxgg", ps, sigma, err, 3000.0, weights=w, wmax=wmax) == len(ps)
+    init, events = read_lhe(path)
     assert len(events) == len(ps)
+    # over-weight events (w > wmax, always kept) carry sigma * w / wmax, the rest sigma
+    xw = np.array([ev[0][2] for ev in events])
+    np.testing.assert_allclose(xw, sigma * np.maximum(1.0, w / wmax), rtol=1e-9)
+    assert init[0][8] == (4.0 if over else 3.0)
 
 
 @pytest.mark.gpu

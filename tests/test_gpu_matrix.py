@@ -16,6 +16,12 @@ PROCS = ["gg_ttx", "gg_ttxg", "gg_ttxgg"]
 NEV = {"gg_ttx": 512, "gg_ttxg": 256, "gg_ttxgg": 64}
 
 
+def _fused(name, **kw):
+    """The process's fused Feynman-diagram kernel (g g > t t~ g g defaults to the
+    colour-ordered kernels, tested in test_gpu_ordered.py)."""
+    return load_matrix(name, kernels="fused", **kw)
+
+
 def _params(matrix, ref_gs=True):
     rp, coup = O.default_params(ref_gs=ref_gs)
     vals = [rp[0], rp[1]] + [coup[c] for c in matrix.params[2:]]
@@ -26,7 +32,7 @@ def _params(matrix, ref_gs=True):
 @pytest.mark.parametrize("name", PROCS)
 def test_smatrix_fp64_parity(dev, name):
     print("case smatrix fp64", name, flush=True)
-    m, _ = load_matrix(name, precision=64, ref_compat=True)
+    m, _ = _fused(name, precision=64, ref_compat=True)
     ps, _ = phase_space_points(name, NEV[name], seed=11)
     rp, coup, vals = _params(m)
     ref = O.smatrix(name, ps, rp, coup, ref_compat=True)
@@ -40,7 +46,7 @@ def test_smatrix_fp64_parity(dev, name):
 @pytest.mark.parametrize("name", PROCS)
 def test_smatrix_fp32_tolerance(dev, name):
     print("case smatrix fp32", name, flush=True)
-    m, _ = load_matrix(name, precision=32, ref_compat=True)
+    m, _ = _fused(name, precision=32, ref_compat=True)
     ps, _ = phase_space_points(name, NEV[name], seed=12)
     rp, coup, vals = _params(m)
     ref = O.smatrix(name, ps, rp, coup, ref_compat=True)
@@ -56,7 +62,7 @@ def test_smatrix_fp32_tolerance(dev, name):
 def test_matrix_jamp_per_helicity(dev, name):
     """matrix(hel) and the jamps of single helicity rows equal the oracle's."""
     print("case matrix/jamp", name, flush=True)
-    m, _ = load_matrix(name, precision=64, ref_compat=True)
+    m, _ = _fused(name, precision=64, ref_compat=True)
     ps, _ = phase_space_points(name, 32, seed=13)
     rp, coup, vals = _params(m)
     tps = torch.from_numpy(ps).to(dev)
@@ -81,7 +87,7 @@ def test_gauge_invariance_gpu(dev, name):
     O(m Gamma / (p^2 - m^2)) ~ 1e-3), and helicity rows whose amplitudes vanish
     identically are skipped by using a row that is non-zero for every process."""
     print("case brst", name, flush=True)
-    m, _ = load_matrix(name, precision=64, ref_compat=False)
+    m, _ = _fused(name, precision=64, ref_compat=False)
     ps, _ = phase_space_points(name, 16, seed=14)
     rp, coup, vals = _params(m, ref_gs=False)
     vals[1] = 0.0
@@ -101,7 +107,7 @@ def test_gauge_invariance_gpu(dev, name):
 def test_running_couplings_per_event(dev):
     """Per-event couplings (running alpha_s) give the oracle's per-event values."""
     name = "gg_ttxg"
-    m, mod = load_matrix(name, precision=64, ref_compat=True)
+    m, mod = _fused(name, precision=64, ref_compat=True)
     model = mod.get_model_param()
     ps, _ = phase_space_points(name, 64, seed=15)
     alphas = np.linspace(0.1, 0.13, 64)

@@ -1,9 +1,10 @@
 """Colour-ordered kernels (Berends-Giele per colour ordering + MFMA colour contraction)
 against the CPU oracle's Feynman-diagram evaluation.
 
-gg_ttxgg_ordered is g g > t t~ g g through the colour-ordered kernels, so the same
-physics is checked against the oracle and against the fused Feynman-diagram kernel;
-gg_ttxggg (120 colour flows) is only evaluated this way.  Bars: FP64 1e-12 relative per
+g g > t t~ g g runs through the colour-ordered kernels by default (the drop-in
+Matrix_1_gg_ttxgg), so the same physics is checked against the oracle and against the
+fused Feynman-diagram kernel (kernels="fused"); gg_ttxggg (120 colour flows) is only
+evaluated this way.  Bars: FP64 1e-12 relative per
 event, jamps 1e-11 of the event's largest jamp; FP32 2e-5 median / 1e-3 worst event.
 """
 
@@ -15,7 +16,7 @@ from oracle import oracle as O
 from tests.helpers import phase_space_points
 from madflow_amd.matrix import load_matrix
 
-CASES = [("gg_ttxgg_ordered", "gg_ttxgg", 128), ("gg_ttxggg", "gg_ttxggg", 32)]
+CASES = [("gg_ttxgg", "gg_ttxgg", 128), ("gg_ttxggg", "gg_ttxggg", 32)]
 
 
 def _vals(m):
@@ -79,8 +80,9 @@ def test_ordered_matches_fused_kernel(dev):
     print("case ordered vs fused", flush=True)
     ps, _ = phase_space_points("gg_ttxgg", 4096, seed=24)
     tps = torch.from_numpy(ps).to(dev)
-    mo, _ = load_matrix("gg_ttxgg_ordered", precision=32, ref_compat=True)
-    mf, _ = load_matrix("gg_ttxgg", precision=32, ref_compat=True)
+    mo, _ = load_matrix("gg_ttxgg", precision=32, ref_compat=True)
+    mf, _ = load_matrix("gg_ttxgg", precision=32, ref_compat=True, kernels="fused")
+    assert (mo.registry_name, mf.registry_name) == ("gg_ttxgg", "gg_ttxgg_fused")
     _, _, vals = _vals(mf)
     a = mo.smatrix(tps, *vals).cpu().numpy()
     b = mf.smatrix(tps, *vals).cpu().numpy()

@@ -12,7 +12,10 @@ from oracle import oracle as O
 from tests.helpers import phase_space_points
 from madflow_amd.matrix import load_matrix
 
-NEV = {"gg_ttx": 8192, "gg_ttxg": 8192, "gg_ttxgg": 8192, "gg_ttxgg_ordered": 8192, "gg_ttxggg": 1024}
+# (process, kernel family): the default kernels of every process, and the fused
+# g g > t t~ g g kernel beside its colour-ordered default
+CASES = [("gg_ttx", None, 8192), ("gg_ttxg", None, 8192), ("gg_ttxgg", None, 8192), ("gg_ttxgg", "fused", 8192),
+         ("gg_ttxggg", None, 1024)]
 
 
 def _passes_cuts(ps, pt_min=30.0, dr_min=0.4):
@@ -31,12 +34,13 @@ def _passes_cuts(ps, pt_min=30.0, dr_min=0.4):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", list(NEV))
-def test_fp32_error_against_oracle(dev, name):
-    print("case fp32 error", name, flush=True)
-    oname = "gg_ttxgg" if name == "gg_ttxgg_ordered" else name
-    m, _ = load_matrix(name, precision=32, ref_compat=True)
-    ps, _ = phase_space_points(oname, NEV[name], seed=31)
+@pytest.mark.parametrize("name,kernels,nev", CASES)
+def test_fp32_error_against_oracle(dev, name, kernels, nev):
+    print("case fp32 error", name, kernels, flush=True)
+    oname = name
+    m, _ = load_matrix(name, precision=32, ref_compat=True, kernels=kernels)
+    name = m.registry_name
+    ps, _ = phase_space_points(oname, nev, seed=31)
     rp, coup = O.default_params(ref_gs=True)
     vals = [rp[0], rp[1]] + [coup[c] for c in m.params[2:]]
     ref = O.smatrix(oname, ps, rp, coup, ref_compat=True, nthreads=16)

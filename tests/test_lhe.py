@@ -39,3 +39,23 @@ def test_lhe_round_trip(tmp_path, name):
 def test_lhe_rejects_wrong_shape(tmp_path):
     with pytest.raises(ValueError):
         write_lhe(tmp_path / "x.lhe", "gg_ttxg", np.zeros((3, 4, 4)), 1.0, 0.1, 1000.0)
+
+
+def test_lhe_overweight_events_keep_their_weight(tmp_path):
+    """Events kept with w > wmax (hit-or-miss keeps them always) carry sigma * w / wmax, the
+    others sigma; the file then declares weighted events (IDWTUP = 4)."""
+    ps, _ = phase_space_points("gg_ttxg", 6, seed=4)
+    w = np.array([0.5, 2.0, 1.0, 3.5, 0.1, 1.0])
+    path = tmp_path / "ow.lhe"
+    write_lhe(path, "gg_ttxg", ps, sigma_pb=10.0, err_pb=0.1, sqrts=2000.0, weights=w, wmax=1.0)
+    init, events = read_lhe(path)
+    assert init[0][8] == 4.0
+    np.testing.assert_allclose([ev[0][2] for ev in events], [10.0, 20.0, 10.0, 35.0, 10.0, 10.0], rtol=1e-12)
+    with open(path) as f:
+        assert 'overweight_events="2"' in f.read()
+    # without over-weight events the file stays unweighted
+    write_lhe(path, "gg_ttxg", ps, sigma_pb=10.0, err_pb=0.1, sqrts=2000.0, weights=np.minimum(w, 1.0), wmax=1.0)
+    init, events = read_lhe(path)
+    assert init[0][8] == 3.0 and all(ev[0][2] == 10.0 for ev in events)
+    with pytest.raises(ValueError):
+        write_lhe(path, "gg_ttxg", ps, 10.0, 0.1, 2000.0, weights=w)

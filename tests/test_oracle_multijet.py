@@ -1,11 +1,13 @@
 """Pins the oracle's multi-gluon processes, for which the reference ships no vectors:
 
 * colour-dressed brute force (tests/bruteforce.py: explicit SU(3) colour, Dirac matrices,
-  Berends-Giele recursion) per helicity row;
+  Berends-Giele recursion) per helicity row, in two forms (open colour tensors up to four
+  gluons; explicit colour assignments up to five, which covers g g > t t~ g g g);
+* the helicity sum divided by 2 x 2 helicities x 8 x 8 colours x k! (k identical final
+  gluons) equals smatrix;
 * gauge (BRST) invariance: a gluon polarisation replaced by its momentum kills every jamp;
 * Lorentz invariance of |M|^2 under random boosts and rotations;
-* Bose symmetry under exchange of final-state gluons;
-* the helicity sum equals the sum of matrix() over the rows.
+* Bose symmetry under exchange of final-state gluons.
 """
 
 import numpy as np
@@ -28,6 +30,33 @@ def test_bruteforce_per_helicity(name):
         bf = BF.matrix_hel(ps[0], hel)
         orc = O.matrix_hel(name, ps, hel, rp, coup)[0]
         assert abs(orc / bf - 1) < 1e-11, (row, orc, bf)
+
+
+@pytest.mark.parametrize("name,row", [("gg_ttxg", 17), ("gg_ttxgg", 6)])
+def test_bruteforce_forms_agree(name, row):
+    """The explicit-assignment brute force equals the open-tensor one, summing all 8^n gluon
+    colour assignments or fixing the first gluon's colour (Schur's lemma, factor 8)."""
+    ps, _ = phase_space_points(name, 1, seed=36)
+    hel = O.load_process(name)["helicities"][row]
+    a = BF.matrix_hel(ps[0], hel)
+    b = BF.matrix_hel_batched(ps[0], hel, fix_first=False)
+    c = BF.matrix_hel_batched(ps[0], hel, fix_first=True)
+    assert abs(b / a - 1) < 1e-12 and abs(c / a - 1) < 1e-12, (a, b, c)
+
+
+@pytest.mark.parametrize("row", [6, 77])
+def test_bruteforce_ttggg(row):
+    """g g > t t~ g g g: the oracle's colour-summed |M|^2 of a helicity row (its 1,240
+    Feynman diagrams, 120 jamps and the 120 x 120 colour matrix over CF_DEN = 324) equals
+    the explicit-colour Berends-Giele brute force."""
+    name = "gg_ttxggg"
+    ps, _ = phase_space_points(name, 1, seed=31)
+    d = O.load_process(name)
+    rp, coup = O.default_params()
+    hel = d["helicities"][row]
+    bf = BF.matrix_hel_batched(ps[0], hel)
+    orc = O.matrix_hel(name, ps, hel, rp, coup)[0]
+    assert abs(orc / bf - 1) < 1e-11, (row, orc, bf)
 
 
 @pytest.mark.parametrize("name", ["gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
@@ -83,12 +112,19 @@ def test_bose_symmetry(name):
     np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), O.smatrix(name, q, rp, coup), rtol=1e-11)
 
 
-def test_helicity_sum():
-    name = "gg_ttxg"
-    ps, _ = phase_space_points(name, 5, seed=35)
+@pytest.mark.parametrize("name,k_final", [("gg_ttx", 0), ("gg_ttxg", 1), ("gg_ttxgg", 2), ("gg_ttxggg", 3)])
+def test_helicity_sum(name, k_final):
+    """smatrix = sum over all 2^nexternal helicity rows of matrix_hel / (4 * 64 * k!): spin
+    average 2 x 2, colour average 8 x 8, k identical final-state gluons."""
+    import itertools
+    import math
+    ps, _ = phase_space_points(name, 3, seed=35)
     d = O.load_process(name)
+    n = d["nexternal"]
+    rows = [list(h) for h in itertools.product([-1, 1], repeat=n)]
+    assert sorted(map(tuple, d["helicities"])) == sorted(map(tuple, rows))
     rp, coup = O.default_params()
-    tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in d["helicities"]) / d["denominator"]
+    tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in rows) / (4 * 64 * math.factorial(k_final))
     np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), tot, rtol=1e-13)
 
 

@@ -48,6 +48,41 @@ def test_refine_monotone_and_concentrates():
     assert w[np.argmin(np.abs((out[0, :-1] + w / 2) - 0.3))] < 0.02 < w.max()
 
 
+def test_refine_hand_worked_smoothing():
+    """Lepage's refinement worked by hand on 5 uniform bins, histogram (0, 0, 1, 0, 0).
+
+    Smoothing (ends: mean of 2, inside: mean of 3) gives (0, 1/3, 1/3, 1/3, 0), so
+    r = (0, 1/3, 1/3, 1/3, 0): bins 1..3 get the same damped importance m and the end bins
+    none (r = 0).  Each new bin holds 3m/5: the new edges fall 3/5, 6/5, 9/5 and 12/5 of
+    an old bin (width 0.2) after x = 0.2, i.e. at 0.32, 0.44, 0.56, 0.68, whatever the
+    damping exponent.  Without the smoothing, r = (0, 0, 1, 0, 0) has no bin with
+    0 < r < 1, every importance is zero and the grid would stay uniform."""
+    edges = np.linspace(0.0, 1.0, 6)[None]
+    for alpha in (1.5, 0.7):
+        out = O.vegas_refine(edges, np.array([[0.0, 0.0, 1.0, 0.0, 0.0]]), alpha=alpha)
+        np.testing.assert_allclose(out[0], [0.0, 0.32, 0.44, 0.56, 0.68, 1.0], rtol=0, atol=1e-15)
+
+
+def test_refine_hand_worked_damping():
+    """Lepage's refinement worked by hand on 3 uniform bins, histogram (0, 0, 6), alpha 1.5.
+
+    Smoothed (0, 2, 3), r = (0, 0.4, 0.6).  Damped importances m = ((1 - r) / ln(1/r))^1.5:
+    m0 = 0, m1 = (0.6 / ln 2.5)^1.5 = 0.529879..., m2 = (0.4 / ln(5/3))^1.5 = 0.692917...
+    Each new bin holds A = (m1 + m2) / 3.  Edge 1 lies in old bin 1 at the fraction A / m1;
+    edge 2 in old bin 2 at the fraction (2 A - m1) / m2:
+        x1 = 1/3 + (1/3) (1 + m2/m1) / 3          = 0.5897430785...
+        x2 = 2/3 + (1/3) (2 (m1 + m2) / 3 - m1) / m2 = 0.8039212697...
+    Without the damping (m = r) the edges would be 0.6111 and 0.8148; without the smoothing
+    r = (0, 0, 1) damps to zero everywhere and the grid would not move."""
+    m1 = (0.6 / math.log(2.5)) ** 1.5
+    m2 = (0.4 / math.log(5.0 / 3.0)) ** 1.5
+    x1 = 1 / 3 + (1 + m2 / m1) / 9
+    x2 = 2 / 3 + (2 * (m1 + m2) / 3 - m1) / m2 / 3
+    assert abs(x1 - 0.5897430785377977) < 1e-15 and abs(x2 - 0.803921269685988) < 1e-15
+    out = O.vegas_refine(np.array([[0.0, 1 / 3, 2 / 3, 1.0]]), np.array([[0.0, 0.0, 6.0]]), alpha=1.5)
+    np.testing.assert_allclose(out[0], [0.0, x1, x2, 1.0], rtol=0, atol=1e-15)
+
+
 def _vegas(f, ndim, n, iters, seed=3):
     edges = np.tile(np.linspace(0, 1, 51), (ndim, 1))
     res = []

@@ -66,6 +66,21 @@ int mfx_smatrix(int proc, const double* all_ps, int64_t nev, const double* mass_
  * its intermediates itself (custom_op/generation.py:206-211). */
 int64_t mfx_smatrix_workspace_size(int proc, int64_t nev, int precision, int per_helicity);
 
+/* mfx_smatrix without a caller workspace, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference.  It allocates nothing: the colour-ordered processes
+ * stage their jamps in a fixed device array of mfx_smatrix_noalloc_scratch_bytes()
+ * (256 MiB) compiled into the library, one copy per GPU, held from the library's first
+ * use on that GPU until the process ends; events and helicity rows are split into passes
+ * that fit it, so any nev is accepted (a process whose single 16-event tile of one
+ * helicity row would not fit is rejected with -8).  The fused processes use no scratch.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from several threads) race and give wrong results.  Arguments,
+ * results and errors are otherwise those of mfx_smatrix. */
+int mfx_smatrix_noalloc(int proc, const double* all_ps, int64_t nev, const double* mass_width, const double* gc,
+                        const double* gc_ev, double* out, int precision, int ref_compat, void* stream);
+int64_t mfx_smatrix_noalloc_scratch_bytes(void);
+
 /* matrix(all_ps, hel, ...) of one helicity vector (matrix_method_python.inc:106-138):
  * colour-summed |M|^2 of that row (no averaging).  hel: host int[nexternal], entries
  * +-1, or 4 on a gluon for the BRST/gauge check (wavefunctions_flow.py:146-152).

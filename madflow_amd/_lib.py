@@ -33,6 +33,8 @@ _SIGNATURES = {
     "mfx_process_info": ([c_int, c_vp], c_int),
     "mfx_smatrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp], c_int),
     "mfx_smatrix_workspace_size": ([c_int, c_i64, c_int, c_int], c_i64),
+    "mfx_smatrix_noalloc": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),
+    "mfx_smatrix_noalloc_scratch_bytes": ([], c_i64),
     "mfx_matrix": ([c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp], c_int),
     "mfx_wavefunction": ([c_int, c_vp, c_i64, c_dbl, c_int, c_int, c_vp, c_int, c_int, c_vp], c_int),
     "mfx_aloha": ([c_int, c_vp, c_int, c_i64, c_vp, c_dbl, c_dbl, c_vp, c_int, c_vp], c_int),

@@ -5,6 +5,7 @@
 #include "../../include/madflow_amd.h"
 #include <string.h>
 #include <stdio.h>
+#include <mutex>
 
 namespace mfx {
 static const int MAXPROC = 16;
@@ -205,6 +206,68 @@ int mfx_smatrix(int proc, const double* all_ps, int64_t nev, const double* mass_
   rc = check_workspace(p, precision, 0, nev, workspace, workspace_bytes);
   if (rc) return rc;
   return hip_status(p.launch(a, precision, 0, workspace, workspace_bytes, (hipStream_t)stream));
+}
+
+}  // extern "C"
+
+namespace mfx {
+// Device scratch of mfx_smatrix_noalloc: the jamp staging of the colour-ordered kernels,
+// one copy per GPU, held from the library's first use on that GPU until the process ends.
+// In host code the name is only a stand-in; its device address comes from
+// hipGetSymbolAddress, once per device.
+constexpr int64_t NOALLOC_SCRATCH_BYTES = 256ll << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+static std::mutex g_scratch_mutex;
+
+// Device address of the current device's scratch array (looked up on the first call on
+// that device); nullptr with mfx_last_error() set on failure.
+static void* scratch_ptr(int* rc) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *rc = hip_status(e);
+    return nullptr;
+  }
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    *rc = fail(-10, "device index out of range for the scratch table%s", "");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *rc = e != hipSuccess ? hip_status(e) : fail(-10, "hipGetSymbolAddress returned no address%s", "");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *rc = 0;
+  return g_scratch_ptr[dev];
+}
+}  // namespace mfx
+
+extern "C" {
+
+int64_t mfx_smatrix_noalloc_scratch_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
+
+int mfx_smatrix_noalloc(int proc, const double* all_ps, int64_t nev, const double* mass_width, const double* gc,
+                        const double* gc_ev, double* out, int precision, int ref_compat, void* stream) {
+  SmatrixArgs a;
+  int rc = fill_args(proc, all_ps, nev, mass_width, gc, gc_ev, out, precision, ref_compat, a);
+  if (rc) return rc;
+  const ProcessEntry& p = process(proc);
+  if (!p.ws_per_row || nev == 0)   // the fused kernels stage nothing
+    return hip_status(p.launch(a, precision, 0, nullptr, 0, (hipStream_t)stream));
+  // the launcher splits events and helicity rows to fit the scratch; it needs room for
+  // one 16-event tile of one helicity row
+  if (16 * p.ws_per_row(precision) > NOALLOC_SCRATCH_BYTES)
+    return fail(-8, "process %s needs more jamp staging than the fixed scratch of mfx_smatrix_noalloc holds", p.name);
+  void* ws = scratch_ptr(&rc);
+  if (!ws) return rc;
+  return hip_status(p.launch(a, precision, 0, ws, NOALLOC_SCRATCH_BYTES, (hipStream_t)stream));
 }
 
 int mfx_matrix(int proc, const double* all_ps, int64_t nev, const int* hel, const double* mass_width,

@@ -70,9 +70,15 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
 
 // Colour-ordered currents of one word.  J[i][j] (i <= j) is the gluon current of the
 // word positions i..j (J[i][i] = the gluon itself), O[j] the top line carrying the top and
-// the first j gluons.  pinv[a][b] (a < b) = 2 k_a.k_b of the gluons at word positions a,
-// b and tinv[a] = 2 k_top.k_a (FP32 forms only).  All indices are compile-time constants
-// after unrolling, so every current lives in registers.
+// the first j gluons.  FP32 forms only: rg[i][j] (i < j) = P^2 of the gluons at word
+// positions i..j and rt[j] = P^2 - m_t^2 of the top with the first j gluons, sums of the
+// pair invariants 2 k_a.k_b (computed in FP64 per event and rounded once).  Incoming
+// momenta enter negated, so these sums can cancel: rt[j] is taken from the shorter of its
+// two equal forms, the top with the first j gluons or the anti-top with the others
+// (momentum conservation), and of two forms of equal length from the one with the smaller
+// sum of magnitudes (summing the top with the first j gluons throughout lost up to 1e-4 of
+// |M|^2 in FP32).  All indices are compile-time constants after unrolling, so every
+// current lives in registers.
 //
 // The four-gluon sum of J(i..j) runs over the splits i <= k < l < j into
 // V2 = J(i..k), V3 = J(k+1..l), V4 = J(l+1..j).  Two of its three terms only involve dot
@@ -85,7 +91,7 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
 
 // J(i..j) (and S(i..j)) for every range ending at j in [J0, J1), shorter ranges first.
 template <int NG, int J0, int J1, class R>
-__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG][NG], const R (&pinv)[NG][NG],
+__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG][NG], const R (&rg)[NG][NG],
                                                const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
   // -i GC_12 multiplies the four-gluon structures (GC_12 = i gs^2: a real gs^2)
@@ -125,11 +131,7 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG]
       // propagator 1/P^2 of the massless gluon
       R den2;
       if constexpr (INV) {
-        den2 = R(0);
-#pragma unroll
-        for (int a = i; a <= j; ++a)
-#pragma unroll
-          for (int b = a + 1; b <= j; ++b) den2 += pinv[a][b];
+        den2 = rg[i][j];
       } else {
         R P[4];
         mom_out(N3, P);
@@ -150,8 +152,8 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG]
 // off-shell momentum of O(j) (top + gluons 0..j-1), so the gamma.J O columns are summed
 // and the top propagator is applied once per j.
 template <int NG, int O0, int O1, class R>
-__device__ __forceinline__ void ordered_top_line(W<R> (&O)[NG], const W<R> (&J)[NG][NG], const R (&pinv)[NG][NG],
-                                                 const R (&tinv)[NG], const Params<R>& prm) {
+__device__ __forceinline__ void ordered_top_line(W<R> (&O)[NG], const W<R> (&J)[NG][NG], const R (&rt)[NG],
+                                                 const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
 #pragma unroll
   for (int j = O0; j <= O1; ++j) {
@@ -164,13 +166,7 @@ __device__ __forceinline__ void ordered_top_line(W<R> (&O)[NG], const W<R> (&J)[
     mom_out(O[j], P);
     R den2;
     if constexpr (INV) {
-      den2 = R(0);
-#pragma unroll
-      for (int a = 0; a < j; ++a) {
-        den2 += tinv[a];
-#pragma unroll
-        for (int b = a + 1; b < j; ++b) den2 += pinv[a][b];
-      }
+      den2 = rt[j];
     } else {
       den2 = den_from_slots(P, prm.mdl_MT);
     }
@@ -214,7 +210,9 @@ template <class T, class R>
 struct OrderedStage {
   static constexpr int NG = T::NG;
   static constexpr int NWF = NG * 12;                  // 5 gluons x 6 complex
-  static constexpr int NINV = StableForms<R>::value ? (NG + 1) * NG / 2 : 0;   // gluon-gluon + top-gluon
+  // FP32: the pair invariants gluon-gluon and top-gluon (pair_index), then anti-top-gluon
+  static constexpr int NPI = StableForms<R>::value ? (NG + 1) * NG / 2 : 0;
+  static constexpr int NINV = StableForms<R>::value ? NPI + NG : 0;
   // FP32: the top and anti-top spinors of both helicities (momentum slots once, spinor
   // components per helicity), staged once per event; FP64 recomputes them (its LDS
   // budget is spent)
@@ -253,6 +251,13 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         const double v = 2.0 * (k[x][0] * k[y][0] - k[x][1] * k[y][1] - k[x][2] * k[y][2] - k[x][3] * k[y][3]);
         lds[(S::NWF + T::pair_index(x, y)) * BLOCK + t] = R(v);
       }
+    // anti-top with each gluon slot (outgoing anti-top: sign +1)
+    const double* ka = src + 4 * T::ATOP_LEG;
+#pragma unroll
+    for (int x = 0; x < NG; ++x) {
+      const double v = 2.0 * (k[x][0] * ka[0] - k[x][1] * ka[1] - k[x][2] * ka[2] - k[x][3] * ka[3]);
+      lds[(S::NWF + S::NPI + x) * BLOCK + t] = R(v);
+    }
   }
   if constexpr (S::NFS > 0) {
     const int* hv = a.s.use_helv ? a.s.helv : nullptr;
@@ -300,7 +305,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
   // to spare) nor for four gluons (the kernel fits two waves per SIMD without them).
   constexpr bool SHARE_TOP = S::NFS > 0 && NG >= 5;
   TopPrefix<R, PRE + 1, SHARE_TOP> OP;
-  R pinv[NG][NG], tinv[NG];
+  R rg[NG][NG], rt[NG];   // FP32 forms: propagator denominators of the ordering (above)
 #pragma unroll 1
   for (int g = r0 >> 2; g <= (r1 - 1) >> 2; ++g) {
     unsigned hbg = 0u;
@@ -336,18 +341,60 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         }
       }
       if constexpr (S::NINV > 0) {
+        // the word's pair invariants: rg[i][j] for the ranges ending at j >= PRE (the
+        // prefix ranges of the second ordering of a pair are the first's), rt[j] for the
+        // top with the first j gluons from the shorter of its two forms
+        R pp[NG][NG], tp[NG], ap[NG];
 #pragma unroll
-        for (int x = 0; x < NG; ++x) tinv[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
+        for (int x = 0; x < NG; ++x) {
+          tp[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
+          ap[x] = lds[(S::NWF + S::NPI + perm[x]) * BLOCK + t];
 #pragma unroll
-        for (int x = 0; x < NG; ++x)
+          for (int y = x + 1; y < NG; ++y) pp[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
+        }
 #pragma unroll
-          for (int y = x + 1; y < NG; ++y) {
-            if (y < PRE && !fresh) continue;
-            pinv[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
+        for (int j = 1; j < NG; ++j) {
+          if (j < PRE && !fresh) continue;
+#pragma unroll
+          for (int i = j - 1; i >= 0; --i) {
+            R acc = R(0);
+#pragma unroll
+            for (int a = i; a <= j; ++a)
+#pragma unroll
+              for (int b = a + 1; b <= j; ++b) acc += pp[a][b];
+            rg[i][j] = acc;
           }
+        }
+#pragma unroll
+        for (int j = 1; j < NG; ++j) {
+          // direct: the top and gluons 0..j-1; complementary: the anti-top and gluons j..NG-1
+          const int nd = j + j * (j - 1) / 2, nc = (NG - j) + (NG - j) * (NG - j - 1) / 2;
+          R sd = R(0), md = R(0), sc = R(0), mc = R(0);
+#pragma unroll
+          for (int a = 0; a < j; ++a) {
+            sd += tp[a];
+            md += fabs(tp[a]);
+#pragma unroll
+            for (int b = a + 1; b < j; ++b) {
+              sd += pp[a][b];
+              md += fabs(pp[a][b]);
+            }
+          }
+#pragma unroll
+          for (int a = j; a < NG; ++a) {
+            sc += ap[a];
+            mc += fabs(ap[a]);
+#pragma unroll
+            for (int b = a + 1; b < NG; ++b) {
+              sc += pp[a][b];
+              mc += fabs(pp[a][b]);
+            }
+          }
+          rt[j] = nd < nc ? sd : (nc < nd ? sc : (md <= mc ? sd : sc));
+        }
       }
-      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, pinv, prm);
-      ordered_ranges<NG, PRE, NG, R>(J, SD, pinv, prm);
+      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, rg, prm);
+      ordered_ranges<NG, PRE, NG, R>(J, SD, rg, prm);
       if constexpr (SHARE_TOP) {
         // unrolled: OP is indexed by ht
 #pragma unroll
@@ -360,14 +407,14 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
           O[0] = top;
           // O(1..PRE) only involve the prefix gluons: computed once per pair of orderings
           if (fresh) {
-            ordered_top_line<NG, 1, PRE, R>(O, J, pinv, tinv, prm);
+            ordered_top_line<NG, 1, PRE, R>(O, J, rt, prm);
 #pragma unroll
             for (int j = 1; j <= PRE; ++j) OP.o[ht][j] = O[j];
           } else {
 #pragma unroll
             for (int j = 1; j <= PRE; ++j) O[j] = OP.o[ht][j];
           }
-          ordered_top_line<NG, PRE + 1, NG - 1, R>(O, J, pinv, tinv, prm);
+          ordered_top_line<NG, PRE + 1, NG - 1, R>(O, J, rt, prm);
           const VslashF<R> x = ordered_close_col<NG, R>(O, J);
 #pragma unroll 1
           for (int ha = 0; ha < 2; ++ha) {
@@ -388,7 +435,7 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
           if constexpr (S::NFS > 0) staged_fermion(0, ht, top);
           else T::template fermions<R>(p, hbg | ((unsigned)ht << T::TOP_LEG), helv, prm, top, atop);
           O[0] = top;
-          ordered_top_line<NG, 1, NG - 1, R>(O, J, pinv, tinv, prm);
+          ordered_top_line<NG, 1, NG - 1, R>(O, J, rt, prm);
           const VslashF<R> x = ordered_close_col<NG, R>(O, J);
 #pragma unroll 1
           for (int ha = 0; ha < 2; ++ha) {

@@ -49,8 +49,10 @@ struct LayerGeom {
   int A[DMAX], B[DMAX];
   // offsets (in floats) of this layer's parameters in the fp32 parameter vector
   int oW1, ob1, oW2, ob2, oW3, ob3;
-  // offsets (in bf16 elements) of the transposed bf16 copies W2t [H][H], W3t [NP][H]
-  int oW2t, oW3t;
+  // offsets (in bf16 elements) of the bf16 copies: transposed W2t [H][H], W3t [NP][H]
+  // (B operands of the forward GEMMs) and in the original layout W2n [H][H], W3n [H][NP]
+  // (B operands of the backward GEMMs)
+  int oW2t, oW3t, oW2n, oW3n;
 };
 
 struct FlowGeom {
@@ -222,8 +224,11 @@ __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
 // One block, 16 samples; wave w computes the output tiles n0 = 16 (w + 4 i).
 // xa: LDS [16][DMAX] conditioning inputs (only nA used); h1s/h2s: LDS bf16 [16][H];
 // raws: LDS fp32 [16][NP].
+// NPC: the layer's NP when known at compile time (0: g.NP at run time).
+template <int NPC>
 __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, const unsigned short* __restrict__ Pb,
                             const float* xa, unsigned short* h1s, unsigned short* h2s, float* raws) {
+  const int NP = NPC ? NPC : g.NP;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   // layer 1 on VALU: 16 x 128 outputs, 8 per thread
   for (int q = tid; q < TILE * H; q += NT) {
@@ -249,7 +254,8 @@ __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, con
   }
   __syncthreads();
   // layer 3: raw = h2 W3 + b3, B operand from W3t [NP][H] (bf16)
-  for (int n0 = 16 * wid; n0 < g.NP; n0 += 16 * WPB) {
+#pragma unroll
+  for (int n0 = 16 * wid; n0 < NP; n0 += 16 * WPB) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < H; k0 += 16) {
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(h2s + r * H + k0 + kq);
@@ -258,7 +264,7 @@ __device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, con
     }
     for (int i = 0; i < 4; ++i) {
       const int row = kq + i, col = n0 + r;
-      raws[row * g.NP + col] = acc[i] + P[g.ob3 + col];
+      raws[row * NP + col] = acc[i] + P[g.ob3 + col];
     }
   }
   __syncthreads();
@@ -272,15 +278,23 @@ struct FwdArgs {
   float* x_out;            // (n, D)
   float* logdet;           // (n,) accumulated
   float* save_x;           // optional (n, D): copy of the layer input
-  unsigned short* save_h1; // optional (n, H) bf16
-  unsigned short* save_h2; // optional (n, H) bf16
+  // optional bf16 activations, tile-transposed [tile][H][TILE] (sample fastest, zero for the
+  // samples past n in the last tile): a backward MFMA operand (four consecutive samples of
+  // one column) is then one 8-byte load
+  unsigned short* save_h1;
+  unsigned short* save_h2;
   float* save_raw;         // optional (n, NPmax)
   int inverse;
 };
 
+// NB: the layer's number of transformed dimensions when known at compile time (its NP
+// is then a constant and the loops over it unroll), 0 for the run-time geometry.
+template <int NB>
 __global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict__ P,
                                               const unsigned short* __restrict__ Pb) {
+  constexpr int NPC = NB ? ((NB * NPD + 15) / 16) * 16 : 0;
   const LayerGeom& g = A.geo.layer[A.layer];
+  const int NP = NPC ? NPC : g.NP, nB = NB ? NB : g.nB;
   const int D = A.geo.D;
   __shared__ float xa[TILE * DMAX];
   __shared__ float xs[TILE * DMAX];
@@ -302,14 +316,14 @@ __global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict
     xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
   }
   __syncthreads();
-  mlp_forward(g, P, Pb, xa, h1s, h2s, raws);
+  mlp_forward<NPC>(g, P, Pb, xa, h1s, h2s, raws);
   // spline on every (sample, transformed dimension); the log-dets of one sample are
   // summed by one thread afterwards in a fixed order
   __shared__ float lds_ld[TILE * DMAX];
-  for (int q = tid; q < TILE * g.nB; q += NT) {
-    const int s = q / g.nB, b = q % g.nB;
+  for (int q = tid; q < TILE * nB; q += NT) {
+    const int s = q / nB, b = q % nB;
     const int d = g.B[b];
-    const float* raw = raws + s * g.NP + b * NPD;
+    const float* raw = raws + s * NP + b * NPD;
     const SplineEval r = A.inverse ? rq_inverse(raw, xs[s * DMAX + d]) : rq_forward(raw, xs[s * DMAX + d]);
     xs[s * DMAX + d] = r.z;
     lds_ld[s * DMAX + b] = r.ld;
@@ -319,7 +333,7 @@ __global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict
     const int64_t e = s0 + tid;
     if (e < A.n) {
       float ld = 0.f;
-      for (int b = 0; b < g.nB; ++b) ld += lds_ld[tid * DMAX + b];
+      for (int b = 0; b < nB; ++b) ld += lds_ld[tid * DMAX + b];
       A.logdet[e] += ld;
     }
   }
@@ -330,15 +344,148 @@ __global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict
   }
   if (A.save_h1) {
     for (int q = tid; q < TILE * H; q += NT) {
-      const int64_t e = s0 + q / H;
-      if (e < A.n) {
-        A.save_h1[e * H + q % H] = h1s[q];
-        A.save_h2[e * H + q % H] = h2s[q];
+      const int j = q / TILE, s = q % TILE;
+      const bool ok = s0 + s < A.n;
+      A.save_h1[s0 * H + q] = ok ? h1s[s * H + j] : (unsigned short)0;
+      A.save_h2[s0 * H + q] = ok ? h2s[s * H + j] : (unsigned short)0;
+    }
+    for (int q = tid; q < TILE * NP; q += NT) {
+      const int64_t e = s0 + q / NP;
+      if (e < A.n) A.save_raw[e * A.geo.NPmax + q % NP] = raws[q];
+    }
+  }
+}
+
+// Forward / inverse of one layer with NB (1..NB_LDS) transformed dimensions, persistent:
+// each block stages the layer's weights in LDS once (W2^T and W3^T in bf16, rows padded
+// to WP = H + 4 elements so that the 16 rows an MFMA operand spans fall in distinct
+// banks; W1 and the biases in fp32), then runs the tiles b, b + G, ... of 16 samples
+// through the same arithmetic as k_layer.
+constexpr int NB_LDS = 6;
+constexpr int WP = H + 4;       // padded row pitch (bf16 elements) of the LDS weights and h1/h2
+
+template <int NB>
+__global__ void __launch_bounds__(NT) k_layer_lds(FwdArgs A, const float* __restrict__ P,
+                                                  const unsigned short* __restrict__ Pb) {
+  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
+  const LayerGeom& g = A.geo.layer[A.layer];
+  const int D = A.geo.D;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  __shared__ __attribute__((aligned(16))) unsigned short W2s[H * WP];
+  __shared__ __attribute__((aligned(16))) unsigned short W3s[NP * WP];
+  __shared__ float W1s[DMAX * H], b1s[H], b2s[H], b3s[NP];
+  __shared__ float xa[TILE * DMAX], xs[TILE * DMAX], lds_ld[TILE * DMAX];
+  __shared__ __attribute__((aligned(16))) unsigned short h1s[TILE * WP];
+  __shared__ __attribute__((aligned(16))) unsigned short h2s[TILE * WP];
+  __shared__ float raws[TILE * NP];
+  // weights: bf16 rows copied 8 bytes at a time
+  for (int q = tid; q < H * (H / 4); q += NT) {
+    const int row = q / (H / 4), c = 4 * (q % (H / 4));
+    *reinterpret_cast<uint2*>(W2s + row * WP + c) = *reinterpret_cast<const uint2*>(Pb + g.oW2t + row * H + c);
+  }
+  for (int q = tid; q < NP * (H / 4); q += NT) {
+    const int row = q / (H / 4), c = 4 * (q % (H / 4));
+    *reinterpret_cast<uint2*>(W3s + row * WP + c) = *reinterpret_cast<const uint2*>(Pb + g.oW3t + row * H + c);
+  }
+  for (int q = tid; q < g.nA * H; q += NT) W1s[q] = P[g.oW1 + q];
+  for (int q = tid; q < H; q += NT) {
+    b1s[q] = P[g.ob1 + q];
+    b2s[q] = P[g.ob2 + q];
+  }
+  for (int q = tid; q < NP; q += NT) b3s[q] = P[g.ob3 + q];
+  const int64_t ntiles = (A.n + TILE - 1) / TILE;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t s0 = tile * TILE;
+    __syncthreads();                 // weights staged / the previous tile is done with the LDS
+    for (int q = tid; q < TILE * DMAX; q += NT) {
+      const int s = q / DMAX, d = q % DMAX;
+      const int64_t e = s0 + s;
+      const float v = (e < A.n && d < D) ? A.x_in[e * D + d] : 0.5f;
+      xs[q] = v;
+      if (A.save_x && e < A.n && d < D) A.save_x[e * D + d] = v;
+    }
+    __syncthreads();
+    for (int q = tid; q < TILE * DMAX; q += NT) {
+      const int s = q / DMAX, i = q % DMAX;
+      xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
+    }
+    __syncthreads();
+    // layer 1 on VALU: 16 x 128 outputs
+    for (int q = tid; q < TILE * H; q += NT) {
+      const int s = q / H, j = q % H;
+      float v = b1s[j];
+      for (int i = 0; i < g.nA; ++i) v += xa[s * DMAX + i] * W1s[i * H + j];
+      h1s[s * WP + j] = f2bf(fmaxf(v, 0.f));
+    }
+    __syncthreads();
+    // layer 2: h2 = relu(h1 W2 + b2); wave w computes the column tile 16 w
+    {
+      const int n0 = 16 * wid;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k0 = 0; k0 < H; k0 += 16) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1s + r * WP + k0 + kq);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(W2s + (n0 + r) * WP + k0 + kq);
+        acc = mfma_bf16(a, b, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = kq + i, col = n0 + r;
+        h2s[row * WP + col] = f2bf(fmaxf(acc[i] + b2s[col], 0.f));
       }
     }
-    for (int q = tid; q < TILE * g.NP; q += NT) {
-      const int64_t e = s0 + q / g.NP;
-      if (e < A.n) A.save_raw[e * A.geo.NPmax + q % g.NP] = raws[q];
+    __syncthreads();
+    // layer 3: raw = h2 W3 + b3
+#pragma unroll
+    for (int n0 = 16 * wid; n0 < NP; n0 += 16 * WPB) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k0 = 0; k0 < H; k0 += 16) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(h2s + r * WP + k0 + kq);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(W3s + (n0 + r) * WP + k0 + kq);
+        acc = mfma_bf16(a, b, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = kq + i, col = n0 + r;
+        raws[row * NP + col] = acc[i] + b3s[col];
+      }
+    }
+    __syncthreads();
+    for (int q = tid; q < TILE * NB; q += NT) {
+      const int s = q / NB, b = q % NB;
+      const int d = g.B[b];
+      const float* raw = raws + s * NP + b * NPD;
+      const SplineEval sp = A.inverse ? rq_inverse(raw, xs[s * DMAX + d]) : rq_forward(raw, xs[s * DMAX + d]);
+      xs[s * DMAX + d] = sp.z;
+      lds_ld[s * DMAX + b] = sp.ld;
+    }
+    __syncthreads();
+    if (tid < TILE) {
+      const int64_t e = s0 + tid;
+      if (e < A.n) {
+        float ld = 0.f;
+        for (int b = 0; b < NB; ++b) ld += lds_ld[tid * DMAX + b];
+        A.logdet[e] += ld;
+      }
+    }
+    for (int q = tid; q < TILE * D; q += NT) {
+      const int s = q / D, d = q % D;
+      const int64_t e = s0 + s;
+      if (e < A.n) A.x_out[e * D + d] = xs[s * DMAX + d];
+    }
+    if (A.save_h1) {
+      for (int q = tid; q < TILE * H; q += NT) {
+        const int j = q / TILE, s = q % TILE;
+        const bool ok = s0 + s < A.n;
+        A.save_h1[s0 * H + q] = ok ? h1s[s * WP + j] : (unsigned short)0;
+        A.save_h2[s0 * H + q] = ok ? h2s[s * WP + j] : (unsigned short)0;
+      }
+      for (int q = tid; q < TILE * NP; q += NT) {
+        const int64_t e = s0 + q / NP;
+        if (e < A.n) A.save_raw[e * A.geo.NPmax + q % NP] = raws[q];
+      }
     }
   }
 }
@@ -375,9 +522,9 @@ __global__ void __launch_bounds__(NT) k_layer_bwd(BwdArgs A, const float* __rest
     gxs[q] = ok ? A.gx[e * D + d] : 0.f;
   }
   for (int q = tid; q < TILE * H; q += NT) {
-    const int64_t e = s0 + q / H;
-    h1f[q] = e < A.n ? bf2f(A.h1[e * H + q % H]) : 0.f;
-    h2f[q] = e < A.n ? bf2f(A.h2[e * H + q % H]) : 0.f;
+    const int j = q / TILE, s = q % TILE;          // tile-transposed saved activations
+    h1f[s * H + j] = bf2f(A.h1[s0 * H + q]);
+    h2f[s * H + j] = bf2f(A.h2[s0 * H + q]);
   }
   __syncthreads();
   // spline backward: d raw (into go) and d x_B (replaces the incoming gradient of x_B)
@@ -469,6 +616,254 @@ __global__ void __launch_bounds__(NT) k_layer_bwd(BwdArgs A, const float* __rest
     const int s = q / D, d = q % D;
     const int64_t e = s0 + s;
     if (e < A.n) A.gx[e * D + d] = gxs[s * DMAX + d];
+  }
+}
+
+// Backward of one layer in two kernels, for layers whose transformed set has NB (1..8)
+// dimensions (NP = pad16(NB (3K+1)) is then a compile-time constant, so every loop over
+// it unrolls).  Both run their GEMMs on the bf16 matrix cores with fp32 accumulation
+// (v_mfma_f32_16x16x16bf16): the activations and weights are the bf16 values of the
+// forward pass, the deltas are rounded to bf16 like them.
+//  * k_layer_bwd_data: per 16-sample tile (one per block), the spline backward and the
+//    two input-gradient products gh2 = (go W3^T) relu'(h2), gh1 = (gh2 W2^T) relu'(h1),
+//    the input gradient gx, and the deltas go, gh2, gh1 written tile-transposed
+//    ([tile][column][sample], like the saved activations);
+//  * k_layer_wgrad: the weight gradients as GEMMs with the samples as K, over splits of
+//    WG_TILES tiles: block (split, y < 8) computes rows 16 y .. 16 y + 15 of dW2 = h1^T gh2
+//    and dW3 = h2^T go (wave w: column tiles w, w + 8, ...); block (split, 8) the bias sums
+//    as products with a row of ones and dW1 = x_A^T gh1 (fp32, from the exact inputs).
+//    One partial per split, reduced in a fixed order.
+constexpr int WG_TILES = 4;                    // tiles (of 16 samples) per weight-gradient split
+constexpr int WG_SPLIT = WG_TILES * TILE;      // samples per split
+
+__device__ __forceinline__ bf16x4 ld_bf16x4(const unsigned short* p) {
+  return *reinterpret_cast<const bf16x4*>(p);
+}
+
+template <int NB>
+__global__ void __launch_bounds__(NT) k_layer_bwd_data(BwdArgs A, const float* __restrict__ P,
+                                                       const unsigned short* __restrict__ Pb,
+                                                       unsigned short* __restrict__ d_o,
+                                                       unsigned short* __restrict__ d_h2,
+                                                       unsigned short* __restrict__ d_h1) {
+  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
+  constexpr int NPP = NP + 4;                  // padded bf16 row of go (distinct banks per row)
+  static_assert(H / 16 == WPB, "one column tile of gh2 / gh1 per wave");
+  const LayerGeom& g = A.geo.layer[A.layer];
+  const int D = A.geo.D;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int r = lane & 15, kq = (lane >> 4) * 4;   // bf16 MFMA: four consecutive k per lane
+  const int n0 = 16 * wid;                         // this wave's column tile of gh2 / gh1
+  const int64_t tile = blockIdx.x, s0 = tile * TILE;
+  __shared__ float xs[TILE * DMAX], gxs[TILE * DMAX];
+  __shared__ __attribute__((aligned(16))) unsigned short h1b[TILE * WP], h2b[TILE * WP], ghb[TILE * WP];
+  __shared__ __attribute__((aligned(16))) unsigned short gob[TILE * NPP];
+  __shared__ float go[TILE * NP], gh1[TILE * H], W1s[DMAX * H];
+  for (int q = tid; q < TILE * DMAX; q += NT) {
+    const int s = q / DMAX, d = q % DMAX;
+    const int64_t e = s0 + s;
+    const bool ok = e < A.n && d < D;
+    xs[q] = ok ? A.x_in[e * D + d] : 0.5f;
+    gxs[q] = ok ? A.gx[e * D + d] : 0.f;
+  }
+  for (int q = tid; q < g.nA * H; q += NT) W1s[q] = P[g.oW1 + q];
+  for (int q = tid; q < TILE * H; q += NT) {
+    const int j = q / TILE, s = q % TILE;        // saved activations are tile-transposed
+    h1b[s * WP + j] = A.h1[s0 * H + q];
+    h2b[s * WP + j] = A.h2[s0 * H + q];
+  }
+  for (int q = tid; q < TILE * NP; q += NT) go[q] = 0.f;
+  __syncthreads();
+  // spline backward: d raw (into go) and d x_B (replaces the incoming gradient of x_B),
+  // written straight into LDS (a private array indexed by the bin would live in scratch)
+  for (int q = tid; q < TILE * NB; q += NT) {
+    const int s = q / NB, b = q % NB;
+    const int64_t e = s0 + s;
+    if (e >= A.n) continue;
+    const int d = g.B[b];
+    gxs[s * DMAX + d] = rq_backward(A.raw + e * A.geo.NPmax + b * NPD, xs[s * DMAX + d], gxs[s * DMAX + d],
+                                    A.gl[e], go + s * NP + b * NPD);
+  }
+  __syncthreads();
+  // go in bf16: the A operand of gh2 (LDS) and the delta of dW3 / b3 (global, tile-transposed)
+  for (int q = tid; q < TILE * NP; q += NT) {
+    const int c = q / TILE, s = q % TILE;
+    const unsigned short v = f2bf(go[s * NP + c]);
+    gob[s * NPP + c] = v;
+    d_o[s0 * NP + q] = v;
+  }
+  __syncthreads();
+  // gh2 = (go W3^T) * relu'(h2): B[k = np][n = h] = W3[h][np], the row h of W3n
+  {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < NP; k0 += 16)
+      acc = mfma_bf16(ld_bf16x4(gob + r * NPP + k0 + kq), ld_bf16x4(Pb + g.oW3n + (n0 + r) * NP + k0 + kq), acc);
+    unsigned short out[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = kq + i, col = n0 + r;
+      const unsigned short v = f2bf(bf2f(h2b[row * WP + col]) > 0.f ? acc[i] : 0.f);
+      ghb[row * WP + col] = v;
+      out[i] = v;
+    }
+    // the four samples of column n0 + r: one 8-byte store
+    *reinterpret_cast<uint2*>(d_h2 + s0 * H + (n0 + r) * TILE + kq) =
+        make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
+  }
+  __syncthreads();
+  // gh1 = (gh2 W2^T) * relu'(h1): B[k = o][n = i] = W2[i][o], the row i of W2n
+  {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < H; k0 += 16)
+      acc = mfma_bf16(ld_bf16x4(ghb + r * WP + k0 + kq), ld_bf16x4(Pb + g.oW2n + (n0 + r) * H + k0 + kq), acc);
+    unsigned short out[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = kq + i, col = n0 + r;
+      const float v = bf2f(h1b[row * WP + col]) > 0.f ? acc[i] : 0.f;
+      gh1[row * H + col] = v;
+      out[i] = f2bf(v);
+    }
+    *reinterpret_cast<uint2*>(d_h1 + s0 * H + (n0 + r) * TILE + kq) =
+        make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
+  }
+  __syncthreads();
+  // d x_A += gh1 W1^T (fp32): 8 lanes per (sample, input), 16 terms each, summed by shuffles
+  // within aligned groups of 8 lanes (the summation order is fixed)
+  for (int q0 = 0; q0 < TILE * DMAX * 8; q0 += NT) {
+    const int q = q0 + tid, o = q >> 3, part = q & 7;
+    const int s = o / DMAX, i = o % DMAX;
+    float v = 0.f;
+    if (i < g.nA) {
+#pragma unroll
+      for (int j = 0; j < H / 8; ++j) v += gh1[s * H + part * (H / 8) + j] * W1s[i * H + part * (H / 8) + j];
+    }
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    if (part == 0 && i < g.nA) gxs[s * DMAX + g.A[i]] += v;
+  }
+  __syncthreads();
+  for (int q = tid; q < TILE * D; q += NT) {
+    const int s = q / D, d = q % D;
+    const int64_t e = s0 + s;
+    if (e < A.n) A.gx[e * D + d] = gxs[s * DMAX + d];
+  }
+}
+
+template <int NB>
+__global__ void __launch_bounds__(NT) k_layer_wgrad(BwdArgs A, const unsigned short* __restrict__ d_o,
+                                                    const unsigned short* __restrict__ d_h2,
+                                                    const unsigned short* __restrict__ d_h1) {
+  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
+  constexpr int NT3 = NP / 16;
+  constexpr int J3 = (NT3 + WPB - 1) / WPB;
+  static_assert(DMAX == 16, "dW1 is one 16-row tile");
+  const LayerGeom& g = A.geo.layer[A.layer];
+  const int D = A.geo.D;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int64_t ntiles = (A.n + TILE - 1) / TILE;
+  const int64_t t0 = (int64_t)blockIdx.x * WG_TILES;
+  const int nt = (int)(t0 + WG_TILES < ntiles ? WG_TILES : ntiles - t0);   // tiles of this split
+  const int nlp = g.ob3 + NP - g.oW1;
+  float* part = A.partial + (size_t)blockIdx.x * nlp;
+  // a wave whose second dW3 column tile does not exist repeats its last one, unstored
+  int jc[J3];
+#pragma unroll
+  for (int u = 0; u < J3; ++u) jc[u] = wid + WPB * u < NT3 ? wid + WPB * u : NT3 - 1;
+  f32x4 acc3[J3], acc2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int u = 0; u < J3; ++u) acc3[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (blockIdx.y < H / 16) {
+    const int m0 = 16 * blockIdx.y;
+    // every operand of the split is loaded before the MFMAs (tiles past the end repeat the
+    // last one with a zero A operand)
+    bf16x4 a3[WG_TILES], a2[WG_TILES], b2[WG_TILES], b3[WG_TILES][J3];
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t) {
+      const int64_t tt = t0 + (t < nt ? t : nt - 1);
+      a3[t] = ld_bf16x4(A.h2 + tt * TILE * H + (m0 + r) * TILE + kq);
+      a2[t] = ld_bf16x4(A.h1 + tt * TILE * H + (m0 + r) * TILE + kq);
+      b2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
+#pragma unroll
+      for (int u = 0; u < J3; ++u) b3[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
+    }
+    const bf16x4 zero = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t) {
+      const bf16x4 x3 = t < nt ? a3[t] : zero, x2 = t < nt ? a2[t] : zero;
+#pragma unroll
+      for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(x3, b3[t][u], acc3[u]);
+      acc2 = mfma_bf16(x2, b2[t], acc2);
+    }
+#pragma unroll
+    for (int u = 0; u < J3; ++u) {
+      const int j = wid + WPB * u;
+      if (j < NT3)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[g.oW3 - g.oW1 + (m0 + kq + i) * NP + 16 * j + r] = acc3[u][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[g.oW2 - g.oW1 + (m0 + kq + i) * H + 16 * wid + r] = acc2[i];
+  } else {
+    // bias sums: ones^T B, every row of which is the column sum of B (row 0 is stored)
+    const short one = (short)0x3f80;            // 1.0 in bf16
+    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, accw1 = {0.f, 0.f, 0.f, 0.f};
+    bf16x4 bh1[WG_TILES], bh2[WG_TILES], bo[WG_TILES][J3];
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t) {
+      const int64_t tt = t0 + (t < nt ? t : nt - 1);
+      bh1[t] = ld_bf16x4(d_h1 + tt * TILE * H + (16 * wid + r) * TILE + kq);
+      bh2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
+#pragma unroll
+      for (int u = 0; u < J3; ++u) bo[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
+    }
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t) {
+      const bf16x4 ones = t < nt ? bf16x4{one, one, one, one} : bf16x4{0, 0, 0, 0};
+      acc1 = mfma_bf16(ones, bh1[t], acc1);
+      acc2 = mfma_bf16(ones, bh2[t], acc2);
+#pragma unroll
+      for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(ones, bo[t][u], acc3[u]);
+    }
+    // dW1 = x_A^T gh1 on the fp32 matrix cores (A[r][k] = x of sample k, input r < nA)
+    const int kf = lane >> 4;                    // fp32 MFMA: one k per lane quarter
+    const bool xrow = r < g.nA;
+    const int da = xrow ? g.A[r] : 0;
+    float xv[WG_TILES][4], gv[WG_TILES][4];
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t) {
+      const int64_t tt = t0 + (t < nt ? t : nt - 1);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t e = tt * TILE + 4 * k + kf;
+        xv[t][k] = A.x_in[(e < A.n ? e : A.n - 1) * D + da];
+        gv[t][k] = bf2f(d_h1[tt * TILE * H + (16 * wid + r) * TILE + 4 * k + kf]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < WG_TILES; ++t)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t e = (t0 + t) * TILE + 4 * k + kf;
+        const bool ok = t < nt && e < A.n && xrow;
+        accw1 = mfma_f32(ok ? xv[t][k] : 0.f, gv[t][k], accw1);
+      }
+    // fp32 MFMA output: D[4 kf + i][r] in element i
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[(4 * kf + i) * H + 16 * wid + r] = accw1[i];
+    if (kq == 0) {
+      part[g.ob1 - g.oW1 + 16 * wid + r] = acc1[0];
+      part[g.ob2 - g.oW1 + 16 * wid + r] = acc2[0];
+#pragma unroll
+      for (int u = 0; u < J3; ++u) {
+        const int j = wid + WPB * u;
+        if (j < NT3) part[g.ob3 - g.oW1 + 16 * j + r] = acc3[u][0];
+      }
+    }
   }
 }
 
@@ -599,10 +994,12 @@ __global__ void k_cast(FlowGeom geo, const float* __restrict__ P, unsigned short
     if (q < H * H) {
       const int o = q / H, i = q % H;       // W2t[o][i] = W2[i][o]
       Pb[g.oW2t + q] = f2bf(P[g.oW2 + i * H + o]);
+      Pb[g.oW2n + q] = f2bf(P[g.oW2 + q]);  // W2n = W2
     } else {
       const int qq = q - H * H;
       const int o = qq / H, i = qq % H;     // W3t[o][i] = W3[i][o]
       Pb[g.oW3t + qq] = f2bf(P[g.oW3 + i * g.NP + o]);
+      Pb[g.oW3n + qq] = f2bf(P[g.oW3 + qq]);  // W3n = W3
     }
   }
 }
@@ -612,6 +1009,92 @@ __global__ void k_cast(FlowGeom geo, const float* __restrict__ P, unsigned short
 
 using namespace mfx;
 using namespace mfx::flow;
+
+// Layers with 1..NB_TEMPLATED transformed dimensions run the kernels compiled for their
+// geometry (NP a constant); wider layers the run-time-geometry kernels.
+constexpr int NB_TEMPLATED = 8;
+
+static void launch_layer(int nb, dim3 grid, size_t shmem, hipStream_t s, const FwdArgs& A, const float* P,
+                         const unsigned short* Pb) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  if (nb >= 1 && nb <= NB_LDS && grid.x <= 8u * (unsigned)cus) {
+    // training-size batches: persistent blocks with the weights in LDS, one per CU at
+    // most, each running a few tiles (the weights are read once per block).  Large
+    // batches (sampling) keep one block per tile, which holds more waves per SIMD.
+    const dim3 gp(grid.x < (unsigned)cus ? grid.x : (unsigned)cus);
+    switch (nb) {
+      case 1: hipLaunchKernelGGL(k_layer_lds<1>, gp, dim3(NT), 0, s, A, P, Pb); break;
+      case 2: hipLaunchKernelGGL(k_layer_lds<2>, gp, dim3(NT), 0, s, A, P, Pb); break;
+      case 3: hipLaunchKernelGGL(k_layer_lds<3>, gp, dim3(NT), 0, s, A, P, Pb); break;
+      case 4: hipLaunchKernelGGL(k_layer_lds<4>, gp, dim3(NT), 0, s, A, P, Pb); break;
+      case 5: hipLaunchKernelGGL(k_layer_lds<5>, gp, dim3(NT), 0, s, A, P, Pb); break;
+      default: hipLaunchKernelGGL(k_layer_lds<6>, gp, dim3(NT), 0, s, A, P, Pb); break;
+    }
+    return;
+  }
+  switch (nb) {
+    case 1: hipLaunchKernelGGL(k_layer<1>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 2: hipLaunchKernelGGL(k_layer<2>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 3: hipLaunchKernelGGL(k_layer<3>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 4: hipLaunchKernelGGL(k_layer<4>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 5: hipLaunchKernelGGL(k_layer<5>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 6: hipLaunchKernelGGL(k_layer<6>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 7: hipLaunchKernelGGL(k_layer<7>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    case 8: hipLaunchKernelGGL(k_layer<8>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+    default: hipLaunchKernelGGL(k_layer<0>, grid, dim3(NT), shmem, s, A, P, Pb); break;
+  }
+}
+
+// The backward of a layer with 1..NB_TEMPLATED transformed dimensions: the data-gradient
+// kernel (one block per 16-sample tile), then the weight-gradient GEMMs (one block row per
+// WG_SPLIT samples).  The deltas live at the start of the partial workspace, the
+// per-split partials after them.
+// The backward of a layer with 1..NB_TEMPLATED transformed dimensions: the data-gradient
+// kernel (one block per 16-sample tile), then the weight-gradient GEMMs (one block row per
+// WG_TILES tiles).  The deltas live at the start of the partial workspace, the per-split
+// partials after them.
+static void launch_layer_bwd_split(int nb, int64_t n, hipStream_t s, const BwdArgs& A, const float* P,
+                                   const unsigned short* Pb, unsigned short* d_o, unsigned short* d_h2,
+                                   unsigned short* d_h1) {
+  const dim3 gd((unsigned)((n + TILE - 1) / TILE)), gw((unsigned)((n + WG_SPLIT - 1) / WG_SPLIT), H / 16 + 1);
+#define MFX_BWD_CASE(K)                                                                                    \
+  case K:                                                                                                 \
+    hipLaunchKernelGGL(k_layer_bwd_data<K>, gd, dim3(NT), 0, s, A, P, Pb, d_o, d_h2, d_h1);               \
+    hipLaunchKernelGGL(k_layer_wgrad<K>, gw, dim3(NT), 0, s, A, (const unsigned short*)d_o,                \
+                       (const unsigned short*)d_h2, (const unsigned short*)d_h1);                         \
+    break;
+  switch (nb) {
+    MFX_BWD_CASE(1)
+    MFX_BWD_CASE(2)
+    MFX_BWD_CASE(3)
+    MFX_BWD_CASE(4)
+    MFX_BWD_CASE(5)
+    MFX_BWD_CASE(6)
+    MFX_BWD_CASE(7)
+    default: MFX_BWD_CASE(8)
+  }
+#undef MFX_BWD_CASE
+}
+
+static int64_t ntiles_of(int64_t n) { return (n + TILE - 1) / TILE; }
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+// Saved state of one layer (training), each piece 256-byte aligned: x (n, D) and raw
+// (n, NPmax) in f32, then the bf16 activations h1 and h2 tile-transposed over whole tiles.
+struct SaveLayout {
+  size_t raw, h1, h2, bytes;   // byte offsets from the layer's base, and the layer's size
+};
+static SaveLayout save_layout(const FlowGeom& G, int64_t n) {
+  SaveLayout s;
+  s.raw = align256((size_t)n * G.D * 4);
+  s.h1 = s.raw + align256((size_t)n * G.NPmax * 4);
+  s.h2 = s.h1 + (size_t)ntiles_of(n) * TILE * H * 2;
+  s.bytes = s.h2 + (size_t)ntiles_of(n) * TILE * H * 2;
+  return s;
+}
+static size_t save_per_layer(const FlowGeom& G, int64_t n) { return save_layout(G, n).bytes; }
 
 static int make_geom(int D, int L, const int* masks, FlowGeom& G) {
   if (D < 2 || D > DMAX || L < 1 || L > 16) return fail(-2, "flow needs 2 <= D <= 16 and 1 <= L <= 16%s", "");
@@ -637,7 +1120,9 @@ static int make_geom(int D, int L, const int* masks, FlowGeom& G) {
     np = g.ob3 + g.NP;
     g.oW2t = nb;
     g.oW3t = nb + H * H;
-    nb += H * H + g.NP * H;
+    g.oW2n = g.oW3t + g.NP * H;
+    g.oW3n = g.oW2n + H * H;
+    nb += 2 * (H * H + g.NP * H);
   }
   G.nparams = np;
   G.nbf16 = nb;
@@ -680,8 +1165,9 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
   if (!params || !params_bf16 || !x_in || !x_out || !logdet) return fail(-3, "null pointer%s", "");
   hipStream_t s = (hipStream_t)stream;
   const int nblk = (int)((n + TILE - 1) / TILE);
-  // per layer saved state (training): x_in (D), h1, h2 (bf16, H each), raw (NPmax)
-  const size_t per_layer = (size_t)n * (D + G.NPmax) * 4 + (size_t)n * H * 2 * 2;
+  // per layer saved state (training): x_in (n, D), raw (n, NPmax) f32, then h1, h2 bf16
+  // tile-transposed over whole tiles
+  const size_t per_layer = save_per_layer(G, n);
   const float* cur = x_in;
   for (int step = 0; step < L; ++step) {
     const int l = inverse ? L - 1 - step : step;
@@ -698,14 +1184,14 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
     A.save_raw = nullptr;
     if (save && !inverse) {
       char* base = (char*)save + per_layer * l;
-      float* xsave = (float*)base;
-      A.save_x = xsave;   // the kernel keeps the layer input for the backward pass
-      A.save_raw = xsave + (size_t)n * D;
-      A.save_h1 = (unsigned short*)(A.save_raw + (size_t)n * G.NPmax);
-      A.save_h2 = A.save_h1 + (size_t)n * H;
+      const SaveLayout sl = save_layout(G, n);
+      A.save_x = (float*)base;   // the kernel keeps the layer input for the backward pass
+      A.save_raw = (float*)(base + sl.raw);
+      A.save_h1 = (unsigned short*)(base + sl.h1);
+      A.save_h2 = (unsigned short*)(base + sl.h2);
     }
     const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-    hipLaunchKernelGGL(k_layer, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
+    launch_layer(G.layer[l].nB, dim3(nblk), shmem, s, A, params, (const unsigned short*)params_bf16);
     cur = x_out;
   }
   return hip_status(hipGetLastError());
@@ -714,7 +1200,7 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
 int64_t mfx_flow_save_bytes(int D, int L, const int* masks, int64_t n) {
   FlowGeom G;
   if (make_geom(D, L, masks, G)) return -1;
-  return (int64_t)L * ((int64_t)n * (D + G.NPmax) * 4 + (int64_t)n * H * 2 * 2);
+  return (int64_t)L * (int64_t)save_per_layer(G, n);
 }
 
 int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n) {
@@ -725,7 +1211,18 @@ int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n) {
     const int lp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
     maxlp = lp > maxlp ? lp : maxlp;
   }
-  return ((n + TILE - 1) / TILE) * (int64_t)maxlp * 4;
+  // the larger of the two layouts: per-tile partials (run-time-geometry backward), or the
+  // deltas of the batch (NP + 2 H floats per sample) followed by one partial per
+  // WG_SPLIT samples (the split backward)
+  int64_t bytes = ((n + TILE - 1) / TILE) * (int64_t)maxlp * 4;
+  for (int l = 0; l < L; ++l) {
+    const LayerGeom& g = G.layer[l];
+    if (g.nB < 1 || g.nB > NB_TEMPLATED) continue;
+    const int64_t lp = g.ob3 + g.NP - g.oW1;
+    const int64_t b = ((n + TILE - 1) / TILE) * TILE * (g.NP + 2 * H) * 2 + (n + WG_SPLIT - 1) / WG_SPLIT * lp * 4;
+    bytes = b > bytes ? b : bytes;
+  }
+  return bytes;
 }
 
 int mfx_flow_backward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
@@ -737,24 +1234,41 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
   if (!params || !params_bf16 || !save || !gl || !gx || !partial || !grad) return fail(-3, "null pointer%s", "");
   hipStream_t s = (hipStream_t)stream;
   const int nblk = (int)((n + TILE - 1) / TILE);
-  const size_t per_layer = (size_t)n * (D + G.NPmax) * 4 + (size_t)n * H * 2 * 2;
+  const size_t per_layer = save_per_layer(G, n);
   for (int l = L - 1; l >= 0; --l) {
     const char* base = (const char*)save + per_layer * l;
+    const SaveLayout sl = save_layout(G, n);
     BwdArgs A;
     A.geo = G;
     A.layer = l;
     A.n = n;
     A.x_in = (const float*)base;
-    A.raw = A.x_in + (size_t)n * D;
-    A.h1 = (const unsigned short*)(A.raw + (size_t)n * G.NPmax);
-    A.h2 = A.h1 + (size_t)n * H;
+    A.raw = (const float*)(base + sl.raw);
+    A.h1 = (const unsigned short*)(base + sl.h1);
+    A.h2 = (const unsigned short*)(base + sl.h2);
     A.gl = gl;
     A.gx = gx;
     A.partial = partial;
-    const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-    hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
     const int nlp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
-    hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nblk, nlp,
+    int nparts = nblk;
+    const int nb = G.layer[l].nB;
+    if (nb >= 1 && nb <= NB_TEMPLATED) {
+      // bf16 deltas over whole tiles (NP + 2 H per sample) first, then one partial per split
+      const size_t nt16 = (size_t)((n + TILE - 1) / TILE) * TILE;
+      unsigned short* d_o = (unsigned short*)partial;
+      unsigned short* d_h2 = d_o + nt16 * G.layer[l].NP;
+      unsigned short* d_h1 = d_h2 + nt16 * H;
+      nparts = (int)((n + WG_SPLIT - 1) / WG_SPLIT);
+      A.partial = (float*)(d_h1 + nt16 * H);
+      launch_layer_bwd_split(nb, n, s, A, params, (const unsigned short*)params_bf16, d_o, d_h2, d_h1);
+      hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, A.partial, nparts, nlp,
+                         grad + G.layer[l].oW1);
+      continue;
+    } else {
+      const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
+      hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
+    }
+    hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nparts, nlp,
                        grad + G.layer[l].oW1);
   }
   return hip_status(hipGetLastError());

@@ -289,6 +289,37 @@ def train_flow(a, torch, ig):
     return flow, t_pairs, time.perf_counter() - t0
 
 
+def flow_grid(a, torch, main=None):
+    """The flow leg's frozen FP32 VEGAS grid: flow_grid_warmup adaptation iterations of nev
+    points (or flow_grid_nev points when given).  Each leg of the comparison adapts its
+    grid with its own best schedule, and that time is counted in its set-up: plain VEGAS
+    needs the full warm-up (with 3 iterations instead of 5 it takes 3 frozen iterations
+    to the target and its rate drops 3x), the flow corrects what a shorter adaptation
+    leaves.  Returns the integrand and the adaptation time."""
+    from madflow_amd.vegas import MEIntegrand, VegasFlow
+    ig = MEIntegrand(a.process, precision=32, mapping=a.mapping, seed=7, events_limit=a.events_limit,
+                     kernels=a.kernels, **ME_ONLY_KIN[a.process])
+    vf = VegasFlow(ig.ndim, a.flow_grid_nev or a.nev)
+    vf.compile(ig)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vf.run_integration(a.flow_grid_warmup, log=log)
+    vf.freeze_grid()
+    torch.cuda.synchronize()
+    return ig, time.perf_counter() - t0
+
+
+def flow_train_only(a, torch):
+    """The flow leg's set-up alone (grid adaptation, training pairs, training): one JSON
+    line of timings, for profiling the training kernels."""
+    ig, t_grid = flow_grid(a, torch)
+    flow, t_pairs, t_train = train_flow(a, torch, ig)
+    print(json.dumps({"flow_train_only": True, "grid_warmup_s": t_grid, "train_pairs_s": t_pairs, "train_s": t_train,
+                      "stage_steps_run": flow.stage_steps, "batch": a.flow_batch,
+                      "ms_per_step": t_train * 1e3 / max(1, sum(flow.stage_steps))}))
+    return 0
+
+
 def flow_iteration(a, torch, ig, flow, k):
     """One iteration of a.nev points from the defensive mixture of the flow and the
     frozen VEGAS cube, each weighted by f(y)/q'(y)."""
@@ -327,10 +358,10 @@ def run_flow_compare(a, torch, main):
     plain["equal_time"] = _rate_at(timed64, a.equal_time_s, a.equal_time_s)
     plain["equal_wall"] = _rate_at(timed64, a.equal_wall_s - setup64, a.equal_wall_s)
     del ig64, vf
-    # ---- VEGAS + flow, FP32 matrix elements, on the main run's frozen grid
-    ig = main["ig"]
+    # ---- VEGAS + flow, FP32 matrix elements, on a frozen grid of its own
+    ig, t_grid = flow_grid(a, torch, main)
     flow, t_pairs, t_train = train_flow(a, torch, ig)
-    setup_flow = main["t_warm"] + t_pairs + t_train
+    setup_flow = t_grid + t_pairs + t_train
     t0 = time.perf_counter()
     itf = lambda k: flow_iteration(a, torch, ig, flow, k)
     resf = _run_to_target(itf, a.target, a.max_iter, "flow fp32")
@@ -339,7 +370,8 @@ def run_flow_compare(a, torch, main):
     timedf = _extend_to(itf, list(resf), learned["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup_flow), torch)
     learned["equal_time"] = _rate_at(timedf, a.equal_time_s, a.equal_time_s)
     learned["equal_wall"] = _rate_at(timedf, a.equal_wall_s - setup_flow, a.equal_wall_s)
-    learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=main["t_warm"],
+    learned.update(train_pairs_s=t_pairs, train_s=t_train, grid_warmup_s=t_grid,
+                   grid_warmup="%d x %d points" % (a.flow_grid_warmup, a.flow_grid_nev or a.nev),
                    best_step=getattr(flow, "best_step", None), mixture=a.flow_mix)
 
     def gain(k):
@@ -451,10 +483,18 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
+    ap.add_argument("--flow-grid-warmup", type=int, default=3,
+                    help="grid adaptation iterations of the flow leg's FP32 grid, counted in its set-up (the plain "
+                         "FP64 leg adapts for --warmup iterations)")
+    ap.add_argument("--flow-grid-nev", type=int, default=0,
+                    help="points per adaptation iteration of the flow leg's grid (0: --nev, like the plain leg)")
+    ap.add_argument("--flow-train-only", action="store_true",
+                    help="run only the flow leg's grid adaptation, training pairs and training, print timings, exit")
     ap.add_argument("--flow-steps", type=int, default=20000)
     ap.add_argument("--flow-batch", type=int, default=8192)
     ap.add_argument("--flow-lr", type=float, default=1e-3)
-    ap.add_argument("--flow-mix", type=float, default=0.2)
+    ap.add_argument("--flow-mix", type=float, default=0.3,
+                    help="weight of the uniform (plain VEGAS) component of the defensive mixture q' = (1-m) q + m")
     ap.add_argument("--flow-eval-every", type=int, default=250)
     ap.add_argument("--flow-seed", type=int, default=1, help="initial weights and minibatch draws of the flow")
     ap.add_argument("--flow-patience", type=int, default=2,
@@ -485,6 +525,8 @@ def main(argv=None):
         return 2
     if a.me_only:
         return me_only(a, torch)
+    if a.flow_train_only:
+        return flow_train_only(a, torch)
     if a.copy_probe:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -1,8 +1,8 @@
 """Per-event error of the FP32 build of every process against the FP64 oracle, on larger
 samples than the parity tests: random RAMBO points, and the subset that passes the
 benchmark cuts (pT(g) > 30 GeV, dR(g,g) > 0.4).  Prints one line per process with the
-median, 99th percentile and maximum relative error; the bars are those of the parity
-tests (median 2e-5, worst event 1e-3)."""
+median, 99th percentile and maximum relative error.  Bars: median 2e-6, 99th percentile
+1e-5, worst event 5e-5 (measured: worst 7e-7 to 1.9e-5 depending on the process)."""
 
 import numpy as np
 import pytest
@@ -54,5 +54,6 @@ def test_fp32_error_against_oracle(dev, name, kernels, nev):
         line += "; %d post-cut median %.2e p99 %.2e max %.2e" % (len(rc), np.median(rc), np.quantile(rc, 0.99),
                                                                  rc.max())
     print(line, flush=True)
-    assert np.median(rel) < 2e-5
-    assert rel.max() < 1e-3
+    assert np.median(rel) < 2e-6
+    assert np.quantile(rel, 0.99) < 1e-5
+    assert rel.max() < 5e-5

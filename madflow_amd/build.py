@@ -38,6 +38,8 @@ UNITS = [
     "procs/gg_ttxgg.hip",
     "procs/gg_ttxgg_fused.hip",
     "procs/gg_ttxggg.hip",
+    "procs/uux_ttx.hip",
+    "procs/uux_ttxg.hip",
 ]
 
 

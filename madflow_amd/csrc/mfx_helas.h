@@ -431,6 +431,21 @@ __device__ __forceinline__ void VVVV4P0_1(const W<R>& a, const W<R>& b, const W<
 }
 
 // ---------------------------------------------------------------------- FFV1P0_3
+// den2 = P^2 - M3^2 of the off-shell vector, supplied by the caller
+template <class R>
+__device__ __forceinline__ void FFV1P0_3(const W<R>& F1, const W<R>& F2, cx<R> COUP, R M3, R W3, R den2, W<R>& V3) {
+  V3[0] = F1[0] + F2[0];
+  V3[1] = F1[1] + F2[1];
+  const cx<R> dmi = mul_mi(prop_denom(COUP, den2, M3, W3));
+  const cx<R> J0 = F1[2] * F2[4] + F1[3] * F2[5] + F1[4] * F2[2] + F1[5] * F2[3];
+  const cx<R> J1 = F1[4] * F2[3] + F1[5] * F2[2] - F1[2] * F2[5] - F1[3] * F2[4];
+  const cx<R> J2 = mul_i(F1[3] * F2[4] + F1[4] * F2[3] - F1[2] * F2[5] - F1[5] * F2[2]);
+  const cx<R> J3 = F1[3] * F2[5] + F1[4] * F2[2] - F1[2] * F2[4] - F1[5] * F2[3];
+  V3[2] = dmi * J0;
+  V3[3] = dmi * J1;
+  V3[4] = dmi * J2;
+  V3[5] = dmi * J3;
+}
 template <class R>
 __device__ __forceinline__ void FFV1P0_3(const W<R>& F1, const W<R>& F2, cx<R> COUP, R M3, R W3, W<R>& V3) {
   V3[0] = F1[0] + F2[0];

@@ -267,6 +267,8 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
         // zero bits inserted at the paired legs' positions, the lower position first
         unsigned hb = ((unsigned)q & ((1u << XA) - 1u)) | (((unsigned)q >> XA) << (XA + 1));
         if constexpr (K == 2) hb = (hb & ((1u << XB) - 1u)) | ((hb >> XB) << (XB + 1));
+        // groups of rows that vanish identically (the paired legs never decide that)
+        if (!T::live(hb)) continue;
         if constexpr (K == 2) {
           static_assert(MFMA, "four rows per pass keep three rows' jamps in the colour-sum LDS");
           // rows 1..3 accumulate in place in the wave's LDS arrays; row 0 stays in
@@ -308,6 +310,7 @@ __global__ void __launch_bounds__(256) k_smatrix(SmatrixArgs a) {
 #pragma unroll
           for (int k = 0; k < T::NINV; ++k) opaque(inv[k]);
         }
+        if (!T::live(T::hel_bits(h))) continue;   // a row that vanishes identically
         cx<R> jamp[T::NCOLOR];
         T::template amplitudes<R>(p, inv, T::hel_bits(h), nullptr, prm, jamp);
         if constexpr (MFMA) ans += colour_sum_mfma<T>(jamp, lds.stage[0][wave], lds.cf, lds.red[wave]);

@@ -34,6 +34,8 @@ _PROC_PARAMS = {
     "gg_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "gg_ttxgg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "gg_ttxggg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
+    "uux_ttx": ["mdl_MT", "mdl_WT", "GC_11"],
+    "uux_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
 }
 _GC_SLOT = {"GC_10": 0, "GC_11": 1, "GC_12": 2}
 

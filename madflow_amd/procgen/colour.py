@@ -194,3 +194,63 @@ def all_orderings(gluons):
     """Colour basis: every ordering of the gluon labels, lexicographic in the given list.
     For g g > t t~ this gives [(0,1),(1,0)] as in the mockup's jamp order."""
     return [tuple(p) for p in permutations(gluons)]
+
+
+# ------------------------------------------------------------ two fermion lines
+
+# Colour basis of q q~ > t t~ + gluons: products of two fundamental strings.  A basis
+# element is (pairing, w_top, w_light): the string w_top starts at the top's colour index,
+# the string w_light at the incoming antiquark's (the fundamental end of the light line).
+# pairing 0 ("straight"): w_top ends at the anti-top, w_light at the incoming quark;
+# pairing 1 ("crossed"): w_top ends at the incoming quark, w_light at the anti-top.
+_ENDS = ({"T": "aT", "L": "aL"}, {"T": "aL", "L": "aT"})
+
+
+def fierz_contract(top, light, label):
+    """sum over the adjoint index ``label`` of (top)_{top, anti-top} (light)_{qbar, q}, with
+    ``top`` and ``light`` TPolys whose every monomial holds ``label`` once.  The Fierz
+    identity T^b_{ab} T^b_{cd} = 1/2 d_{ad} d_{cb} - 1/(2N) d_{ab} d_{cd} gives
+    (X b Y)(Z b W) = 1/2 (ZY)(XW) [crossed] - 1/(2N) (ZW)(XY) [straight].
+    Returns {(pairing, w_top, w_light): (Fraction re, Fraction im)}."""
+    out = {}
+    half, sub = Fraction(1, 2), Fraction(-1, 2 * NC)
+    for mt, (ar, ai) in top.terms.items():
+        k = mt.index(label)
+        Z, W = mt[:k], mt[k + 1:]
+        for ml, (br, bi) in light.terms.items():
+            j = ml.index(label)
+            X, Y = ml[:j], ml[j + 1:]
+            cr, ci = ar * br - ai * bi, ar * bi + ai * br
+            for key, f in (((1, Z + Y, X + W), half), ((0, Z + W, X + Y), sub)):
+                o = out.get(key, (Fraction(0), Fraction(0)))
+                out[key] = (o[0] + f * cr, o[1] + f * ci)
+    return {k: v for k, v in out.items() if v != (0, 0)}
+
+
+def colour_matrix_two_lines(basis):
+    """CF[s][t] = sum over colours of B_s conj(B_t) for two-string basis elements: the
+    open indices close into cycles (string of s, then the reversed string of t that ends
+    at the same anti-fundamental index, ...), each cycle one trace."""
+    n = len(basis)
+    cf = [[None] * n for _ in range(n)]
+    for i, s in enumerate(basis):
+        ws = {"T": tuple(s[1]), "L": tuple(s[2])}
+        es = _ENDS[s[0]]
+        for j, t in enumerate(basis):
+            wt = {"T": tuple(t[1]), "L": tuple(t[2])}
+            back = {v: k for k, v in _ENDS[t[0]].items()}
+            traces, seen = [], set()
+            for f0 in ("T", "L"):
+                if f0 in seen:
+                    continue
+                word, f = (), f0
+                while True:
+                    seen.add(f)
+                    word += ws[f]
+                    f = back[es[f]]
+                    word += tuple(reversed(wt[f]))
+                    if f == f0:
+                        break
+                traces.append(word)
+            cf[i][j] = _trace_prod(_canon(traces))
+    return cf

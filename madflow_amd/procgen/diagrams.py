@@ -322,11 +322,11 @@ def _product_expand(d):
 
 def helicity_table(kinds):
     """Helicity combinations in the emitted order (first leg slowest): vectors and
-    outgoing fermions run (-1, +1), the outgoing anti-fermion (+1, -1) — the order of
+    fermions run (-1, +1), anti-fermions (incoming or outgoing) (+1, -1) — the order of
     the mockup's ``helicities`` (``mockup_debug_me.py:424-442``)."""
     states = []
     for k in kinds:
-        if k == "atop":
+        if k in ("atop", "antiquark"):
             states.append((1, -1))
         else:
             states.append((-1, 1))
@@ -390,4 +390,208 @@ def gg_ttx_ng(ng):
         color_num=nums,
         initial_states=[[21, 21]],
         mirror_initial_states=False,
+    )
+
+
+# ---------------------------------------------------------------------------
+# q q~ > t t~ + gluons: two fermion lines
+# ---------------------------------------------------------------------------
+
+LIGHT = -1    # leaf of a bridge tree that stands for the light-quark line's current
+
+
+def _cluster_sequences(glist):
+    """Every ordered sequence of gluon clusters (one tree per block) over ``glist``."""
+    import itertools
+    out = []
+    for part in ordered_set_partitions(glist):
+        for combo in itertools.product(*[trees(b) for b in part]):
+            out.append(list(combo))
+    return out
+
+
+def enumerate_qq_diagrams(gluons):
+    """Tree diagrams of q q~ > t t~ + gluons as (light, pb, bridge, top, pt):
+
+    * ``light``: the gluon clusters on the light-quark line, from the incoming antiquark
+      (the end the fermion flow leaves by) to the incoming quark, and ``pb`` the position
+      of the bridge vertex among them (clusters light[:pb] sit on the antiquark side);
+    * ``bridge``: the gluon tree from the light line to the top line, over LIGHT and the
+      gluons it collects through three- and four-gluon vertices;
+    * ``top``: the clusters on the top line from the top to the anti-top, and ``pt`` the
+      position of the bridge among them.
+
+    Every tree-level QCD diagram has this form: the two lines are joined by exactly one
+    gluon tree, and every other gluon hangs off one of the lines in a tree of its own.
+    Diagram counts: 1, 5, 36 for 0, 1, 2 gluons."""
+    import itertools
+    diags = []
+    for assign in itertools.product(range(3), repeat=len(gluons)):
+        gl = [g for g, a in zip(gluons, assign) if a == 0]
+        gb = [g for g, a in zip(gluons, assign) if a == 1]
+        gt = [g for g, a in zip(gluons, assign) if a == 2]
+        for lseq in _cluster_sequences(gl):
+            for pb in range(len(lseq) + 1):
+                for btree in trees([LIGHT] + gb):
+                    for tseq in _cluster_sequences(gt):
+                        for pt in range(len(tseq) + 1):
+                            diags.append((lseq, pb, btree, tseq, pt))
+    return diags
+
+
+def _subst_leaf(tree, old, new):
+    """An expanded tree with the leaf ``old`` replaced by ``new``."""
+    if isinstance(tree, int):
+        return new if tree == old else tree
+    if tree[0] == "V3":
+        return ("V3", _subst_leaf(tree[1], old, new), _subst_leaf(tree[2], old, new))
+    return ("V4", tree[1]) + tuple(_subst_leaf(c, old, new) for c in tree[2:])
+
+
+def _current_with_leaves(prog, tree, leaves):
+    """_gluon_current for trees whose leaves may be off-shell currents: ``leaves`` maps
+    such a leaf (a tuple tagged "L") to its (wavefunction index, colour polynomial)."""
+    if isinstance(tree, tuple) and tree[0] == "L":
+        return leaves[tree]
+    if isinstance(tree, int):
+        return _gluon_current(prog, tree)
+    if tree[0] == "V3":
+        wa, pa = _current_with_leaves(prog, tree[1], leaves)
+        wb, pb = _current_with_leaves(prog, tree[2], leaves)
+        w = prog.current(("G", tree), "VVV1P0_1", (wa, wb), "GC_10", "ZERO", "ZERO")
+        return w, f_contract(pa, pb)
+    k = tree[1]
+    (wa, pa), (wb, pb), (wc, pc) = [_current_with_leaves(prog, c, leaves) for c in tree[2:]]
+    w = prog.current(("G", tree), V4_NAMES[k] + "P0_1", (wa, wb, wc), "GC_12", "ZERO", "ZERO")
+    if k == 0:
+        poly = pa.commutator(pb.commutator(pc))
+    elif k == 1:
+        poly = pb.commutator(pa.commutator(pc))
+    else:
+        poly = pc.commutator(pa.commutator(pb))
+    return w, poly.scale(-1, 0)
+
+
+BRIDGE_LABEL = "b"
+
+
+def build_program_qq(gluons, q_leg, qb_leg, top_leg, atop_leg, externals, nexternal):
+    """HELAS program of q q~ > t t~ + gluons (incoming quark ``q_leg`` from ixxxxx,
+    incoming antiquark ``qb_leg`` from oxxxxx).  Per diagram: the light line's clusters
+    are applied to the antiquark (FFV1_1) and to the quark (FFV1_2) up to the bridge
+    vertex, which FFV1P0_3 closes into the light current; the bridge tree carries it to the
+    top line, whose amplitude closes at the bridge with FFV1_0.  The colour factor is the
+    product of the two lines' generator strings with the bridge index contracted by the
+    Fierz identity, expanded in the two-string basis of ``colour.fierz_contract``.
+    Returns (program, number of diagrams, {amplitude: {basis element: coefficient}})."""
+    from .colour import fierz_contract
+    prog = Program(externals, nexternal)
+    diags = enumerate_qq_diagrams(gluons)
+    diags.sort(key=lambda d: (len(d[0]) + len(d[3]), repr(d)))
+    colours = []
+    for lseq, pb, btree, tseq, pt in diags:
+        for lexp in _product_expand(lseq):
+            curl = [_gluon_current(prog, t) for t in lexp]
+            w_o, key = qb_leg, ()
+            for i in range(pb):
+                key = key + (lexp[i],)
+                w_o = prog.current(("LO", key), "FFV1_1", (w_o, curl[i][0]), "GC_11", "ZERO", "ZERO")
+            w_i, key = q_leg, ()
+            for i in range(len(lexp) - 1, pb - 1, -1):
+                key = key + (lexp[i],)
+                w_i = prog.current(("LI", key), "FFV1_2", (w_i, curl[i][0]), "GC_11", "ZERO", "ZERO")
+            lkey = ("L", tuple(lexp[:pb]), tuple(lexp[pb:]))
+            w_l = prog.current(("LC", lkey), "FFV1P0_3", (w_i, w_o), "GC_11", "ZERO", "ZERO")
+            lpoly = TPoly.one()
+            for _, p in curl[:pb]:
+                lpoly = lpoly * p
+            lpoly = lpoly * TPoly.gen(BRIDGE_LABEL)
+            for _, p in curl[pb:]:
+                lpoly = lpoly * p
+            leaves = {lkey: (w_l, TPoly.gen(BRIDGE_LABEL))}
+            for bexp in _expand_v4(btree):
+                bexp = _subst_leaf(bexp, LIGHT, lkey)
+                for texp in _product_expand(tseq):
+                    curt = [_gluon_current(prog, t) for t in texp]
+                    w_b, bpoly = _current_with_leaves(prog, bexp, leaves)
+                    w_top, key = top_leg, ()
+                    for i in range(pt):
+                        key = key + (texp[i],)
+                        w_top = prog.current(("O", key), "FFV1_1", (w_top, curt[i][0]), "GC_11", "mdl_MT", "mdl_WT")
+                    w_atop, key = atop_leg, ()
+                    for i in range(len(texp) - 1, pt - 1, -1):
+                        key = key + (texp[i],)
+                        w_atop = prog.current(("I", key), "FFV1_2", (w_atop, curt[i][0]), "GC_11", "mdl_MT",
+                                              "mdl_WT")
+                    tpoly = TPoly.one()
+                    for _, p in curt[:pt]:
+                        tpoly = tpoly * p
+                    tpoly = tpoly * bpoly
+                    for _, p in curt[pt:]:
+                        tpoly = tpoly * p
+                    prog.amplitude("FFV1_0", (w_atop, w_top, w_b), "GC_11", None)
+                    colours.append(fierz_contract(tpoly, lpoly, BRIDGE_LABEL))
+    return prog, len(diags), colours
+
+
+def _coef_json(x):
+    """A rational coefficient for the process description: an int when integral, else the
+    exact fraction as a string ("-1/6")."""
+    from fractions import Fraction
+    x = Fraction(x)
+    return int(x) if x.denominator == 1 else "%d/%d" % (x.numerator, x.denominator)
+
+
+def qq_ttx_ng(ng):
+    """Process description dict for u u~ > t t~ + ng gluons.  With massless quarks and QCD
+    vertices only, the same matrix element serves c c~, d d~ and s s~ (``initial_states``),
+    and the mirror processes with the antiquark in beam 1 (``mirror_initial_states``)."""
+    from math import factorial, lcm
+    from fractions import Fraction
+    from .colour import colour_matrix_two_lines
+    nexternal = 4 + ng
+    kinds = ["quark", "antiquark", "top", "atop"] + ["gluon"] * ng
+    externals = [dict(leg=0, routine="ixxxxx", mass="ZERO", sign=+1),
+                 dict(leg=1, routine="oxxxxx", mass="ZERO", sign=-1),
+                 dict(leg=2, routine="oxxxxx", mass="mdl_MT", sign=+1),
+                 dict(leg=3, routine="ixxxxx", mass="mdl_MT", sign=-1)]
+    externals += [dict(leg=l, routine="vxxxxx", mass="ZERO", sign=+1) for l in range(4, nexternal)]
+    gluons = list(range(4, nexternal))
+    prog, ndiags, colours = build_program_qq(gluons, 0, 1, 2, 3, externals, nexternal)
+    prog.ops = schedule_ops(prog.ops, nexternal)
+    basis = sorted({key for c in colours for key in c})
+    index = {b: i for i, b in enumerate(basis)}
+    jamp = [[] for _ in basis]
+    for a, col in enumerate(colours):
+        for key, (re, im) in sorted(col.items()):
+            jamp[index[key]].append((a, _coef_json(re), _coef_json(im)))
+    cf = colour_matrix_two_lines(basis)
+    denoms, nums = line_numerators(cf)
+    name = "uux_ttx" + "g" * ng
+    used = {op["coup"] for op in prog.ops}
+    params = ["mdl_MT", "mdl_WT"] + [c for c in ("GC_10", "GC_11", "GC_12") if c in used]
+    return dict(
+        name=name,
+        process="u u~ > t t~" + " g" * ng,
+        class_name="Matrix_1_" + name,
+        nexternal=nexternal,
+        ninitial=2,
+        ndiags=ndiags,
+        ngraphs=prog.namps,
+        nwavefuncs=prog.nwf,
+        ncolor=len(basis),
+        ncomb=2 ** nexternal,
+        kinds=kinds,
+        externals=externals,
+        helicities=helicity_table(kinds),
+        denominator=4 * 9 * factorial(ng),
+        params=params,
+        masses=["ZERO", "ZERO", "mdl_MT", "mdl_MT"] + ["ZERO"] * ng,
+        ops=prog.ops,
+        jamp=jamp,
+        colour_basis=[[p, list(w1), list(w2)] for p, w1, w2 in basis],
+        color_denom=denoms,
+        color_num=nums,
+        initial_states=[[2, -2], [4, -4], [1, -1], [3, -3]],
+        mirror_initial_states=True,
     )

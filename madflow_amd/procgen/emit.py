@@ -1,6 +1,7 @@
 """Emit generated process descriptions.
 
-``python -m madflow_amd.procgen.emit`` regenerates, for g g > t t~ + 0..3 g:
+``python -m madflow_amd.procgen.emit`` regenerates, for g g > t t~ + 0..3 g and
+u u~ > t t~ + 0..1 g:
 
 * ``madflow_amd/processes/<name>.json`` — the process description (helicity table,
   external wavefunctions, HELAS program, jamp coefficients, colour matrix,
@@ -18,16 +19,34 @@ import re as _re
 import os
 import sys
 
-from .diagrams import gg_ttx_ng, schedule_ops
+from fractions import Fraction
+
+from .diagrams import gg_ttx_ng, qq_ttx_ng, schedule_ops
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROC_DIR = os.path.join(PKG, "processes")
 INC_DIR = os.path.join(PKG, "csrc", "procs")
 
-NGLUONS = (0, 1, 2, 3)
+NGLUONS = (0, 1, 2, 3)        # g g > t t~ + ng g
+QQ_NGLUONS = (0, 1)           # u u~ > t t~ + ng g
 
 
 # ---------------------------------------------------------------- HIP emission
+
+def _exact(x):
+    """A rational coefficient as an int when it is integral (the Gaussian-integer
+    coefficients of the gluon processes stay ints), else as a Fraction."""
+    x = Fraction(x)
+    return int(x) if x.denominator == 1 else x
+
+
+def _lit(x):
+    """C++ literal of a rational coefficient in the kernel's real type R."""
+    x = Fraction(x)
+    if x.denominator == 1:
+        return "R(%d)" % x.numerator
+    return "R(%d.0 / %d.0)" % (x.numerator, x.denominator)
+
 
 def _cname(name):
     return name.upper()
@@ -71,7 +90,7 @@ def merged_ops(desc):
     coef = {}
     for c, terms in enumerate(desc["jamp"]):
         for (a, re, im) in terms:
-            coef.setdefault(a, {})[c] = (re, im)
+            coef.setdefault(a, {})[c] = (Fraction(re), Fraction(im))
     groups = {}
     order = []
     for op in desc["ops"]:
@@ -80,14 +99,15 @@ def merged_ops(desc):
         cs = sorted(coef.get(op["out"], {}).items())
         if not cs:
             continue
+        # factor: the first coefficient c0; norm: every coefficient divided by c0
         r0, i0 = cs[0][1]
-        assert r0 * r0 + i0 * i0 == 1
-        norm = tuple((c, r * r0 + i * i0, i * r0 - r * i0) for c, (r, i) in cs)
+        n0 = r0 * r0 + i0 * i0
+        norm = tuple((c, _exact((r * r0 + i * i0) / n0), _exact((i * r0 - r * i0) / n0)) for c, (r, i) in cs)
         key = (op["routine"], op["args"][0], op["args"][1], op["coup"], norm)
         if key not in groups:
             groups[key] = []
             order.append(key)
-        groups[key].append((op, (r0, i0)))
+        groups[key].append((op, (_exact(r0), _exact(i0))))
     ops = [op for op in desc["ops"] if op["kind"] == "wf"]
     for key in order:
         routine, f1, f2, coup, norm = key
@@ -170,10 +190,26 @@ def _mom_expr(legs, overall, nin):
 
 
 def _scaled(expr, re, im):
-    """expr times a unit Gaussian integer, as a sign or an i-multiplication."""
+    """expr times a coefficient that is real or imaginary: a unit as a sign or an
+    i-multiplication, any other rational as a product with its literal as well."""
+    assert re == 0 or im == 0
     if im == 0:
-        return expr if re > 0 else "-" + expr
-    return ("mul_i(%s)" if im > 0 else "mul_mi(%s)") % expr
+        if re in (1, -1):
+            return expr if re > 0 else "-" + expr
+        return "(%s * %s)" % (_lit(re), expr)
+    if im in (1, -1):
+        return ("mul_i(%s)" if im > 0 else "mul_mi(%s)") % expr
+    return "(%s * mul_i(%s))" % (_lit(im), expr)
+
+
+def _jamp_update(w, indent, jn, c, re, im, a):
+    """jamp[c] += (re + i im) * a for a real or imaginary rational coefficient."""
+    if im == 0 and re in (1, -1):
+        w("%s%s[%d] %s= %s;" % (indent, jn, c, "+" if re > 0 else "-", a))
+    elif re == 0 and im in (1, -1):
+        w("%s%s[%d] %s= mul_i(%s);" % (indent, jn, c, "+" if im > 0 else "-", a))
+    else:
+        w("%s%s[%d] += %s;" % (indent, jn, c, _scaled(a, re, im)))
 
 
 def _pair_side(a, b):
@@ -230,10 +266,7 @@ def _emit_ops(w, prog, desc, legs_of):
             w(post)
             for jn, a in (("jamp", "amp"), ("jamp2", "ampt")):
                 for (c, re, im) in op["norm"]:
-                    if im == 0:
-                        w("    %s[%d] %s= %s;" % (jn, c, "+" if re > 0 else "-", a))
-                    else:
-                        w("    %s[%d] %s= mul_i(%s);" % (jn, c, "+" if im > 0 else "-", a))
+                    _jamp_update(w, "    ", jn, c, re, im, a)
             w("  }")
             w("  __builtin_amdgcn_sched_barrier(0);")
             continue
@@ -288,13 +321,21 @@ def _emit_ops(w, prog, desc, legs_of):
                     _scaled("w%d[q]" % v, r, i) for v, (r, i) in zip(vs, op["factors"])))
                 w("    const cx<R> amp = %s(w%d, w%d, vs, %s);" % (op["routine"], f1, f2, coup))
             for (c, re, im) in op["norm"]:
-                if im == 0:
-                    w("    %s[%d] %s= amp;" % (jn, c, "+" if re > 0 else "-"))
-                else:
-                    w("    %s[%d] %s= mul_i(amp);" % (jn, c, "+" if im > 0 else "-"))
+                _jamp_update(w, "    ", jn, c, re, im, "amp")
             w("  }")
             # keep the machine scheduler from hoisting later currents above this point
             w("  __builtin_amdgcn_sched_barrier(0);")
+
+
+def zero_helicity_pairs(desc):
+    """Pairs of external legs (quark, antiquark) of the massless quark lines.  A vector
+    coupling conserves chirality along a massless line, so every helicity row in which the
+    incoming quark and antiquark carry the same helicity label vanishes identically (the
+    helicity filtering MadGraph does at run time, decided here by the process)."""
+    kinds = desc.get("kinds", [])
+    if "quark" in kinds and "antiquark" in kinds:
+        return [(kinds.index("quark"), kinds.index("antiquark"))]
+    return []
 
 
 def emit_hip(desc, uname=None):
@@ -431,6 +472,19 @@ def emit_hip(desc, uname=None):
         w("    %s::%s(p, inv, hb, prm, %s);" % (name, fname, ", ".join(jnames)))
         w("  }")
     w("  static __device__ __forceinline__ unsigned hel_bits(int h) { return HEL_BITS[h]; }")
+    pairs = zero_helicity_pairs(desc)
+    if dual:
+        assert not any(X in pq for X in legs for pq in pairs), "a paired leg decides a vanishing row"
+    if pairs:
+        w("  // rows that vanish identically: a massless quark line conserves chirality, so its")
+        w("  // two external legs need opposite helicities (bit l of hb: leg l at helicity +1)")
+        w("  static constexpr bool FILTERED = true;")
+        w("  static __device__ __forceinline__ bool live(unsigned hb) {")
+        w("    return %s;" % " && ".join("(((hb >> %d) ^ (hb >> %d)) & 1u)" % pq for pq in pairs))
+        w("  }")
+    else:
+        w("  static constexpr bool FILTERED = false;")
+        w("  static __device__ __forceinline__ bool live(unsigned) { return true; }")
     w("  static __device__ __forceinline__ float cf(float, int i, int j) { return CF_NUM_F[i][j]; }")
     w("  static __device__ __forceinline__ double cf(double, int i, int j) { return CF_NUM_D[i][j]; }")
     w("};")
@@ -457,7 +511,7 @@ def emit_hip_unit(uname):
 # both precisions; for g g > t t~ g g g the n! jamps of the fused form do not fit the
 # register file next to a 1,890-amplitude program.
 KERNELS = {"gg_ttx": ("fused",), "gg_ttxg": ("fused",), "gg_ttxgg": ("ordered", "fused"),
-           "gg_ttxggg": ("ordered",)}
+           "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",)}
 # helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
 MULTI_K = {}
 
@@ -642,11 +696,15 @@ def emitted_files(desc):
     return out
 
 
+def descriptions():
+    """Process descriptions of every generated process, in emission order."""
+    return [gg_ttx_ng(ng) for ng in NGLUONS] + [qq_ttx_ng(ng) for ng in QQ_NGLUONS]
+
+
 def main(argv=None):
     os.makedirs(PROC_DIR, exist_ok=True)
     os.makedirs(INC_DIR, exist_ok=True)
-    for ng in NGLUONS:
-        desc = gg_ttx_ng(ng)
+    for desc in descriptions():
         for rel, text in emitted_files(desc).items():
             with open(os.path.join(PKG, rel), "w") as f:
                 f.write(text)

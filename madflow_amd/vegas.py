@@ -56,6 +56,8 @@ PROCESS_SETUP = {
     "gg_ttxg": (3, [173.0, 173.0, 0.0]),
     "gg_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
     "gg_ttxggg": (5, [173.0, 173.0, 0.0, 0.0, 0.0]),
+    "uux_ttx": (2, [173.0, 173.0]),
+    "uux_ttxg": (3, [173.0, 173.0, 0.0]),
 }
 
 

@@ -17,12 +17,18 @@ OUT = os.path.join(os.path.dirname(O.HERE), "madflow_amd", "processes", "flops_p
 
 def main():
     res = {}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg"):
         try:
             O.load_process(name)
         except FileNotFoundError:
             continue
         res[name] = O.flops_per_me(name)
+        # rows that do not vanish identically (the kernels skip the others; the count above
+        # is over every row, as the reference evaluates them)
+        d = O.load_process(name)
+        q = [i for i, k in enumerate(d.get("kinds", [])) if k in ("quark", "antiquark")]
+        res[name]["live_rows"] = sum(1 for h in d["helicities"] if len(q) < 2 or h[q[0]] != h[q[1]])
+        res[name]["rows"] = len(d["helicities"])
         print(name, res[name])
     with open(OUT, "w") as f:
         json.dump(res, f, indent=1)

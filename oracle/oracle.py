@@ -24,6 +24,7 @@ Parity status of each oracle function (how it is pinned):
 
 import ctypes
 import json
+from fractions import Fraction
 import os
 import subprocess
 import numpy as np
@@ -91,12 +92,15 @@ class EncodedProcess:
                 cidx[op["coup"]], midx(op["mass"]), midx(op["width"])]
         self.ops = np.array(ops, dtype=np.int32)
         self.nops = len(desc["ops"])
-        jt = []
+        jt, jc = [], []
         for c, terms in enumerate(desc["jamp"]):
             for (a, re, im) in terms:
-                jt += [c, a, re, im]
+                # coefficients: ints, or exact fractions written as "p/q"
+                jt += [c, a]
+                jc += [float(Fraction(re)), float(Fraction(im))]
         self.jterms = np.array(jt, dtype=np.int32)
-        self.njterms = len(jt) // 4
+        self.jcoef = np.array(jc, dtype=np.float64)
+        self.njterms = len(jt) // 2
         self.cf_num = np.ascontiguousarray(np.array(desc["color_num"], dtype=np.float64))
         self.cf_den = np.array(desc["color_denom"], dtype=np.float64)
 
@@ -105,7 +109,7 @@ class EncodedProcess:
         return [
             ctypes.c_int(d["nexternal"]), ctypes.c_int(d["ncomb"]), _p(self.hel), _p(self.ext),
             ctypes.c_int(self.nops), _p(self.ops), ctypes.c_int(d["nwavefuncs"]), ctypes.c_int(d["ngraphs"]),
-            ctypes.c_int(d["ncolor"]), ctypes.c_int(self.njterms), _p(self.jterms), _p(self.cf_num),
+            ctypes.c_int(d["ncolor"]), ctypes.c_int(self.njterms), _p(self.jterms), _p(self.jcoef), _p(self.cf_num),
             _p(self.cf_den), ctypes.c_double(d["denominator"]),
         ]
 

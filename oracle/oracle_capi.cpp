@@ -33,19 +33,22 @@ inline bool operator!=(Flop a, Flop b) { return a.v != b.v; }
 inline Flop sqrt(Flop a) { ++Flop::sqrt_; return Flop(std::sqrt(a.v)); }
 
 static Proc make_proc(int nexternal, int ncomb, const double* hel, const int* ext, int nops, const int* ops, int nwf,
-                      int namps, int ncolor, int njterms, const int* jterms, const double* cf_num,
-                      const double* cf_den, double denominator) {
+                      int namps, int ncolor, int njterms, const int* jterms, const double* jcoef,
+                      const double* cf_num, const double* cf_den, double denominator) {
   Proc P;
   P.nexternal = nexternal; P.ncomb = ncomb; P.helicities = hel; P.externals = ext;
   P.nops = nops; P.ops = ops; P.nwf = nwf; P.namps = namps; P.ncolor = ncolor;
-  P.njterms = njterms; P.jterms = jterms; P.cf_num = cf_num; P.cf_den = cf_den; P.denominator = denominator;
+  P.njterms = njterms; P.jterms = jterms; P.jcoef = jcoef;
+  P.cf_num = cf_num; P.cf_den = cf_den; P.denominator = denominator;
   return P;
 }
 
 #define PROC_ARGS                                                                                          \
   int nexternal, int ncomb, const double *hel, const int *ext, int nops, const int *ops, int nwf, int namps, \
-      int ncolor, int njterms, const int *jterms, const double *cf_num, const double *cf_den, double denominator
-#define PROC_PASS nexternal, ncomb, hel, ext, nops, ops, nwf, namps, ncolor, njterms, jterms, cf_num, cf_den, denominator
+      int ncolor, int njterms, const int *jterms, const double *jcoef, const double *cf_num,                 \
+      const double *cf_den, double denominator
+#define PROC_PASS \
+  nexternal, ncomb, hel, ext, nops, ops, nwf, namps, ncolor, njterms, jterms, jcoef, cf_num, cf_den, denominator
 
 extern "C" {
 

@@ -25,7 +25,8 @@ struct Proc {
   const double* helicities;  // ncomb * nexternal
   const int* externals;      // nexternal * 3: (kind, mass_idx, sign)
   const int* ops;            // nops * 9: (routine, out, a0, a1, a2, a3, coup, mass_idx, width_idx)
-  const int* jterms;         // njterms * 4: (colour, amp, re, im)
+  const int* jterms;         // njterms * 2: (colour, amp)
+  const double* jcoef;       // njterms * 2: (re, im) of the coefficient, real or imaginary
   const double* cf_num;      // ncolor * ncolor
   const double* cf_den;      // ncolor
   double denominator;
@@ -76,13 +77,17 @@ T matrix_one(const Proc& P, const T* p /*nexternal*4*/, const T* hel, const T* r
       default: break;
     }
   }
-  // jamp lines (PyOut_exporter.py:334-375): coefficients are +-1 or +-i
+  // jamp lines (PyOut_exporter.py:334-375): coefficients +-1 or +-i (one quark line), or
+  // rationals times 1 or i (two quark lines: the Fierz factors 1/2 and -1/6)
   for (int t = 0; t < P.njterms; ++t) {
-    const int* j = P.jterms + 4 * t;
+    const int* j = P.jterms + 2 * t;
+    const double re = P.jcoef[2 * t], im = P.jcoef[2 * t + 1];
     const cx<T>& a = amp[j[1]];
     cx<T> term;
-    if (j[3] == 0) term = (j[2] > 0) ? a : -a;
-    else term = (j[3] > 0) ? cx<T>(-a.im, a.re) : cx<T>(a.im, -a.re);
+    if (im == 0.0) term = (re == 1.0) ? a : (re == -1.0) ? -a : cx<T>(T(re) * a.re, T(re) * a.im);
+    else if (im == 1.0) term = cx<T>(-a.im, a.re);
+    else if (im == -1.0) term = cx<T>(a.im, -a.re);
+    else term = cx<T>(-(T(im) * a.im), T(im) * a.re);
     jamp[j[0]] = jamp[j[0]] + term;
   }
   // colour sum (matrix_method_python.inc:137): sum_ij J_i cf_ij conj(J_j) / denom.

@@ -369,3 +369,65 @@ def matrix_hel_batched(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, r
     sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
     M, _, weight = amplitude_batched(p, hel, mt, wt, gs, sqh, fix_first=fix_first)
     return float(np.sum(np.abs(M) ** 2) * weight)
+
+
+# ----------------------------------------------------- q q~ > t t~ (+ one gluon)
+# Two fermion lines: every Feynman diagram written out with explicit Dirac matrices and
+# explicit colour tensors (Gell-Mann T, the f^{abc} above), independent of the generator's
+# diagram enumeration, its Fierz expansion and its two-string colour basis.
+#   light line: row spinor v-bar of the incoming antiquark (oxxxxx), column spinor u of
+#   the incoming quark (ixxxxx); colour string T[c_qbar, c_q].  Top line as above.
+
+def _vvv(e2, e3, P2, P3):
+    """Lorentz part of the three-gluon current of children e2 (momentum P2) and e3 (P3),
+    the structure of the recursion above: parent momentum P1 = -(P2 + P3)."""
+    P1 = -(P2 + P3)
+    return e2 * mdot(P1 - P2, e3) + e3 * mdot(P3 - P1, e2) + mdot(e2, e3) * (P2 - P3)
+
+
+def qq_amplitude_tensor(p, hel, mt, wt, gs, sqh):
+    """M[c_qbar, c_q, c_t, c_tbar(, a)] of u u~ > t t~ (g): legs 0 u, 1 u~, 2 t, 3 t~, 4 g."""
+    n = p.shape[0]
+    ng = n - 4
+    assert ng in (0, 1)
+    GC10, GC11 = -gs, 1j * gs
+    ref = sqh != np.sqrt(0.5)
+    u = O.wavefunction("i", p[0:1], 0.0, hel[0], 1)[0][2:6]
+    vb = O.wavefunction("o", p[1:2], 0.0, hel[1], -1)[0][2:6]
+    ub = O.wavefunction("o", p[2:3], mt, hel[2], 1)[0][2:6]
+    v = O.wavefunction("i", p[3:4], mt, hel[3], -1)[0][2:6]
+    K = [-p[0], -p[1], p[2], p[3]]
+
+    def fprop(P, m, w):
+        return 1j * (slash(P.astype(complex)) + m * np.eye(4)) / (mdot(P, P) - m * m + 1j * m * w)
+
+    KL, KT = K[0] + K[1], K[2] + K[3]
+    # each line's current as a gluon with its propagator (colour T^b on the line, not included)
+    Lprop = (-1j / mdot(KL, KL)) * GC11 * np.array([vb @ G[m] @ u for m in range(4)])
+    Tprop = (-1j / mdot(KT, KT)) * GC11 * np.array([ub @ G[m] @ v for m in range(4)])
+    TT = np.einsum("bij,bkl->ijkl", T, T)                   # sum_b T^b[qb, q] T^b[t, tb]
+    if ng == 0:
+        return GC11 * (ub @ slash(Lprop) @ v) * TT
+    eps = O.wavefunction("v", p[4:5], 0.0, hel[4], 1, ref_compat=ref)[0][2:6]
+    kg = p[4]
+    se, sL, sT = slash(eps), slash(Lprop), slash(Tprop)
+    A1 = GC11 ** 2 * ub @ se @ fprop(K[2] + kg, mt, wt) @ sL @ v           # gluon off the top
+    A2 = GC11 ** 2 * ub @ sL @ fprop(K[2] + KL, mt, wt) @ se @ v           # off the anti-top
+    A3 = GC11 ** 2 * vb @ se @ fprop(K[1] + kg, 0.0, 0.0) @ sT @ u        # off the antiquark
+    A4 = GC11 ** 2 * vb @ sT @ fprop(K[1] + KT, 0.0, 0.0) @ se @ u        # off the quark
+    J = GC10 * _vvv(Lprop, eps, KL, kg) * (-1j / mdot(KL + kg, KL + kg))  # the s-channel gluon
+    A5 = GC11 * ub @ slash(J) @ v
+    TaTb = np.einsum("aij,bjk->abik", T, T)                # (T^a T^b)[i, k]
+    M = (A1 * np.einsum("abik,bqr->qrika", TaTb, T)        # (T^a T^b)[t, tb] T^b[qb, q]
+         + A2 * np.einsum("baik,bqr->qrika", TaTb, T)      # (T^b T^a)[t, tb] T^b[qb, q]
+         + A3 * np.einsum("abqr,bik->qrika", TaTb, T)      # (T^a T^b)[qb, q] T^b[t, tb]
+         + A4 * np.einsum("baqr,bik->qrika", TaTb, T)      # (T^b T^a)[qb, q] T^b[t, tb]
+         + A5 * np.einsum("cba,cik,bqr->qrika", F, T, T))  # f^{c b a} T^c[t, tb] T^b[qb, q]
+    return M
+
+
+def qq_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False):
+    """Colour-summed |M|^2 of one helicity row of u u~ > t t~ (g)."""
+    gs = 2 * np.sqrt(np.pi * alpha_s)
+    sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
+    return float(np.sum(np.abs(qq_amplitude_tensor(p, hel, mt, wt, gs, sqh)) ** 2))

@@ -4,14 +4,15 @@ import numpy as np
 
 from oracle import oracle as O
 
-SQRTS = {"gg_ttx": 1000.0, "gg_ttxg": 2000.0, "gg_ttxgg": 3000.0, "gg_ttxggg": 5000.0}
-NOUT = {"gg_ttx": 2, "gg_ttxg": 3, "gg_ttxgg": 4, "gg_ttxggg": 5}
+SQRTS = {"gg_ttx": 1000.0, "gg_ttxg": 2000.0, "gg_ttxgg": 3000.0, "gg_ttxggg": 5000.0,
+         "uux_ttx": 1000.0, "uux_ttxg": 2000.0}
+NOUT = {"gg_ttx": 2, "gg_ttxg": 3, "gg_ttxgg": 4, "gg_ttxggg": 5, "uux_ttx": 2, "uux_ttxg": 3}
 MT = 173.0
 
 
 def phase_space_points(name, nev, seed=7, sqrts=None):
-    """(nev, nexternal, 4) momenta: incoming gluons along +-z, outgoing t, t~, gluons
-    from the oracle's massive RAMBO (per-event Newton)."""
+    """(nev, nexternal, 4) momenta: incoming partons (gluons or u, u~) along +-z,
+    outgoing t, t~, gluons from the oracle's massive RAMBO (per-event Newton)."""
     sqrts = sqrts or SQRTS[name]
     n = NOUT[name]
     rng = np.random.default_rng(seed)

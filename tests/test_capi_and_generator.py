@@ -3,6 +3,7 @@ arguments without touching a GPU) and of the process generator."""
 
 import ctypes
 import json
+import math
 import os
 
 import numpy as np
@@ -26,7 +27,7 @@ def test_registry_and_argument_checks():
     names = [lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())]
     # sorted by (external legs, name): the process names select the default kernels,
     # gg_ttxgg_fused the fused kernel of g g > t t~ g g
-    assert names == ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxgg_fused", "gg_ttxggg"]
+    assert names == ["gg_ttx", "uux_ttx", "gg_ttxg", "uux_ttxg", "gg_ttxgg", "gg_ttxgg_fused", "gg_ttxggg"]
     info = np.zeros(5, dtype=np.int32)
     ttgg = lib.mfx_process_index(b"gg_ttxgg")
     assert lib.mfx_process_info(ttgg, info.ctypes.data_as(ctypes.c_void_p)) == 0
@@ -214,12 +215,13 @@ def test_trace_sum_closed_forms():
     assert colour.trace_sum([0, 1, 1, 0]) == Fraction(16, 3)
 
 
-@pytest.mark.parametrize("ng", [0, 1, 2, 3])
-def test_emitted_files_are_current(ng):
+@pytest.mark.parametrize("kind,ng", [("gg", 0), ("gg", 1), ("gg", 2), ("gg", 3), ("qq", 0), ("qq", 1)])
+def test_emitted_files_are_current(kind, ng):
     """Every committed generated file (process description, drop-in module, HIP kernels of
     each kernel family) equals what the generator produces now."""
     from madflow_amd.procgen.emit import emitted_files
-    d = gg_ttx_ng(ng)
+    from madflow_amd.procgen.diagrams import qq_ttx_ng
+    d = gg_ttx_ng(ng) if kind == "gg" else qq_ttx_ng(ng)
     files = emitted_files(d)
     assert os.path.join("processes", d["name"] + ".json") in files
     for rel, text in files.items():
@@ -237,7 +239,7 @@ def test_kernel_families_are_registered():
     import importlib
     lib = _lib.load()
     names = {lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg"):
         mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % name)
         cls = getattr(mod, "Matrix_1_%s" % name)
         for fam in cls.kernels:
@@ -288,7 +290,7 @@ def test_dual_row_program(ng):
             defined.add(op["out"])
 
 
-@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg"])
+@pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg"])
 def test_process_module_model_params(name):
     """Every drop-in module's get_model_param() evaluates exactly the parameters its
     smatrix takes, in that order."""
@@ -299,3 +301,59 @@ def test_process_module_model_params(name):
     model.freeze_alpha_s(0.118)
     vals = list(model.evaluate())
     assert len(vals) == len(cls.params)
+
+
+def test_qq_generator_counts_and_colour():
+    """u u~ > t t~ + k g: 1, 5 and 36 diagrams and 2, 4 and 12 colour flows for k = 0, 1, 2
+    (MadGraph's counts); the colour matrix of the two-string basis for k = 0 is
+    [[N^2, N], [N, N^2]] = [[9, 3], [3, 9]], and the single amplitude's jamp coefficients
+    are the Fierz factors -1/(2N) (straight) and 1/2 (crossed)."""
+    from fractions import Fraction
+    from madflow_amd.procgen.diagrams import qq_ttx_ng
+    for k, (nd, nc) in enumerate([(1, 2), (5, 4), (36, 12)]):
+        d = qq_ttx_ng(k)
+        assert (d["ndiags"], d["ncolor"]) == (nd, nc)
+        assert d["denominator"] == 36 * math.factorial(k)
+    d = qq_ttx_ng(0)
+    assert d["color_num"] == [[9, 3], [3, 9]] and d["color_denom"] == [1, 1]
+    assert [[(a, Fraction(re), Fraction(im)) for a, re, im in t] for t in d["jamp"]] == [
+        [(0, Fraction(-1, 6), 0)], [(0, Fraction(1, 2), 0)]]
+    # gluon on each line, and through the three-gluon vertex: every flow is reached
+    d = qq_ttx_ng(1)
+    assert all(d["jamp"])
+    assert sorted(op["routine"] for op in d["ops"] if op["kind"] == "wf") == sorted(
+        ["FFV1P0_3", "FFV1P0_3", "FFV1P0_3", "FFV1_1", "FFV1_2", "FFV1_1", "FFV1_2", "VVV1P0_1"])
+
+
+def test_two_line_colour_matrix_brute_force():
+    """The two-string colour matrix of u u~ > t t~ g equals the explicit Gell-Mann sum
+    sum over all colour indices of B_s conj(B_t)."""
+    from madflow_amd.procgen.diagrams import qq_ttx_ng
+    from madflow_amd.procgen.colour import colour_matrix_two_lines
+    from tests.bruteforce import T
+    d = qq_ttx_ng(1)
+    basis = [(p, tuple(w1), tuple(w2)) for p, w1, w2 in d["colour_basis"]]
+    eye = np.eye(3)
+
+    def string(word, a):
+        m = eye.astype(complex)
+        for _ in word:
+            m = m @ T[a]
+        return m
+
+    def tensor(b):
+        # B[c_t, c_tbar, c_qbar, c_q, a]
+        pairing, w1, w2 = b
+        out = np.zeros((3, 3, 3, 3, 8), complex)
+        for a in range(8):
+            s1, s2 = string(w1, a), string(w2, a)
+            if pairing == 0:
+                out[..., a] = np.einsum("ij,kl->ijkl", s1, s2)
+            else:   # w1 from the top to the quark, w2 from the antiquark to the anti-top
+                out[..., a] = np.einsum("il,kj->ijkl", s1, s2)
+        return out
+
+    Bs = [tensor(b) for b in basis]
+    brute = np.array([[np.sum(x * np.conj(y)).real for y in Bs] for x in Bs])
+    exact = np.array(colour_matrix_two_lines(basis), dtype=float)
+    np.testing.assert_allclose(brute, exact, atol=1e-12)

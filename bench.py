@@ -204,8 +204,13 @@ def _rate_at(timed, t_cut, wall):
 def run_main(a, torch):
     """Plain VEGAS at the main precision: W warm-up iterations, then exactly K timed ones."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
-    ig = MEIntegrand(a.process, precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
-                     kernels=a.kernels, **ME_ONLY_KIN[a.process])
+    if a.mapping == "multichannel":
+        from madflow_amd.multichannel import MultiChannelIntegrand
+        ig = MultiChannelIntegrand(a.process, precision=a.precision, seed=3, events_limit=a.events_limit,
+                                   kernels=a.kernels, **ME_ONLY_KIN[a.process])
+    else:
+        ig = MEIntegrand(a.process, precision=a.precision, mapping=a.mapping, seed=3, events_limit=a.events_limit,
+                         kernels=a.kernels, **ME_ONLY_KIN[a.process])
     vf = VegasFlow(ig.ndim, a.nev)
     vf.compile(ig)
     log("warm-up: %d VEGAS iterations of %d points (grid adaptation)" % (a.warmup, a.nev))
@@ -485,7 +490,8 @@ def main(argv=None):
                     help="kernel family of the matrix element (default: the drop-in class's, the faster form; "
                          "colour-ordered for gg_ttxgg)")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
-    ap.add_argument("--mapping", default="diet", choices=["diet", "rambo"])
+    ap.add_argument("--mapping", default="diet", choices=["diet", "rambo", "multichannel"],
+                    help="phase space of the plain VEGAS leg; multichannel skips the flow leg")
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--max-iter", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -597,7 +603,7 @@ def main(argv=None):
                                            m["ig"].params,
                                            list(m["ig"].matrix.params))
         res["cpu_baseline"]["c1_end_to_end"] = cpu_c1_end_to_end(res["cpu_baseline"]["cores"])
-    if not a.no_flow:
+    if not a.no_flow and a.mapping != "multichannel":
         fc = run_flow_compare(a, torch, m)
         res["flow_comparison"] = fc
         # the product path: FP32 matrix elements with the learned proposal, to the target

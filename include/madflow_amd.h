@@ -174,6 +174,42 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
                          const int* slot, const double* me, const double* wps, const double* f_in, double* f_out,
                          void* workspace, int64_t workspace_bytes, double* result, void* stream);
 
+/* ------------------------------------------------ multi-channel phase space */
+/* No counterpart in the reference (doc/source/usage.rst:84-99 names the limitation: RAMBO
+ * and VEGAS only).  A channel maps the 3*n_out-4 dimensional cube onto the fixed-sqrt(s)
+ * phase space (outgoing legs numbered from 0: t, t~, gluons):
+ *   type 0 flat:       RAMBO on diet over every outgoing leg;
+ *   type 1 t-channel:  massless leg a with pT ~ 1/pT^2 on [lo, hi] (hi = the largest
+ *                      energy the leg can carry), rapidity and azimuth flat; the other
+ *                      legs by diet in the rest frame of P - k_a;
+ *   type 2 s-channel:  massless legs a, b with m_ab^2 ~ 1/m^2 on [lo, hi], isotropic in
+ *                      their rest frame; the other legs and the pair by diet. */
+typedef struct mfx_mc_channel {
+  int type;
+  int a, b;
+  int reserved;
+  double lo, hi;
+} mfx_mc_channel;
+
+/* mfx_ps_generate for the nev points of one channel (prm: the diet parameters, mapping 1):
+ * Philox uniforms (counter ev0+e, iteration) -> VEGAS map of that channel's grid (grids:
+ * device [nch][ndim][nbins+1]) -> the channel's map; wps = scale / sum_j alpha_j g_j(p)
+ * x gev2pb / (2 s), with g_j the density of channel j at the point (its inverse map and
+ * grid); ratios (device, nev x nch) receives g_j / g.  scale = alpha_channel * N / nev
+ * makes the mean of f over all channels' points the stratified estimate.  The other
+ * outputs, the workspace and *npass_host are those of mfx_ps_generate. */
+int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                    double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
+                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
+
+/* result (device, nch doubles) = sum over the nev points of f^2 ratios[e][j],
+ * f = me[slot[e]] * wps[e] (0 when slot[e] < 0): the moments that adapt the channel
+ * weights (Kleiss and Pittau).  Fixed-order compensated sums; workspace of at least
+ * 1024 * nch * 16 bytes (mfx_integrand_workspace_size for the same nev suffices). */
+int mfx_mc_moments(int64_t nev, int nch, const int* slot, const double* me, const double* wps, const double* ratios,
+                   void* workspace, int64_t workspace_bytes, double* result, void* stream);
+
 /* VEGAS grid refinement from the f^2 histogram: smoothing, ((1-r)/ln(1/r))^alpha damping
  * and equal-weight rebinning, per dimension (device grid updated in place). */
 int mfx_vegas_refine(double* grid, const double* hist, int ndim, int nbins, double alpha, void* stream);

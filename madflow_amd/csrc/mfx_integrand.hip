@@ -194,6 +194,251 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
   if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
 }
 
+// ---------------------------------------------------------------- multi-channel
+// Multi-channel phase space (Kleiss and Pittau, Comput. Phys. Commun. 83 (1994) 141;
+// one VEGAS grid per channel as in T. Ohl, Comput. Phys. Commun. 120 (1999) 13).  The
+// density of a point is g(p) = sum_j alpha_j g_j(p), g_j = 1 / (VEGAS jacobian of grid j
+// at x_j(p) x dPhi/dx of channel j's map at p), where x_j(p) is the inverse of channel j's
+// map; the weight is 1 / g(p).  Maps (outgoing legs 0 = t, 1 = t~, 2.. = gluons):
+//   0 flat:  RAMBO on diet over every outgoing leg;
+//   1 t-channel of leg a: pT ~ 1/pT^2 on [lo, hi = E_max], rapidity flat in the kinematic
+//     range |y| <= acosh(E_max / pT), azimuth flat; the other legs by diet in the rest
+//     frame of P - k_a  (Phi_n = int d^3k / ((2pi)^3 2E) Phi_{n-1}(P - k));
+//   2 s-channel of legs (a, b): m_ab^2 ~ 1/m^2 on [lo, hi], isotropic decay in the pair's
+//     rest frame; the other legs and the pair by diet in the c.m. frame
+//     (Phi_n = int dm^2 / (2pi) Phi_{n-1}(P; ..., G(m)) Phi_2(G), Phi_2 = 1 / (8 pi)).
+constexpr int MC_MAXCH = 8;
+
+struct McChannel {
+  int type, a, b, nsub;
+  int legs[RAMBO_MAXN];   // outgoing leg of each diet slot; the pair G is the last slot
+  double lo, hi;
+  RamboConst sub;         // diet constants of the sub-system (s-channel: G's mass per event)
+};
+
+struct McConst {
+  int nch;
+  double scale;           // weight factor alpha_c N / n_c of the stratified estimator
+  double alpha[MC_MAXCH];
+  McChannel ch[MC_MAXCH];
+};
+
+__device__ __forceinline__ double grid_jacobian(const double* __restrict__ grid, int ndim, int nb, const double* x) {
+  double jac = 1.0;
+  for (int d = 0; d < ndim; ++d) {
+    const double* e = grid + d * (nb + 1);
+    int lo = 0, hi = nb;            // e[lo] <= x < e[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (e[mid] <= x[d]) lo = mid; else hi = mid;
+    }
+    jac *= (e[lo + 1] - e[lo]) * nb;
+  }
+  return jac;
+}
+
+// channel map: cube point x -> outgoing momenta p (c.m. frame); returns dPhi/dx
+__device__ double mc_forward(const McChannel& ch, double sqrts, const double* x, double (*p)[4], bool& conv) {
+  conv = true;
+  if (ch.type == 0) return diet_event(ch.sub, x, sqrts, p, conv);
+  double q[RAMBO_MAXN][4];
+  if (ch.type == 1) {
+    const double A = 1.0 / ch.lo, B = 1.0 / ch.hi;
+    const double pt = 1.0 / (A - x[0] * (A - B));
+    const double ymax = acosh(fmax(ch.hi / pt, 1.0));
+    const double y = (2.0 * x[1] - 1.0) * ymax;
+    double sp, cp;
+    sincos(2.0 * MFX_PI * x[2], &sp, &cp);
+    const double k[4] = {pt * cosh(y), pt * cp, pt * sp, pt * sinh(y)};
+    const double X[4] = {sqrts - k[0], -k[1], -k[2], -k[3]};
+    const double MX = sqrt(fmax(X[0] * X[0] - X[1] * X[1] - X[2] * X[2] - X[3] * X[3], 0.0));
+    if (!(MX > ch.sub.total_mass)) {
+      conv = false;
+      return 0.0;
+    }
+    const double w = diet_event(ch.sub, x + 3, MX, q, conv);
+    for (int j = 0; j < ch.nsub; ++j) boost_by(q[j], X, MX, 1.0, p[ch.legs[j]]);
+    for (int c = 0; c < 4; ++c) p[ch.a][c] = k[c];
+    return pt * pt * pt * (A - B) * (2.0 * ymax) / (8.0 * MFX_PI * MFX_PI) * w;
+  }
+  const double L = log(ch.hi / ch.lo);
+  const double m2 = ch.lo * exp(x[0] * L), m = sqrt(m2);
+  RamboConst rc = ch.sub;
+  rc.masses[ch.nsub - 1] = m;
+  rc.total_mass += m;
+  rc.massive = 1;
+  if (!(sqrts > rc.total_mass)) {
+    conv = false;
+    return 0.0;
+  }
+  const double w = diet_event(rc, x + 3, sqrts, q, conv);
+  const double* G = q[ch.nsub - 1];
+  const double ct = 2.0 * x[1] - 1.0, st = sqrt(fmax(0.0, 1.0 - ct * ct));
+  double sp, cp;
+  sincos(2.0 * MFX_PI * x[2], &sp, &cp);
+  const double h = 0.5 * m;
+  const double ka[4] = {h, h * st * cp, h * st * sp, h * ct}, kb[4] = {h, -h * st * cp, -h * st * sp, -h * ct};
+  for (int j = 0; j < ch.nsub - 1; ++j)
+    for (int c = 0; c < 4; ++c) p[ch.legs[j]][c] = q[j][c];
+  boost_by(ka, G, m, 1.0, p[ch.a]);
+  boost_by(kb, G, m, 1.0, p[ch.b]);
+  return w * m2 * L / (16.0 * MFX_PI * MFX_PI);
+}
+
+// inverse of mc_forward: outgoing momenta (c.m. frame) -> x and dPhi/dx; ok = false
+// outside the channel's support (its density is zero there)
+__device__ double mc_inverse(const McChannel& ch, double sqrts, const double (*p)[4], double* x, bool& ok) {
+  double q[RAMBO_MAXN][4];
+  ok = true;
+  if (ch.type == 0) {
+    for (int j = 0; j < ch.nsub; ++j)
+      for (int c = 0; c < 4; ++c) q[j][c] = p[j][c];
+    return diet_inverse(ch.sub, sqrts, q, x);
+  }
+  if (ch.type == 1) {
+    const double* k = p[ch.a];
+    const double pt = sqrt(k[1] * k[1] + k[2] * k[2]);
+    ok = pt >= ch.lo && pt <= ch.hi;
+    const double ptc = fmin(fmax(pt, ch.lo), ch.hi);
+    const double A = 1.0 / ch.lo, B = 1.0 / ch.hi;
+    const double ymax = acosh(fmax(ch.hi / ptc, 1.0));
+    const double y = asinh(k[3] / ptc);
+    ok = ok && fabs(y) <= ymax;
+    x[0] = (A - 1.0 / ptc) / (A - B);
+    x[1] = 0.5 * (ymax > 0.0 ? y / ymax : 0.0) + 0.5;
+    double ph = atan2(k[2], k[1]);
+    if (ph < 0.0) ph += 2.0 * MFX_PI;
+    x[2] = ph / (2.0 * MFX_PI);
+    const double X[4] = {sqrts - k[0], -k[1], -k[2], -k[3]};
+    const double MX = sqrt(fmax(X[0] * X[0] - X[1] * X[1] - X[2] * X[2] - X[3] * X[3], 0.0));
+    ok = ok && MX > ch.sub.total_mass;
+    if (!ok) return 0.0;
+    for (int j = 0; j < ch.nsub; ++j) boost_by(p[ch.legs[j]], X, MX, -1.0, q[j]);
+    const double w = diet_inverse(ch.sub, MX, q, x + 3);
+    return ptc * ptc * ptc * (A - B) * (2.0 * ymax) / (8.0 * MFX_PI * MFX_PI) * w;
+  }
+  double G[4];
+  for (int c = 0; c < 4; ++c) G[c] = p[ch.a][c] + p[ch.b][c];
+  const double m2 = G[0] * G[0] - G[1] * G[1] - G[2] * G[2] - G[3] * G[3];
+  ok = m2 >= ch.lo && m2 <= ch.hi;
+  if (!ok) return 0.0;
+  const double L = log(ch.hi / ch.lo), m = sqrt(m2);
+  x[0] = log(m2 / ch.lo) / L;
+  double kr[4];
+  boost_by(p[ch.a], G, m, -1.0, kr);
+  const double ka = sqrt(kr[1] * kr[1] + kr[2] * kr[2] + kr[3] * kr[3]);
+  x[1] = 0.5 * (kr[3] / ka + 1.0);
+  double ph = atan2(kr[2], kr[1]);
+  if (ph < 0.0) ph += 2.0 * MFX_PI;
+  x[2] = ph / (2.0 * MFX_PI);
+  for (int j = 0; j < ch.nsub - 1; ++j)
+    for (int c = 0; c < 4; ++c) q[j][c] = p[ch.legs[j]][c];
+  for (int c = 0; c < 4; ++c) q[ch.nsub - 1][c] = G[c];
+  RamboConst rc = ch.sub;
+  rc.masses[ch.nsub - 1] = m;
+  rc.total_mass += m;
+  rc.massive = 1;
+  const double w = diet_inverse(rc, sqrts, q, x + 3);
+  return w * m2 * L / (16.0 * MFX_PI * MFX_PI);
+}
+
+// Points of channel c: VEGAS map of grid c, channel c's map, the densities of every
+// channel at the point (ratios[e][j] = g_j / g) and the weight scale / g; then cuts, as
+// k_generate (fixed sqrt(s), c.m. frame).
+__global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst mc, int c,
+                                                           const double* __restrict__ grids, uint64_t ev0, int64_t nev,
+                                                           uint32_t iteration, double* __restrict__ all_ps,
+                                                           double* __restrict__ wps, uint8_t* __restrict__ flag,
+                                                           double* __restrict__ ratios, int* __restrict__ block_count) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  bool ok = false;
+  if (e < nev) {
+    const int D = g.ndim, NB = g.nbins;
+    double y[MAXDIM], x[MAXDIM];
+    uniforms(g, ev0 + e, iteration, y);
+    const double jac = vmap(g, grids + (size_t)c * D * (NB + 1), y, x, nullptr);
+    double p[RAMBO_MAXN + 2][4];
+    bool conv = true;
+    const double w = mc_forward(mc.ch[c], g.sqrts, x, p + 2, conv);
+    double gj[MC_MAXCH], gsum = 0.0;
+    for (int j = 0; j < mc.nch; ++j) {
+      double v = 0.0;
+      if (j == c) {
+        v = 1.0 / (jac * w);
+      } else if (conv) {
+        double xj[MAXDIM];
+        bool in = false;
+        const double wj = mc_inverse(mc.ch[j], g.sqrts, p + 2, xj, in);
+        if (in && wj > 0.0) v = 1.0 / (grid_jacobian(grids + (size_t)j * D * (NB + 1), D, NB, xj) * wj);
+      }
+      if (!isfinite(v)) v = 0.0;
+      gj[j] = v;
+      gsum += mc.alpha[j] * v;
+    }
+    const double eh = 0.5 * g.sqrts;
+    p[0][0] = eh; p[0][1] = 0.0; p[0][2] = 0.0; p[0][3] = eh;
+    p[1][0] = eh; p[1][1] = 0.0; p[1][2] = 0.0; p[1][3] = -eh;
+    const double wt = mc.scale / gsum;
+    ok = conv && w > 0.0 && gsum > 0.0 && isfinite(wt) && pass_cuts(g, p + 2);
+    double* out = all_ps + e * (g.nexternal * 4);
+    for (int i = 0; i < g.nexternal; ++i)
+      for (int k = 0; k < 4; ++k) out[4 * i + k] = p[i][k];
+    wps[e] = ok ? wt * g.norm : 0.0;
+    flag[e] = ok ? 1 : (conv ? 0 : 2);
+    for (int j = 0; j < mc.nch; ++j) ratios[e * mc.nch + j] = gsum > 0.0 ? gj[j] / gsum : 0.0;
+  }
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const unsigned long long bal = __ballot(ok);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&cnt, __popcll(bal));
+  __syncthreads();
+  if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
+}
+
+// sum over events of f^2 r_j, f = me[slot] * wps: the channel-weight moments
+// W_j = <(f/g)^2 g_j/g> of the Kleiss-Pittau update (per-block compensated partials)
+__global__ void __launch_bounds__(ACC_BLOCK) k_mc_moments(int64_t nev, int nch, const int* __restrict__ slot,
+                                                          const double* __restrict__ me, const double* __restrict__ wps,
+                                                          const double* __restrict__ ratios,
+                                                          double* __restrict__ partials) {
+  KSum acc[MC_MAXCH];
+  for (int64_t e = (int64_t)blockIdx.x * ACC_BLOCK + threadIdx.x; e < nev; e += (int64_t)gridDim.x * ACC_BLOCK) {
+    const int s = slot[e];
+    double f = s >= 0 ? me[s] * wps[e] : 0.0;
+    if (!isfinite(f)) f = 0.0;
+    for (int j = 0; j < nch; ++j) acc[j].add(f * f * ratios[e * nch + j]);
+  }
+  __shared__ double red[ACC_BLOCK][2];
+  for (int j = 0; j < nch; ++j) {
+    red[threadIdx.x][0] = acc[j].s;
+    red[threadIdx.x][1] = acc[j].c;
+    __syncthreads();
+    for (int off = ACC_BLOCK / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) {
+        KSum a{red[threadIdx.x][0], red[threadIdx.x][1]};
+        a.merge(KSum{red[threadIdx.x + off][0], red[threadIdx.x + off][1]});
+        red[threadIdx.x][0] = a.s;
+        red[threadIdx.x][1] = a.c;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partials[((size_t)blockIdx.x * nch + j) * 2] = red[0][0];
+      partials[((size_t)blockIdx.x * nch + j) * 2 + 1] = red[0][1];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_mc_reduce(int nparts, int nch, const double* __restrict__ partials, double* __restrict__ result) {
+  const int j = threadIdx.x;
+  if (j >= nch) return;
+  KSum a;
+  for (int b = 0; b < nparts; ++b) a.merge(KSum{partials[((size_t)b * nch + j) * 2], partials[((size_t)b * nch + j) * 2 + 1]});
+  result[j] = a.s + a.c;
+}
+
 // ---------------------------------------------------------------- unweighting
 // Hit-or-miss unweighting (the reference hands its weighted events to MadGraph's
 // lhe_parser unweight, lhe_writer.py:319-356): event e is kept with probability
@@ -587,6 +832,111 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
   hipLaunchKernelGGL(k_accumulate, dim3(ACC_GRID), dim3(ACC_BLOCK), shmem, s, g, ev0, nev, iteration, y_in, slot, me,
                      wps, f_in, f_out, partials);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, ACC_GRID, hb, partials, result);
+  return hip_status(hipGetLastError());
+}
+
+static int make_mc(const mfx_ps_params* prm, const GenConst& g, const mfx_mc_channel* chs, int nch,
+                   const double* alpha, double scale, McConst& mc) {
+  if (g.mapping != 1) return fail(-2, "multi-channel phase space needs the diet parameters (mapping 1)%s", "");
+  if (nch < 1 || nch > MC_MAXCH) return fail(-2, "between 1 and 8 channels%s", "");
+  if (!chs || !alpha) return fail(-3, "null pointer%s", "");
+  if (!(scale > 0.0) || !isfinite(scale)) return fail(-2, "scale must be > 0%s", "");
+  memset(&mc, 0, sizeof(mc));
+  mc.nch = nch;
+  mc.scale = scale;
+  const int n = g.rc.n;
+  for (int c = 0; c < nch; ++c) {
+    const mfx_mc_channel& in = chs[c];
+    McChannel& ch = mc.ch[c];
+    if (!(alpha[c] >= 0.0) || !isfinite(alpha[c])) return fail(-2, "channel weights must be >= 0%s", "");
+    mc.alpha[c] = alpha[c];
+    ch.type = in.type;
+    ch.a = in.a;
+    ch.b = in.b;
+    ch.lo = in.lo;
+    ch.hi = in.hi;
+    int m = 0;
+    if (in.type == 0) {
+      for (int i = 0; i < n; ++i) ch.legs[m++] = i;
+    } else if (in.type == 1 || in.type == 2) {
+      if (in.a < 0 || in.a >= n || prm->masses[in.a] != 0.0) return fail(-2, "channel leg a must be a massless outgoing leg%s", "");
+      if (in.type == 2 && (in.b < 0 || in.b >= n || in.b == in.a || prm->masses[in.b] != 0.0))
+        return fail(-2, "s-channel leg b must be another massless outgoing leg%s", "");
+      if (!(in.lo > 0.0 && in.hi > in.lo) || !isfinite(in.hi)) return fail(-2, "channel range must be 0 < lo < hi%s", "");
+      for (int i = 0; i < n; ++i)
+        if (i != in.a && !(in.type == 2 && i == in.b)) ch.legs[m++] = i;
+      if (in.type == 2) ch.legs[m++] = -1;     // the pair
+      if (m < 2) return fail(-2, "a channel's sub-system needs two or more legs%s", "");
+    } else {
+      return fail(-2, "channel type must be 0 (flat), 1 (t-channel) or 2 (s-channel)%s", "");
+    }
+    ch.nsub = m;
+    RamboConst& rc = ch.sub;
+    rc.n = m;
+    rc.massive = in.type == 2 ? 1 : 0;
+    rc.total_mass = 0.0;
+    for (int j = 0; j < m; ++j) {
+      rc.masses[j] = ch.legs[j] >= 0 ? prm->masses[ch.legs[j]] : 0.0;
+      rc.total_mass += rc.masses[j];
+      if (rc.masses[j] != 0.0) rc.massive = 1;
+    }
+    rc.logw0 = log(MFX_PI / 2.0) * (m - 1) - 2.0 * lgamma((double)(m - 1)) - log((double)(m - 1));
+    rc.inorm = 1.0 / pow(2.0 * MFX_PI, 3 * m - 4);
+  }
+  return 0;
+}
+
+int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                    double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
+                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  GenConst g;
+  int rc = make_gen(prm, g);
+  if (rc) return rc;
+  McConst mc;
+  rc = make_mc(prm, g, channels, nch, alpha, scale, mc);
+  if (rc) return rc;
+  if (channel < 0 || channel >= nch) return fail(-2, "channel out of range%s", "");
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (nev > 0x7fffffff) return fail(-2, "nev must fit in 31 bits%s", "");
+  if (workspace_bytes < mfx_integrand_workspace_size(nev, prm->ndim, prm->nbins))
+    return fail(-8, "workspace too small (see mfx_integrand_workspace_size)%s", "");
+  if (nev == 0) {
+    if (npass_host) *npass_host = 0;
+    return 0;
+  }
+  if (!grids || !all_ps || !wps || !flag || !slot || !dense || !ratios || !workspace) return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
+  int* counts = (int*)workspace;
+  int* offsets = counts + nblk + 1;
+  hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
+                     all_ps, wps, flag, ratios, counts);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
+  hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
+                     slot);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_status(e);
+  if (npass_host) {
+    int tot = 0;
+    e = hipMemcpyAsync(&tot, offsets + nblk, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e);
+    *npass_host = tot;
+  }
+  return 0;
+}
+
+int mfx_mc_moments(int64_t nev, int nch, const int* slot, const double* me, const double* wps, const double* ratios,
+                   void* workspace, int64_t workspace_bytes, double* result, void* stream) {
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (nch < 1 || nch > MC_MAXCH) return fail(-2, "between 1 and 8 channels%s", "");
+  if (workspace_bytes < (int64_t)ACC_GRID * nch * 2 * 8) return fail(-8, "workspace too small%s", "");
+  if (!result || !workspace || (nev > 0 && (!slot || !wps || !ratios))) return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  double* partials = (double*)workspace;
+  hipLaunchKernelGGL(k_mc_moments, dim3(ACC_GRID), dim3(ACC_BLOCK), 0, s, nev, nch, slot, me, wps, ratios, partials);
+  hipLaunchKernelGGL(k_mc_reduce, dim3(1), dim3(64), 0, s, ACC_GRID, nch, partials, result);
   return hip_status(hipGetLastError());
 }
 

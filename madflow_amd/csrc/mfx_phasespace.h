@@ -166,4 +166,71 @@ __device__ __forceinline__ double diet_event(const RamboConst& rc, const double*
   return exp(logw) * rc.inorm;
 }
 
+// Pure boost of k by the velocity of Q (mass M); sign = -1 boosts into Q's rest frame.
+__device__ __forceinline__ void boost_by(const double* k, const double* Q, double M, double sign, double* out) {
+  const double b1 = sign * Q[1] / M, b2 = sign * Q[2] / M, b3 = sign * Q[3] / M, g = Q[0] / M;
+  const double bk = b1 * k[1] + b2 * k[2] + b3 * k[3];
+  const double c = bk / (g + 1.0) + k[0];
+  out[0] = g * k[0] + bk;
+  out[1] = k[1] + b1 * c;
+  out[2] = k[2] + b2 * c;
+  out[3] = k[3] + b3 * c;
+}
+
+// Inverse of diet_event: n momenta p (overwritten) in the rest frame of their sum
+// (M, 0, 0, 0) -> the 3n-4 cube coordinates x that diet_event maps onto them, and the
+// weight diet_event gives there.  Massive particles are first mapped back to the massless
+// momenta q_i = p_i / xi, xi = sum |p_i| / M (the inverse of RAMBO's rescaling); the
+// massive weight factor follows from the p_i alone.  The decay masses then give
+// x[i] = F(M_{i+1}^2 / M_i^2) with F the CDF diet_solve inverts, and the direction of q_i
+// in the rest frame of Q_i = q_i + ... + q_{n-1} gives the two angle coordinates.
+__device__ __forceinline__ double diet_inverse(const RamboConst& rc, double M, double (*p)[4], double* x) {
+  const int n = rc.n;
+  double logw = rc.logw0 + (2 * n - 4) * log(M);
+  if (rc.massive) {
+    double sa = 0.0, prod = 1.0, s2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double pa = sqrt(p[i][1] * p[i][1] + p[i][2] * p[i][2] + p[i][3] * p[i][3]);
+      sa += pa;
+      prod *= pa / p[i][0];
+      s2 += pa * pa / p[i][0];
+    }
+    const double xi = sa / M;
+    logw += (2 * n - 3) * log(xi) + log(prod / s2 * M);
+    const double ixi = 1.0 / xi;
+    for (int i = 0; i < n; ++i) {
+      const double pa = sqrt(p[i][1] * p[i][1] + p[i][2] * p[i][2] + p[i][3] * p[i][3]);
+      p[i][0] = pa * ixi;
+      p[i][1] *= ixi;
+      p[i][2] *= ixi;
+      p[i][3] *= ixi;
+    }
+  }
+  double Q[4] = {M, 0.0, 0.0, 0.0};
+  double Mc = M;
+  for (int i = 0; i < n - 1; ++i) {
+    double Qn[4];
+    for (int j = 0; j < 4; ++j) Qn[j] = Q[j] - p[i][j];
+    double Mn = 0.0;
+    if (i < n - 2) {
+      Mn = sqrt(fmax(Qn[0] * Qn[0] - Qn[1] * Qn[1] - Qn[2] * Qn[2] - Qn[3] * Qn[3], 0.0));
+      const double u = (Mn / Mc) * (Mn / Mc);
+      const int k = n - 2 - i;
+      double uk = 1.0;
+      for (int j = 0; j < k; ++j) uk *= u;
+      x[i] = (k + 1) * uk - k * uk * u;
+    }
+    double kr[4];
+    boost_by(p[i], Q, Mc, -1.0, kr);
+    const double ka = sqrt(kr[1] * kr[1] + kr[2] * kr[2] + kr[3] * kr[3]);
+    x[n - 2 + 2 * i] = 0.5 * (kr[3] / ka + 1.0);
+    double ph = atan2(kr[2], kr[1]);
+    if (ph < 0.0) ph += 2.0 * MFX_PI;
+    x[n - 1 + 2 * i] = ph / (2.0 * MFX_PI);
+    for (int j = 0; j < 4; ++j) Q[j] = Qn[j];
+    Mc = Mn;
+  }
+  return exp(logw) * rc.inorm;
+}
+
 }  // namespace mfx

@@ -1,0 +1,185 @@
+"""Multi-channel phase space with one VEGAS grid per channel (no reference counterpart:
+``doc/source/usage.rst:84-99`` names RAMBO + VEGAS as madflow's only phase space).
+
+The channels of g g > t t~ + gluons (and u u~ > t t~ + gluons) at fixed sqrt(s):
+
+* flat: RAMBO on diet over every outgoing leg (the plain ``mapping="diet"`` sampling);
+* one t-channel per outgoing gluon: its pT ~ 1/pT^2 from the pT cut to the kinematic
+  limit, rapidity and azimuth flat, the other legs by diet (initial-state emission);
+* one s-channel per gluon pair: m^2 ~ 1/m^2 from the smallest mass the pT and Delta R
+  cuts allow, isotropic decay (final-state g -> g g splitting).
+
+Each iteration draws n_c = alpha_c N points from channel c (stratified), each with weight
+f / g, g = sum_j alpha_j g_j (Kleiss and Pittau, Comput. Phys. Commun. 83 (1994) 141); the
+kernels evaluate every channel's density at every point through the inverse maps.  Grid c
+is refined from the histogram of its own points (Ohl, Comput. Phys. Commun. 120 (1999)
+13), and the weights follow alpha_c <- alpha_c sqrt(W_c), W_c = <(f/g)^2 g_c / g>, with a
+floor alpha_min so that no channel dies.  The kernels are ``mfx_mc_generate`` and
+``mfx_mc_moments``; smatrix and the accumulation are the plain integrand's.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .vegas import MEIntegrand, _merge, _ptr, ALPHA, PROCESS_SETUP
+
+
+class McChannel(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("a", ctypes.c_int), ("b", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("lo", ctypes.c_double), ("hi", ctypes.c_double)]
+
+
+def _mc_sig(lib):
+    vp, i64, u64, u32, i, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
+    from .vegas import PsParams
+    lib.mfx_mc_generate.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp, vp, vp,
+                                    vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_mc_generate.restype = i
+    lib.mfx_mc_moments.argtypes = [i64, i, vp, vp, vp, vp, vp, i64, vp, vp]
+    lib.mfx_mc_moments.restype = i
+
+
+def default_channels(masses, sqrts, pt_min=0.0, dr_min=0.0):
+    """[(type, a, b, lo, hi)] for outgoing masses: flat, a t-channel per massless leg, an
+    s-channel per pair of massless legs.  The lower ends follow the cuts (pT > pt_min;
+    m_ab^2 >= (pT dR)^2 roughly, with 10% margin), else 1e-3 sqrt(s)."""
+    masses = [float(m) for m in masses]
+    s = float(sqrts) ** 2
+    gl = [i for i, m in enumerate(masses) if m == 0.0]
+    out = [(0, 0, 0, 0.0, 0.0)]
+    emax = (s - sum(masses) ** 2) / (2.0 * sqrts)
+    pt_lo = pt_min if pt_min > 0 else 1e-3 * sqrts
+    for a in gl:
+        out.append((1, a, 0, pt_lo, emax))
+    m_lo = 0.9 * pt_min * dr_min if pt_min > 0 and dr_min > 0 else 1e-3 * sqrts
+    for k, a in enumerate(gl):
+        for b in gl[k + 1:]:
+            others = sum(masses) - masses[a] - masses[b]
+            out.append((2, a, b, m_lo * m_lo, (sqrts - others) ** 2))
+    return out
+
+
+class MultiChannelIntegrand(MEIntegrand):
+    """MEIntegrand over the multi-channel phase space (fixed sqrt(s)); the keywords are
+    MEIntegrand's (the mapping is the diet cube of 3 n_out - 4 dimensions).  ``channels``
+    overrides default_channels; ``adapt_alpha`` switches the channel-weight adaptation;
+    ``alpha_min`` is the weight floor."""
+
+    def __init__(self, process, sqrts, channels=None, adapt_alpha=True, alpha_min=0.02, **kw):
+        if kw.get("mapping", "diet") != "diet":
+            raise ValueError("the multi-channel phase space runs on the diet cube (fixed sqrt(s))")
+        kw["mapping"] = "diet"
+        super().__init__(process, sqrts, **kw)
+        _mc_sig(self.lib)
+        n_out, masses = PROCESS_SETUP[process]
+        pt_min, dr_min = kw.get("pt_min", 0.0), kw.get("dr_min", 0.0)
+        self.channels = list(channels) if channels is not None else default_channels(masses, sqrts, pt_min, dr_min)
+        self.nch = len(self.channels)
+        if not 1 <= self.nch <= 8:
+            raise ValueError("between 1 and 8 channels")
+        arr = (McChannel * self.nch)()
+        for c, (t, a, b, lo, hi) in enumerate(self.channels):
+            arr[c].type, arr[c].a, arr[c].b, arr[c].lo, arr[c].hi = t, a, b, lo, hi
+        self._chan_arr = arr
+        self.alpha = np.full(self.nch, 1.0 / self.nch)
+        self.adapt_alpha = adapt_alpha
+        self.alpha_min = float(alpha_min)
+        self.grids = self.grid.unsqueeze(0).repeat(self.nch, 1, 1).contiguous()
+        self._ratios = None
+        self.history_alpha = []
+
+    def _counts(self, n_events):
+        """Points per channel: alpha_c N, rounded, at least 1000, summing to N."""
+        n = np.maximum(np.floor(self.alpha * n_events).astype(np.int64), min(1000, n_events // self.nch))
+        n[np.argmax(self.alpha)] += n_events - int(n.sum())
+        return n
+
+    def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False):
+        """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f)."""
+        b = self._alloc(n)
+        if self._ratios is None or self._ratios.shape[0] < n:
+            self._ratios = torch.empty((n, self.nch), dtype=torch.float64, device=self.device)
+        s = _lib.stream_ptr(self.device)
+        npass = ctypes.c_int64(0)
+        alpha = np.ascontiguousarray(self.alpha, dtype=np.float64)
+        _lib.check(self.lib.mfx_mc_generate(
+            ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
+            alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
+            _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(self._ratios),
+            _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+        npass = npass.value
+        self.last_chunk_npass = npass
+        if self.me_timer is not None and npass:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
+            e1.record()
+            self.me_timer.append((e0, e1, npass))
+        else:
+            me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+        _lib.check(self.lib.mfx_vegas_accumulate(
+            ctypes.byref(self.p), ev0, n, iteration, None, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
+        mom = torch.empty(self.nch, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.mfx_mc_moments(n, self.nch, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]),
+                                           _ptr(self._ratios), _ptr(b["ws"]), b["ws"].numel(), _ptr(mom), s))
+        return (b["result"].cpu().numpy().copy(), npass, mom.cpu().numpy(),
+                (b["f"][:n].clone() if keep_f else None))
+
+    @staticmethod
+    def channel_offset(c):
+        """Philox counter offset of channel c's points (disjoint streams per channel)."""
+        return c << 40
+
+    def iteration(self, n_events, iteration, train=True, keep_f=False):
+        """One iteration of n_events points over the channels; the same dict as
+        MEIntegrand.iteration plus the channel weights used (``alpha``)."""
+        counts = self._counts(int(n_events))
+        N = float(n_events)
+        alpha_used = self.alpha.copy()
+        I_parts, var_parts, wmax, nonzero, npass_tot = [], [], 0.0, 0.0, 0
+        moments = np.zeros(self.nch)
+        fs = []
+        for c in range(self.nch):
+            nc = int(counts[c])
+            scale = alpha_used[c] * N / nc
+            tot = None
+            ev = 0
+            while ev < nc:
+                n = min(self.events_limit, nc - ev)
+                r, k, mom, f = self.chunk_channel(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
+                tot = r if tot is None else _merge(tot, r)
+                moments += mom / N
+                npass_tot += k
+                if keep_f:
+                    fs.append(f)
+                ev += n
+            # stratum c: the mean of its f estimates alpha_c * N / n_c * int g_c f / g
+            mean_c = tot[0] / nc
+            var_c = max((tot[1] / nc - mean_c * mean_c) / (nc - 1.0), 0.0)
+            I_parts.append(tot[0] / N)
+            var_parts.append(var_c * (nc / N) ** 2)
+            wmax = max(wmax, tot[2])
+            nonzero += tot[3]
+            if train:
+                hist = torch.from_numpy(tot[6:].copy()).to(self.device)
+                _lib.check(self.lib.mfx_vegas_refine(_ptr(self.grids[c]), _ptr(hist), self.ndim, self.nbins, ALPHA,
+                                                     _lib.stream_ptr(self.device)))
+        I = math.fsum(I_parts)
+        var = math.fsum(var_parts)
+        if train and self.adapt_alpha and np.all(np.isfinite(moments)) and moments.sum() > 0:
+            a = alpha_used * np.sqrt(np.maximum(moments, 0.0))
+            a = a / a.sum()
+            a = np.maximum(a, self.alpha_min)
+            self.alpha = a / a.sum()
+        self.history_alpha.append(alpha_used)
+        out = dict(I=I, var=var, sigma=math.sqrt(var), wmax=wmax, nonzero=nonzero, npass=npass_tot, n=int(n_events),
+                   sum_f=I * N, alpha=alpha_used.tolist(), counts=counts.tolist())
+        if keep_f:
+            out["f"] = torch.cat(fs)
+        return out

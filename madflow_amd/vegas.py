@@ -65,6 +65,19 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def initial_flavours(matrix):
+    """(beam-1 flavours, beam-2 flavours) of a drop-in class's initial states, the mirror
+    pairs appended when mirror_initial_states is set (madflow_exec.py:141-155); None for
+    processes with two incoming gluons only."""
+    states = [tuple(s) for s in matrix.initial_states]
+    if all(s == (21, 21) for s in states):
+        return None
+    f1, f2 = [a for a, _ in states], [b for _, b in states]
+    if matrix.mirror_initial_states:
+        f1, f2 = f1 + [b for _, b in states], f2 + [a for a, _ in states]
+    return f1, f2
+
+
 class MEIntegrand:
     """sigma = int dPhi |M|^2 / (2 s) * GeV^-2->pb for one process at fixed sqrt(s)
     (mapping "rambo" or "diet"), or the hadronic
@@ -73,7 +86,10 @@ class MEIntegrand:
     integrand of madflow_exec.py:422-470).
 
     Hadronic mode: ``pdf(x, q2)`` returns x f_g(x, Q^2) for device tensors (the gluon of
-    both beams; pdfflow's xfxQ2 in the reference); ``scale`` None is the reference's dynamic
+    both beams; pdfflow's xfxQ2 in the reference) for processes with two incoming gluons;
+    for quark-initiated processes ``pdf(x, q2, pid)`` returns x f(x, Q^2) of the PDG flavour
+    pid, and the luminosity sums the process's flavour pairs, mirrors included (the
+    reference's luminosity, madflow_exec.py:141-155,440-454); ``scale`` None is the reference's dynamic
     scale Q = sum of the outgoing transverse masses / 2 (madflow_exec.py:427-430), a number
     a fixed Q; ``alphas(q2)`` (optional) gives alpha_s(Q^2) per event, which then sets the
     couplings of every event, else alpha_s stays frozen.  ``lab_frame`` boosts the
@@ -97,6 +113,7 @@ class MEIntegrand:
         if self.hadronic and pdf is None:
             raise ValueError("the hadronic mapping needs a pdf(x, q2) -> x f(x, Q^2)")
         self.pdf, self.scale, self.alphas = pdf, scale, alphas
+        self.flavours = initial_flavours(self.matrix)
         if alphas is None:
             model.freeze_alpha_s(alpha_s)
             self.params = list(model.evaluate())
@@ -275,7 +292,20 @@ class MEIntegrand:
             if self.width is not None:
                 params[1] = float(self.width)
         x1, x2 = x[:, 0], x[:, 1]
-        lumi = self.pdf(x1, q2) * self.pdf(x2, q2) / (x1 * x2)
+        if self.flavours is None:
+            lumi = self.pdf(x1, q2) * self.pdf(x2, q2) / (x1 * x2)
+        else:
+            # sum over the initial-state flavour pairs of the process, mirror pairs included
+            # (madflow_exec.py:141-155,440-454; pdfflow's xfxQ2(pid, x, q2) per beam)
+            f1, f2 = self.flavours
+            cache = {}
+
+            def xf(pid, xv, side):
+                if (pid, side) not in cache:
+                    cache[(pid, side)] = self.pdf(xv, q2, pid)
+                return cache[(pid, side)]
+
+            lumi = sum(xf(a, x1, 1) * xf(b, x2, 2) for a, b in zip(f1, f2)) / (x1 * x2)
         return params, lumi
 
     def proposal_iteration(self, flow, n_events, iteration, mix=0.2, seed=1234):

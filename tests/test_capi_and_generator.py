@@ -357,3 +357,13 @@ def test_two_line_colour_matrix_brute_force():
     brute = np.array([[np.sum(x * np.conj(y)).real for y in Bs] for x in Bs])
     exact = np.array(colour_matrix_two_lines(basis), dtype=float)
     np.testing.assert_allclose(brute, exact, atol=1e-12)
+
+
+def test_initial_flavours_with_mirror():
+    """The hadronic luminosity's flavour pairs follow madflow_exec.py:141-155: the initial
+    states, then (mirror_initial_states) the same pairs with the beams swapped."""
+    from madflow_amd.vegas import initial_flavours
+    from madflow_amd.processes.matrix_1_uux_ttx import Matrix_1_uux_ttx
+    from madflow_amd.processes.matrix_1_gg_ttx import Matrix_1_gg_ttx
+    assert initial_flavours(Matrix_1_uux_ttx) == ([2, 4, 1, 3, -2, -4, -1, -3], [-2, -4, -1, -3, 2, 4, 1, 3])
+    assert initial_flavours(Matrix_1_gg_ttx) is None

@@ -112,3 +112,50 @@ def test_uux_ttx_cross_section(dev):
     print("sigma", res, "+-", err, "closed", closed, flush=True)
     assert abs(res - closed) < 3 * err
     assert err / res < 2e-3
+
+
+def _toy_quark_pdf(x, q2, pid):
+    """x f(x) of a toy valence-like quark (pid > 0) and a softer antiquark (pid < 0),
+    flavour-independent; Q^2 is ignored."""
+    import torch as _t
+    if pid > 0:
+        return 2.0 * _t.sqrt(x) * (1 - x) ** 3
+    return 0.5 * (1 - x) ** 7
+
+
+@pytest.mark.gpu
+def test_hadronic_uux_ttx_cross_section(dev):
+    """Hadronic mode with a quark-initiated process: the luminosity sums the four flavour
+    pairs (u u~, c c~, d d~, s s~) and their mirrors (madflow_exec.py:141-155,440-454),
+    so sigma = 8 int dx1 dx2 q(x1) qbar(x2) sigma_hat(x1 x2 s) for the toy densities
+    (q(x) qbar(y) + qbar(x) q(y) summed over 4 pairs, the two orders equal after x1 <-> x2
+    in the symmetric integral); VEGAS must agree with the direct 2D integral within 3 sigma."""
+    import math
+    from scipy import integrate
+    from madflow_amd.vegas import MEIntegrand, VegasFlow, GEV2PB
+    print("case hadronic sigma uux_ttx", flush=True)
+    ig = MEIntegrand("uux_ttx", precision=64, mapping="hadronic", seed=3, sqrts=13000.0, ref_compat=False,
+                     width=0.0, pdf=_toy_quark_pdf, scale=173.0)
+    assert ig.flavours == ([2, 4, 1, 3, -2, -4, -1, -3], [-2, -4, -1, -3, 2, 4, 1, 3])
+    vf = VegasFlow(ig.ndim, 1000000)
+    vf.compile(ig)
+    vf.run_integration(3)
+    vf.freeze_grid()
+    res, err = vf.run_integration(5)
+    m, a, s = 173.0, 0.118, 13000.0 ** 2
+
+    def sig_hat(sh):
+        rho = 4 * m * m / sh
+        return 8 * math.pi * a * a / (27 * sh) * math.sqrt(max(1 - rho, 0.0)) * (1 + rho / 2) * GEV2PB
+
+    t = 4 * m * m / s
+    q = lambda x: 2.0 * math.sqrt(x) * (1 - x) ** 3 / x
+    qb = lambda x: 0.5 * (1 - x) ** 7 / x
+    direct, _ = integrate.dblquad(
+        lambda u2, u1: (q(math.exp(u1)) * qb(math.exp(u2)) + qb(math.exp(u1)) * q(math.exp(u2)))
+        * math.exp(u1 + u2) * sig_hat(math.exp(u1 + u2) * s),
+        math.log(t), 0.0, lambda u1: math.log(t) - u1, lambda u1: 0.0, epsabs=0, epsrel=1e-7)
+    direct *= 4
+    print("hadronic sigma", res, "+-", err, "direct", direct, flush=True)
+    assert abs(res - direct) < 3 * err + 1e-6 * direct
+    assert err / res < 1e-2

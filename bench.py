@@ -43,8 +43,14 @@ PEAK_TFLOPS = {32: 157.3, 64: 78.6}   # MI355X published vector peaks (FP32 pack
 # HBM bytes per matrix element, (process, precision): rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a 1 GiB copy (profiles/pmc_k_smatrix_gg_ttxgg_f32.md for the fused
 # Feynman-diagram kernel, profiles/pmc_k_ordered_gg_ttxgg_f32.md for the colour-ordered
-# kernels, jamps staged through HBM included)
-HBM_BYTES_PER_ME = {("gg_ttxgg_fused", 32): 28105.2, ("gg_ttxgg", 32): 24776.7}
+# kernels, jamps staged through HBM included; 1 GiB copy calibration)
+HBM_BYTES_PER_ME = {("gg_ttxgg_fused", 32): 28180.3, ("gg_ttxgg", 32): 24776.7}
+# VALU wave instructions issued per ME (SQ_INSTS_VALU summed over the call's kernels / MEs,
+# same profiles): the executed-work side of the roofline.  A wave64 VALU instruction holds
+# a 16-lane SIMD for 4 cycles; 4 SIMDs per CU at the 2.4 GHz peak engine clock.
+VALU_WAVE_INSTS_PER_ME = {("gg_ttxgg", 32): 18802.7}
+VALU_CYCLES_PER_INST = 4
+ENGINE_HZ = 2.4e9
 METRIC = "unweighted events/sec at sigma/sigma=1e-3 and ME evals/sec, gg->ttgg on 1 MI355X"
 
 
@@ -597,6 +603,13 @@ def main(argv=None):
                      "flops_per_me": flops, "mes_per_launch": m["me_n"] / max(1, m["launches"]),
                      "kernel_ms_per_launch": m["me_ms"] / max(1, m["launches"])},
     }
+    if (reg_name, a.precision) in VALU_WAVE_INSTS_PER_ME and kernel_s > 0:
+        # executed VALU issue against the issue rate of every SIMD: the oracle-flop fraction
+        # above counts the reference algorithm's work, this one the instructions run
+        nsimd = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        insts = VALU_WAVE_INSTS_PER_ME[(reg_name, a.precision)] * m["me_n"]
+        res["roofline"]["valu_issue_frac"] = insts * VALU_CYCLES_PER_INST / (nsimd * ENGINE_HZ) / kernel_s
+        res["roofline"]["valu_wave_insts_per_me"] = VALU_WAVE_INSTS_PER_ME[(reg_name, a.precision)]
     if not a.no_cpu_baseline:
         b = m["ig"]._bufs
         res["cpu_baseline"] = cpu_baseline(a.process, b["dense"], m["ig"].last_chunk_npass,

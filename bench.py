@@ -387,7 +387,8 @@ def run_flow_compare(a, torch, main):
 
     def gain(k):
         return learned[k] / plain[k] if plain[k] > 0 else None
-    return {
+    mc = run_multichannel_leg(a, torch) if (a.multichannel_leg and a.mapping == "diet") else None
+    out = {
         "workload": "%s, %s, %d-dim %s VEGAS cube, %d points/iteration, target sigma/sigma=%g" % (
             LABEL[a.process], _kin_text(a.process), ig.ndim, a.mapping, a.nev, a.target),
         "plain_vegas_fp64": plain,
@@ -408,6 +409,49 @@ def run_flow_compare(a, torch, main):
                                              if plain["equal_wall"]["unweighted_per_s"] > 0 else None),
         "sigma_pull": (learned["sigma_pb"] - plain["sigma_pb"]) / math.hypot(learned["err_pb"], plain["err_pb"]),
     }
+    if mc is not None:
+        # the multi-channel phase space with plain VEGAS per channel (no learned part)
+        out["multichannel_fp32"] = mc
+        out["gain_multichannel_per_s_to_target"] = (mc["unweighted_per_s_to_target"] / plain["unweighted_per_s_to_target"]
+                                                    if plain["unweighted_per_s_to_target"] > 0 else None)
+        out["gain_multichannel_per_s_equal_time"] = (mc["equal_time"]["unweighted_per_s"] /
+                                                     plain["equal_time"]["unweighted_per_s"]
+                                                     if plain["equal_time"]["unweighted_per_s"] > 0 else None)
+        out["sigma_pull_multichannel"] = ((mc["sigma_pb"] - plain["sigma_pb"]) /
+                                          math.hypot(mc["err_pb"], plain["err_pb"]))
+    return out
+
+
+def run_multichannel_leg(a, torch):
+    """FP32 matrix elements on the multi-channel phase space (flat diet, a t-channel per
+    gluon, an s-channel per gluon pair, one VEGAS grid each, adapted channel weights): the
+    same warm-up, run to the target and equal-time extension as the plain leg."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.vegas import VegasFlow
+    sync = torch.cuda.synchronize
+    ig = MultiChannelIntegrand(a.process, precision=32, seed=7, events_limit=a.events_limit, kernels=a.kernels,
+                               **ME_ONLY_KIN[a.process])
+    vf = VegasFlow(ig.ndim, a.nev)
+    vf.compile(ig)
+    sync()
+    t0 = time.perf_counter()
+    vf.run_integration(a.warmup, log=log)
+    vf.freeze_grid()
+    sync()
+    setup = time.perf_counter() - t0
+
+    def it(k):
+        vf.run_integration(1)
+        return vf.history[-1]
+    t0 = time.perf_counter()
+    res = _run_to_target(it, a.target, a.max_iter, "multichannel fp32")
+    sync()
+    out = _unweighted(res, setup, time.perf_counter() - t0, a.target)
+    timed = _extend_to(it, list(res), out["sample_s"], a.equal_time_s, torch)
+    out["equal_time"] = _rate_at(timed, a.equal_time_s, a.equal_time_s)
+    out["channels"] = [list(c) for c in ig.channels]
+    out["alpha"] = [float(x) for x in ig.alpha]
+    return out
 
 
 ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5),
@@ -502,6 +546,8 @@ def main(argv=None):
     ap.add_argument("--max-iter", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--no-multichannel-leg", dest="multichannel_leg", action="store_false",
+                    help="skip the multi-channel FP32 leg of the flow comparison")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
     ap.add_argument("--flow-grid-warmup", type=int, default=3,
                     help="grid adaptation iterations of the flow leg's FP32 grid, counted in its set-up (the plain "

@@ -248,6 +248,8 @@ def train_flow(a, torch, ig):
     from madflow_amd.flow import RQSFlow
     from madflow_amd.phasespace_gpu import philox_uniforms
     D = ig.ndim
+    # on the multi-channel phase space the flow learns channel 0's (the flat channel's) cube
+    chunk = ig.cube0_chunk if hasattr(ig, "cube0_chunk") else ig.chunk
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ntot = a.flow_pairs + a.flow_pairs // 10
@@ -257,7 +259,7 @@ def train_flow(a, torch, ig):
     while done < ntot:
         n = min(a.events_limit, ntot - done)
         yy = philox_uniforms(n, D, 99, 0, ev0=done).float().contiguous()
-        _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=yy)
+        _, _, fb = chunk(done, n, 0, keep_f=True, y_in=yy)
         y[done:done + n] = yy
         f[done:done + n] = fb
         done += n
@@ -283,7 +285,7 @@ def train_flow(a, torch, ig):
             u = philox_uniforms(n, D, 98 - rnd, 0, ev0=done).float()
             ys, winv = flow.sample_mixture(u, u, a.flow_mix)
             ys = ys.contiguous()
-            _, _, fb = ig.chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
+            _, _, fb = chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
             yq[done:done + n] = ys
             wq[done:done + n] = fb
             if fq is not None:
@@ -388,6 +390,7 @@ def run_flow_compare(a, torch, main):
     def gain(k):
         return learned[k] / plain[k] if plain[k] > 0 else None
     mc = run_multichannel_leg(a, torch) if (a.multichannel_leg and a.mapping == "diet") else None
+    mcf = run_flow_mc_leg(a, torch) if (a.flow_mc_leg and a.mapping == "diet") else None
     out = {
         "workload": "%s, %s, %d-dim %s VEGAS cube, %d points/iteration, target sigma/sigma=%g" % (
             LABEL[a.process], _kin_text(a.process), ig.ndim, a.mapping, a.nev, a.target),
@@ -419,6 +422,55 @@ def run_flow_compare(a, torch, main):
                                                      if plain["equal_time"]["unweighted_per_s"] > 0 else None)
         out["sigma_pull_multichannel"] = ((mc["sigma_pb"] - plain["sigma_pb"]) /
                                           math.hypot(mc["err_pb"], plain["err_pb"]))
+    if mcf is not None:
+        # the learned proposal as a channel of the multi-channel phase space
+        out["multichannel_flow_fp32"] = mcf
+        for k, key in (("to_target", "unweighted_per_s_to_target"),):
+            out["gain_multichannel_flow_per_s_" + k] = (mcf[key] / plain[key] if plain[key] > 0 else None)
+        out["gain_multichannel_flow_per_s_equal_time"] = (mcf["equal_time"]["unweighted_per_s"] /
+                                                          plain["equal_time"]["unweighted_per_s"]
+                                                          if plain["equal_time"]["unweighted_per_s"] > 0 else None)
+        out["gain_multichannel_flow_per_s_equal_wall"] = (mcf["equal_wall"]["unweighted_per_s"] /
+                                                          plain["equal_wall"]["unweighted_per_s"]
+                                                          if plain["equal_wall"]["unweighted_per_s"] > 0 else None)
+        out["sigma_pull_multichannel_flow"] = ((mcf["sigma_pb"] - plain["sigma_pb"]) /
+                                               math.hypot(mcf["err_pb"], plain["err_pb"]))
+    return out
+
+
+def run_flow_mc_leg(a, torch):
+    """FP32 matrix elements on the multi-channel phase space with the learned proposal as
+    one more channel: the channels' warm-up (grids and weights), then the flow trained on
+    channel 0's cube as in the flow leg, attached with weight flow_mc_alpha; all of it
+    counted in the set-up."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.vegas import VegasFlow
+    sync = torch.cuda.synchronize
+    ig = MultiChannelIntegrand(a.process, precision=32, seed=9, events_limit=a.events_limit, kernels=a.kernels,
+                               **ME_ONLY_KIN[a.process])
+    vf = VegasFlow(ig.ndim, a.nev)
+    vf.compile(ig)
+    sync()
+    t0 = time.perf_counter()
+    vf.run_integration(a.warmup, log=log)
+    vf.freeze_grid()
+    sync()
+    t_warm = time.perf_counter() - t0
+    flow, t_pairs, t_train = train_flow(a, torch, ig)
+    ig.attach_flow(flow, a.flow_mc_alpha)
+    setup = t_warm + t_pairs + t_train
+
+    def it(k):
+        return ig.iteration(a.nev, 200 + k, train=False)
+    t0 = time.perf_counter()
+    res = _run_to_target(it, a.target, a.max_iter, "multichannel + flow fp32")
+    sync()
+    out = _unweighted(res, setup, time.perf_counter() - t0, a.target)
+    timed = _extend_to(it, list(res), out["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup), torch)
+    out["equal_time"] = _rate_at(timed, a.equal_time_s, a.equal_time_s)
+    out["equal_wall"] = _rate_at(timed, a.equal_wall_s - setup, a.equal_wall_s)
+    out.update(warmup_s=t_warm, train_pairs_s=t_pairs, train_s=t_train, alpha_learned=ig.alpha_learned,
+               alpha=[float(x) for x in ig.alpha], stage_steps_run=list(getattr(flow, "stage_steps", [])))
     return out
 
 
@@ -548,6 +600,10 @@ def main(argv=None):
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-multichannel-leg", dest="multichannel_leg", action="store_false",
                     help="skip the multi-channel FP32 leg of the flow comparison")
+    ap.add_argument("--flow-mc-leg", action="store_true",
+                    help="add a leg with the learned proposal as a channel of the multi-channel phase space")
+    ap.add_argument("--flow-mc-alpha", type=float, default=0.5,
+                    help="weight of the learned channel in the multi-channel mixture")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
     ap.add_argument("--flow-grid-warmup", type=int, default=3,
                     help="grid adaptation iterations of the flow leg's FP32 grid, counted in its set-up (the plain "

@@ -203,6 +203,25 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
                     double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
                     void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
 
+/* A learned channel beside the analytic ones (channel 0 must be flat): its density on the
+ * phase space is q(y0(p)) g_0(p), with q the learned density on channel 0's cube and y0(p)
+ * the point of that cube that channel 0's grid and map send to p.  With y_in (nev x ndim
+ * f32, channel 0) the points are the learned channel's samples mapped through channel 0;
+ * without, they are channel `channel`'s.  Writes y0_out (nev x ndim f32, the cube point
+ * of every point, for the caller to evaluate q), the raw densities g_j in ratios (nev x
+ * (nch + 1)), gdens (nev x 2: sum_j alpha_j g_j, g_0) and wps = scale x gev2pb / (2 s);
+ * mfx_mc_combine then divides by the full density. */
+int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                            double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
+                            uint32_t iteration, const float* y_in, double* all_ps, double* wps, uint8_t* flag,
+                            int* slot, double* dense, double* ratios, float* y0_out, double* gdens, void* workspace,
+                            int64_t workspace_bytes, int64_t* npass_host, void* stream);
+
+/* g = gdens[e][0] + alpha_learned exp(lq[e]) gdens[e][1]: wps[e] /= g, ratios[e][j] /= g for
+ * the nch analytic channels and ratios[e][nch] = exp(lq[e]) gdens[e][1] / g. */
+int mfx_mc_combine(int64_t nev, int nch, double alpha_learned, const float* lq, const double* gdens, double* wps,
+                   double* ratios, void* stream);
+
 /* result (device, nch doubles) = sum over the nev points of f^2 ratios[e][j],
  * f = me[slot[e]] * wps[e] (0 when slot[e] < 0): the moments that adapt the channel
  * weights (Kleiss and Pittau).  Fixed-order compensated sums; workspace of at least

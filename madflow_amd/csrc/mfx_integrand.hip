@@ -342,20 +342,54 @@ __device__ double mc_inverse(const McChannel& ch, double sqrts, const double (*p
   return w * m2 * L / (16.0 * MFX_PI * MFX_PI);
 }
 
+// VEGAS jacobian of a grid at the mapped point x, and (y != nullptr) the cube point y that
+// the grid maps onto x
+__device__ __forceinline__ double grid_inverse(const double* __restrict__ grid, int ndim, int nb, const double* x,
+                                               double* y) {
+  double jac = 1.0;
+  for (int d = 0; d < ndim; ++d) {
+    const double* e = grid + d * (nb + 1);
+    int lo = 0, hi = nb;            // e[lo] <= x < e[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (e[mid] <= x[d]) lo = mid; else hi = mid;
+    }
+    const double w = e[lo + 1] - e[lo];
+    jac *= w * nb;
+    if (y) {
+      const double t = w > 0.0 ? (x[d] - e[lo]) / w : 0.0;
+      const double v = (lo + fmin(fmax(t, 0.0), 1.0)) / nb;
+      y[d] = v;
+    }
+  }
+  return jac;
+}
+
 // Points of channel c: VEGAS map of grid c, channel c's map, the densities of every
 // channel at the point (ratios[e][j] = g_j / g) and the weight scale / g; then cuts, as
 // k_generate (fixed sqrt(s), c.m. frame).
+// Learned mode (y0 != nullptr; channel 0 is the flat channel): the points of channel c
+// come from y_in when it is given (the learned channel's samples on channel 0's cube),
+// and the kernel leaves the division by g to mfx_mc_combine, once the learned density at
+// each point's channel-0 cube point y0 is known: it writes y0, the raw densities g_j in
+// ratios (row stride nch + 1), gdens[e] = (sum_j alpha_j g_j, g_0) and wps = scale x norm.
 __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst mc, int c,
                                                            const double* __restrict__ grids, uint64_t ev0, int64_t nev,
-                                                           uint32_t iteration, double* __restrict__ all_ps,
-                                                           double* __restrict__ wps, uint8_t* __restrict__ flag,
-                                                           double* __restrict__ ratios, int* __restrict__ block_count) {
+                                                           uint32_t iteration, const float* __restrict__ y_in,
+                                                           double* __restrict__ all_ps, double* __restrict__ wps,
+                                                           uint8_t* __restrict__ flag, double* __restrict__ ratios,
+                                                           float* __restrict__ y0_out, double* __restrict__ gdens,
+                                                           int* __restrict__ block_count) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
   bool ok = false;
   if (e < nev) {
     const int D = g.ndim, NB = g.nbins;
-    double y[MAXDIM], x[MAXDIM];
-    uniforms(g, ev0 + e, iteration, y);
+    const bool learned = y0_out != nullptr;
+    const int rs = learned ? mc.nch + 1 : mc.nch;
+    double y[MAXDIM], x[MAXDIM], y0[MAXDIM];
+    bool y_ok = true;
+    if (y_in) y_ok = external_point(y_in + e * D, D, y);
+    else uniforms(g, ev0 + e, iteration, y);
     const double jac = vmap(g, grids + (size_t)c * D * (NB + 1), y, x, nullptr);
     double p[RAMBO_MAXN + 2][4];
     bool conv = true;
@@ -365,11 +399,16 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
       double v = 0.0;
       if (j == c) {
         v = 1.0 / (jac * w);
+        if (learned && j == 0)
+          for (int d = 0; d < D; ++d) y0[d] = y[d];
       } else if (conv) {
         double xj[MAXDIM];
         bool in = false;
         const double wj = mc_inverse(mc.ch[j], g.sqrts, p + 2, xj, in);
-        if (in && wj > 0.0) v = 1.0 / (grid_jacobian(grids + (size_t)j * D * (NB + 1), D, NB, xj) * wj);
+        const double J = grid_inverse(grids + (size_t)j * D * (NB + 1), D, NB, xj, (learned && j == 0) ? y0 : nullptr);
+        if (in && wj > 0.0) v = 1.0 / (J * wj);
+      } else if (learned && j == 0) {
+        for (int d = 0; d < D; ++d) y0[d] = 0.5;
       }
       if (!isfinite(v)) v = 0.0;
       gj[j] = v;
@@ -378,14 +417,22 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
     const double eh = 0.5 * g.sqrts;
     p[0][0] = eh; p[0][1] = 0.0; p[0][2] = 0.0; p[0][3] = eh;
     p[1][0] = eh; p[1][1] = 0.0; p[1][2] = 0.0; p[1][3] = -eh;
-    const double wt = mc.scale / gsum;
-    ok = conv && w > 0.0 && gsum > 0.0 && isfinite(wt) && pass_cuts(g, p + 2);
+    const double wt = learned ? mc.scale : mc.scale / gsum;
+    ok = y_ok && conv && w > 0.0 && (learned ? gj[0] > 0.0 : gsum > 0.0) && isfinite(wt) && pass_cuts(g, p + 2);
     double* out = all_ps + e * (g.nexternal * 4);
     for (int i = 0; i < g.nexternal; ++i)
       for (int k = 0; k < 4; ++k) out[4 * i + k] = p[i][k];
     wps[e] = ok ? wt * g.norm : 0.0;
     flag[e] = ok ? 1 : (conv ? 0 : 2);
-    for (int j = 0; j < mc.nch; ++j) ratios[e * mc.nch + j] = gsum > 0.0 ? gj[j] / gsum : 0.0;
+    if (learned) {
+      for (int j = 0; j < mc.nch; ++j) ratios[e * rs + j] = gj[j];
+      ratios[e * rs + mc.nch] = 0.0;
+      for (int d = 0; d < D; ++d) y0_out[e * D + d] = (float)fmin(fmax(y0[d], 0.0), 1.0);
+      gdens[2 * e] = gsum;
+      gdens[2 * e + 1] = gj[0];
+    } else {
+      for (int j = 0; j < mc.nch; ++j) ratios[e * rs + j] = gsum > 0.0 ? gj[j] / gsum : 0.0;
+    }
   }
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
@@ -394,6 +441,24 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
   if ((threadIdx.x & 63) == 0) atomicAdd(&cnt, __popcll(bal));
   __syncthreads();
   if (threadIdx.x == 0) block_count[blockIdx.x] = cnt;
+}
+
+// The learned channel's share of the density: g = gdens0 + alpha_learned q(y0) gdens1 with
+// q = exp(lq) the learned density on channel 0's cube; wps /= g, the raw densities in
+// ratios become g_j / g, and the last column q g_0 / g.
+__global__ void __launch_bounds__(GEN_BLOCK) k_mc_combine(int64_t n, int nch, double alpha_learned,
+                                                          const float* __restrict__ lq, const double* __restrict__ gdens,
+                                                          double* __restrict__ wps, double* __restrict__ ratios) {
+  const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
+  if (e >= n) return;
+  const int rs = nch + 1;
+  const double qg0 = exp((double)lq[e]) * gdens[2 * e + 1];
+  const double gt = gdens[2 * e] + alpha_learned * qg0;
+  const double inv = gt > 0.0 && isfinite(gt) ? 1.0 / gt : 0.0;
+  const double w = wps[e] * inv;
+  wps[e] = isfinite(w) ? w : 0.0;
+  for (int j = 0; j < nch; ++j) ratios[e * rs + j] *= inv;
+  ratios[e * rs + nch] = isfinite(qg0 * inv) ? qg0 * inv : 0.0;
 }
 
 // sum over events of f^2 r_j, f = me[slot] * wps: the channel-weight moments
@@ -911,7 +976,7 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
-                     all_ps, wps, flag, ratios, counts);
+                     (const float*)nullptr, all_ps, wps, flag, ratios, (float*)nullptr, (double*)nullptr, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);
@@ -925,6 +990,65 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
     *npass_host = tot;
   }
   return 0;
+}
+
+int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                            double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
+                            uint32_t iteration, const float* y_in, double* all_ps, double* wps, uint8_t* flag,
+                            int* slot, double* dense, double* ratios, float* y0_out, double* gdens, void* workspace,
+                            int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  GenConst g;
+  int rc = make_gen(prm, g);
+  if (rc) return rc;
+  McConst mc;
+  rc = make_mc(prm, g, channels, nch, alpha, scale, mc);
+  if (rc) return rc;
+  if (nch + 1 > MC_MAXCH) return fail(-2, "at most 7 channels beside the learned one%s", "");
+  if (mc.ch[0].type != 0) return fail(-2, "the learned channel lives on channel 0's cube, which must be flat%s", "");
+  if (channel < 0 || channel >= nch) return fail(-2, "channel out of range%s", "");
+  if (y_in && channel != 0) return fail(-2, "learned points are mapped through channel 0%s", "");
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (nev > 0x7fffffff) return fail(-2, "nev must fit in 31 bits%s", "");
+  if (workspace_bytes < mfx_integrand_workspace_size(nev, prm->ndim, prm->nbins))
+    return fail(-8, "workspace too small (see mfx_integrand_workspace_size)%s", "");
+  if (nev == 0) {
+    if (npass_host) *npass_host = 0;
+    return 0;
+  }
+  if (!grids || !all_ps || !wps || !flag || !slot || !dense || !ratios || !y0_out || !gdens || !workspace)
+    return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
+  int* counts = (int*)workspace;
+  int* offsets = counts + nblk + 1;
+  hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
+                     y_in, all_ps, wps, flag, ratios, y0_out, gdens, counts);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
+  hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
+                     slot);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_status(e);
+  if (npass_host) {
+    int tot = 0;
+    e = hipMemcpyAsync(&tot, offsets + nblk, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e);
+    *npass_host = tot;
+  }
+  return 0;
+}
+
+int mfx_mc_combine(int64_t nev, int nch, double alpha_learned, const float* lq, const double* gdens, double* wps,
+                   double* ratios, void* stream) {
+  if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
+  if (nch < 1 || nch + 1 > MC_MAXCH) return fail(-2, "between 1 and 7 channels beside the learned one%s", "");
+  if (!(alpha_learned >= 0.0) || !isfinite(alpha_learned)) return fail(-2, "alpha_learned must be >= 0%s", "");
+  if (nev == 0) return 0;
+  if (!lq || !gdens || !wps || !ratios) return fail(-3, "null pointer%s", "");
+  const unsigned nblk = (unsigned)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
+  hipLaunchKernelGGL(k_mc_combine, dim3(nblk), dim3(GEN_BLOCK), 0, (hipStream_t)stream, nev, nch, alpha_learned, lq,
+                     gdens, wps, ratios);
+  return hip_status(hipGetLastError());
 }
 
 int mfx_mc_moments(int64_t nev, int nch, const int* slot, const double* me, const double* wps, const double* ratios,

@@ -41,6 +41,11 @@ def _mc_sig(lib):
     lib.mfx_mc_generate.restype = i
     lib.mfx_mc_moments.argtypes = [i64, i, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_mc_moments.restype = i
+    lib.mfx_mc_generate_learned.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp,
+                                            vp, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_mc_generate_learned.restype = i
+    lib.mfx_mc_combine.argtypes = [i64, i, d, vp, vp, vp, vp, vp]
+    lib.mfx_mc_combine.restype = i
 
 
 def default_channels(masses, sqrts, pt_min=0.0, dr_min=0.0):
@@ -91,12 +96,99 @@ class MultiChannelIntegrand(MEIntegrand):
         self.grids = self.grid.unsqueeze(0).repeat(self.nch, 1, 1).contiguous()
         self._ratios = None
         self.history_alpha = []
+        self.flow = None          # learned channel (attach_flow)
+        self.alpha_learned = 0.0
 
-    def _counts(self, n_events):
+    def _alpha_all(self):
+        """Weights of every channel of the mixture: the analytic ones, then the learned one."""
+        if self.flow is None:
+            return self.alpha.copy()
+        return np.concatenate([(1.0 - self.alpha_learned) * self.alpha, [self.alpha_learned]])
+
+    def _counts(self, alpha, n_events):
         """Points per channel: alpha_c N, rounded, at least 1000, summing to N."""
-        n = np.maximum(np.floor(self.alpha * n_events).astype(np.int64), min(1000, n_events // self.nch))
-        n[np.argmax(self.alpha)] += n_events - int(n.sum())
+        k = len(alpha)
+        n = np.maximum(np.floor(alpha * n_events).astype(np.int64), min(1000, n_events // k))
+        n[np.argmax(alpha)] += n_events - int(n.sum())
         return n
+
+    # ---------------------------------------------------------------- learned channel
+    def cube0_chunk(self, ev0, n, iteration, keep_f=False, y_in=None, w_in=None):
+        """MEIntegrand.chunk on channel 0's cube alone (its frozen grid, the flat map, no
+        mixture weight): the integrand values that train a learned density on that cube."""
+        if self.channels[0][0] != 0:
+            raise ValueError("the learned channel needs channel 0 to be the flat one")
+        saved = self.grid
+        self.grid = self.grids[0].contiguous()
+        try:
+            return MEIntegrand.chunk(self, ev0, n, iteration, keep_f=keep_f, y_in=y_in, w_in=w_in)
+        finally:
+            self.grid = saved
+
+    def attach_flow(self, flow, alpha_learned=0.5):
+        """Add a learned density on channel 0's cube as one more channel of the mixture,
+        with weight alpha_learned (the analytic channels share the rest in their current
+        proportions).  Grids are not refined while it is attached."""
+        if self.channels[0][0] != 0:
+            raise ValueError("the learned channel needs channel 0 to be the flat one")
+        if self.nch + 1 > 8:
+            raise ValueError("at most 7 analytic channels beside the learned one")
+        if flow.D != self.ndim:
+            raise ValueError("the flow has %d dimensions, the cube %d" % (flow.D, self.ndim))
+        if not 0.0 < alpha_learned < 1.0:
+            raise ValueError("alpha_learned must lie in (0, 1)")
+        self.flow = flow
+        self.alpha_learned = float(alpha_learned)
+
+    def _chunk_learned(self, c, ev0, n, iteration, scale, keep_f):
+        """One chunk of channel c of the mixture with the learned channel (c == nch)."""
+        from .phasespace_gpu import philox_uniforms
+        b = self._alloc(n)
+        ncol = self.nch + 1
+        if self._ratios is None or self._ratios.numel() < n * ncol:
+            self._ratios = torch.empty((n, ncol), dtype=torch.float64, device=self.device)
+        if getattr(self, "_y0", None) is None or self._y0.shape[0] < n:
+            self._y0 = torch.empty((n, self.ndim), dtype=torch.float32, device=self.device)
+            self._gd = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        s = _lib.stream_ptr(self.device)
+        y_in = None
+        if c == self.nch:
+            u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0).float().contiguous()
+            y_in, _ = self.flow.sample(u)
+            y_in = y_in.contiguous()
+        alpha = np.ascontiguousarray((1.0 - self.alpha_learned) * self.alpha, dtype=np.float64)
+        npass = ctypes.c_int64(0)
+        _lib.check(self.lib.mfx_mc_generate_learned(
+            ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
+            alpha.ctypes.data_as(ctypes.c_void_p), float(scale), 0 if c == self.nch else c, _ptr(self.grids), ev0, n,
+            iteration, _ptr(y_in), _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]),
+            _ptr(b["dense"]), _ptr(self._ratios), _ptr(self._y0), _ptr(self._gd), _ptr(b["ws"]), b["ws"].numel(),
+            ctypes.byref(npass), s))
+        lq = self.flow.log_prob(self._y0[:n].contiguous())
+        _lib.check(self.lib.mfx_mc_combine(n, self.nch, self.alpha_learned, _ptr(lq), _ptr(self._gd), _ptr(b["wps"]),
+                                           _ptr(self._ratios), s))
+        npass = npass.value
+        self.last_chunk_npass = npass
+        me = self._smatrix_timed(b, npass)
+        _lib.check(self.lib.mfx_vegas_accumulate(
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
+        mom = torch.empty(ncol, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.mfx_mc_moments(n, ncol, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), _ptr(self._ratios),
+                                           _ptr(b["ws"]), b["ws"].numel(), _ptr(mom), s))
+        return (b["result"].cpu().numpy().copy(), npass, mom.cpu().numpy(),
+                (b["f"][:n].clone() if keep_f else None))
+
+    def _smatrix_timed(self, b, npass):
+        if self.me_timer is not None and npass:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
+            e1.record()
+            self.me_timer.append((e0, e1, npass))
+            return me
+        return self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
 
     def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False):
         """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f)."""
@@ -113,15 +205,7 @@ class MultiChannelIntegrand(MEIntegrand):
             _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
         npass = npass.value
         self.last_chunk_npass = npass
-        if self.me_timer is not None and npass:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
-            e1.record()
-            self.me_timer.append((e0, e1, npass))
-        else:
-            me = self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+        me = self._smatrix_timed(b, npass)
         _lib.check(self.lib.mfx_vegas_accumulate(
             ctypes.byref(self.p), ev0, n, iteration, None, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
             _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
@@ -139,20 +223,25 @@ class MultiChannelIntegrand(MEIntegrand):
     def iteration(self, n_events, iteration, train=True, keep_f=False):
         """One iteration of n_events points over the channels; the same dict as
         MEIntegrand.iteration plus the channel weights used (``alpha``)."""
-        counts = self._counts(int(n_events))
+        learned = self.flow is not None
+        alpha_used = self._alpha_all()
+        counts = self._counts(alpha_used, int(n_events))
         N = float(n_events)
-        alpha_used = self.alpha.copy()
+        K = len(alpha_used)
         I_parts, var_parts, wmax, nonzero, npass_tot = [], [], 0.0, 0.0, 0
-        moments = np.zeros(self.nch)
+        moments = np.zeros(K)
         fs = []
-        for c in range(self.nch):
+        for c in range(K):
             nc = int(counts[c])
             scale = alpha_used[c] * N / nc
             tot = None
             ev = 0
             while ev < nc:
                 n = min(self.events_limit, nc - ev)
-                r, k, mom, f = self.chunk_channel(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
+                if learned:
+                    r, k, mom, f = self._chunk_learned(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
+                else:
+                    r, k, mom, f = self.chunk_channel(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
                 tot = r if tot is None else _merge(tot, r)
                 moments += mom / N
                 npass_tot += k
@@ -166,7 +255,7 @@ class MultiChannelIntegrand(MEIntegrand):
             var_parts.append(var_c * (nc / N) ** 2)
             wmax = max(wmax, tot[2])
             nonzero += tot[3]
-            if train:
+            if train and not learned:
                 hist = torch.from_numpy(tot[6:].copy()).to(self.device)
                 _lib.check(self.lib.mfx_vegas_refine(_ptr(self.grids[c]), _ptr(hist), self.ndim, self.nbins, ALPHA,
                                                      _lib.stream_ptr(self.device)))
@@ -176,7 +265,12 @@ class MultiChannelIntegrand(MEIntegrand):
             a = alpha_used * np.sqrt(np.maximum(moments, 0.0))
             a = a / a.sum()
             a = np.maximum(a, self.alpha_min)
-            self.alpha = a / a.sum()
+            a = a / a.sum()
+            if learned:
+                self.alpha_learned = float(a[-1])
+                self.alpha = a[:-1] / a[:-1].sum()
+            else:
+                self.alpha = a
         self.history_alpha.append(alpha_used)
         out = dict(I=I, var=var, sigma=math.sqrt(var), wmax=wmax, nonzero=nonzero, npass=npass_tot, n=int(n_events),
                    sum_f=I * N, alpha=alpha_used.tolist(), counts=counts.tolist())

@@ -106,3 +106,67 @@ def test_mc_cross_section_against_diet(dev):
               eff, "alpha", getattr(ig, "alpha", None), flush=True)
     (a, ea), (b, eb) = res["diet"], res["multichannel"]
     assert abs(a - b) < 3 * np.hypot(ea, eb)
+
+
+@pytest.mark.gpu
+def test_mc_learned_channel_matches_restatement(dev):
+    """The learned channel (a flow on channel 0's cube, here untrained) as one more channel:
+    for points of an analytic channel and of the learned one, the kernel's channel-0 cube
+    point y0 and the weight 1 / ((1 - aL) sum_j alpha_j g_j + aL q(y0) g_0) equal the
+    restatement's (q from the GPU flow's density at y0), and the density ratios sum to 1."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.flow import RQSFlow
+    print("case mc learned channel parity", flush=True)
+    ig = MultiChannelIntegrand("gg_ttxgg", SQRTS, precision=64, seed=7, **CUTS)
+    chs = R.Channels(4, MASSES, SQRTS, **CUTS)
+    rng = np.random.default_rng(4)
+    grids = _random_grids(ig.nch, ig.ndim, ig.nbins, rng)
+    ig.grids.copy_(torch.from_numpy(grids))
+    ig.alpha = np.array([0.4, 0.2, 0.25, 0.15])
+    flow = RQSFlow(ig.ndim, layers=8, seed=3)
+    ig.attach_flow(flow, 0.4)
+    n = 3000
+    a_an = 0.6 * ig.alpha
+    for c in (1, ig.nch):                       # an analytic channel, then the learned one
+        ev0 = ig.channel_offset(c) + 5
+        ig._chunk_learned(c, ev0, n, 3, 1.0, False)
+        b = ig._bufs
+        ps = b["all_ps"][:n].cpu().numpy()[:, 2:]
+        flag = b["flag"][:n].cpu().numpy()
+        wps = b["wps"][:n].cpu().numpy()
+        y0 = ig._y0[:n].cpu().numpy()
+        ratios = ig._ratios.reshape(-1)[: n * (ig.nch + 1)].cpu().numpy().reshape(n, ig.nch + 1)
+        gan, g0, y0_ref, _ = R.learned_terms(chs, grids, a_an, ps)
+        ok = flag == 1
+        assert ok.mean() > 0.4
+        assert np.max(np.abs(y0[ok] - y0_ref[ok])) < 2e-6
+        q = np.exp(flow.log_prob(torch.from_numpy(y0).to(dev).float()).double().cpu().numpy())
+        w_ref = 1.0 / (gan + 0.4 * q * g0)
+        norm = R.GEV2PB_REF / (2 * SQRTS ** 2)
+        np.testing.assert_allclose(wps[ok], w_ref[ok] * norm, rtol=1e-8)
+        alpha_all = np.concatenate([a_an, [0.4]])
+        np.testing.assert_allclose(ratios[ok] @ alpha_all, 1.0, rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_mc_learned_channel_unbiased(dev):
+    """With an untrained flow as the learned channel (any density is a valid proposal)
+    sigma agrees with the analytic channels alone within 3 combined standard deviations."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.flow import RQSFlow
+    from madflow_amd.vegas import VegasFlow
+    print("case mc learned channel sigma", flush=True)
+    res = {}
+    for name in ("analytic", "learned"):
+        ig = MultiChannelIntegrand("gg_ttxgg", SQRTS, precision=32, seed=11, **CUTS)
+        vf = VegasFlow(ig.ndim, 500000)
+        vf.compile(ig)
+        vf.run_integration(3)
+        vf.freeze_grid()
+        if name == "learned":
+            ig.attach_flow(RQSFlow(ig.ndim, layers=8, seed=2), 0.3)
+        r = [ig.iteration(1000000, 20 + k, train=False) for k in range(2)]
+        res[name] = (np.mean([x["I"] for x in r]), np.sqrt(np.sum([x["var"] for x in r])) / 2)
+        print(name, res[name], flush=True)
+    (a, ea), (b, eb) = res["analytic"], res["learned"]
+    assert abs(a - b) < 3 * np.hypot(ea, eb)

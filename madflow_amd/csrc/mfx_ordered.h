@@ -70,9 +70,10 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
 
 // Colour-ordered currents of one word.  J[i][j] (i <= j) is the gluon current of the
 // word positions i..j (J[i][i] = the gluon itself), O[j] the top line carrying the top and
-// the first j gluons.  FP32 forms only: rg[i][j] (i < j) = P^2 of the gluons at word
-// positions i..j and rt[j] = P^2 - m_t^2 of the top with the first j gluons, sums of the
-// pair invariants 2 k_a.k_b (computed in FP64 per event and rounded once).  Incoming
+// the first j gluons.  FP32 forms only: P^2 of the gluons at word positions i..j (summed
+// where the range's propagator is applied) and rt[j] = P^2 - m_t^2 of the top with the
+// first j gluons are sums of the pair invariants pp[a][b] = 2 k_a.k_b (computed in FP64
+// per event and rounded once).  Incoming
 // momenta enter negated, so these sums can cancel: rt[j] is taken from the shorter of its
 // two equal forms, the top with the first j gluons or the anti-top with the others
 // (momentum conservation), and of two forms of equal length from the one with the smaller
@@ -91,7 +92,7 @@ __device__ __forceinline__ void add_components(W<R>& acc, const W<R>& t) {
 
 // J(i..j) (and S(i..j)) for every range ending at j in [J0, J1), shorter ranges first.
 template <int NG, int J0, int J1, class R>
-__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG][NG], const R (&rg)[NG][NG],
+__device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG][NG], const R (&pp)[NG][NG],
                                                const Params<R>& prm) {
   constexpr bool INV = StableForms<R>::value;
   // -i GC_12 multiplies the four-gluon structures (GC_12 = i gs^2: a real gs^2)
@@ -131,7 +132,12 @@ __device__ __forceinline__ void ordered_ranges(W<R> (&J)[NG][NG], cx<R> (&S)[NG]
       // propagator 1/P^2 of the massless gluon
       R den2;
       if constexpr (INV) {
-        den2 = rg[i][j];
+        // P^2 of the gluons at word positions i..j: the sum of their pair invariants
+        den2 = R(0);
+#pragma unroll
+        for (int a = i; a <= j; ++a)
+#pragma unroll
+          for (int b = a + 1; b <= j; ++b) den2 += pp[a][b];
       } else {
         R P[4];
         mom_out(N3, P);
@@ -305,7 +311,10 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
   // to spare) nor for four gluons (the kernel fits two waves per SIMD without them).
   constexpr bool SHARE_TOP = S::NFS > 0 && NG >= 5;
   TopPrefix<R, PRE + 1, SHARE_TOP> OP;
-  R rg[NG][NG], rt[NG];   // FP32 forms: propagator denominators of the ordering (above)
+  // FP32 forms: the word's pair invariants (gluon-gluon pp, top-gluon tp, anti-top-gluon
+  // ap) and the top-line denominators rt (above); pp of the prefix pairs carries over to
+  // the second ordering of a pair
+  R pp[NG][NG], tp[NG], ap[NG], rt[NG];
 #pragma unroll 1
   for (int g = r0 >> 2; g <= (r1 - 1) >> 2; ++g) {
     unsigned hbg = 0u;
@@ -341,28 +350,17 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
         }
       }
       if constexpr (S::NINV > 0) {
-        // the word's pair invariants: rg[i][j] for the ranges ending at j >= PRE (the
-        // prefix ranges of the second ordering of a pair are the first's), rt[j] for the
-        // top with the first j gluons from the shorter of its two forms
-        R pp[NG][NG], tp[NG], ap[NG];
+        // the word's pair invariants (the prefix pairs of a pair's second ordering are the
+        // first's), and rt[j] for the top with the first j gluons from the shorter of its
+        // two forms
 #pragma unroll
         for (int x = 0; x < NG; ++x) {
           tp[x] = lds[(S::NWF + T::pair_index(perm[x], NG)) * BLOCK + t];
           ap[x] = lds[(S::NWF + S::NPI + perm[x]) * BLOCK + t];
 #pragma unroll
-          for (int y = x + 1; y < NG; ++y) pp[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
-        }
-#pragma unroll
-        for (int j = 1; j < NG; ++j) {
-          if (j < PRE && !fresh) continue;
-#pragma unroll
-          for (int i = j - 1; i >= 0; --i) {
-            R acc = R(0);
-#pragma unroll
-            for (int a = i; a <= j; ++a)
-#pragma unroll
-              for (int b = a + 1; b <= j; ++b) acc += pp[a][b];
-            rg[i][j] = acc;
+          for (int y = x + 1; y < NG; ++y) {
+            if (y < PRE && !fresh) continue;
+            pp[x][y] = lds[(S::NWF + T::pair_index(perm[x], perm[y])) * BLOCK + t];
           }
         }
 #pragma unroll
@@ -393,8 +391,8 @@ __global__ void __launch_bounds__(BLOCK) k_ordered_jamps(OrderedArgs a) {
           rt[j] = nd < nc ? sd : (nc < nd ? sc : (md <= mc ? sd : sc));
         }
       }
-      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, rg, prm);
-      ordered_ranges<NG, PRE, NG, R>(J, SD, rg, prm);
+      if (fresh) ordered_ranges<NG, 1, PRE, R>(J, SD, pp, prm);
+      ordered_ranges<NG, PRE, NG, R>(J, SD, pp, prm);
       if constexpr (SHARE_TOP) {
         // unrolled: OP is indexed by ht
 #pragma unroll

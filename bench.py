@@ -303,12 +303,12 @@ def train_flow(a, torch, ig):
 
 
 def flow_grid(a, torch, main=None):
-    """The flow leg's frozen FP32 VEGAS grid: flow_grid_warmup adaptation iterations of nev
-    points (or flow_grid_nev points when given).  Each leg of the comparison adapts its
-    grid with its own best schedule, and that time is counted in its set-up: plain VEGAS
-    needs the full warm-up (with 3 iterations instead of 5 it takes 3 frozen iterations
-    to the target and its rate drops 3x), the flow corrects what a shorter adaptation
-    leaves.  Returns the integrand and the adaptation time."""
+    """The flow leg's frozen FP32 VEGAS grid: flow_grid_warmup adaptation iterations of
+    flow_grid_nev points (nev when 0), counted in its set-up.  The flow corrects what a
+    short adaptation leaves, so its leg adapts on smaller iterations than the plain leg of
+    the reference's schedule; the plain leg is also run with a short warm-up of its own
+    (plain_leg), and the gains are reported against both.  Returns the integrand and the
+    adaptation time."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
     ig = MEIntegrand(a.process, precision=32, mapping=a.mapping, seed=7, events_limit=a.events_limit,
                      kernels=a.kernels, **ME_ONLY_KIN[a.process])
@@ -339,19 +339,16 @@ def flow_iteration(a, torch, ig, flow, k):
     return ig.proposal_iteration(flow, a.nev, 100 + k, a.flow_mix)
 
 
-def run_flow_compare(a, torch, main):
-    """Plain VEGAS with FP64 matrix elements against VEGAS + learned proposal with FP32
-    matrix elements on the same cube, each run to sigma/sigma = target.  The flow path
-    reuses the FP32 grid of the main run (its warm-up time is counted in its set-up),
-    then generates the training pairs, trains, and samples y ~ q with weights f(y)/q(y).
-    After the target, both keep sampling to the same sampling time (equal_time_s), which
-    gives the unweighted rates at equal wall time over samples of comparable size."""
+def plain_leg(a, torch, warm_nev, tag):
+    """Plain VEGAS with FP64 matrix elements: a.warmup adaptation iterations of warm_nev points
+    (the reference's schedule runs them with the iteration size itself,
+    madflow_exec.py:489-497), then frozen iterations of a.nev points to the target, then on
+    to equal sampling time and equal wall time."""
     from madflow_amd.vegas import MEIntegrand, VegasFlow
     sync = torch.cuda.synchronize
-    # ---- plain VEGAS, FP64 matrix elements
     ig64 = MEIntegrand(a.process, precision=64, mapping=a.mapping, seed=5, events_limit=a.events_limit,
                        kernels=a.kernels, **ME_ONLY_KIN[a.process])
-    vf = VegasFlow(ig64.ndim, a.nev)
+    vf = VegasFlow(ig64.ndim, warm_nev)
     vf.compile(ig64)
     sync()
     t0 = time.perf_counter()
@@ -359,18 +356,35 @@ def run_flow_compare(a, torch, main):
     vf.freeze_grid()
     sync()
     setup64 = time.perf_counter() - t0
+    vf.n_events = a.nev
 
     def it64(k):
         vf.run_integration(1)
         return vf.history[-1]
     t0 = time.perf_counter()
-    res64 = _run_to_target(it64, a.target, a.max_iter, "plain fp64")
+    res64 = _run_to_target(it64, a.target, a.max_iter, tag)
     sync()
-    plain = _unweighted(res64, setup64, time.perf_counter() - t0, a.target)
-    timed64 = _extend_to(it64, list(res64), plain["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup64), torch)
-    plain["equal_time"] = _rate_at(timed64, a.equal_time_s, a.equal_time_s)
-    plain["equal_wall"] = _rate_at(timed64, a.equal_wall_s - setup64, a.equal_wall_s)
+    out = _unweighted(res64, setup64, time.perf_counter() - t0, a.target)
+    timed64 = _extend_to(it64, list(res64), out["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup64), torch)
+    out["equal_time"] = _rate_at(timed64, a.equal_time_s, a.equal_time_s)
+    out["equal_wall"] = _rate_at(timed64, a.equal_wall_s - setup64, a.equal_wall_s)
+    out["warmup"] = "%d x %d points" % (a.warmup, warm_nev)
     del ig64, vf
+    return out
+
+
+def run_flow_compare(a, torch, main):
+    """Plain VEGAS with FP64 matrix elements (the reference's schedule, and a short warm-up)
+    against VEGAS + learned proposal with FP32 matrix elements on the same cube, each run to
+    sigma/sigma = target.  The flow path adapts an FP32 grid of its own (counted in its
+    set-up), generates the training pairs, trains, and samples y from the defensive
+    mixture with weights f(y)/q'(y).  After the target every leg keeps sampling to equal
+    sampling time (equal_time_s) and to equal wall time with its set-up counted
+    (equal_wall_s)."""
+    sync = torch.cuda.synchronize
+    # ---- plain VEGAS, FP64 matrix elements: the reference's schedule, then a short warm-up
+    plain = plain_leg(a, torch, a.nev, "plain fp64")
+    plain_short = plain_leg(a, torch, a.short_warmup_nev, "plain fp64, short warm-up") if a.short_warmup_nev else None
     # ---- VEGAS + flow, FP32 matrix elements, on a frozen grid of its own
     ig, t_grid = flow_grid(a, torch, main)
     flow, t_pairs, t_train = train_flow(a, torch, ig)
@@ -387,14 +401,16 @@ def run_flow_compare(a, torch, main):
                    grid_warmup="%d x %d points" % (a.flow_grid_warmup, a.flow_grid_nev or a.nev),
                    best_step=getattr(flow, "best_step", None), mixture=a.flow_mix)
 
-    def gain(k):
-        return learned[k] / plain[k] if plain[k] > 0 else None
+    def gain(k, base=None):
+        base = plain if base is None else base
+        return learned[k] / base[k] if base[k] > 0 else None
     mc = run_multichannel_leg(a, torch) if (a.multichannel_leg and a.mapping == "diet") else None
     mcf = run_flow_mc_leg(a, torch) if (a.flow_mc_leg and a.mapping == "diet") else None
     out = {
         "workload": "%s, %s, %d-dim %s VEGAS cube, %d points/iteration, target sigma/sigma=%g" % (
             LABEL[a.process], _kin_text(a.process), ig.ndim, a.mapping, a.nev, a.target),
         "plain_vegas_fp64": plain,
+        "plain_vegas_fp64_short_warmup": plain_short,
         "vegas_flow_fp32": learned,
         "flow": {"layers": 8, "bins": 16, "hidden": "2x128", "weights": "bf16 (fp32 master)",
                  "train_pairs": a.flow_pairs, "validation_pairs": a.flow_pairs // 10,
@@ -404,6 +420,8 @@ def run_flow_compare(a, torch, main):
                  "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience, "seed": a.flow_seed,
                  "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
+        "gain_unweighted_per_s_to_target_vs_short_warmup": (gain("unweighted_per_s_to_target", plain_short)
+                                                            if plain_short else None),
         "gain_unweighted_per_s_equal_time": (learned["equal_time"]["unweighted_per_s"] /
                                              plain["equal_time"]["unweighted_per_s"]
                                              if plain["equal_time"]["unweighted_per_s"] > 0 else None),
@@ -417,6 +435,10 @@ def run_flow_compare(a, torch, main):
         out["multichannel_fp32"] = mc
         out["gain_multichannel_per_s_to_target"] = (mc["unweighted_per_s_to_target"] / plain["unweighted_per_s_to_target"]
                                                     if plain["unweighted_per_s_to_target"] > 0 else None)
+        if plain_short:
+            out["gain_multichannel_per_s_to_target_vs_short_warmup"] = (
+                mc["unweighted_per_s_to_target"] / plain_short["unweighted_per_s_to_target"]
+                if plain_short["unweighted_per_s_to_target"] > 0 else None)
         out["gain_multichannel_per_s_equal_time"] = (mc["equal_time"]["unweighted_per_s"] /
                                                      plain["equal_time"]["unweighted_per_s"]
                                                      if plain["equal_time"]["unweighted_per_s"] > 0 else None)
@@ -483,7 +505,7 @@ def run_multichannel_leg(a, torch):
     sync = torch.cuda.synchronize
     ig = MultiChannelIntegrand(a.process, precision=32, seed=7, events_limit=a.events_limit, kernels=a.kernels,
                                **ME_ONLY_KIN[a.process])
-    vf = VegasFlow(ig.ndim, a.nev)
+    vf = VegasFlow(ig.ndim, a.mc_warmup_nev or a.nev)
     vf.compile(ig)
     sync()
     t0 = time.perf_counter()
@@ -491,6 +513,7 @@ def run_multichannel_leg(a, torch):
     vf.freeze_grid()
     sync()
     setup = time.perf_counter() - t0
+    vf.n_events = a.nev
 
     def it(k):
         vf.run_integration(1)
@@ -501,6 +524,7 @@ def run_multichannel_leg(a, torch):
     out = _unweighted(res, setup, time.perf_counter() - t0, a.target)
     timed = _extend_to(it, list(res), out["sample_s"], a.equal_time_s, torch)
     out["equal_time"] = _rate_at(timed, a.equal_time_s, a.equal_time_s)
+    out["warmup"] = "%d x %d points" % (a.warmup, a.mc_warmup_nev or a.nev)
     out["channels"] = [list(c) for c in ig.channels]
     out["alpha"] = [float(x) for x in ig.alpha]
     return out
@@ -579,7 +603,7 @@ def me_only(a, torch):
     return 0
 
 
-def main(argv=None):
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -608,8 +632,13 @@ def main(argv=None):
     ap.add_argument("--flow-grid-warmup", type=int, default=3,
                     help="grid adaptation iterations of the flow leg's FP32 grid, counted in its set-up (the plain "
                          "FP64 leg adapts for --warmup iterations)")
-    ap.add_argument("--flow-grid-nev", type=int, default=0,
+    ap.add_argument("--flow-grid-nev", type=int, default=4_000_000,
                     help="points per adaptation iteration of the flow leg's grid (0: --nev, like the plain leg)")
+    ap.add_argument("--short-warmup-nev", type=int, default=4_000_000,
+                    help="points per warm-up iteration of the second plain FP64 leg (0: no second leg); the first "
+                         "follows the reference's schedule, warm-up iterations of --nev points")
+    ap.add_argument("--mc-warmup-nev", type=int, default=4_000_000,
+                    help="points per warm-up iteration of the multi-channel leg (0: --nev)")
     ap.add_argument("--flow-train-only", action="store_true",
                     help="run only the flow leg's grid adaptation, training pairs and training, print timings, exit")
     ap.add_argument("--flow-steps", type=int, default=20000)
@@ -636,7 +665,11 @@ def main(argv=None):
     ap.add_argument("--repeats", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 4 times (calibrates HBM byte counters) and exit")
-    a = ap.parse_args(argv)
+    return ap.parse_args(argv)
+
+
+def main(argv=None):
+    a = parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: the event loop shards by Philox counter range, but this build "
               "runs and reports a single MI355X only (--gpus 1)", file=sys.stderr)

@@ -252,16 +252,28 @@ int mfx_unweight(const double* w, int64_t nev, double wmax, uint32_t seed, uint3
  * sizes[] = {nparams, nbf16, NPmax, then per layer (offset, count)}. */
 int mfx_flow_sizes(int D, int L, const int* masks, int64_t* sizes);
 int mfx_flow_cast(int D, int L, const int* masks, const float* params, uint16_t* params_bf16, void* stream);
-/* x -> u with log|du/dx| accumulated into logdet (inverse = 0), or u -> x with
- * log|dx/du| (inverse = 1).  save: NULL, or mfx_flow_save_bytes of device memory that
- * receives what mfx_flow_backward needs (forward direction only). */
+/* x -> u with the sum of the layers' log|du/dx| written to logdet (inverse = 0), or
+ * u -> x with log|dx/du| (inverse = 1); one launch for all layers.  save: NULL, or
+ * mfx_flow_save_bytes of device memory that receives what mfx_flow_backward needs
+ * (forward direction only). */
 int mfx_flow_apply(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
                    const float* x_in, float* x_out, float* logdet, int inverse, float* save, void* stream);
+/* The forward pass of a training step with its minibatch drawn in the same launch: sample
+ * e of the batch takes pool point k, the first with cdf[k] >= u_e (cdf: normalised running
+ * sum of the npool training weights, f64), u_e the first 53-bit uniform of the Philox
+ * stream (counter (e, 0, 0, draw), key (seed, 0x6d66)).  logdet (batch,) and save as in
+ * mfx_flow_apply; y_out (batch, D) and f_out (batch,) optional (f_out needs pool_f).
+ * No reference counterpart (the learned proposal is new). */
+int mfx_flow_draw_forward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16,
+                          const double* cdf, int64_t npool, const float* pool_y, const double* pool_f, uint32_t seed,
+                          uint32_t draw, int64_t batch, float* y_out, double* f_out, float* logdet, float* save,
+                          void* stream);
 int64_t mfx_flow_save_bytes(int D, int L, const int* masks, int64_t n);
 int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n);
 /* Gradient of sum_i gl_i * logdet_i (+ gx . u) w.r.t. the parameters (grad, fp32) and
- * the inputs (gx in/out), from a saved forward pass.  partial: workspace of
- * mfx_flow_partial_bytes; the reduction over samples has a fixed order. */
+ * the inputs (gx in/out; NULL: no gradient of the output, none returned), from a saved
+ * forward pass.  partial: workspace of mfx_flow_partial_bytes; the reduction over samples
+ * has a fixed order. */
 int mfx_flow_backward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
                       const float* save, const float* gl, float* gx, float* partial, float* grad, void* stream);
 /* Minibatch of a weighted training set: for each of the batch uniforms r, the first
@@ -276,6 +288,11 @@ int mfx_flow_seed(const double* w, const double* f, const float* logq, int64_t n
 /* Adam update of the fp32 parameters (gradient clipped to +-clip, non-finite -> 0). */
 int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
                   float beta2, float eps, int step, float clip, void* stream);
+/* mfx_flow_adam on every layer's parameters followed by mfx_flow_cast's bf16 copies of the
+ * updated weights, in one launch. */
+int mfx_flow_adam_cast(int D, int L, const int* masks, float* params, uint16_t* params_bf16, const float* grad,
+                       float* m1, float* m2, float lr, float beta1, float beta2, float eps, int step, float clip,
+                       void* stream);
 
 #ifdef __cplusplus
 }

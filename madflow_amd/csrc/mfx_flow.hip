@@ -11,15 +11,18 @@
 //   inverse  u -> x  (sampling; analytic root of the RQ quadratic)
 //   backward gradients of a per-sample weighted objective through every layer
 // Kernels work on one block of 8 waves per 16 samples (the waves split the MFMA output
-// tiles, so that a small training batch still puts two waves on every SIMD): the 128x128
-// and 128xNP GEMMs of the forward
-// pass run on bf16 MFMA (v_mfma_f32_16x16x16bf16, fp32 accumulation, bf16 weights),
-// the backward GEMMs on fp32 MFMA (v_mfma_f32_16x16x4f32); the spline math is per-lane
-// VALU.  Weight-gradient partials are written per block and reduced in a fixed order.
+// tiles, so that a small training batch still puts two waves on every SIMD) and take the
+// tile through every layer in one launch.  The 128x128 and 128xNP GEMMs run on bf16 MFMA
+// (v_mfma_f32_16x16x16bf16, fp32 accumulation, bf16 weights), dW1 on fp32 MFMA
+// (v_mfma_f32_16x16x4f32); the spline math is per-lane VALU.  Weight-gradient partials
+// are written per split of tiles and reduced in a fixed order.  A training step is five
+// launches: the forward pass with the minibatch draw, the per-sample seed, the data and
+// weight gradients, their reduction, and Adam with the refresh of the bf16 weights.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
 #include "mfx_api_common.h"
+#include "mfx_rng.h"
 #include "../../include/madflow_amd.h"
 
 namespace mfx {
@@ -62,13 +65,7 @@ struct FlowGeom {
 };
 
 // ------------------------------------------------------------------ spline math
-struct Knots {
-  float X, w, Y, h, a, c;   // bin left edge, width, height offset, height, derivatives
-  int k;
-};
-
-// widths/heights from raw logits, softmax with a minimum; returns the cumulative edge
-// below ``v`` (searching in the cumulative sums) and the bin.
+// bin widths / heights from raw logits: softmax with a minimum value minv per bin
 __device__ __forceinline__ void softmax_bins(const float* r, float minv, float* out) {
   float m = r[0];
   for (int j = 1; j < K; ++j) m = fmaxf(m, r[j]);
@@ -83,59 +80,83 @@ __device__ __forceinline__ void softmax_bins(const float* r, float minv, float* 
 __device__ __forceinline__ float softplus(float x) { return x > 20.f ? x : log1pf(__expf(x)); }
 __device__ __forceinline__ float sigmoid(float x) { return 1.f / (1.f + __expf(-x)); }
 
-// Locate the bin of ``v`` along the cumulative sums of ``wdt`` (edge search variable:
-// x for the forward map, the heights for the inverse).
-__device__ __forceinline__ int find_bin(const float* cum, float v) {
-  int k = 0;
-  for (int j = 1; j < K; ++j) k += (v >= cum[j]) ? 1 : 0;
-  return k;
+// Bin of v along the cumulative sums c[j] = w[0] + ... + w[j-1] (c[0] = 0, c[K] = 1, summed
+// in that order): k = the number of j in 1..K-1 with v >= c[j] (the sums increase, so c[k] <=
+// v < c[k+1]), with the edges c[k] and c[k+1].  The sums are formed on the fly, one at a time,
+// rather than kept in an array.
+struct Edge {
+  int k;
+  float lo, hi;
+};
+__device__ __forceinline__ Edge bin_of(const float* w, float v) {
+  Edge e{0, 0.f, w[0]};
+  float c = 0.f;
+#pragma unroll
+  for (int j = 1; j < K; ++j) {
+    c = c + w[j - 1];                                  // c[j]
+    const float cn = j == K - 1 ? 1.f : c + w[j];      // c[j + 1]
+    if (v >= c) {
+      e.k = j;
+      e.lo = c;
+      e.hi = cn;
+    }
+  }
+  return e;
+}
+// the edges c[k], c[k + 1] of bin k of the same cumulative sums
+__device__ __forceinline__ Edge edges_at(const float* w, int k) {
+  Edge e{k, 0.f, w[0]};
+  float c = 0.f;
+#pragma unroll
+  for (int j = 1; j < K; ++j) {
+    c = c + w[j - 1];
+    const float cn = j == K - 1 ? 1.f : c + w[j];
+    if (j == k) {
+      e.lo = c;
+      e.hi = cn;
+    }
+  }
+  return e;
 }
 
 struct SplineEval {
   float z, ld;  // output and log derivative
 };
 
-__device__ __forceinline__ void knots_of(const float* raw, float* w, float* h, float* cw, float* ch) {
+__device__ __forceinline__ SplineEval rq_forward(const float* raw, float x) {
+  float w[K], h[K];
   softmax_bins(raw, MINW, w);
   softmax_bins(raw + K, MINH, h);
-  cw[0] = 0.f;
-  ch[0] = 0.f;
-  for (int j = 0; j < K; ++j) {
-    cw[j + 1] = cw[j] + w[j];
-    ch[j + 1] = ch[j] + h[j];
-  }
-  cw[K] = 1.f;
-  ch[K] = 1.f;
-}
-
-__device__ __forceinline__ SplineEval rq_forward(const float* raw, float x) {
-  float w[K], h[K], cw[K + 1], ch[K + 1];
-  knots_of(raw, w, h, cw, ch);
   x = fminf(fmaxf(x, 0.f), 1.f);
-  const int k = find_bin(cw, x);
+  const Edge ex = bin_of(w, x);
+  const int k = ex.k;
+  const Edge ey = edges_at(h, k);
   const float a = MIND + softplus(raw[2 * K + k]), c = MIND + softplus(raw[2 * K + k + 1]);
-  const float wk = cw[k + 1] - cw[k], hk = ch[k + 1] - ch[k];
-  const float xi = fminf(fmaxf((x - cw[k]) / wk, 0.f), 1.f);
+  const float wk = ex.hi - ex.lo, hk = ey.hi - ey.lo;
+  const float xi = fminf(fmaxf((x - ex.lo) / wk, 0.f), 1.f);
   const float s = hk / wk;
   const float t = xi * (1.f - xi);
   const float num = hk * (s * xi * xi + a * t);
   const float den = s + (a + c - 2.f * s) * t;
   const float P = c * xi * xi + 2.f * s * t + a * (1.f - xi) * (1.f - xi);
   SplineEval r;
-  r.z = ch[k] + num / den;
+  r.z = ey.lo + num / den;
   r.ld = 2.f * __logf(s) + __logf(P) - 2.f * __logf(den);
   return r;
 }
 
 __device__ __forceinline__ SplineEval rq_inverse(const float* raw, float z) {
-  float w[K], h[K], cw[K + 1], ch[K + 1];
-  knots_of(raw, w, h, cw, ch);
+  float w[K], h[K];
+  softmax_bins(raw, MINW, w);
+  softmax_bins(raw + K, MINH, h);
   z = fminf(fmaxf(z, 0.f), 1.f);
-  const int k = find_bin(ch, z);
+  const Edge ey = bin_of(h, z);
+  const int k = ey.k;
+  const Edge ex = edges_at(w, k);
   const float a = MIND + softplus(raw[2 * K + k]), c = MIND + softplus(raw[2 * K + k + 1]);
-  const float wk = cw[k + 1] - cw[k], hk = ch[k + 1] - ch[k];
+  const float wk = ex.hi - ex.lo, hk = ey.hi - ey.lo;
   const float s = hk / wk;
-  const float d = z - ch[k];
+  const float d = z - ey.lo;
   const float g = a + c - 2.f * s;
   const float qa = hk * (s - a) + d * g;
   const float qb = hk * a - d * g;
@@ -147,22 +168,25 @@ __device__ __forceinline__ SplineEval rq_inverse(const float* raw, float z) {
   const float den = s + g * t;
   const float P = c * xi * xi + 2.f * s * t + a * (1.f - xi) * (1.f - xi);
   SplineEval r;
-  r.z = cw[k] + xi * wk;                                          // x
+  r.z = ex.lo + xi * wk;                                          // x
   r.ld = -(2.f * __logf(s) + __logf(P) - 2.f * __logf(den));     // log|dx/du|
   return r;
 }
 
 // Gradient of gz*z + gl*ld w.r.t. the raw parameters (written to graw[NPD]) and x.
 __device__ __forceinline__ float rq_backward(const float* raw, float x, float gz, float gl, float* graw) {
-  float w[K], h[K], cw[K + 1], ch[K + 1];
-  knots_of(raw, w, h, cw, ch);
+  float w[K], h[K];
+  softmax_bins(raw, MINW, w);
+  softmax_bins(raw + K, MINH, h);
   const bool inside = x > 0.f && x < 1.f;
   x = fminf(fmaxf(x, 0.f), 1.f);
-  const int k = find_bin(cw, x);
+  const Edge ex = bin_of(w, x);
+  const int k = ex.k;
+  const Edge ey = edges_at(h, k);
   const float ra = raw[2 * K + k], rc = raw[2 * K + k + 1];
   const float a = MIND + softplus(ra), c = MIND + softplus(rc);
-  const float wk = cw[k + 1] - cw[k], hk = ch[k + 1] - ch[k];
-  const float xi = fminf(fmaxf((x - cw[k]) / wk, 0.f), 1.f);
+  const float wk = ex.hi - ex.lo, hk = ey.hi - ey.lo;
+  const float xi = fminf(fmaxf((x - ex.lo) / wk, 0.f), 1.f);
   const float s = hk / wk;
   const float t = xi * (1.f - xi);
   const float g2 = a + c - 2.f * s;
@@ -183,25 +207,22 @@ __device__ __forceinline__ float rq_backward(const float* raw, float x, float gz
   const float GY = gz;
   const float Ga = gz * (Na * Dn - N * Da) * iD2 + gl * (Pa * iP - 2.f * Da * iD);
   const float Gc = gz * (-N * Dc) * iD2 + gl * (Pc * iP - 2.f * Dc * iD);
-  // chain to bin quantities
+  // chain to bin quantities: d/d w_j and d/d h_j are GX (GY) below bin k, Gw (Ghk) at k
   const float GX = -Gxi / wk;
   const float Gw = -Gxi * xi / wk - Gs * s / wk;
   const float Ghk = Gh + Gs / wk;
-  float gw[K], gh[K];
-  for (int j = 0; j < K; ++j) {
-    gw[j] = (j < k ? GX : 0.f) + (j == k ? Gw : 0.f);
-    gh[j] = (j < k ? GY : 0.f) + (j == k ? Ghk : 0.f);
-  }
   // through the softmax: w = min + (1 - K min) sm
   float sw = 0.f, sh = 0.f;
+#pragma unroll
   for (int j = 0; j < K; ++j) {
-    sw += (w[j] - MINW) * gw[j];
-    sh += (h[j] - MINH) * gh[j];
+    sw += (w[j] - MINW) * ((j < k ? GX : 0.f) + (j == k ? Gw : 0.f));
+    sh += (h[j] - MINH) * ((j < k ? GY : 0.f) + (j == k ? Ghk : 0.f));
   }
   const float nw = 1.f - K * MINW, nh = 1.f - K * MINH;
+#pragma unroll
   for (int j = 0; j < K; ++j) {
-    graw[j] = (w[j] - MINW) * (gw[j] - sw / nw);
-    graw[K + j] = (h[j] - MINH) * (gh[j] - sh / nh);
+    graw[j] = (w[j] - MINW) * (((j < k ? GX : 0.f) + (j == k ? Gw : 0.f)) - sw / nw);
+    graw[K + j] = (h[j] - MINH) * (((j < k ? GY : 0.f) + (j == k ? Ghk : 0.f)) - sh / nh);
   }
   for (int j = 0; j <= K; ++j) graw[2 * K + j] = 0.f;
   graw[2 * K + k] = Ga * sigmoid(ra);
@@ -220,584 +241,492 @@ __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// ------------------------------------------------------------------ MLP forward
-// One block, 16 samples; wave w computes the output tiles n0 = 16 (w + 4 i).
-// xa: LDS [16][DMAX] conditioning inputs (only nA used); h1s/h2s: LDS bf16 [16][H];
-// raws: LDS fp32 [16][NP].
-// NPC: the layer's NP when known at compile time (0: g.NP at run time).
-template <int NPC>
-__device__ void mlp_forward(const LayerGeom& g, const float* __restrict__ P, const unsigned short* __restrict__ Pb,
-                            const float* xa, unsigned short* h1s, unsigned short* h2s, float* raws) {
-  const int NP = NPC ? NPC : g.NP;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  // layer 1 on VALU: 16 x 128 outputs, 8 per thread
-  for (int q = tid; q < TILE * H; q += NT) {
-    const int s = q / H, j = q % H;
-    float v = P[g.ob1 + j];
-    for (int i = 0; i < g.nA; ++i) v += xa[s * DMAX + i] * P[g.oW1 + i * H + j];
-    h1s[s * H + j] = f2bf(fmaxf(v, 0.f));
-  }
-  __syncthreads();
-  const int r = lane & 15, kq = (lane >> 4) * 4;
-  // layer 2: h2 = relu(h1 W2 + b2), B operand from W2t [out][in] (bf16)
-  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < H; k0 += 16) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1s + r * H + k0 + kq);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(Pb + g.oW2t + (n0 + r) * H + k0 + kq);
-      acc = mfma_bf16(a, b, acc);
-    }
-    for (int i = 0; i < 4; ++i) {
-      const int row = kq + i, col = n0 + r;
-      h2s[row * H + col] = f2bf(fmaxf(acc[i] + P[g.ob2 + col], 0.f));
-    }
-  }
-  __syncthreads();
-  // layer 3: raw = h2 W3 + b3, B operand from W3t [NP][H] (bf16)
-#pragma unroll
-  for (int n0 = 16 * wid; n0 < NP; n0 += 16 * WPB) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < H; k0 += 16) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(h2s + r * H + k0 + kq);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(Pb + g.oW3t + (n0 + r) * H + k0 + kq);
-      acc = mfma_bf16(a, b, acc);
-    }
-    for (int i = 0; i < 4; ++i) {
-      const int row = kq + i, col = n0 + r;
-      raws[row * NP + col] = acc[i] + P[g.ob3 + col];
-    }
-  }
-  __syncthreads();
+// ------------------------------------------------------------------ whole-flow kernels
+// Every kernel below takes its 16-sample tile (or, for the weight gradients, a split of
+// tiles) through all L layers in one launch.  The coupling layers act on each sample
+// separately, so a tile's state between layers (coordinates, log-det, incoming gradient)
+// stays in LDS, and only what the backward pass needs goes to memory.  The layers' weights
+// are MFMA B operands read straight from the bf16 copies (8 layers of 84 KB stay in L2).
+// NBM bounds the transformed set of every layer (it sizes the LDS arrays); each layer's
+// own geometry is read at run time.
+constexpr int WP = H + 8;       // padded row pitch (bf16 elements) of h1/h2 in LDS: 16-byte
+                                // rows, and the 16 rows of an MFMA operand in distinct banks
+
+// Saved state of one layer for the backward pass, at a fixed per-layer stride: the layer
+// input x (n, D) and the spline parameters raw (n, NPmax) in fp32, then the bf16 activations
+// h1 and h2 tile-transposed ([tile][column][sample]) over whole tiles.
+struct SaveLayout {
+  int64_t raw, h1, h2, bytes;   // byte offsets from the layer's base, and the stride
+};
+// The bf16 deltas of one layer, tile-transposed over whole tiles: go [nt16][NP], gh2 and
+// gh1 [nt16][H], at a fixed per-layer stride.
+struct DeltaLayout {
+  int64_t o, h2, h1, bytes;
+};
+
+__device__ __forceinline__ bf16x4 ld_bf16x4(const unsigned short* p) {
+  return *reinterpret_cast<const bf16x4*>(p);
+}
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 ld_bf16x8(const unsigned short* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
+}
+// acc += A B over 32 consecutive k: the lanes of quarter g hold k = K0 + 8 g .. K0 + 8 g + 7 of
+// their row (A) and column (B) in one 16-byte load each, and two MFMAs take the lower and
+// upper four (the k order inside the sum is a fixed permutation, the same for A and B).
+__device__ __forceinline__ f32x4 mfma_k32(bf16x8 a, bf16x8 b, f32x4 acc) {
+  acc = mfma_bf16(__builtin_shufflevector(a, a, 0, 1, 2, 3), __builtin_shufflevector(b, b, 0, 1, 2, 3), acc);
+  return mfma_bf16(__builtin_shufflevector(a, a, 4, 5, 6, 7), __builtin_shufflevector(b, b, 4, 5, 6, 7), acc);
 }
 
-struct FwdArgs {
+// first index k of the normalised running sum with cdf[k] >= u
+__device__ __forceinline__ int64_t cdf_search(const double* __restrict__ cdf, int64_t n, double u) {
+  int64_t lo = 0, hi = n - 1;          // invariant: the answer lies in [lo, hi]
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] >= u) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct FlowArgs {
   FlowGeom geo;
-  int layer;
   int64_t n;
-  const float* x_in;       // (n, D)
-  float* x_out;            // (n, D)
-  float* logdet;           // (n,) accumulated
-  float* save_x;           // optional (n, D): copy of the layer input
-  // optional bf16 activations, tile-transposed [tile][H][TILE] (sample fastest, zero for the
-  // samples past n in the last tile): a backward MFMA operand (four consecutive samples of
-  // one column) is then one 8-byte load
-  unsigned short* save_h1;
-  unsigned short* save_h2;
-  float* save_raw;         // optional (n, NPmax)
+  const float* x_in;       // (n, D); null when the batch is drawn from a pool
+  // minibatch draw: sample e takes pool point k, the first with cdf[k] >= u_e, where u_e
+  // is the first 53-bit Philox uniform of counter (e, 0, 0, draw) under key (seed, 0x6d66)
+  const double* cdf;
+  int64_t npool;
+  const float* pool_y;     // (npool, D)
+  const double* pool_f;    // optional (npool): integrand values
+  uint32_t seed, draw;
+  float* y_out;            // optional (n, D): the drawn points
+  double* f_out;           // optional (n): their integrand values
+  float* x_out;            // optional (n, D): the flow's output
+  float* logdet;           // (n): sum over the layers of the log-dets (written)
+  char* save;              // optional: saved state of layer l at save + l * sl.bytes
+  SaveLayout sl;
   int inverse;
 };
 
-// NB: the layer's number of transformed dimensions when known at compile time (its NP
-// is then a constant and the loops over it unroll), 0 for the run-time geometry.
-template <int NB>
-__global__ void __launch_bounds__(NT) k_layer(FwdArgs A, const float* __restrict__ P,
-                                              const unsigned short* __restrict__ Pb) {
-  constexpr int NPC = NB ? ((NB * NPD + 15) / 16) * 16 : 0;
-  const LayerGeom& g = A.geo.layer[A.layer];
-  const int NP = NPC ? NPC : g.NP, nB = NB ? NB : g.nB;
-  const int D = A.geo.D;
-  __shared__ float xa[TILE * DMAX];
-  __shared__ float xs[TILE * DMAX];
-  __shared__ __attribute__((aligned(16))) unsigned short h1s[TILE * H];
-  __shared__ __attribute__((aligned(16))) unsigned short h2s[TILE * H];
-  extern __shared__ float raws[];
-  const int tid = threadIdx.x;
+template <int NBM>
+__global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict__ P,
+                                             const unsigned short* __restrict__ Pb) {
+  constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
+  const int D = A.geo.D, L = A.geo.L;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int r = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
+  __shared__ float xs[TILE * DMAX], xa[TILE * DMAX], lds_ld[TILE * DMAX], ldacc[TILE];
+  __shared__ __attribute__((aligned(16))) unsigned short h1s[TILE * WP];
+  __shared__ __attribute__((aligned(16))) unsigned short h2s[TILE * WP];
+  __shared__ float raws[TILE * NPM];
+  __shared__ int64_t pick[TILE];
   const int64_t s0 = (int64_t)blockIdx.x * TILE;
+  if (tid < TILE) {
+    const int64_t e = s0 + tid;
+    int64_t k = 0;
+    if (A.cdf && e < A.n) {
+      double u0, u1;
+      philox_pair((uint64_t)e, 0u, A.seed, A.draw, u0, u1);
+      k = cdf_search(A.cdf, A.npool, u0);
+      if (A.f_out) A.f_out[e] = A.pool_f[k];
+    }
+    pick[tid] = k;
+    ldacc[tid] = 0.f;
+  }
+  __syncthreads();
   for (int q = tid; q < TILE * DMAX; q += NT) {
     const int s = q / DMAX, d = q % DMAX;
     const int64_t e = s0 + s;
-    const float v = (e < A.n && d < D) ? A.x_in[e * D + d] : 0.5f;
+    const bool ok = e < A.n && d < D;
+    const float v = ok ? (A.cdf ? A.pool_y[pick[s] * D + d] : A.x_in[e * D + d]) : 0.5f;
     xs[q] = v;
-    if (A.save_x && e < A.n && d < D) A.save_x[e * D + d] = v;
+    if (ok && A.y_out) A.y_out[e * D + d] = v;
   }
-  __syncthreads();
-  for (int q = tid; q < TILE * DMAX; q += NT) {
-    const int s = q / DMAX, i = q % DMAX;
-    xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
-  }
-  __syncthreads();
-  mlp_forward<NPC>(g, P, Pb, xa, h1s, h2s, raws);
-  // spline on every (sample, transformed dimension); the log-dets of one sample are
-  // summed by one thread afterwards in a fixed order
-  __shared__ float lds_ld[TILE * DMAX];
-  for (int q = tid; q < TILE * nB; q += NT) {
-    const int s = q / nB, b = q % nB;
-    const int d = g.B[b];
-    const float* raw = raws + s * NP + b * NPD;
-    const SplineEval r = A.inverse ? rq_inverse(raw, xs[s * DMAX + d]) : rq_forward(raw, xs[s * DMAX + d]);
-    xs[s * DMAX + d] = r.z;
-    lds_ld[s * DMAX + b] = r.ld;
-  }
-  __syncthreads();
-  if (tid < TILE) {
-    const int64_t e = s0 + tid;
-    if (e < A.n) {
-      float ld = 0.f;
-      for (int b = 0; b < nB; ++b) ld += lds_ld[tid * DMAX + b];
-      A.logdet[e] += ld;
-    }
-  }
-  for (int q = tid; q < TILE * D; q += NT) {
-    const int s = q / D, d = q % D;
-    const int64_t e = s0 + s;
-    if (e < A.n) A.x_out[e * D + d] = xs[s * DMAX + d];
-  }
-  if (A.save_h1) {
-    for (int q = tid; q < TILE * H; q += NT) {
-      const int j = q / TILE, s = q % TILE;
-      const bool ok = s0 + s < A.n;
-      A.save_h1[s0 * H + q] = ok ? h1s[s * H + j] : (unsigned short)0;
-      A.save_h2[s0 * H + q] = ok ? h2s[s * H + j] : (unsigned short)0;
-    }
-    for (int q = tid; q < TILE * NP; q += NT) {
-      const int64_t e = s0 + q / NP;
-      if (e < A.n) A.save_raw[e * A.geo.NPmax + q % NP] = raws[q];
-    }
-  }
-}
-
-// Forward / inverse of one layer with NB (1..NB_LDS) transformed dimensions, persistent:
-// each block stages the layer's weights in LDS once (W2^T and W3^T in bf16, rows padded
-// to WP = H + 4 elements so that the 16 rows an MFMA operand spans fall in distinct
-// banks; W1 and the biases in fp32), then runs the tiles b, b + G, ... of 16 samples
-// through the same arithmetic as k_layer.
-constexpr int NB_LDS = 6;
-constexpr int WP = H + 4;       // padded row pitch (bf16 elements) of the LDS weights and h1/h2
-
-template <int NB>
-__global__ void __launch_bounds__(NT) k_layer_lds(FwdArgs A, const float* __restrict__ P,
-                                                  const unsigned short* __restrict__ Pb) {
-  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
-  const LayerGeom& g = A.geo.layer[A.layer];
-  const int D = A.geo.D;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int r = lane & 15, kq = (lane >> 4) * 4;
-  __shared__ __attribute__((aligned(16))) unsigned short W2s[H * WP];
-  __shared__ __attribute__((aligned(16))) unsigned short W3s[NP * WP];
-  __shared__ float W1s[DMAX * H], b1s[H], b2s[H], b3s[NP];
-  __shared__ float xa[TILE * DMAX], xs[TILE * DMAX], lds_ld[TILE * DMAX];
-  __shared__ __attribute__((aligned(16))) unsigned short h1s[TILE * WP];
-  __shared__ __attribute__((aligned(16))) unsigned short h2s[TILE * WP];
-  __shared__ float raws[TILE * NP];
-  // weights: bf16 rows copied 8 bytes at a time
-  for (int q = tid; q < H * (H / 4); q += NT) {
-    const int row = q / (H / 4), c = 4 * (q % (H / 4));
-    *reinterpret_cast<uint2*>(W2s + row * WP + c) = *reinterpret_cast<const uint2*>(Pb + g.oW2t + row * H + c);
-  }
-  for (int q = tid; q < NP * (H / 4); q += NT) {
-    const int row = q / (H / 4), c = 4 * (q % (H / 4));
-    *reinterpret_cast<uint2*>(W3s + row * WP + c) = *reinterpret_cast<const uint2*>(Pb + g.oW3t + row * H + c);
-  }
-  for (int q = tid; q < g.nA * H; q += NT) W1s[q] = P[g.oW1 + q];
-  for (int q = tid; q < H; q += NT) {
-    b1s[q] = P[g.ob1 + q];
-    b2s[q] = P[g.ob2 + q];
-  }
-  for (int q = tid; q < NP; q += NT) b3s[q] = P[g.ob3 + q];
-  const int64_t ntiles = (A.n + TILE - 1) / TILE;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t s0 = tile * TILE;
-    __syncthreads();                 // weights staged / the previous tile is done with the LDS
-    for (int q = tid; q < TILE * DMAX; q += NT) {
-      const int s = q / DMAX, d = q % DMAX;
-      const int64_t e = s0 + s;
-      const float v = (e < A.n && d < D) ? A.x_in[e * D + d] : 0.5f;
-      xs[q] = v;
-      if (A.save_x && e < A.n && d < D) A.save_x[e * D + d] = v;
-    }
-    __syncthreads();
+  for (int step = 0; step < L; ++step) {
+    const int l = A.inverse ? L - 1 - step : step;
+    const LayerGeom& g = A.geo.layer[l];
+    const int nB = g.nB, NP = g.NP;
+    char* base = A.save ? A.save + A.sl.bytes * l : nullptr;
+    __syncthreads();                       // xs holds this layer's input
     for (int q = tid; q < TILE * DMAX; q += NT) {
       const int s = q / DMAX, i = q % DMAX;
       xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
+      const int64_t e = s0 + s;
+      if (base && e < A.n && i < D) reinterpret_cast<float*>(base)[e * D + i] = xs[q];
     }
     __syncthreads();
-    // layer 1 on VALU: 16 x 128 outputs
-    for (int q = tid; q < TILE * H; q += NT) {
-      const int s = q / H, j = q % H;
-      float v = b1s[j];
-      for (int i = 0; i < g.nA; ++i) v += xa[s * DMAX + i] * W1s[i * H + j];
-      h1s[s * WP + j] = f2bf(fmaxf(v, 0.f));
+    // layer 1 on VALU: 16 x 128 outputs; thread t computes column t % H for the samples
+    // t / H + 4 u, with that column of W1 loaded once (all rows in flight together)
+    {
+      static_assert(NT == 4 * H, "four samples per thread in layer 1");
+      const int j = tid & (H - 1), sb = tid / H;
+      float w1[DMAX];
+#pragma unroll
+      for (int i = 0; i < DMAX; ++i) w1[i] = i < g.nA ? P[g.oW1 + i * H + j] : 0.f;
+      const float bj = P[g.ob1 + j];
+#pragma unroll
+      for (int u = 0; u < TILE / 4; ++u) {
+        const int s = sb + 4 * u;
+        float v = bj;
+#pragma unroll
+        for (int i = 0; i < DMAX; ++i)
+          if (i < g.nA) v += xa[s * DMAX + i] * w1[i];
+        h1s[s * WP + j] = f2bf(fmaxf(v, 0.f));
+      }
     }
     __syncthreads();
-    // layer 2: h2 = relu(h1 W2 + b2); wave w computes the column tile 16 w
+    // layer 2: h2 = relu(h1 W2 + b2), B operand W2t [out][in]; wave w: column tile 16 w
     {
       const int n0 = 16 * wid;
+      const float bias = P[g.ob2 + n0 + r];
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int k0 = 0; k0 < H; k0 += 16) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1s + r * WP + k0 + kq);
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(W2s + (n0 + r) * WP + k0 + kq);
-        acc = mfma_bf16(a, b, acc);
-      }
+      for (int K0 = 0; K0 < H; K0 += 32)
+        acc = mfma_k32(ld_bf16x8(h1s + r * WP + K0 + k8), ld_bf16x8(Pb + g.oW2t + (n0 + r) * H + K0 + k8), acc);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = kq + i, col = n0 + r;
-        h2s[row * WP + col] = f2bf(fmaxf(acc[i] + b2s[col], 0.f));
+        h2s[row * WP + col] = f2bf(fmaxf(acc[i] + bias, 0.f));
       }
     }
     __syncthreads();
-    // layer 3: raw = h2 W3 + b3
+    // layer 3: raw = h2 W3 + b3, B operand W3t [NP][H]; wave w: column tiles w, w + 8, ...
+    // (unrolled to the bound, so that the operand loads of every tile are in flight together)
 #pragma unroll
-    for (int n0 = 16 * wid; n0 < NP; n0 += 16 * WPB) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < (NPM / 16 + WPB - 1) / WPB; ++u) {
+      const int n0 = 16 * (wid + WPB * u);
+      if (n0 < NP) {
+        const float bias = P[g.ob3 + n0 + r];
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int k0 = 0; k0 < H; k0 += 16) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(h2s + r * WP + k0 + kq);
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(W3s + (n0 + r) * WP + k0 + kq);
-        acc = mfma_bf16(a, b, acc);
-      }
+        for (int K0 = 0; K0 < H; K0 += 32)
+          acc = mfma_k32(ld_bf16x8(h2s + r * WP + K0 + k8), ld_bf16x8(Pb + g.oW3t + (n0 + r) * H + K0 + k8), acc);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = kq + i, col = n0 + r;
-        raws[row * NP + col] = acc[i] + b3s[col];
+        for (int i = 0; i < 4; ++i) {
+          const int row = kq + i, col = n0 + r;
+          raws[row * NPM + col] = acc[i] + bias;
+        }
       }
     }
     __syncthreads();
-    for (int q = tid; q < TILE * NB; q += NT) {
-      const int s = q / NB, b = q % NB;
+    // spline on every (sample, transformed dimension); the log-dets of one sample are
+    // summed by one thread afterwards in a fixed order
+    for (int q = tid; q < TILE * nB; q += NT) {
+      const int s = q / nB, b = q % nB;
       const int d = g.B[b];
-      const float* raw = raws + s * NP + b * NPD;
+      const float* raw = raws + s * NPM + b * NPD;
       const SplineEval sp = A.inverse ? rq_inverse(raw, xs[s * DMAX + d]) : rq_forward(raw, xs[s * DMAX + d]);
       xs[s * DMAX + d] = sp.z;
       lds_ld[s * DMAX + b] = sp.ld;
     }
     __syncthreads();
     if (tid < TILE) {
-      const int64_t e = s0 + tid;
-      if (e < A.n) {
-        float ld = 0.f;
-        for (int b = 0; b < NB; ++b) ld += lds_ld[tid * DMAX + b];
-        A.logdet[e] += ld;
+      float ld = 0.f;
+      for (int b = 0; b < nB; ++b) ld += lds_ld[tid * DMAX + b];
+      ldacc[tid] += ld;
+    }
+    if (base) {
+      unsigned short* sh1 = reinterpret_cast<unsigned short*>(base + A.sl.h1);
+      unsigned short* sh2 = reinterpret_cast<unsigned short*>(base + A.sl.h2);
+      float* sraw = reinterpret_cast<float*>(base + A.sl.raw);
+#pragma unroll
+      for (int u = 0; u < TILE * H / NT; ++u) {
+        const int q = tid + u * NT;
+        const int j = q / TILE, s = q % TILE;
+        const bool ok = s0 + s < A.n;
+        sh1[s0 * H + q] = ok ? h1s[s * WP + j] : (unsigned short)0;
+        sh2[s0 * H + q] = ok ? h2s[s * WP + j] : (unsigned short)0;
+      }
+#pragma unroll
+      for (int u = 0; u < (TILE * NPM + NT - 1) / NT; ++u) {
+        const int q = tid + u * NT;
+        if (q < TILE * NP) {
+          const int s = q / NP, c = q % NP;
+          const int64_t e = s0 + s;
+          if (e < A.n) sraw[e * A.geo.NPmax + c] = raws[s * NPM + c];
+        }
       }
     }
+  }
+  __syncthreads();
+  if (tid < TILE && s0 + tid < A.n) A.logdet[s0 + tid] = ldacc[tid];
+  if (A.x_out)
     for (int q = tid; q < TILE * D; q += NT) {
       const int s = q / D, d = q % D;
       const int64_t e = s0 + s;
       if (e < A.n) A.x_out[e * D + d] = xs[s * DMAX + d];
     }
-    if (A.save_h1) {
-      for (int q = tid; q < TILE * H; q += NT) {
-        const int j = q / TILE, s = q % TILE;
-        const bool ok = s0 + s < A.n;
-        A.save_h1[s0 * H + q] = ok ? h1s[s * WP + j] : (unsigned short)0;
-        A.save_h2[s0 * H + q] = ok ? h2s[s * WP + j] : (unsigned short)0;
-      }
-      for (int q = tid; q < TILE * NP; q += NT) {
-        const int64_t e = s0 + q / NP;
-        if (e < A.n) A.save_raw[e * A.geo.NPmax + q % NP] = raws[q];
-      }
-    }
-  }
 }
 
 // ------------------------------------------------------------------ backward
+// Two kernels, both on the bf16 matrix cores with fp32 accumulation: the activations and
+// weights are the bf16 values of the forward pass, the deltas are rounded to bf16 like them.
+//  * k_flow_bwd: per 16-sample tile, the layers in reverse: the spline backward, the
+//    input-gradient products gh2 = (go W3^T) relu'(h2) and gh1 = (gh2 W2^T) relu'(h1), the
+//    input gradient (carried to the next layer in LDS), and the deltas go, gh2, gh1 written
+//    tile-transposed for the weight gradients;
+//  * k_flow_wgrad: the weight gradients of every layer as GEMMs with the samples as K, over
+//    splits of split_tiles tiles: block (split, y < 8, layer) computes rows 16 y .. 16 y + 15
+//    of dW2 = h1^T gh2 and dW3 = h2^T go (wave w: column tiles w, w + 8, ...); block
+//    (split, 8, layer) the bias sums as products with a row of ones and dW1 = x_A^T gh1 (fp32,
+//    from the exact inputs).  One partial gradient per split, reduced in a fixed order.
 struct BwdArgs {
   FlowGeom geo;
-  int layer;
   int64_t n;
-  const float* x_in;               // layer input (n, D)
-  const unsigned short* h1;        // saved (n, H)
-  const unsigned short* h2;        // saved (n, H)
-  const float* raw;                // saved (n, NPmax)
-  const float* gl;                 // (n,) d objective / d log-det (per sample)
-  float* gx;                       // (n, D): in: d/d layer output, out: d/d layer input
-  float* partial;                  // (nblocks, layer nparams) weight-gradient partials
+  const char* save;        // the forward pass's saved state
+  SaveLayout sl;
+  char* delta;             // layer l's deltas at delta + l * dl.bytes
+  DeltaLayout dl;
+  const float* gl;         // (n): d objective / d log q, per sample
+  const float* gx_in;      // optional (n, D): d objective / d flow output (null: zero)
+  float* gx_out;           // optional (n, D): d objective / d flow input
+  float* partial;          // (nsplit, nparams): weight-gradient partials
+  int split_tiles;
 };
 
-__global__ void __launch_bounds__(NT) k_layer_bwd(BwdArgs A, const float* __restrict__ P,
-                                                  const unsigned short* __restrict__ Pb) {
-  const LayerGeom& g = A.geo.layer[A.layer];
-  const int D = A.geo.D;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int r = lane & 15, kq = lane >> 4;  // fp32 MFMA: k index = lane/16
-  const int64_t s0 = (int64_t)blockIdx.x * TILE;
-  __shared__ float xs[TILE * DMAX], gxs[TILE * DMAX];
-  __shared__ float h1f[TILE * H], h2f[TILE * H], gh[TILE * H];
-  extern __shared__ float go[];   // [TILE][NP]
-  for (int q = tid; q < TILE * DMAX; q += NT) {
-    const int s = q / DMAX, d = q % DMAX;
-    const int64_t e = s0 + s;
-    const bool ok = e < A.n && d < D;
-    xs[q] = ok ? A.x_in[e * D + d] : 0.5f;
-    gxs[q] = ok ? A.gx[e * D + d] : 0.f;
-  }
-  for (int q = tid; q < TILE * H; q += NT) {
-    const int j = q / TILE, s = q % TILE;          // tile-transposed saved activations
-    h1f[s * H + j] = bf2f(A.h1[s0 * H + q]);
-    h2f[s * H + j] = bf2f(A.h2[s0 * H + q]);
-  }
-  __syncthreads();
-  // spline backward: d raw (into go) and d x_B (replaces the incoming gradient of x_B)
-  for (int q = tid; q < TILE * g.NP; q += NT) go[q] = 0.f;
-  __syncthreads();
-  for (int q = tid; q < TILE * g.nB; q += NT) {
-    const int s = q / g.nB, b = q % g.nB;
-    const int64_t e = s0 + s;
-    if (e >= A.n) continue;
-    const int d = g.B[b];
-    float graw[NPD];
-    const float gxb = rq_backward(A.raw + e * A.geo.NPmax + b * NPD, xs[s * DMAX + d], gxs[s * DMAX + d], A.gl[e],
-                                  graw);
-    for (int j = 0; j < NPD; ++j) go[s * g.NP + b * NPD + j] = graw[j];
-    gxs[s * DMAX + d] = gxb;
-  }
-  __syncthreads();
-  const int nlp = g.ob3 + g.NP - g.oW1;   // parameters of this layer
-  float* part = A.partial + (size_t)blockIdx.x * nlp;
-  // db3 and dW3 = h2^T go  (M = H, N = NP, K = 16 samples)
-  for (int q = tid; q < g.NP; q += NT) {
-    float v = 0.f;
-    for (int s = 0; s < TILE; ++s) v += go[s * g.NP + q];
-    part[g.ob3 - g.oW1 + q] = v;
-  }
-  const int nt3 = g.NP / 16;
-  for (int t = wid; t < (H / 16) * nt3; t += WPB) {
-      const int m0 = 16 * (t / nt3), n0 = 16 * (t % nt3);
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k0 = 0; k0 < TILE; k0 += 4) acc = mfma_f32(h2f[(k0 + kq) * H + m0 + r], go[(k0 + kq) * g.NP + n0 + r], acc);
-      for (int i = 0; i < 4; ++i) part[g.oW3 - g.oW1 + (m0 + 4 * kq + i) * g.NP + n0 + r] = acc[i];
-    }
-  // gh2 = (go W3^T) * relu'(h2)  (M = 16, N = H, K = NP), with the bf16 weights the
-  // forward pass used: W3^T[np][j] is the transposed bf16 copy W3t
-  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.NP; k0 += 4)
-      acc = mfma_f32(go[r * g.NP + k0 + kq], bf2f(Pb[g.oW3t + (k0 + kq) * H + n0 + r]), acc);
-    for (int i = 0; i < 4; ++i) {
-      const int row = 4 * kq + i, col = n0 + r;
-      gh[row * H + col] = h2f[row * H + col] > 0.f ? acc[i] : 0.f;
-    }
-  }
-  __syncthreads();
-  // db2, dW2 = h1^T gh2
-  for (int q = tid; q < H; q += NT) {
-    float v = 0.f;
-    for (int s = 0; s < TILE; ++s) v += gh[s * H + q];
-    part[g.ob2 - g.oW1 + q] = v;
-  }
-  for (int t = wid; t < (H / 16) * (H / 16); t += WPB) {
-      const int m0 = 16 * (t / (H / 16)), n0 = 16 * (t % (H / 16));
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k0 = 0; k0 < TILE; k0 += 4) acc = mfma_f32(h1f[(k0 + kq) * H + m0 + r], gh[(k0 + kq) * H + n0 + r], acc);
-      for (int i = 0; i < 4; ++i) part[g.oW2 - g.oW1 + (m0 + 4 * kq + i) * H + n0 + r] = acc[i];
-    }
-  // gh1 = (gh2 W2^T) * relu'(h1); W2^T[o][i] is the bf16 copy W2t
-  __shared__ float gh1[TILE * H];
-  for (int n0 = 16 * wid; n0 < H; n0 += 16 * WPB) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < H; k0 += 4) acc = mfma_f32(gh[r * H + k0 + kq], bf2f(Pb[g.oW2t + (k0 + kq) * H + n0 + r]), acc);
-    for (int i = 0; i < 4; ++i) {
-      const int row = 4 * kq + i, col = n0 + r;
-      gh1[row * H + col] = h1f[row * H + col] > 0.f ? acc[i] : 0.f;
-    }
-  }
-  __syncthreads();
-  // db1, dW1 = x_A^T gh1, and d x_A += gh1 W1^T
-  for (int q = tid; q < H; q += NT) {
-    float v = 0.f;
-    for (int s = 0; s < TILE; ++s) v += gh1[s * H + q];
-    part[g.ob1 - g.oW1 + q] = v;
-  }
-  for (int q = tid; q < DMAX * H; q += NT) {
-    const int i = q / H, j = q % H;
-    float v = 0.f;
-    if (i < g.nA)
-      for (int s = 0; s < TILE; ++s) v += xs[s * DMAX + g.A[i]] * gh1[s * H + j];
-    part[q] = v;   // W1 block is the first DMAX*H entries of the layer
-  }
-  for (int q = tid; q < TILE * g.nA; q += NT) {
-    const int s = q / g.nA, i = q % g.nA;
-    float v = 0.f;
-    for (int j = 0; j < H; ++j) v += gh1[s * H + j] * P[g.oW1 + i * H + j];
-    gxs[s * DMAX + g.A[i]] += v;
-  }
-  __syncthreads();
-  for (int q = tid; q < TILE * D; q += NT) {
-    const int s = q / D, d = q % D;
-    const int64_t e = s0 + s;
-    if (e < A.n) A.gx[e * D + d] = gxs[s * DMAX + d];
-  }
-}
+// The data-gradient kernel runs 4 waves per tile (each wave two column tiles of gh2 / gh1):
+// its register use allows 12 waves per CU, so three tiles are in flight on every CU.
+constexpr int WPB_B = 4;
+constexpr int NT_B = 64 * WPB_B;
 
-// Backward of one layer in two kernels, for layers whose transformed set has NB (1..8)
-// dimensions (NP = pad16(NB (3K+1)) is then a compile-time constant, so every loop over
-// it unrolls).  Both run their GEMMs on the bf16 matrix cores with fp32 accumulation
-// (v_mfma_f32_16x16x16bf16): the activations and weights are the bf16 values of the
-// forward pass, the deltas are rounded to bf16 like them.
-//  * k_layer_bwd_data: per 16-sample tile (one per block), the spline backward and the
-//    two input-gradient products gh2 = (go W3^T) relu'(h2), gh1 = (gh2 W2^T) relu'(h1),
-//    the input gradient gx, and the deltas go, gh2, gh1 written tile-transposed
-//    ([tile][column][sample], like the saved activations);
-//  * k_layer_wgrad: the weight gradients as GEMMs with the samples as K, over splits of
-//    WG_TILES tiles: block (split, y < 8) computes rows 16 y .. 16 y + 15 of dW2 = h1^T gh2
-//    and dW3 = h2^T go (wave w: column tiles w, w + 8, ...); block (split, 8) the bias sums
-//    as products with a row of ones and dW1 = x_A^T gh1 (fp32, from the exact inputs).
-//    One partial per split, reduced in a fixed order.
-constexpr int WG_TILES = 4;                    // tiles (of 16 samples) per weight-gradient split
-constexpr int WG_SPLIT = WG_TILES * TILE;      // samples per split
-
-__device__ __forceinline__ bf16x4 ld_bf16x4(const unsigned short* p) {
-  return *reinterpret_cast<const bf16x4*>(p);
-}
-
-template <int NB>
-__global__ void __launch_bounds__(NT) k_layer_bwd_data(BwdArgs A, const float* __restrict__ P,
-                                                       const unsigned short* __restrict__ Pb,
-                                                       unsigned short* __restrict__ d_o,
-                                                       unsigned short* __restrict__ d_h2,
-                                                       unsigned short* __restrict__ d_h1) {
-  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
-  constexpr int NPP = NP + 4;                  // padded bf16 row of go (distinct banks per row)
-  static_assert(H / 16 == WPB, "one column tile of gh2 / gh1 per wave");
-  const LayerGeom& g = A.geo.layer[A.layer];
-  const int D = A.geo.D;
+template <int NBM>
+__global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) k_flow_bwd(BwdArgs A, const float* __restrict__ P,
+                                                   const unsigned short* __restrict__ Pb) {
+  constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
+  constexpr int NPP = NPM + 8;                 // padded bf16 row of go (16-byte rows, distinct banks)
+  static_assert(H / 16 == 2 * WPB_B, "two column tiles of gh2 / gh1 per wave");
+  static_assert(TILE * DMAX == NT_B, "one coordinate per thread");
+  const int D = A.geo.D, L = A.geo.L;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int r = lane & 15, kq = (lane >> 4) * 4;   // bf16 MFMA: four consecutive k per lane
-  const int n0 = 16 * wid;                         // this wave's column tile of gh2 / gh1
-  const int64_t tile = blockIdx.x, s0 = tile * TILE;
+  const int k8 = (lane >> 4) * 8;                  // ... eight in the 16-byte loads of mfma_k32
+  const int64_t s0 = (int64_t)blockIdx.x * TILE;
   __shared__ float xs[TILE * DMAX], gxs[TILE * DMAX];
   __shared__ __attribute__((aligned(16))) unsigned short h1b[TILE * WP], h2b[TILE * WP], ghb[TILE * WP];
   __shared__ __attribute__((aligned(16))) unsigned short gob[TILE * NPP];
-  __shared__ float go[TILE * NP], gh1[TILE * H], W1s[DMAX * H];
-  for (int q = tid; q < TILE * DMAX; q += NT) {
-    const int s = q / DMAX, d = q % DMAX;
-    const int64_t e = s0 + s;
-    const bool ok = e < A.n && d < D;
-    xs[q] = ok ? A.x_in[e * D + d] : 0.5f;
-    gxs[q] = ok ? A.gx[e * D + d] : 0.f;
-  }
-  for (int q = tid; q < g.nA * H; q += NT) W1s[q] = P[g.oW1 + q];
-  for (int q = tid; q < TILE * H; q += NT) {
-    const int j = q / TILE, s = q % TILE;        // saved activations are tile-transposed
-    h1b[s * WP + j] = A.h1[s0 * H + q];
-    h2b[s * WP + j] = A.h2[s0 * H + q];
-  }
-  for (int q = tid; q < TILE * NP; q += NT) go[q] = 0.f;
-  __syncthreads();
-  // spline backward: d raw (into go) and d x_B (replaces the incoming gradient of x_B),
-  // written straight into LDS (a private array indexed by the bin would live in scratch)
-  for (int q = tid; q < TILE * NB; q += NT) {
-    const int s = q / NB, b = q % NB;
-    const int64_t e = s0 + s;
-    if (e >= A.n) continue;
-    const int d = g.B[b];
-    gxs[s * DMAX + d] = rq_backward(A.raw + e * A.geo.NPmax + b * NPD, xs[s * DMAX + d], gxs[s * DMAX + d],
-                                    A.gl[e], go + s * NP + b * NPD);
-  }
-  __syncthreads();
-  // go in bf16: the A operand of gh2 (LDS) and the delta of dW3 / b3 (global, tile-transposed)
-  for (int q = tid; q < TILE * NP; q += NT) {
-    const int c = q / TILE, s = q % TILE;
-    const unsigned short v = f2bf(go[s * NP + c]);
-    gob[s * NPP + c] = v;
-    d_o[s0 * NP + q] = v;
-  }
-  __syncthreads();
-  // gh2 = (go W3^T) * relu'(h2): B[k = np][n = h] = W3[h][np], the row h of W3n
+  // go: the tile's spline parameters, overwritten in place by their gradient
+  __shared__ __attribute__((aligned(16))) float go[TILE * NPM], gh1[TILE * H], W1s[DMAX * H];
   {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k0 = 0; k0 < NP; k0 += 16)
-      acc = mfma_bf16(ld_bf16x4(gob + r * NPP + k0 + kq), ld_bf16x4(Pb + g.oW3n + (n0 + r) * NP + k0 + kq), acc);
-    unsigned short out[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = kq + i, col = n0 + r;
-      const unsigned short v = f2bf(bf2f(h2b[row * WP + col]) > 0.f ? acc[i] : 0.f);
-      ghb[row * WP + col] = v;
-      out[i] = v;
-    }
-    // the four samples of column n0 + r: one 8-byte store
-    *reinterpret_cast<uint2*>(d_h2 + s0 * H + (n0 + r) * TILE + kq) =
-        make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
-  }
-  __syncthreads();
-  // gh1 = (gh2 W2^T) * relu'(h1): B[k = o][n = i] = W2[i][o], the row i of W2n
-  {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k0 = 0; k0 < H; k0 += 16)
-      acc = mfma_bf16(ld_bf16x4(ghb + r * WP + k0 + kq), ld_bf16x4(Pb + g.oW2n + (n0 + r) * H + k0 + kq), acc);
-    unsigned short out[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = kq + i, col = n0 + r;
-      const float v = bf2f(h1b[row * WP + col]) > 0.f ? acc[i] : 0.f;
-      gh1[row * H + col] = v;
-      out[i] = f2bf(v);
-    }
-    *reinterpret_cast<uint2*>(d_h1 + s0 * H + (n0 + r) * TILE + kq) =
-        make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
-  }
-  __syncthreads();
-  // d x_A += gh1 W1^T (fp32): 8 lanes per (sample, input), 16 terms each, summed by shuffles
-  // within aligned groups of 8 lanes (the summation order is fixed)
-  for (int q0 = 0; q0 < TILE * DMAX * 8; q0 += NT) {
-    const int q = q0 + tid, o = q >> 3, part = q & 7;
-    const int s = o / DMAX, i = o % DMAX;
-    float v = 0.f;
-    if (i < g.nA) {
-#pragma unroll
-      for (int j = 0; j < H / 8; ++j) v += gh1[s * H + part * (H / 8) + j] * W1s[i * H + part * (H / 8) + j];
-    }
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    if (part == 0 && i < g.nA) gxs[s * DMAX + g.A[i]] += v;
-  }
-  __syncthreads();
-  for (int q = tid; q < TILE * D; q += NT) {
-    const int s = q / D, d = q % D;
+    const int s = tid / DMAX, d = tid % DMAX;
     const int64_t e = s0 + s;
-    if (e < A.n) A.gx[e * D + d] = gxs[s * DMAX + d];
+    gxs[tid] = (A.gx_in && e < A.n && d < D) ? A.gx_in[e * D + d] : 0.f;
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    const LayerGeom& g = A.geo.layer[l];
+    const int nB = g.nB, NP = g.NP;
+    const char* base = A.save + A.sl.bytes * l;
+    const float* sx = reinterpret_cast<const float*>(base);
+    const float* sraw = reinterpret_cast<const float*>(base + A.sl.raw);
+    const unsigned short* sh1 = reinterpret_cast<const unsigned short*>(base + A.sl.h1);
+    const unsigned short* sh2 = reinterpret_cast<const unsigned short*>(base + A.sl.h2);
+    char* dbase = A.delta + A.dl.bytes * l;
+    unsigned short* d_o = reinterpret_cast<unsigned short*>(dbase + A.dl.o);
+    unsigned short* d_h2 = reinterpret_cast<unsigned short*>(dbase + A.dl.h2);
+    unsigned short* d_h1 = reinterpret_cast<unsigned short*>(dbase + A.dl.h1);
+    __syncthreads();                             // the previous layer is done with the LDS
+    // the loads of this layer's state, unrolled to their bounds so that they are all in
+    // flight together
+    {
+      const int s = tid / DMAX, d = tid % DMAX;
+      const int64_t e = s0 + s;
+      xs[tid] = (e < A.n && d < D) ? sx[e * D + d] : 0.5f;
+    }
+#pragma unroll
+    for (int u = 0; u < DMAX * H / 4 / NT_B; ++u) {
+      const int q = tid + u * NT_B;            // four floats of W1 (g.oW1 is a multiple of 4)
+      if (q < g.nA * H / 4) *reinterpret_cast<float4*>(W1s + 4 * q) = *reinterpret_cast<const float4*>(P + g.oW1 + 4 * q);
+    }
+#pragma unroll
+    for (int u = 0; u < TILE * H / 2 / NT_B; ++u) {
+      const int q = tid + u * NT_B;            // samples s, s + 1 of column j: one 4-byte load
+      const int j = (2 * q) / TILE, s = (2 * q) % TILE;   // saved activations are tile-transposed
+      const unsigned v1 = *reinterpret_cast<const unsigned*>(sh1 + s0 * H + 2 * q);
+      const unsigned v2 = *reinterpret_cast<const unsigned*>(sh2 + s0 * H + 2 * q);
+      h1b[s * WP + j] = (unsigned short)v1;
+      h1b[(s + 1) * WP + j] = (unsigned short)(v1 >> 16);
+      h2b[s * WP + j] = (unsigned short)v2;
+      h2b[(s + 1) * WP + j] = (unsigned short)(v2 >> 16);
+    }
+    // spline parameters of the transformed dimensions, four at a time; the padding columns
+    // (and samples past n) are zero and stay zero in the gradient
+    const int nraw = nB * NPD, np4 = NP / 4;
+#pragma unroll
+    for (int u = 0; u < (TILE * NPM / 4 + NT_B - 1) / NT_B; ++u) {
+      const int q = tid + u * NT_B;
+      if (q < TILE * np4) {
+        const int s = q / np4, c = 4 * (q % np4);
+        const int64_t e = s0 + s;
+        float4 v = e < A.n ? *reinterpret_cast<const float4*>(sraw + e * A.geo.NPmax + c) : float4{0.f, 0.f, 0.f, 0.f};
+        if (c + 0 >= nraw) v.x = 0.f;
+        if (c + 1 >= nraw) v.y = 0.f;
+        if (c + 2 >= nraw) v.z = 0.f;
+        if (c + 3 >= nraw) v.w = 0.f;
+        *reinterpret_cast<float4*>(go + s * NPM + c) = v;
+      }
+    }
+    __syncthreads();
+    // spline backward: d raw (in place in go) and d x_B (replaces the incoming gradient of
+    // x_B); each item reads its parameters before it writes their gradient
+    for (int q = tid; q < TILE * nB; q += NT_B) {
+      const int s = q / nB, b = q % nB;
+      const int64_t e = s0 + s;
+      if (e >= A.n) continue;
+      const int d = g.B[b];
+      float* gr = go + s * NPM + b * NPD;
+      gxs[s * DMAX + d] = rq_backward(gr, xs[s * DMAX + d], gxs[s * DMAX + d], A.gl[e], gr);
+    }
+    __syncthreads();
+    // go in bf16: the A operand of gh2 (LDS) and the delta of dW3 / b3 (tile-transposed)
+    for (int q = tid; q < TILE * NP; q += NT_B) {
+      const int c = q / TILE, s = q % TILE;
+      const unsigned short v = f2bf(go[s * NPM + c]);
+      gob[s * NPP + c] = v;
+      d_o[s0 * NP + q] = v;
+    }
+    __syncthreads();
+    // gh2 = (go W3^T) * relu'(h2): B[k = np][n = h] = W3[h][np], the row h of W3n; wave w
+    // computes the column tiles w and w + 4 (one A operand, two B operands per k step)
+    {
+      f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+      const unsigned short* w3[2] = {Pb + g.oW3n + (16 * wid + r) * NP, Pb + g.oW3n + (16 * (wid + WPB_B) + r) * NP};
+#pragma unroll
+      for (int K0 = 0; K0 + 32 <= NPM; K0 += 32)
+        if (K0 + 32 <= NP) {
+          const bf16x8 a = ld_bf16x8(gob + r * NPP + K0 + k8);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) acc[u] = mfma_k32(a, ld_bf16x8(w3[u] + K0 + k8), acc[u]);
+        }
+      if (NP % 32) {                               // NP = 32 m + 16: the last 16 k
+        const int k0 = NP - 16;
+        const bf16x4 a = ld_bf16x4(gob + r * NPP + k0 + kq);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[u] = mfma_bf16(a, ld_bf16x4(w3[u] + k0 + kq), acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int n0 = 16 * (wid + WPB_B * u);
+        unsigned short out[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = kq + i, col = n0 + r;
+          const unsigned short v = f2bf(bf2f(h2b[row * WP + col]) > 0.f ? acc[u][i] : 0.f);
+          ghb[row * WP + col] = v;
+          out[i] = v;
+        }
+        // the four samples of column n0 + r: one 8-byte store
+        *reinterpret_cast<uint2*>(d_h2 + s0 * H + (n0 + r) * TILE + kq) =
+            make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
+      }
+    }
+    __syncthreads();
+    // gh1 = (gh2 W2^T) * relu'(h1): B[k = o][n = i] = W2[i][o], the row i of W2n
+    {
+      f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int K0 = 0; K0 < H; K0 += 32) {
+        const bf16x8 a = ld_bf16x8(ghb + r * WP + K0 + k8);
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          acc[u] = mfma_k32(a, ld_bf16x8(Pb + g.oW2n + (16 * (wid + WPB_B * u) + r) * H + K0 + k8), acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int n0 = 16 * (wid + WPB_B * u);
+        unsigned short out[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = kq + i, col = n0 + r;
+          const float v = bf2f(h1b[row * WP + col]) > 0.f ? acc[u][i] : 0.f;
+          gh1[row * H + col] = v;
+          out[i] = f2bf(v);
+        }
+        *reinterpret_cast<uint2*>(d_h1 + s0 * H + (n0 + r) * TILE + kq) =
+            make_uint2((unsigned)out[0] | ((unsigned)out[1] << 16), (unsigned)out[2] | ((unsigned)out[3] << 16));
+      }
+    }
+    __syncthreads();
+    // d x_A += gh1 W1^T (fp32): 8 lanes per (sample, input), 16 terms each, summed by
+    // shuffles within aligned groups of 8 lanes (the summation order is fixed)
+    for (int q0 = 0; q0 < TILE * DMAX * 8; q0 += NT_B) {
+      const int q = q0 + tid, o = q >> 3, part = q & 7;
+      const int s = o / DMAX, i = o % DMAX;
+      float v = 0.f;
+      if (i < g.nA) {
+#pragma unroll
+        for (int j = 0; j < H / 8; ++j) v += gh1[s * H + part * (H / 8) + j] * W1s[i * H + part * (H / 8) + j];
+      }
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      if (part == 0 && i < g.nA) gxs[s * DMAX + g.A[i]] += v;
+    }
+  }
+  __syncthreads();
+  if (A.gx_out && tid < TILE * D) {
+    const int s = tid / D, d = tid % D;
+    const int64_t e = s0 + s;
+    if (e < A.n) A.gx_out[e * D + d] = gxs[s * DMAX + d];
   }
 }
 
-template <int NB>
-__global__ void __launch_bounds__(NT) k_layer_wgrad(BwdArgs A, const unsigned short* __restrict__ d_o,
-                                                    const unsigned short* __restrict__ d_h2,
-                                                    const unsigned short* __restrict__ d_h1) {
-  constexpr int NP = ((NB * NPD + 15) / 16) * 16;
-  constexpr int NT3 = NP / 16;
-  constexpr int J3 = (NT3 + WPB - 1) / WPB;
+constexpr int WG_CHUNK = 4;     // tiles whose operands a weight-gradient block loads together
+
+template <int NBM>
+__global__ void __launch_bounds__(NT) k_flow_wgrad(BwdArgs A) {
+  constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
+  constexpr int J3 = (NPM / 16 + WPB - 1) / WPB;
   static_assert(DMAX == 16, "dW1 is one 16-row tile");
-  const LayerGeom& g = A.geo.layer[A.layer];
+  const int l = blockIdx.z;
+  const LayerGeom& g = A.geo.layer[l];
+  const int NP = g.NP, NT3 = NP / 16;
   const int D = A.geo.D;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int r = lane & 15, kq = (lane >> 4) * 4;
+  const char* base = A.save + A.sl.bytes * l;
+  const float* sx = reinterpret_cast<const float*>(base);
+  const unsigned short* sh1 = reinterpret_cast<const unsigned short*>(base + A.sl.h1);
+  const unsigned short* sh2 = reinterpret_cast<const unsigned short*>(base + A.sl.h2);
+  const char* dbase = A.delta + A.dl.bytes * l;
+  const unsigned short* d_o = reinterpret_cast<const unsigned short*>(dbase + A.dl.o);
+  const unsigned short* d_h2 = reinterpret_cast<const unsigned short*>(dbase + A.dl.h2);
+  const unsigned short* d_h1 = reinterpret_cast<const unsigned short*>(dbase + A.dl.h1);
   const int64_t ntiles = (A.n + TILE - 1) / TILE;
-  const int64_t t0 = (int64_t)blockIdx.x * WG_TILES;
-  const int nt = (int)(t0 + WG_TILES < ntiles ? WG_TILES : ntiles - t0);   // tiles of this split
-  const int nlp = g.ob3 + NP - g.oW1;
-  float* part = A.partial + (size_t)blockIdx.x * nlp;
-  // a wave whose second dW3 column tile does not exist repeats its last one, unstored
+  const int64_t t0 = (int64_t)blockIdx.x * A.split_tiles;
+  const int nt = (int)(t0 + A.split_tiles < ntiles ? A.split_tiles : ntiles - t0);   // tiles of this split
+  float* part = A.partial + (size_t)blockIdx.x * A.geo.nparams + g.oW1;
+  // a wave whose later dW3 column tiles do not exist repeats its last one, unstored
   int jc[J3];
 #pragma unroll
   for (int u = 0; u < J3; ++u) jc[u] = wid + WPB * u < NT3 ? wid + WPB * u : NT3 - 1;
   f32x4 acc3[J3], acc2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int u = 0; u < J3; ++u) acc3[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x4 zero = {0, 0, 0, 0};
   if (blockIdx.y < H / 16) {
     const int m0 = 16 * blockIdx.y;
-    // every operand of the split is loaded before the MFMAs (tiles past the end repeat the
-    // last one with a zero A operand)
-    bf16x4 a3[WG_TILES], a2[WG_TILES], b2[WG_TILES], b3[WG_TILES][J3];
+    for (int c0 = 0; c0 < nt; c0 += WG_CHUNK) {
+      // the operands of WG_CHUNK tiles are loaded before their MFMAs (tiles past the end
+      // of the split repeat its last one with a zero A operand)
+      bf16x4 a3[WG_CHUNK], a2[WG_CHUNK], b2[WG_CHUNK], b3[WG_CHUNK][J3];
 #pragma unroll
-    for (int t = 0; t < WG_TILES; ++t) {
-      const int64_t tt = t0 + (t < nt ? t : nt - 1);
-      a3[t] = ld_bf16x4(A.h2 + tt * TILE * H + (m0 + r) * TILE + kq);
-      a2[t] = ld_bf16x4(A.h1 + tt * TILE * H + (m0 + r) * TILE + kq);
-      b2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
+      for (int t = 0; t < WG_CHUNK; ++t) {
+        const int64_t tt = t0 + (c0 + t < nt ? c0 + t : nt - 1);
+        a3[t] = ld_bf16x4(sh2 + tt * TILE * H + (m0 + r) * TILE + kq);
+        a2[t] = ld_bf16x4(sh1 + tt * TILE * H + (m0 + r) * TILE + kq);
+        b2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
 #pragma unroll
-      for (int u = 0; u < J3; ++u) b3[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
-    }
-    const bf16x4 zero = {0, 0, 0, 0};
+        for (int u = 0; u < J3; ++u) b3[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
+      }
 #pragma unroll
-    for (int t = 0; t < WG_TILES; ++t) {
-      const bf16x4 x3 = t < nt ? a3[t] : zero, x2 = t < nt ? a2[t] : zero;
+      for (int t = 0; t < WG_CHUNK; ++t) {
+        const bool ok = c0 + t < nt;
+        const bf16x4 x3 = ok ? a3[t] : zero, x2 = ok ? a2[t] : zero;
 #pragma unroll
-      for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(x3, b3[t][u], acc3[u]);
-      acc2 = mfma_bf16(x2, b2[t], acc2);
+        for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(x3, b3[t][u], acc3[u]);
+        acc2 = mfma_bf16(x2, b2[t], acc2);
+      }
     }
 #pragma unroll
     for (int u = 0; u < J3; ++u) {
@@ -812,46 +741,43 @@ __global__ void __launch_bounds__(NT) k_layer_wgrad(BwdArgs A, const unsigned sh
     // bias sums: ones^T B, every row of which is the column sum of B (row 0 is stored)
     const short one = (short)0x3f80;            // 1.0 in bf16
     f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, accw1 = {0.f, 0.f, 0.f, 0.f};
-    bf16x4 bh1[WG_TILES], bh2[WG_TILES], bo[WG_TILES][J3];
-#pragma unroll
-    for (int t = 0; t < WG_TILES; ++t) {
-      const int64_t tt = t0 + (t < nt ? t : nt - 1);
-      bh1[t] = ld_bf16x4(d_h1 + tt * TILE * H + (16 * wid + r) * TILE + kq);
-      bh2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
-#pragma unroll
-      for (int u = 0; u < J3; ++u) bo[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
-    }
-#pragma unroll
-    for (int t = 0; t < WG_TILES; ++t) {
-      const bf16x4 ones = t < nt ? bf16x4{one, one, one, one} : bf16x4{0, 0, 0, 0};
-      acc1 = mfma_bf16(ones, bh1[t], acc1);
-      acc2 = mfma_bf16(ones, bh2[t], acc2);
-#pragma unroll
-      for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(ones, bo[t][u], acc3[u]);
-    }
     // dW1 = x_A^T gh1 on the fp32 matrix cores (A[r][k] = x of sample k, input r < nA)
     const int kf = lane >> 4;                    // fp32 MFMA: one k per lane quarter
     const bool xrow = r < g.nA;
     const int da = xrow ? g.A[r] : 0;
-    float xv[WG_TILES][4], gv[WG_TILES][4];
+    for (int c0 = 0; c0 < nt; c0 += WG_CHUNK) {
+      bf16x4 bh1[WG_CHUNK], bh2[WG_CHUNK], bo[WG_CHUNK][J3];
+      float xv[WG_CHUNK][4], gv[WG_CHUNK][4];
 #pragma unroll
-    for (int t = 0; t < WG_TILES; ++t) {
-      const int64_t tt = t0 + (t < nt ? t : nt - 1);
+      for (int t = 0; t < WG_CHUNK; ++t) {
+        const int64_t tt = t0 + (c0 + t < nt ? c0 + t : nt - 1);
+        bh1[t] = ld_bf16x4(d_h1 + tt * TILE * H + (16 * wid + r) * TILE + kq);
+        bh2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int64_t e = tt * TILE + 4 * k + kf;
-        xv[t][k] = A.x_in[(e < A.n ? e : A.n - 1) * D + da];
-        gv[t][k] = bf2f(d_h1[tt * TILE * H + (16 * wid + r) * TILE + 4 * k + kf]);
+        for (int u = 0; u < J3; ++u) bo[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t e = tt * TILE + 4 * k + kf;
+          xv[t][k] = sx[(e < A.n ? e : A.n - 1) * D + da];
+          gv[t][k] = bf2f(d_h1[tt * TILE * H + (16 * wid + r) * TILE + 4 * k + kf]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < WG_CHUNK; ++t) {
+        const bool ok = c0 + t < nt;
+        const bf16x4 ones = ok ? bf16x4{one, one, one, one} : zero;
+        acc1 = mfma_bf16(ones, bh1[t], acc1);
+        acc2 = mfma_bf16(ones, bh2[t], acc2);
+#pragma unroll
+        for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(ones, bo[t][u], acc3[u]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t e = (t0 + c0 + t) * TILE + 4 * k + kf;
+          const bool okx = ok && e < A.n && xrow;
+          accw1 = mfma_f32(okx ? xv[t][k] : 0.f, gv[t][k], accw1);
+        }
       }
     }
-#pragma unroll
-    for (int t = 0; t < WG_TILES; ++t)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int64_t e = (t0 + t) * TILE + 4 * k + kf;
-        const bool ok = t < nt && e < A.n && xrow;
-        accw1 = mfma_f32(ok ? xv[t][k] : 0.f, gv[t][k], accw1);
-      }
     // fp32 MFMA output: D[4 kf + i][r] in element i
 #pragma unroll
     for (int i = 0; i < 4; ++i) part[(4 * kf + i) * H + 16 * wid + r] = accw1[i];
@@ -867,8 +793,8 @@ __global__ void __launch_bounds__(NT) k_layer_wgrad(BwdArgs A, const unsigned sh
   }
 }
 
-// fixed-order reduction of the per-block partials into the gradient vector
-// Sum of the per-block partial gradients: 64 parameters per block, the partials of each
+
+// Sum of the per-split partial gradients: 64 parameters per block, the partials of each
 // split over 4 waves (Kahan sums over blocks w, w+4, ...), combined in a fixed order in
 // LDS, so the result does not depend on scheduling.
 __global__ void __launch_bounds__(256) k_reduce_grad(const float* __restrict__ partial, int nblocks, int nlp,
@@ -921,15 +847,9 @@ __global__ void k_minibatch(const double* __restrict__ cdf, int64_t n, const dou
                             double* __restrict__ fb) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= batch) return;
-  const double u = r[i];
-  int64_t lo = 0, hi = n - 1;          // invariant: the answer lies in [lo, hi]
-  while (lo < hi) {
-    const int64_t mid = lo + (hi - lo) / 2;
-    if (cdf[mid] >= u) hi = mid;
-    else lo = mid + 1;
-  }
-  for (int d = 0; d < D; ++d) yb[i * D + d] = y[lo * D + d];
-  if (f) fb[i] = f[lo];
+  const int64_t k = cdf_search(cdf, n, r[i]);
+  for (int d = 0; d < D; ++d) yb[i * D + d] = y[k * D + d];
+  if (f) fb[i] = f[k];
 }
 
 // Per-sample seed of the backward pass for the self-normalised objective
@@ -1004,100 +924,78 @@ __global__ void k_cast(FlowGeom geo, const float* __restrict__ P, unsigned short
   }
 }
 
+// Adam step on the fp32 master parameters of every layer (block row y = layer) together with
+// the bf16 copies of the updated hidden/output weights (the same values k_cast writes).
+__global__ void k_adam_cast(FlowGeom geo, float* __restrict__ P, const float* __restrict__ grad,
+                            float* __restrict__ m1, float* __restrict__ m2, float lr, float b1, float b2, float eps,
+                            float bc1, float bc2, float clip, unsigned short* __restrict__ Pb) {
+  const LayerGeom& g = geo.layer[blockIdx.y];
+  const int nlp = g.ob3 + g.NP - g.oW1;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nlp; q += gridDim.x * blockDim.x) {
+    const int i = g.oW1 + q;
+    float gi = grad[i];
+    if (!isfinite(gi)) gi = 0.f;
+    gi = fminf(fmaxf(gi, -clip), clip);
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float p = P[i] - lr * (a / bc1) / (sqrtf(b / bc2) + eps);
+    P[i] = p;
+    if (i >= g.oW2 && i < g.ob2) {
+      const int qq = i - g.oW2, in = qq / H, o = qq % H;   // W2[in][o]
+      Pb[g.oW2n + qq] = f2bf(p);
+      Pb[g.oW2t + o * H + in] = f2bf(p);
+    } else if (i >= g.oW3 && i < g.ob3) {
+      const int qq = i - g.oW3, in = qq / g.NP, o = qq % g.NP;   // W3[in][o]
+      Pb[g.oW3n + qq] = f2bf(p);
+      Pb[g.oW3t + o * H + in] = f2bf(p);
+    }
+  }
+}
+
 }  // namespace flow
 }  // namespace mfx
 
 using namespace mfx;
 using namespace mfx::flow;
 
-// Layers with 1..NB_TEMPLATED transformed dimensions run the kernels compiled for their
-// geometry (NP a constant); wider layers the run-time-geometry kernels.
-constexpr int NB_TEMPLATED = 8;
-
-static void launch_layer(int nb, dim3 grid, size_t shmem, hipStream_t s, const FwdArgs& A, const float* P,
-                         const unsigned short* Pb) {
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-    cus = 256;
-  if (nb >= 1 && nb <= NB_LDS && grid.x <= 8u * (unsigned)cus) {
-    // training-size batches: persistent blocks with the weights in LDS, one per CU at
-    // most, each running a few tiles (the weights are read once per block).  Large
-    // batches (sampling) keep one block per tile, which holds more waves per SIMD.
-    const dim3 gp(grid.x < (unsigned)cus ? grid.x : (unsigned)cus);
-    switch (nb) {
-      case 1: hipLaunchKernelGGL(k_layer_lds<1>, gp, dim3(NT), 0, s, A, P, Pb); break;
-      case 2: hipLaunchKernelGGL(k_layer_lds<2>, gp, dim3(NT), 0, s, A, P, Pb); break;
-      case 3: hipLaunchKernelGGL(k_layer_lds<3>, gp, dim3(NT), 0, s, A, P, Pb); break;
-      case 4: hipLaunchKernelGGL(k_layer_lds<4>, gp, dim3(NT), 0, s, A, P, Pb); break;
-      case 5: hipLaunchKernelGGL(k_layer_lds<5>, gp, dim3(NT), 0, s, A, P, Pb); break;
-      default: hipLaunchKernelGGL(k_layer_lds<6>, gp, dim3(NT), 0, s, A, P, Pb); break;
-    }
-    return;
-  }
-  switch (nb) {
-    case 1: hipLaunchKernelGGL(k_layer<1>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 2: hipLaunchKernelGGL(k_layer<2>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 3: hipLaunchKernelGGL(k_layer<3>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 4: hipLaunchKernelGGL(k_layer<4>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 5: hipLaunchKernelGGL(k_layer<5>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 6: hipLaunchKernelGGL(k_layer<6>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 7: hipLaunchKernelGGL(k_layer<7>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    case 8: hipLaunchKernelGGL(k_layer<8>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-    default: hipLaunchKernelGGL(k_layer<0>, grid, dim3(NT), shmem, s, A, P, Pb); break;
-  }
-}
-
-// The backward of a layer with 1..NB_TEMPLATED transformed dimensions: the data-gradient
-// kernel (one block per 16-sample tile), then the weight-gradient GEMMs (one block row per
-// WG_SPLIT samples).  The deltas live at the start of the partial workspace, the
-// per-split partials after them.
-// The backward of a layer with 1..NB_TEMPLATED transformed dimensions: the data-gradient
-// kernel (one block per 16-sample tile), then the weight-gradient GEMMs (one block row per
-// WG_TILES tiles).  The deltas live at the start of the partial workspace, the per-split
-// partials after them.
-static void launch_layer_bwd_split(int nb, int64_t n, hipStream_t s, const BwdArgs& A, const float* P,
-                                   const unsigned short* Pb, unsigned short* d_o, unsigned short* d_h2,
-                                   unsigned short* d_h1) {
-  const dim3 gd((unsigned)((n + TILE - 1) / TILE)), gw((unsigned)((n + WG_SPLIT - 1) / WG_SPLIT), H / 16 + 1);
-#define MFX_BWD_CASE(K)                                                                                    \
-  case K:                                                                                                 \
-    hipLaunchKernelGGL(k_layer_bwd_data<K>, gd, dim3(NT), 0, s, A, P, Pb, d_o, d_h2, d_h1);               \
-    hipLaunchKernelGGL(k_layer_wgrad<K>, gw, dim3(NT), 0, s, A, (const unsigned short*)d_o,                \
-                       (const unsigned short*)d_h2, (const unsigned short*)d_h1);                         \
-    break;
-  switch (nb) {
-    MFX_BWD_CASE(1)
-    MFX_BWD_CASE(2)
-    MFX_BWD_CASE(3)
-    MFX_BWD_CASE(4)
-    MFX_BWD_CASE(5)
-    MFX_BWD_CASE(6)
-    MFX_BWD_CASE(7)
-    default: MFX_BWD_CASE(8)
-  }
-#undef MFX_BWD_CASE
-}
-
 static int64_t ntiles_of(int64_t n) { return (n + TILE - 1) / TILE; }
-static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
-// Saved state of one layer (training), each piece 256-byte aligned: x (n, D) and raw
-// (n, NPmax) in f32, then the bf16 activations h1 and h2 tile-transposed over whole tiles.
-struct SaveLayout {
-  size_t raw, h1, h2, bytes;   // byte offsets from the layer's base, and the layer's size
-};
+static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
 static SaveLayout save_layout(const FlowGeom& G, int64_t n) {
   SaveLayout s;
-  s.raw = align256((size_t)n * G.D * 4);
-  s.h1 = s.raw + align256((size_t)n * G.NPmax * 4);
-  s.h2 = s.h1 + (size_t)ntiles_of(n) * TILE * H * 2;
-  s.bytes = s.h2 + (size_t)ntiles_of(n) * TILE * H * 2;
+  s.raw = align256(n * G.D * 4);
+  s.h1 = s.raw + align256(n * G.NPmax * 4);
+  s.h2 = s.h1 + align256(ntiles_of(n) * TILE * H * 2);
+  s.bytes = s.h2 + align256(ntiles_of(n) * TILE * H * 2);
   return s;
 }
-static size_t save_per_layer(const FlowGeom& G, int64_t n) { return save_layout(G, n).bytes; }
+
+static DeltaLayout delta_layout(const FlowGeom& G, int64_t n) {
+  const int64_t nt16 = ntiles_of(n) * TILE;
+  DeltaLayout d;
+  d.o = 0;
+  d.h2 = align256(nt16 * G.NPmax * 2);
+  d.h1 = d.h2 + align256(nt16 * H * 2);
+  d.bytes = d.h1 + align256(nt16 * H * 2);
+  return d;
+}
+
+// Tiles per weight-gradient split: about 32 splits (enough blocks for the chip with the 9
+// block rows and the layers), at least WG_CHUNK tiles each.
+static int split_tiles_of(int64_t n) {
+  const int64_t t = (ntiles_of(n) + 31) / 32;
+  return (int)(t < WG_CHUNK ? WG_CHUNK : (t > 4096 ? 4096 : t));
+}
+static int64_t nsplit_of(int64_t n) {
+  const int st = split_tiles_of(n);
+  return (ntiles_of(n) + st - 1) / st;
+}
 
 static int make_geom(int D, int L, const int* masks, FlowGeom& G) {
   if (D < 2 || D > DMAX || L < 1 || L > 16) return fail(-2, "flow needs 2 <= D <= 16 and 1 <= L <= 16%s", "");
+  if (!masks) return fail(-3, "null masks%s", "");
   G.D = D;
   G.L = L;
   int np = 0, nb = 0, npmax = 0;
@@ -1130,6 +1028,43 @@ static int make_geom(int D, int L, const int* masks, FlowGeom& G) {
   return 0;
 }
 
+// The kernels are compiled for transformed sets of up to 4, 8 and 16 dimensions (the LDS
+// arrays are sized for the bound); a flow runs the smallest that holds its widest layer.
+static int nb_bound(const FlowGeom& G) {
+  int m = 0;
+  for (int l = 0; l < G.L; ++l) m = G.layer[l].nB > m ? G.layer[l].nB : m;
+  return m <= 4 ? 4 : (m <= 8 ? 8 : 16);
+}
+
+static void launch_flow(const FlowArgs& A, hipStream_t s, const float* P, const unsigned short* Pb) {
+  const dim3 grid((unsigned)ntiles_of(A.n));
+  switch (nb_bound(A.geo)) {
+    case 4: hipLaunchKernelGGL(k_flow<4>, grid, dim3(NT), 0, s, A, P, Pb); break;
+    case 8: hipLaunchKernelGGL(k_flow<8>, grid, dim3(NT), 0, s, A, P, Pb); break;
+    default: hipLaunchKernelGGL(k_flow<16>, grid, dim3(NT), 0, s, A, P, Pb); break;
+  }
+}
+
+static FlowArgs flow_args(const FlowGeom& G, int64_t n) {
+  FlowArgs A;
+  A.geo = G;
+  A.n = n;
+  A.x_in = nullptr;
+  A.cdf = nullptr;
+  A.npool = 0;
+  A.pool_y = nullptr;
+  A.pool_f = nullptr;
+  A.seed = A.draw = 0;
+  A.y_out = nullptr;
+  A.f_out = nullptr;
+  A.x_out = nullptr;
+  A.logdet = nullptr;
+  A.save = nullptr;
+  A.sl = save_layout(G, n);
+  A.inverse = 0;
+  return A;
+}
+
 extern "C" {
 
 int mfx_flow_sizes(int D, int L, const int* masks, int64_t* sizes) {
@@ -1151,6 +1086,7 @@ int mfx_flow_cast(int D, int L, const int* masks, const float* params, uint16_t*
   FlowGeom G;
   int rc = make_geom(D, L, masks, G);
   if (rc) return rc;
+  if (!params || !params_bf16) return fail(-3, "null pointer%s", "");
   hipLaunchKernelGGL(k_cast, dim3(64, L), dim3(256), 0, (hipStream_t)stream, G, params, params_bf16);
   return hip_status(hipGetLastError());
 }
@@ -1163,66 +1099,53 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
   if (n < 0) return fail(-2, "n must be >= 0%s", "");
   if (n == 0) return 0;
   if (!params || !params_bf16 || !x_in || !x_out || !logdet) return fail(-3, "null pointer%s", "");
-  hipStream_t s = (hipStream_t)stream;
-  const int nblk = (int)((n + TILE - 1) / TILE);
-  // per layer saved state (training): x_in (n, D), raw (n, NPmax) f32, then h1, h2 bf16
-  // tile-transposed over whole tiles
-  const size_t per_layer = save_per_layer(G, n);
-  const float* cur = x_in;
-  for (int step = 0; step < L; ++step) {
-    const int l = inverse ? L - 1 - step : step;
-    FwdArgs A;
-    A.geo = G;
-    A.layer = l;
-    A.n = n;
-    A.x_in = cur;
-    A.x_out = x_out;
-    A.logdet = logdet;
-    A.inverse = inverse;
-    A.save_x = nullptr;
-    A.save_h1 = A.save_h2 = nullptr;
-    A.save_raw = nullptr;
-    if (save && !inverse) {
-      char* base = (char*)save + per_layer * l;
-      const SaveLayout sl = save_layout(G, n);
-      A.save_x = (float*)base;   // the kernel keeps the layer input for the backward pass
-      A.save_raw = (float*)(base + sl.raw);
-      A.save_h1 = (unsigned short*)(base + sl.h1);
-      A.save_h2 = (unsigned short*)(base + sl.h2);
-    }
-    const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-    launch_layer(G.layer[l].nB, dim3(nblk), shmem, s, A, params, (const unsigned short*)params_bf16);
-    cur = x_out;
-  }
+  FlowArgs A = flow_args(G, n);
+  A.x_in = x_in;
+  A.x_out = x_out;
+  A.logdet = logdet;
+  A.inverse = inverse ? 1 : 0;
+  A.save = (save && !inverse) ? (char*)save : nullptr;
+  launch_flow(A, (hipStream_t)stream, params, (const unsigned short*)params_bf16);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_draw_forward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16,
+                          const double* cdf, int64_t npool, const float* pool_y, const double* pool_f, uint32_t seed,
+                          uint32_t draw, int64_t batch, float* y_out, double* f_out, float* logdet, float* save,
+                          void* stream) {
+  FlowGeom G;
+  int rc = make_geom(D, L, masks, G);
+  if (rc) return rc;
+  if (npool <= 0 || batch < 0) return fail(-2, "npool > 0 and batch >= 0 required%s", "");
+  if (batch == 0) return 0;
+  if (!params || !params_bf16 || !cdf || !pool_y || !logdet || (f_out && !pool_f))
+    return fail(-3, "null pointer (pool_f is needed for f_out)%s", "");
+  FlowArgs A = flow_args(G, batch);
+  A.cdf = cdf;
+  A.npool = npool;
+  A.pool_y = pool_y;
+  A.pool_f = pool_f;
+  A.seed = seed;
+  A.draw = draw;
+  A.y_out = y_out;
+  A.f_out = f_out;
+  A.logdet = logdet;
+  A.save = (char*)save;
+  launch_flow(A, (hipStream_t)stream, params, (const unsigned short*)params_bf16);
   return hip_status(hipGetLastError());
 }
 
 int64_t mfx_flow_save_bytes(int D, int L, const int* masks, int64_t n) {
   FlowGeom G;
-  if (make_geom(D, L, masks, G)) return -1;
-  return (int64_t)L * (int64_t)save_per_layer(G, n);
+  if (make_geom(D, L, masks, G) || n < 0) return -1;
+  return (int64_t)L * save_layout(G, n).bytes;
 }
 
 int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n) {
   FlowGeom G;
-  if (make_geom(D, L, masks, G)) return -1;
-  int maxlp = 0;
-  for (int l = 0; l < L; ++l) {
-    const int lp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
-    maxlp = lp > maxlp ? lp : maxlp;
-  }
-  // the larger of the two layouts: per-tile partials (run-time-geometry backward), or the
-  // deltas of the batch (NP + 2 H floats per sample) followed by one partial per
-  // WG_SPLIT samples (the split backward)
-  int64_t bytes = ((n + TILE - 1) / TILE) * (int64_t)maxlp * 4;
-  for (int l = 0; l < L; ++l) {
-    const LayerGeom& g = G.layer[l];
-    if (g.nB < 1 || g.nB > NB_TEMPLATED) continue;
-    const int64_t lp = g.ob3 + g.NP - g.oW1;
-    const int64_t b = ((n + TILE - 1) / TILE) * TILE * (g.NP + 2 * H) * 2 + (n + WG_SPLIT - 1) / WG_SPLIT * lp * 4;
-    bytes = b > bytes ? b : bytes;
-  }
-  return bytes;
+  if (make_geom(D, L, masks, G) || n < 0) return -1;
+  // the deltas of every layer, then one partial gradient per split
+  return (int64_t)L * delta_layout(G, n).bytes + nsplit_of(n) * (int64_t)G.nparams * 4;
 }
 
 int mfx_flow_backward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16, int64_t n,
@@ -1231,46 +1154,37 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
   int rc = make_geom(D, L, masks, G);
   if (rc) return rc;
   if (n <= 0) return n < 0 ? fail(-2, "n must be >= 0%s", "") : 0;
-  if (!params || !params_bf16 || !save || !gl || !gx || !partial || !grad) return fail(-3, "null pointer%s", "");
+  if (!params || !params_bf16 || !save || !gl || !partial || !grad) return fail(-3, "null pointer%s", "");
   hipStream_t s = (hipStream_t)stream;
-  const int nblk = (int)((n + TILE - 1) / TILE);
-  const size_t per_layer = save_per_layer(G, n);
-  for (int l = L - 1; l >= 0; --l) {
-    const char* base = (const char*)save + per_layer * l;
-    const SaveLayout sl = save_layout(G, n);
-    BwdArgs A;
-    A.geo = G;
-    A.layer = l;
-    A.n = n;
-    A.x_in = (const float*)base;
-    A.raw = (const float*)(base + sl.raw);
-    A.h1 = (const unsigned short*)(base + sl.h1);
-    A.h2 = (const unsigned short*)(base + sl.h2);
-    A.gl = gl;
-    A.gx = gx;
-    A.partial = partial;
-    const int nlp = G.layer[l].ob3 + G.layer[l].NP - G.layer[l].oW1;
-    int nparts = nblk;
-    const int nb = G.layer[l].nB;
-    if (nb >= 1 && nb <= NB_TEMPLATED) {
-      // bf16 deltas over whole tiles (NP + 2 H per sample) first, then one partial per split
-      const size_t nt16 = (size_t)((n + TILE - 1) / TILE) * TILE;
-      unsigned short* d_o = (unsigned short*)partial;
-      unsigned short* d_h2 = d_o + nt16 * G.layer[l].NP;
-      unsigned short* d_h1 = d_h2 + nt16 * H;
-      nparts = (int)((n + WG_SPLIT - 1) / WG_SPLIT);
-      A.partial = (float*)(d_h1 + nt16 * H);
-      launch_layer_bwd_split(nb, n, s, A, params, (const unsigned short*)params_bf16, d_o, d_h2, d_h1);
-      hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, A.partial, nparts, nlp,
-                         grad + G.layer[l].oW1);
-      continue;
-    } else {
-      const size_t shmem = (size_t)TILE * G.layer[l].NP * 4;
-      hipLaunchKernelGGL(k_layer_bwd, dim3(nblk), dim3(NT), shmem, s, A, params, (const unsigned short*)params_bf16);
-    }
-    hipLaunchKernelGGL(k_reduce_grad, dim3((nlp + 63) / 64), dim3(256), 0, s, partial, nparts, nlp,
-                       grad + G.layer[l].oW1);
+  BwdArgs A;
+  A.geo = G;
+  A.n = n;
+  A.save = (const char*)save;
+  A.sl = save_layout(G, n);
+  A.delta = (char*)partial;
+  A.dl = delta_layout(G, n);
+  A.gl = gl;
+  A.gx_in = gx;
+  A.gx_out = gx;
+  A.partial = (float*)((char*)partial + (int64_t)L * A.dl.bytes);
+  A.split_tiles = split_tiles_of(n);
+  const int nsplit = (int)nsplit_of(n);
+  const dim3 gd((unsigned)ntiles_of(n)), gw((unsigned)nsplit, H / 16 + 1, (unsigned)L);
+  switch (nb_bound(G)) {
+    case 4:
+      hipLaunchKernelGGL(k_flow_bwd<4>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
+      hipLaunchKernelGGL(k_flow_wgrad<4>, gw, dim3(NT), 0, s, A);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_flow_bwd<8>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
+      hipLaunchKernelGGL(k_flow_wgrad<8>, gw, dim3(NT), 0, s, A);
+      break;
+    default:
+      hipLaunchKernelGGL(k_flow_bwd<16>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
+      hipLaunchKernelGGL(k_flow_wgrad<16>, gw, dim3(NT), 0, s, A);
+      break;
   }
+  hipLaunchKernelGGL(k_reduce_grad, dim3((G.nparams + 63) / 64), dim3(256), 0, s, A.partial, nsplit, G.nparams, grad);
   return hip_status(hipGetLastError());
 }
 
@@ -1294,9 +1208,24 @@ int mfx_flow_seed(const double* w, const double* f, const float* logq, int64_t n
 int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
                   float beta2, float eps, int step, float clip, void* stream) {
   if (n <= 0 || step < 1) return fail(-2, "n > 0 and step >= 1 required%s", "");
+  if (!params || !grad || !m1 || !m2) return fail(-3, "null pointer%s", "");
   const float bc1 = 1.f - powf(beta1, (float)step), bc2 = 1.f - powf(beta2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params, grad, m1,
                      m2, (int)n, lr, beta1, beta2, eps, bc1, bc2, clip);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_adam_cast(int D, int L, const int* masks, float* params, uint16_t* params_bf16, const float* grad,
+                       float* m1, float* m2, float lr, float beta1, float beta2, float eps, int step, float clip,
+                       void* stream) {
+  FlowGeom G;
+  int rc = make_geom(D, L, masks, G);
+  if (rc) return rc;
+  if (step < 1) return fail(-2, "step >= 1 required%s", "");
+  if (!params || !params_bf16 || !grad || !m1 || !m2) return fail(-3, "null pointer%s", "");
+  const float bc1 = 1.f - powf(beta1, (float)step), bc2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam_cast, dim3(64, L), dim3(256), 0, (hipStream_t)stream, G, params, grad, m1, m2, lr, beta1,
+                     beta2, eps, bc1, bc2, clip, (unsigned short*)params_bf16);
   return hip_status(hipGetLastError());
 }
 

@@ -62,6 +62,11 @@ def _sig(lib):
     lib.mfx_flow_minibatch.restype = i
     lib.mfx_flow_seed.argtypes = [vp, vp, vp, i64, f, vp, vp]
     lib.mfx_flow_seed.restype = i
+    u32 = ctypes.c_uint32
+    lib.mfx_flow_draw_forward.argtypes = [i, i, vp, vp, vp, vp, i64, vp, vp, u32, u32, i64, vp, vp, vp, vp, vp]
+    lib.mfx_flow_draw_forward.restype = i
+    lib.mfx_flow_adam_cast.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, f, f, f, i, f, vp]
+    lib.mfx_flow_adam_cast.restype = i
 
 
 def _p(t):
@@ -133,7 +138,7 @@ class RQSFlow:
         x = _lib.require_device_tensor(x, torch.float32, "x")
         n = x.shape[0]
         out = torch.empty_like(x)
-        ld = torch.zeros(n, dtype=torch.float32, device=x.device)
+        ld = torch.empty(n, dtype=torch.float32, device=x.device)     # written by the kernel
         _lib.check(self.lib.mfx_flow_apply(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), n, _p(x),
                                            _p(out), _p(ld), int(inverse), _p(save), _lib.stream_ptr(x.device)))
         return out, ld
@@ -181,10 +186,11 @@ class RQSFlow:
         return loss, grad
 
     def adam_step(self, grad, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=10.0):
+        """One Adam step on the fp32 parameters; the same launch refreshes their bf16 copies."""
         self.step += 1
-        _lib.check(self.lib.mfx_flow_adam(_p(self.params), _p(grad), _p(self.m1), _p(self.m2), self.nparams, lr,
-                                          betas[0], betas[1], eps, self.step, clip, _lib.stream_ptr(self.device)))
-        self._cast()
+        _lib.check(self.lib.mfx_flow_adam_cast(self.D, self.L, self._m(), _p(self.params), _p(self.pbf16), _p(grad),
+                                               _p(self.m1), _p(self.m2), lr, betas[0], betas[1], eps, self.step, clip,
+                                               _lib.stream_ptr(self.device)))
 
     def is_second_moment(self, y, f, mix=0.0, chunk=1 << 20):
         """E_uniform[f^2 / q'] over uniform points y with integrand values f, where
@@ -203,7 +209,8 @@ class RQSFlow:
         """Minibatch weighted maximum likelihood on device-resident pairs (y, w).
 
         resample=True draws each minibatch with probability proportional to w (inverse-CDF
-        lookup on the device) and gives every drawn point unit weight: the same expected
+        lookup inside the forward kernel, Philox uniforms keyed on ``seed`` and the step) and
+        gives every drawn point unit weight: the same expected
         loss as the w-weighted uniform minibatch, with far less gradient noise when the
         weights are heavy-tailed (matrix elements near collinear/soft limits).
 
@@ -233,8 +240,20 @@ class RQSFlow:
             cdf /= total
             y = _lib.require_device_tensor(y, torch.float32, "y")
             fc = _lib.require_device_tensor(f, None, "f").to(torch.float64) if beta else None
-            yb = torch.empty((batch, self.D), dtype=torch.float32, device=y.device)
-            fb = torch.empty(batch, dtype=torch.float64, device=y.device) if beta else None
+            # device workspace of the fused step, allocated once: the minibatch is drawn
+            # inside the forward kernel (Philox counter (sample, step), key seed)
+            dev = y.device
+            fb = torch.empty(batch, dtype=torch.float64, device=dev) if beta else None
+            ld = torch.empty(batch, dtype=torch.float32, device=dev)
+            gl = torch.empty(batch, dtype=torch.float32, device=dev)
+            save = torch.empty(self.lib.mfx_flow_save_bytes(self.D, self.L, self._m(), batch) // 4 + 1,
+                               dtype=torch.float32, device=dev)
+            partial = torch.empty(self.lib.mfx_flow_partial_bytes(self.D, self.L, self._m(), batch) // 4 + 1,
+                                  dtype=torch.float32, device=dev)
+            gbuf = torch.empty_like(self.params)
+            P = dict(par=_p(self.params), pb=_p(self.pbf16), cdf=_p(cdf), y=_p(y), fc=_p(fc), fb=_p(fb), ld=_p(ld),
+                     gl=_p(gl), save=_p(save), partial=_p(partial), grad=_p(gbuf), st=_lib.stream_ptr(dev))
+            seed32 = int(seed) & 0xFFFFFFFF
         for it in range(steps + 1):
             if val is not None and (it % eval_every == 0 or it == steps):
                 m2 = self.is_second_moment(val[0], val[1], mix)
@@ -246,11 +265,19 @@ class RQSFlow:
                     break
             if it == steps:
                 break
+            cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
+            show = log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1)
+            need_loss = show or it % 50 == 0
             if resample:
-                r = torch.rand(batch, generator=g, device=y.device, dtype=torch.float64)
-                _lib.check(self.lib.mfx_flow_minibatch(_p(cdf), n, _p(r), batch, _p(y), self.D, _p(fc), _p(yb),
-                                                       _p(fb), _lib.stream_ptr(y.device)))
-                wb = None
+                m = self._m()
+                _lib.check(self.lib.mfx_flow_draw_forward(self.D, self.L, m, P["par"], P["pb"], P["cdf"], n, P["y"],
+                                                          P["fc"], seed32, it, batch, None, P["fb"], P["ld"],
+                                                          P["save"], P["st"]))
+                _lib.check(self.lib.mfx_flow_seed(None, P["fb"], P["ld"], batch, float(beta), P["gl"], P["st"]))
+                _lib.check(self.lib.mfx_flow_backward(self.D, self.L, m, P["par"], P["pb"], batch, P["save"], P["gl"],
+                                                      None, P["partial"], P["grad"], P["st"]))
+                loss = (gl.to(torch.float64) * ld.to(torch.float64)).sum() if need_loss else None
+                grad = gbuf
             else:
                 idx = torch.randint(0, n, (batch,), generator=g, device=y.device)
                 yb = y[idx].contiguous()
@@ -258,9 +285,7 @@ class RQSFlow:
                 fb = f[idx].contiguous() if beta else None
                 if float(wb.sum()) <= 0:
                     continue
-            cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
-            show = log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1)
-            loss, grad = self.grad(yb, wb, fb, beta, need_loss=show or it % 50 == 0)
+                loss, grad = self.grad(yb, wb, fb, beta, need_loss=need_loss)
             self.adam_step(grad, lr=cur_lr)
             if show:
                 log("flow step %d loss %.5f" % (it, float(loss)))

@@ -170,3 +170,95 @@ def test_minibatch_draw_and_seed(dev):
                                      beta, vp(gl), _lib.stream_ptr(dev)))
         v = (wt if wt is not None else np.ones(batch)) * (f[idx] * np.exp(-ld.astype(np.float64))) ** beta
         np.testing.assert_allclose(gl.cpu().numpy(), -v / v.sum(), rtol=1e-6, atol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("beta", [0.0, 2.0])
+def test_draw_forward_and_fused_step(dev, beta):
+    """mfx_flow_draw_forward draws sample e as the first pool index whose normalised running
+    weight reaches the Philox uniform of counter (e, 0, 0, draw) (the oracle's stream),
+    returns the point's integrand value, and its log q and saved state equal those of a
+    forward pass over the drawn points: the gradient from its save equals flow.grad of the
+    drawn batch bitwise.  mfx_flow_adam_cast equals mfx_flow_adam then mfx_flow_cast bitwise."""
+    import ctypes
+    from madflow_amd import _lib
+    from oracle import oracle as O
+    print("case flow draw-forward / fused step beta=%g" % beta, flush=True)
+    D, npool, batch, seed, draw = 8, 20000, 3000, 77, 5
+    flow = RQSFlow(D, layers=8, seed=3, device=dev)
+    _perturbed(flow, seed=4)
+    lib = flow.lib
+    rng = np.random.default_rng(31)
+    w = rng.random(npool) ** 4 * (rng.random(npool) > 0.2)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    y = rng.random((npool, D)).astype(np.float32)
+    f = rng.random(npool) + 0.05
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    cdf_t, y_t, f_t = t(cdf), t(y), t(f)
+    yo = torch.empty((batch, D), dtype=torch.float32, device=dev)
+    fo = torch.empty(batch, dtype=torch.float64, device=dev)
+    ld = torch.empty(batch, dtype=torch.float32, device=dev)
+    save = torch.empty(lib.mfx_flow_save_bytes(D, 8, flow._m(), batch) // 4 + 1, dtype=torch.float32, device=dev)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.check(lib.mfx_flow_draw_forward(D, 8, flow._m(), vp(flow.params), vp(flow.pbf16), vp(cdf_t), npool, vp(y_t),
+                                         vp(f_t), seed, draw, batch, vp(yo), vp(fo), vp(ld), vp(save),
+                                         _lib.stream_ptr(dev)))
+    u = O.philox_uniforms(batch, 2, seed, draw)[:, 0]
+    idx = np.minimum(np.searchsorted(cdf, u, side="left"), npool - 1)
+    assert np.all(w[idx] > 0)
+    np.testing.assert_array_equal(yo.cpu().numpy(), y[idx])
+    np.testing.assert_array_equal(fo.cpu().numpy(), f[idx])
+    assert torch.equal(ld, flow.log_prob(yo))
+    # the gradient from the draw's saved state equals that of a separate forward pass
+    gl = torch.empty(batch, dtype=torch.float32, device=dev)
+    _lib.check(lib.mfx_flow_seed(None, vp(fo), vp(ld), batch, beta, vp(gl), _lib.stream_ptr(dev)))
+    partial = torch.empty(lib.mfx_flow_partial_bytes(D, 8, flow._m(), batch) // 4 + 1, dtype=torch.float32,
+                          device=dev)
+    grad = torch.empty_like(flow.params)
+    _lib.check(lib.mfx_flow_backward(D, 8, flow._m(), vp(flow.params), vp(flow.pbf16), batch, vp(save), vp(gl), None,
+                                     vp(partial), vp(grad), _lib.stream_ptr(dev)))
+    _, grad_ref = flow.grad(yo, None, fo if beta else None, beta)
+    assert torch.equal(grad, grad_ref)
+    # Adam with the bf16 refresh in one launch
+    p0, m1, m2 = flow.params.clone(), flow.m1.clone(), flow.m2.clone()
+    flow.step = 0
+    flow.adam_step(grad, lr=1e-2)
+    p_fused, pb_fused = flow.params.clone(), flow.pbf16.clone()
+    flow.params.copy_(p0)
+    flow.m1.copy_(m1)
+    flow.m2.copy_(m2)
+    _lib.check(lib.mfx_flow_adam(vp(flow.params), vp(grad), vp(flow.m1), vp(flow.m2), flow.nparams, 1e-2, 0.9, 0.999,
+                                 1e-8, 1, 10.0, _lib.stream_ptr(dev)))
+    flow._cast()
+    assert torch.equal(flow.params, p_fused)
+    assert torch.equal(flow.pbf16, pb_fused)
+
+
+@pytest.mark.gpu
+def test_flow_wide_layers(dev):
+    """A flow whose layers transform 9..15 of 16 dimensions (the kernels sized for 16):
+    forward, inverse and the gradient against the restatement."""
+    print("case flow wide layers D=16", flush=True)
+    D, L = 16, 4
+    masks = np.zeros((L, D), dtype=np.int32)
+    for l, nb in enumerate((9, 15, 12, 10)):
+        masks[l, (np.arange(D) + 3 * l) % D < nb] = 1
+    flow = RQSFlow(D, layers=L, seed=5, device=dev, masks=masks)
+    p = _perturbed(flow, seed=6)
+    x = torch.rand((700, D), generator=torch.Generator().manual_seed(7), dtype=torch.float32)
+    u, ld = flow.forward(x.to(dev))
+    u_ref, ld_ref = R.apply(flow, p, x)
+    np.testing.assert_allclose(u.cpu().numpy(), u_ref.numpy(), atol=2e-4)
+    np.testing.assert_allclose(ld.cpu().numpy(), ld_ref.numpy(), atol=4e-3, rtol=1e-3)
+    xb, _ = flow.sample(u)
+    np.testing.assert_allclose(xb.cpu().numpy(), x.numpy(), atol=3e-4)
+    w = torch.rand(700, generator=torch.Generator().manual_seed(8), dtype=torch.float64) ** 2
+    _, grad = flow.grad(x.to(dev), w.to(dev))
+    pr = p.clone().requires_grad_(True)
+    _, ldr = R.apply(flow, pr, x)
+    (-(w.float() * ldr).sum() / w.float().sum()).backward()
+    g, g_ref = grad.cpu().numpy(), pr.grad.numpy()
+    cos = float(np.dot(g, g_ref) / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
+    print("wide-layer grad cosine", cos, flush=True)
+    assert cos > 0.999

@@ -467,11 +467,8 @@ __global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict
 //    input-gradient products gh2 = (go W3^T) relu'(h2) and gh1 = (gh2 W2^T) relu'(h1), the
 //    input gradient (carried to the next layer in LDS), and the deltas go, gh2, gh1 written
 //    tile-transposed for the weight gradients;
-//  * k_flow_wgrad: the weight gradients of every layer as GEMMs with the samples as K, over
-//    splits of split_tiles tiles: block (split, y < 8, layer) computes rows 16 y .. 16 y + 15
-//    of dW2 = h1^T gh2 and dW3 = h2^T go (wave w: column tiles w, w + 8, ...); block
-//    (split, 8, layer) the bias sums as products with a row of ones and dW1 = x_A^T gh1 (fp32,
-//    from the exact inputs).  One partial gradient per split, reduced in a fixed order.
+//  * k_flow_wgrad: the weight gradients of every layer as GEMMs with the samples as K, one
+//    partial gradient per split of tiles, reduced in a fixed order (k_reduce_grad).
 struct BwdArgs {
   FlowGeom geo;
   int64_t n;
@@ -483,7 +480,6 @@ struct BwdArgs {
   const float* gx_in;      // optional (n, D): d objective / d flow output (null: zero)
   float* gx_out;           // optional (n, D): d objective / d flow input
   float* partial;          // (nsplit, nparams): weight-gradient partials
-  int split_tiles;
 };
 
 // The data-gradient kernel runs 4 waves per tile (each wave two column tiles of gh2 / gh1):
@@ -502,6 +498,7 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int r = lane & 15, kq = (lane >> 4) * 4;   // bf16 MFMA: four consecutive k per lane
   const int k8 = (lane >> 4) * 8;                  // ... eight in the 16-byte loads of mfma_k32
+  const int cw = wid;                              // this wave's column tiles: cw and cw + 4
   const int64_t s0 = (int64_t)blockIdx.x * TILE;
   __shared__ float xs[TILE * DMAX], gxs[TILE * DMAX];
   __shared__ __attribute__((aligned(16))) unsigned short h1b[TILE * WP], h2b[TILE * WP], ghb[TILE * WP];
@@ -587,10 +584,10 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
     }
     __syncthreads();
     // gh2 = (go W3^T) * relu'(h2): B[k = np][n = h] = W3[h][np], the row h of W3n; wave w
-    // computes the column tiles w and w + 4 (one A operand, two B operands per k step)
+    // computes the column tiles cw and cw + 4 (one A operand, two B operands per k step)
     {
       f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-      const unsigned short* w3[2] = {Pb + g.oW3n + (16 * wid + r) * NP, Pb + g.oW3n + (16 * (wid + WPB_B) + r) * NP};
+      const unsigned short* w3[2] = {Pb + g.oW3n + (16 * cw + r) * NP, Pb + g.oW3n + (16 * (cw + WPB_B) + r) * NP};
 #pragma unroll
       for (int K0 = 0; K0 + 32 <= NPM; K0 += 32)
         if (K0 + 32 <= NP) {
@@ -606,7 +603,7 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const int n0 = 16 * (wid + WPB_B * u);
+        const int n0 = 16 * (cw + WPB_B * u);
         unsigned short out[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -629,11 +626,11 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
         const bf16x8 a = ld_bf16x8(ghb + r * WP + K0 + k8);
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-          acc[u] = mfma_k32(a, ld_bf16x8(Pb + g.oW2n + (16 * (wid + WPB_B * u) + r) * H + K0 + k8), acc[u]);
+          acc[u] = mfma_k32(a, ld_bf16x8(Pb + g.oW2n + (16 * (cw + WPB_B * u) + r) * H + K0 + k8), acc[u]);
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const int n0 = 16 * (wid + WPB_B * u);
+        const int n0 = 16 * (cw + WPB_B * u);
         unsigned short out[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -671,19 +668,24 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
   }
 }
 
-constexpr int WG_CHUNK = 4;     // tiles whose operands a weight-gradient block loads together
+// Weight gradients: one block per (split of WG_ST tiles, layer).  The split's saved
+// activations h1, h2 (the A operands, shared by every wave) and inputs x_A are staged in LDS
+// once; wave w owns whole column tiles (dW3 tiles w, w + 8, ...; dW2, b2, b1 and dW1 tile w)
+// and streams its deltas (the B operands) from memory, so every byte of the split is read
+// from memory once.  Two tiles form one 32-sample k step (mfma_k32); a lone last tile
+// takes one 16-sample step.
+constexpr int WG_ST = 16;
 
-template <int NBM>
 __global__ void __launch_bounds__(NT) k_flow_wgrad(BwdArgs A) {
-  constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
-  constexpr int J3 = (NPM / 16 + WPB - 1) / WPB;
-  static_assert(DMAX == 16, "dW1 is one 16-row tile");
-  const int l = blockIdx.z;
+  static_assert(DMAX == 16 && H / 16 == WPB, "dW1 is one 16-row tile; one dW2 column tile per wave");
+  const int l = blockIdx.y;
   const LayerGeom& g = A.geo.layer[l];
   const int NP = g.NP, NT3 = NP / 16;
   const int D = A.geo.D;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int r = lane & 15, kq = (lane >> 4) * 4, kf = lane >> 4;
+  // pair k step: lane quarter g takes samples 8 (g % 2) .. + 7 of tile t + g / 2
+  const int ph = (lane >> 5), po = ((lane >> 4) & 1) * 8;
   const char* base = A.save + A.sl.bytes * l;
   const float* sx = reinterpret_cast<const float*>(base);
   const unsigned short* sh1 = reinterpret_cast<const unsigned short*>(base + A.sl.h1);
@@ -693,106 +695,120 @@ __global__ void __launch_bounds__(NT) k_flow_wgrad(BwdArgs A) {
   const unsigned short* d_h2 = reinterpret_cast<const unsigned short*>(dbase + A.dl.h2);
   const unsigned short* d_h1 = reinterpret_cast<const unsigned short*>(dbase + A.dl.h1);
   const int64_t ntiles = (A.n + TILE - 1) / TILE;
-  const int64_t t0 = (int64_t)blockIdx.x * A.split_tiles;
-  const int nt = (int)(t0 + A.split_tiles < ntiles ? A.split_tiles : ntiles - t0);   // tiles of this split
+  const int64_t t0 = (int64_t)blockIdx.x * WG_ST;
+  const int nt = (int)(t0 + WG_ST < ntiles ? WG_ST : ntiles - t0);   // tiles of this split
   float* part = A.partial + (size_t)blockIdx.x * A.geo.nparams + g.oW1;
-  // a wave whose later dW3 column tiles do not exist repeats its last one, unstored
-  int jc[J3];
+  __shared__ __attribute__((aligned(16))) unsigned short h1l[WG_ST * TILE * H], h2l[WG_ST * TILE * H];
+  __shared__ float xl[WG_ST * TILE * DMAX];
+  // stage the split's activations ([tile][column][sample], as saved: samples past n are
+  // zero there and in the deltas) and its conditioning inputs (zero past n and for i >= nA)
+  for (int q = tid; q < nt * TILE * H / 8; q += NT) {
+    *reinterpret_cast<uint4*>(h1l + 8 * q) = *reinterpret_cast<const uint4*>(sh1 + t0 * TILE * H + 8 * q);
+    *reinterpret_cast<uint4*>(h2l + 8 * q) = *reinterpret_cast<const uint4*>(sh2 + t0 * TILE * H + 8 * q);
+  }
+  for (int q = tid; q < nt * TILE * DMAX; q += NT) {
+    const int i = q % DMAX;
+    const int64_t e = t0 * TILE + q / DMAX;
+    xl[q] = (i < g.nA && e < A.n) ? sx[e * D + g.A[i]] : 0.f;
+  }
+  __syncthreads();
+  const short one = (short)0x3f80;            // 1.0 in bf16
+  const bf16x8 ones8 = {one, one, one, one, one, one, one, one};
+  const bf16x4 ones4 = {one, one, one, one};
+  const bf16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bf16x4 zero4 = {0, 0, 0, 0};
+  // ---- dW3 = h2^T go and b3 = ones^T go: column tiles ja = wid + 16 u and jb = ja + 8
+  for (int ja = wid; ja < NT3; ja += 2 * WPB) {
+    const int jb = ja + WPB;
+    const bool hb = jb < NT3;
+    f32x4 acc[2][H / 16], bacc[2];
 #pragma unroll
-  for (int u = 0; u < J3; ++u) jc[u] = wid + WPB * u < NT3 ? wid + WPB * u : NT3 - 1;
-  f32x4 acc3[J3], acc2 = {0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < H / 16; ++m) acc[0][m] = acc[1][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bacc[0] = bacc[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int t = 0;
+    for (; t + 1 < nt; t += 2) {
+      const int64_t tt = t0 + t + ph;
+      const bf16x8 b0 = ld_bf16x8(d_o + tt * TILE * NP + (16 * ja + r) * TILE + po);
+      const bf16x8 b1 = hb ? ld_bf16x8(d_o + tt * TILE * NP + (16 * jb + r) * TILE + po) : zero8;
 #pragma unroll
-  for (int u = 0; u < J3; ++u) acc3[u] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16x4 zero = {0, 0, 0, 0};
-  if (blockIdx.y < H / 16) {
-    const int m0 = 16 * blockIdx.y;
-    for (int c0 = 0; c0 < nt; c0 += WG_CHUNK) {
-      // the operands of WG_CHUNK tiles are loaded before their MFMAs (tiles past the end
-      // of the split repeat its last one with a zero A operand)
-      bf16x4 a3[WG_CHUNK], a2[WG_CHUNK], b2[WG_CHUNK], b3[WG_CHUNK][J3];
-#pragma unroll
-      for (int t = 0; t < WG_CHUNK; ++t) {
-        const int64_t tt = t0 + (c0 + t < nt ? c0 + t : nt - 1);
-        a3[t] = ld_bf16x4(sh2 + tt * TILE * H + (m0 + r) * TILE + kq);
-        a2[t] = ld_bf16x4(sh1 + tt * TILE * H + (m0 + r) * TILE + kq);
-        b2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
-#pragma unroll
-        for (int u = 0; u < J3; ++u) b3[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
+      for (int m = 0; m < H / 16; ++m) {
+        const bf16x8 a = ld_bf16x8(h2l + ((t + ph) * H + 16 * m + r) * TILE + po);
+        acc[0][m] = mfma_k32(a, b0, acc[0][m]);
+        acc[1][m] = mfma_k32(a, b1, acc[1][m]);
       }
+      bacc[0] = mfma_k32(ones8, b0, bacc[0]);
+      bacc[1] = mfma_k32(ones8, b1, bacc[1]);
+    }
+    if (t < nt) {                                // a lone last tile: one 16-sample step
+      const int64_t tt = t0 + t;
+      const bf16x4 b0 = ld_bf16x4(d_o + tt * TILE * NP + (16 * ja + r) * TILE + kq);
+      const bf16x4 b1 = hb ? ld_bf16x4(d_o + tt * TILE * NP + (16 * jb + r) * TILE + kq) : zero4;
 #pragma unroll
-      for (int t = 0; t < WG_CHUNK; ++t) {
-        const bool ok = c0 + t < nt;
-        const bf16x4 x3 = ok ? a3[t] : zero, x2 = ok ? a2[t] : zero;
-#pragma unroll
-        for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(x3, b3[t][u], acc3[u]);
-        acc2 = mfma_bf16(x2, b2[t], acc2);
+      for (int m = 0; m < H / 16; ++m) {
+        const bf16x4 a = ld_bf16x4(h2l + (t * H + 16 * m + r) * TILE + kq);
+        acc[0][m] = mfma_bf16(a, b0, acc[0][m]);
+        acc[1][m] = mfma_bf16(a, b1, acc[1][m]);
       }
+      bacc[0] = mfma_bf16(ones4, b0, bacc[0]);
+      bacc[1] = mfma_bf16(ones4, b1, bacc[1]);
     }
 #pragma unroll
-    for (int u = 0; u < J3; ++u) {
-      const int j = wid + WPB * u;
-      if (j < NT3)
+    for (int u = 0; u < 2; ++u) {
+      const int j = u ? jb : ja;
+      if (u && !hb) break;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) part[g.oW3 - g.oW1 + (m0 + kq + i) * NP + 16 * j + r] = acc3[u][i];
+      for (int m = 0; m < H / 16; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[g.oW3 - g.oW1 + (16 * m + kq + i) * NP + 16 * j + r] = acc[u][m][i];
+      if (kq == 0) part[g.ob3 - g.oW1 + 16 * j + r] = bacc[u][0];
+    }
+  }
+  // ---- dW2 = h1^T gh2 and b2 = ones^T gh2 (column tile wid); b1 = ones^T gh1 and
+  // dW1 = x_A^T gh1 (fp32 matrix cores, from the exact inputs) of column tile wid
+  {
+    const int j = wid;
+    f32x4 acc[H / 16], b2acc = {0.f, 0.f, 0.f, 0.f}, b1acc = {0.f, 0.f, 0.f, 0.f}, w1acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int m = 0; m < H / 16; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int t = 0;
+    for (; t + 1 < nt; t += 2) {
+      const int64_t tt = t0 + t + ph;
+      const bf16x8 b = ld_bf16x8(d_h2 + tt * TILE * H + (16 * j + r) * TILE + po);
+      const bf16x8 c = ld_bf16x8(d_h1 + tt * TILE * H + (16 * j + r) * TILE + po);
+#pragma unroll
+      for (int m = 0; m < H / 16; ++m)
+        acc[m] = mfma_k32(ld_bf16x8(h1l + ((t + ph) * H + 16 * m + r) * TILE + po), b, acc[m]);
+      b2acc = mfma_k32(ones8, b, b2acc);
+      b1acc = mfma_k32(ones8, c, b1acc);
+    }
+    if (t < nt) {
+      const int64_t tt = t0 + t;
+      const bf16x4 b = ld_bf16x4(d_h2 + tt * TILE * H + (16 * j + r) * TILE + kq);
+      const bf16x4 c = ld_bf16x4(d_h1 + tt * TILE * H + (16 * j + r) * TILE + kq);
+#pragma unroll
+      for (int m = 0; m < H / 16; ++m) acc[m] = mfma_bf16(ld_bf16x4(h1l + (t * H + 16 * m + r) * TILE + kq), b, acc[m]);
+      b2acc = mfma_bf16(ones4, b, b2acc);
+      b1acc = mfma_bf16(ones4, c, b1acc);
+    }
+    // dW1: A[i][k] = x_A of sample k (row i < 16), B[k][c] = gh1; four samples per step
+    for (int s4 = 0; s4 < nt * TILE; s4 += 4) {
+      const int sk = s4 + kf;
+      const float gv = bf2f(d_h1[(t0 + sk / TILE) * TILE * H + (16 * j + r) * TILE + sk % TILE]);
+      w1acc = mfma_f32(xl[sk * DMAX + r], gv, w1acc);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) part[g.oW2 - g.oW1 + (m0 + kq + i) * H + 16 * wid + r] = acc2[i];
-  } else {
-    // bias sums: ones^T B, every row of which is the column sum of B (row 0 is stored)
-    const short one = (short)0x3f80;            // 1.0 in bf16
-    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, accw1 = {0.f, 0.f, 0.f, 0.f};
-    // dW1 = x_A^T gh1 on the fp32 matrix cores (A[r][k] = x of sample k, input r < nA)
-    const int kf = lane >> 4;                    // fp32 MFMA: one k per lane quarter
-    const bool xrow = r < g.nA;
-    const int da = xrow ? g.A[r] : 0;
-    for (int c0 = 0; c0 < nt; c0 += WG_CHUNK) {
-      bf16x4 bh1[WG_CHUNK], bh2[WG_CHUNK], bo[WG_CHUNK][J3];
-      float xv[WG_CHUNK][4], gv[WG_CHUNK][4];
+    for (int m = 0; m < H / 16; ++m)
 #pragma unroll
-      for (int t = 0; t < WG_CHUNK; ++t) {
-        const int64_t tt = t0 + (c0 + t < nt ? c0 + t : nt - 1);
-        bh1[t] = ld_bf16x4(d_h1 + tt * TILE * H + (16 * wid + r) * TILE + kq);
-        bh2[t] = ld_bf16x4(d_h2 + tt * TILE * H + (16 * wid + r) * TILE + kq);
-#pragma unroll
-        for (int u = 0; u < J3; ++u) bo[t][u] = ld_bf16x4(d_o + tt * TILE * NP + (16 * jc[u] + r) * TILE + kq);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int64_t e = tt * TILE + 4 * k + kf;
-          xv[t][k] = sx[(e < A.n ? e : A.n - 1) * D + da];
-          gv[t][k] = bf2f(d_h1[tt * TILE * H + (16 * wid + r) * TILE + 4 * k + kf]);
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < WG_CHUNK; ++t) {
-        const bool ok = c0 + t < nt;
-        const bf16x4 ones = ok ? bf16x4{one, one, one, one} : zero;
-        acc1 = mfma_bf16(ones, bh1[t], acc1);
-        acc2 = mfma_bf16(ones, bh2[t], acc2);
-#pragma unroll
-        for (int u = 0; u < J3; ++u) acc3[u] = mfma_bf16(ones, bo[t][u], acc3[u]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int64_t e = (t0 + c0 + t) * TILE + 4 * k + kf;
-          const bool okx = ok && e < A.n && xrow;
-          accw1 = mfma_f32(okx ? xv[t][k] : 0.f, gv[t][k], accw1);
-        }
-      }
-    }
+      for (int i = 0; i < 4; ++i) part[g.oW2 - g.oW1 + (16 * m + kq + i) * H + 16 * j + r] = acc[m][i];
     // fp32 MFMA output: D[4 kf + i][r] in element i
 #pragma unroll
-    for (int i = 0; i < 4; ++i) part[(4 * kf + i) * H + 16 * wid + r] = accw1[i];
+    for (int i = 0; i < 4; ++i) part[(4 * kf + i) * H + 16 * j + r] = w1acc[i];
     if (kq == 0) {
-      part[g.ob1 - g.oW1 + 16 * wid + r] = acc1[0];
-      part[g.ob2 - g.oW1 + 16 * wid + r] = acc2[0];
-#pragma unroll
-      for (int u = 0; u < J3; ++u) {
-        const int j = wid + WPB * u;
-        if (j < NT3) part[g.ob3 - g.oW1 + 16 * j + r] = acc3[u][0];
-      }
+      part[g.ob2 - g.oW1 + 16 * j + r] = b2acc[0];
+      part[g.ob1 - g.oW1 + 16 * j + r] = b1acc[0];
     }
   }
 }
-
 
 // Sum of the per-split partial gradients: 64 parameters per block, the partials of each
 // split over 4 waves (Kahan sums over blocks w, w+4, ...), combined in a fixed order in
@@ -982,16 +998,8 @@ static DeltaLayout delta_layout(const FlowGeom& G, int64_t n) {
   return d;
 }
 
-// Tiles per weight-gradient split: about 32 splits (enough blocks for the chip with the 9
-// block rows and the layers), at least WG_CHUNK tiles each.
-static int split_tiles_of(int64_t n) {
-  const int64_t t = (ntiles_of(n) + 31) / 32;
-  return (int)(t < WG_CHUNK ? WG_CHUNK : (t > 4096 ? 4096 : t));
-}
-static int64_t nsplit_of(int64_t n) {
-  const int st = split_tiles_of(n);
-  return (ntiles_of(n) + st - 1) / st;
-}
+// Weight-gradient splits of WG_ST tiles
+static int64_t nsplit_of(int64_t n) { return (ntiles_of(n) + WG_ST - 1) / WG_ST; }
 
 static int make_geom(int D, int L, const int* masks, FlowGeom& G) {
   if (D < 2 || D > DMAX || L < 1 || L > 16) return fail(-2, "flow needs 2 <= D <= 16 and 1 <= L <= 16%s", "");
@@ -1167,23 +1175,20 @@ int mfx_flow_backward(int D, int L, const int* masks, const float* params, const
   A.gx_in = gx;
   A.gx_out = gx;
   A.partial = (float*)((char*)partial + (int64_t)L * A.dl.bytes);
-  A.split_tiles = split_tiles_of(n);
   const int nsplit = (int)nsplit_of(n);
-  const dim3 gd((unsigned)ntiles_of(n)), gw((unsigned)nsplit, H / 16 + 1, (unsigned)L);
+  const dim3 gd((unsigned)ntiles_of(n)), gw((unsigned)nsplit, (unsigned)L);
   switch (nb_bound(G)) {
     case 4:
       hipLaunchKernelGGL(k_flow_bwd<4>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
-      hipLaunchKernelGGL(k_flow_wgrad<4>, gw, dim3(NT), 0, s, A);
       break;
     case 8:
       hipLaunchKernelGGL(k_flow_bwd<8>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
-      hipLaunchKernelGGL(k_flow_wgrad<8>, gw, dim3(NT), 0, s, A);
       break;
     default:
       hipLaunchKernelGGL(k_flow_bwd<16>, gd, dim3(NT_B), 0, s, A, params, (const unsigned short*)params_bf16);
-      hipLaunchKernelGGL(k_flow_wgrad<16>, gw, dim3(NT), 0, s, A);
       break;
   }
+  hipLaunchKernelGGL(k_flow_wgrad, gw, dim3(NT), 0, s, A);
   hipLaunchKernelGGL(k_reduce_grad, dim3((G.nparams + 63) / 64), dim3(256), 0, s, A.partial, nsplit, G.nparams, grad);
   return hip_status(hipGetLastError());
 }

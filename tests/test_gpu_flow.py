@@ -51,11 +51,13 @@ def test_flow_logdet_is_jacobian(dev):
 
 
 @pytest.mark.gpu
-def test_flow_gradients_match_autograd(dev):
+@pytest.mark.parametrize("n", [512, 1000])
+def test_flow_gradients_match_autograd(dev, n):
+    """n = 1000: the last 32-sample block of the forward pass holds one tile of the batch."""
+    print("case flow gradients n=%d" % n, flush=True)
     D = 8
     flow = RQSFlow(D, layers=8, seed=7, device=dev)
     p = _perturbed(flow, seed=8)
-    n = 512
     x = torch.rand((n, D), generator=torch.Generator().manual_seed(9), dtype=torch.float32)
     w = torch.rand(n, generator=torch.Generator().manual_seed(10), dtype=torch.float64) ** 4
     loss, grad = flow.grad(x.to(dev), w.to(dev))

@@ -258,7 +258,7 @@ def train_flow(a, torch, ig):
     done = 0
     while done < ntot:
         n = min(a.events_limit, ntot - done)
-        yy = philox_uniforms(n, D, 99, 0, ev0=done).float().contiguous()
+        yy = philox_uniforms(n, D, 99, 0, ev0=done, dtype=torch.float32)
         _, _, fb = chunk(done, n, 0, keep_f=True, y_in=yy)
         y[done:done + n] = yy
         f[done:done + n] = fb
@@ -267,13 +267,17 @@ def train_flow(a, torch, ig):
     t_pairs = time.perf_counter() - t0
     t0 = time.perf_counter()
     flow = RQSFlow(D, layers=8, seed=a.flow_seed)
+    stage_s = []
     nt = a.flow_pairs
     s0 = 1000 * (a.flow_seed - 1)   # minibatch-draw seeds of the stages
     flow.train(y[:nt], f[:nt], steps=a.flow_steps, batch=a.flow_batch, lr=a.flow_lr, seed=s0, log=log,
                val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every, mix=a.flow_mix, beta=a.flow_beta,
                patience=a.flow_patience)
     flow.stage_steps = [flow.steps_run]
+    torch.cuda.synchronize()
+    stage_s.append(time.perf_counter() - t0)
     for rnd in range(a.flow_refine_rounds if a.flow_refine_steps > 0 else 0):
+        t1 = time.perf_counter()
         # further stages: points drawn from the current proposal, weighted by f/q', carry
         # far more of the information where the integrand is large than uniform points
         done = 0
@@ -282,7 +286,7 @@ def train_flow(a, torch, ig):
         fq = torch.empty(nt, dtype=torch.float64, device="cuda") if a.flow_refine_beta else None
         while done < nt:
             n = min(a.events_limit, nt - done)
-            u = philox_uniforms(n, D, 98 - rnd, 0, ev0=done).float()
+            u = philox_uniforms(n, D, 98 - rnd, 0, ev0=done, dtype=torch.float32)
             ys, winv = flow.sample_mixture(u, u, a.flow_mix)
             ys = ys.contiguous()
             _, _, fb = chunk(done, n, 0, keep_f=True, y_in=ys, w_in=winv.contiguous())
@@ -291,14 +295,19 @@ def train_flow(a, torch, ig):
             if fq is not None:
                 fq[done:done + n] = fb / winv
             done += n
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
         flow.train(yq, wq, steps=a.flow_refine_steps, batch=a.flow_batch, lr=a.flow_lr * 0.5, seed=s0 + 1 + rnd,
                    log=log,
                    val=(y[nt:], f[nt:]), eval_every=a.flow_eval_every // 2, mix=a.flow_mix,
                    f=fq, beta=a.flow_refine_beta)
         flow.stage_steps.append(flow.steps_run)
+        torch.cuda.synchronize()
+        stage_s += [t2 - t1, time.perf_counter() - t2]
         del yq, wq, fq
     del y, f
     torch.cuda.synchronize()
+    flow.stage_seconds = stage_s      # stage 1, then per later stage: its pairs, its training
     return flow, t_pairs, time.perf_counter() - t0
 
 
@@ -417,6 +426,7 @@ def run_flow_compare(a, torch, main):
                  "stage_steps_max": [a.flow_steps] + [a.flow_refine_steps] * (a.flow_refine_rounds
                                                                               if a.flow_refine_steps else 0),
                  "stage_steps_run": list(getattr(flow, "stage_steps", [])),
+                 "stage_seconds": list(getattr(flow, "stage_seconds", [])),
                  "batch": a.flow_batch, "defensive_mixture": a.flow_mix, "patience": a.flow_patience, "seed": a.flow_seed,
                  "loss_beta": [a.flow_beta] + [a.flow_refine_beta] * (a.flow_refine_rounds if a.flow_refine_steps else 0)},
         "gain_unweighted_per_s_to_target": gain("unweighted_per_s_to_target"),
@@ -648,7 +658,7 @@ def parse_args(argv=None):
                     help="weight of the uniform (plain VEGAS) component of the defensive mixture q' = (1-m) q + m")
     ap.add_argument("--flow-eval-every", type=int, default=250)
     ap.add_argument("--flow-seed", type=int, default=1, help="initial weights and minibatch draws of the flow")
-    ap.add_argument("--flow-patience", type=int, default=2,
+    ap.add_argument("--flow-patience", type=int, default=1,
                     help="stage 1 stops after this many held-out evaluations without a new best")
     ap.add_argument("--flow-refine-steps", type=int, default=1000)
     ap.add_argument("--flow-refine-rounds", type=int, default=1)

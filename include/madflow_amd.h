@@ -119,6 +119,9 @@ int mfx_rambo(const double* xrand, int64_t nev, int xstride, int n, const double
 /* Philox4x32-10 uniforms in (0,1): out (nev, ndim) f64, counter (ev0+e, dim/2, iteration), key (seed). */
 int mfx_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration, double* out,
                         void* stream);
+/* The same uniforms rounded to fp32 (round to nearest). */
+int mfx_philox_uniforms_f32(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration, float* out,
+                            void* stream);
 
 /* ------------------------------------------------------ VEGAS integration */
 /* Phase-space / integrand configuration (plain C struct). */
@@ -288,6 +291,19 @@ int mfx_flow_seed(const double* w, const double* f, const float* logq, int64_t n
 /* Adam update of the fp32 parameters (gradient clipped to +-clip, non-finite -> 0). */
 int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
                   float beta2, float eps, int step, float clip, void* stream);
+/* Normalised running sum of n training weights (f64; negative or non-finite weights count as
+ * 0), the cdf that mfx_flow_minibatch and mfx_flow_draw_forward search; the sums run in a
+ * fixed order.  workspace: mfx_flow_cdf_workspace(n) bytes of device memory, whose last
+ * double receives the total (0: every weight is 0, and cdf is 0).  No reference counterpart. */
+int64_t mfx_flow_cdf_workspace(int64_t n);
+int mfx_flow_cdf(const double* w, int64_t n, double* cdf, double* workspace, int64_t workspace_bytes, void* stream);
+/* sum_i f_i^2 / ((1 - mix) exp(logq_i) + mix) in f64, fixed order: the second moment of the
+ * importance weights of the defensive mixture at n points.  work: 257 doubles of device
+ * memory; the sum lands in work[256]. */
+int mfx_flow_moment(const float* logq, const double* f, int64_t n, double mix, double* work, void* stream);
+/* winv_i = 1 / ((1 - mix) q_i + mix), f64 (n,), from the log-dets ld of mfx_flow_apply:
+ * log q = ld after a forward pass (inverse = 0), -ld after an inverse pass (inverse = 1). */
+int mfx_flow_mixture_winv(const float* ld, int64_t n, int inverse, double mix, double* winv, void* stream);
 /* mfx_flow_adam on every layer's parameters followed by mfx_flow_cast's bf16 copies of the
  * updated weights, in one launch. */
 int mfx_flow_adam_cast(int D, int L, const int* masks, float* params, uint16_t* params_bf16, const float* grad,

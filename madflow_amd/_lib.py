@@ -40,6 +40,7 @@ _SIGNATURES = {
     "mfx_aloha": ([c_int, c_vp, c_int, c_i64, c_vp, c_dbl, c_dbl, c_vp, c_int, c_vp], c_int),
     "mfx_rambo": ([c_vp, c_i64, c_int, c_int, c_vp, ctypes.c_double, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "mfx_philox_uniforms": ([c_u64, c_i64, c_int, c_u32, c_u32, c_vp, c_vp], c_int),
+    "mfx_philox_uniforms_f32": ([c_u64, c_i64, c_int, c_u32, c_u32, c_vp, c_vp], c_int),
     "mfx_integrand_workspace_size": ([c_i64, c_int, c_int], c_i64),
     "mfx_unweight_workspace_size": ([c_i64], c_i64),
     "mfx_unweight": ([c_vp, c_i64, c_dbl, c_u32, c_u32, c_u64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp], c_int),

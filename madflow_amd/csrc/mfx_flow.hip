@@ -970,6 +970,108 @@ __global__ void k_adam_cast(FlowGeom geo, float* __restrict__ P, const float* __
   }
 }
 
+// ------------------------------------------------------------------ training-set and sampling helpers
+// Normalised running sum of the training weights, max(w, 0) (a weight that is not a number
+// counts as 0), in fp64: per-block inclusive sums over CDF_T * CDF_V consecutive weights
+// (each thread sums CDF_V in order, the thread totals are scanned in LDS in a fixed order),
+// then the block totals are scanned by one block, then every entry gets its block's offset
+// and is divided by the total.  The summation order is fixed, so the result is reproducible.
+constexpr int CDF_T = 256, CDF_V = 16, CDF_B = CDF_T * CDF_V;
+
+__device__ __forceinline__ double wclamp(double w) { return w > 0.0 ? w : 0.0; }
+
+__global__ void __launch_bounds__(CDF_T) k_cdf_blocks(const double* __restrict__ w, int64_t n,
+                                                      double* __restrict__ cdf, double* __restrict__ btot) {
+  __shared__ double ts[CDF_T];
+  const int t = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * CDF_B + (int64_t)t * CDF_V;
+  double run = 0.0;
+  for (int v = 0; v < CDF_V; ++v) {
+    const int64_t i = i0 + v;
+    if (i < n) {
+      run += wclamp(w[i]);
+      cdf[i] = run;                     // the thread's own running sum, offset below
+    }
+  }
+  ts[t] = run;
+  __syncthreads();
+  if (t == 0) {                          // exclusive scan of the thread totals, in order
+    double acc = 0.0;
+    for (int k = 0; k < CDF_T; ++k) {
+      const double v = ts[k];
+      ts[k] = acc;
+      acc += v;
+    }
+    btot[blockIdx.x] = acc;
+  }
+  __syncthreads();
+  const double off = ts[t];
+  for (int v = 0; v < CDF_V; ++v) {
+    const int64_t i = i0 + v;
+    if (i < n) cdf[i] += off;
+  }
+}
+
+// exclusive scan of the block totals in place, in order (one thread); the total goes to
+// btot[nb]
+__global__ void k_cdf_totals(double* __restrict__ btot, int nb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double v = btot[b];
+    btot[b] = acc;
+    acc += v;
+  }
+  btot[nb] = acc;
+}
+
+__global__ void k_cdf_finish(double* __restrict__ cdf, int64_t n, const double* __restrict__ btot, int nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tot = btot[nb];
+  const double v = cdf[i] + btot[i / CDF_B];
+  cdf[i] = tot > 0.0 ? (i == n - 1 ? 1.0 : v / tot) : 0.0;
+}
+
+// Second moment of defensive-mixture importance weights: sum_i f_i^2 / ((1 - mix) e^{logq_i} + mix),
+// in fp64: MOM_G blocks each sum a contiguous range (thread partials in order, then the
+// block's threads in order), one thread sums the block partials in order.
+constexpr int MOM_G = 256, MOM_T = 256;
+
+__global__ void __launch_bounds__(MOM_T) k_moment(const float* __restrict__ logq, const double* __restrict__ f,
+                                                  int64_t n, double mix, double* __restrict__ part) {
+  __shared__ double ts[MOM_T];
+  const int64_t per = (n + MOM_G - 1) / MOM_G;
+  const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  double acc = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += MOM_T) {
+    const double q = (1.0 - mix) * exp((double)logq[i]) + mix;
+    acc += f[i] * f[i] / q;
+  }
+  ts[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < MOM_T; ++k) s += ts[k];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ void k_moment_sum(double* __restrict__ part) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < MOM_G; ++b) s += part[b];
+  part[MOM_G] = s;
+}
+
+// 1 / q'(y) of the defensive mixture q' = (1 - mix) q + mix, from the log-dets of a forward
+// pass (log q = ld) or of an inverse pass (log q = -ld)
+__global__ void k_mixture_winv(const float* __restrict__ ld, int64_t n, double sign, double mix,
+                               double* __restrict__ winv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) winv[i] = 1.0 / ((1.0 - mix) * exp(sign * (double)ld[i]) + mix);
+}
+
 }  // namespace flow
 }  // namespace mfx
 
@@ -1231,6 +1333,43 @@ int mfx_flow_adam_cast(int D, int L, const int* masks, float* params, uint16_t* 
   const float bc1 = 1.f - powf(beta1, (float)step), bc2 = 1.f - powf(beta2, (float)step);
   hipLaunchKernelGGL(k_adam_cast, dim3(64, L), dim3(256), 0, (hipStream_t)stream, G, params, grad, m1, m2, lr, beta1,
                      beta2, eps, bc1, bc2, clip, (unsigned short*)params_bf16);
+  return hip_status(hipGetLastError());
+}
+
+
+int64_t mfx_flow_cdf_workspace(int64_t n) {
+  if (n < 0) return -1;
+  return (int64_t)(((n + CDF_B - 1) / CDF_B + 1) * 8);
+}
+
+int mfx_flow_cdf(const double* w, int64_t n, double* cdf, double* workspace, int64_t workspace_bytes, void* stream) {
+  if (n <= 0) return fail(-2, "n > 0 required%s", "");
+  if (!w || !cdf || !workspace) return fail(-3, "null pointer%s", "");
+  const int64_t nb = (n + CDF_B - 1) / CDF_B;
+  if (nb > (1 << 30) || workspace_bytes < (nb + 1) * 8) return fail(-2, "workspace smaller than mfx_flow_cdf_workspace%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cdf_blocks, dim3((unsigned)nb), dim3(CDF_T), 0, s, w, n, cdf, workspace);
+  hipLaunchKernelGGL(k_cdf_totals, dim3(1), dim3(64), 0, s, workspace, (int)nb);
+  hipLaunchKernelGGL(k_cdf_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cdf, n, (const double*)workspace,
+                     (int)nb);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_moment(const float* logq, const double* f, int64_t n, double mix, double* work, void* stream) {
+  if (n <= 0) return fail(-2, "n > 0 required%s", "");
+  if (!logq || !f || !work) return fail(-3, "null pointer%s", "");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_moment, dim3(MOM_G), dim3(MOM_T), 0, s, logq, f, n, mix, work);
+  hipLaunchKernelGGL(k_moment_sum, dim3(1), dim3(64), 0, s, work);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_mixture_winv(const float* ld, int64_t n, int inverse, double mix, double* winv, void* stream) {
+  if (n < 0) return fail(-2, "n >= 0 required%s", "");
+  if (n == 0) return 0;
+  if (!ld || !winv) return fail(-3, "null pointer%s", "");
+  hipLaunchKernelGGL(k_mixture_winv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ld, n,
+                     inverse ? -1.0 : 1.0, mix, winv);
   return hip_status(hipGetLastError());
 }
 

@@ -67,6 +67,19 @@ __global__ void k_philox(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uin
   }
 }
 
+// the same uniforms rounded to fp32 (round to nearest), for consumers that work in fp32
+__global__ void k_philox_f32(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration,
+                             float* __restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nev) return;
+  for (int j = 0; 2 * j < ndim; ++j) {
+    double u0, u1;
+    philox_pair(ev0 + e, (uint32_t)j, seed, iteration, u0, u1);
+    out[e * ndim + 2 * j] = (float)u0;
+    if (2 * j + 1 < ndim) out[e * ndim + 2 * j + 1] = (float)u1;
+  }
+}
+
 }  // namespace mfx
 
 using namespace mfx;
@@ -103,6 +116,16 @@ int mfx_philox_uniforms(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint
   if (!out) return fail(-3, "null pointer%s", "");
   dim3 grid((unsigned)((nev + 255) / 256)), blk(256);
   hipLaunchKernelGGL(k_philox, grid, blk, 0, (hipStream_t)stream, ev0, nev, ndim, seed, iteration, out);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_philox_uniforms_f32(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, uint32_t iteration, float* out,
+                            void* stream) {
+  if (nev < 0 || ndim < 1) return fail(-2, "nev must be >= 0 and ndim >= 1%s", "");
+  if (nev == 0) return 0;
+  if (!out) return fail(-3, "null pointer%s", "");
+  dim3 grid((unsigned)((nev + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(k_philox_f32, grid, blk, 0, (hipStream_t)stream, ev0, nev, ndim, seed, iteration, out);
   return hip_status(hipGetLastError());
 }
 

@@ -67,6 +67,15 @@ def _sig(lib):
     lib.mfx_flow_draw_forward.restype = i
     lib.mfx_flow_adam_cast.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, f, f, f, i, f, vp]
     lib.mfx_flow_adam_cast.restype = i
+    d = ctypes.c_double
+    lib.mfx_flow_cdf_workspace.argtypes = [i64]
+    lib.mfx_flow_cdf_workspace.restype = i64
+    lib.mfx_flow_cdf.argtypes = [vp, i64, vp, vp, i64, vp]
+    lib.mfx_flow_cdf.restype = i
+    lib.mfx_flow_moment.argtypes = [vp, vp, i64, d, vp, vp]
+    lib.mfx_flow_moment.restype = i
+    lib.mfx_flow_mixture_winv.argtypes = [vp, i64, i, d, vp, vp]
+    lib.mfx_flow_mixture_winv.restype = i
 
 
 def _p(t):
@@ -150,10 +159,10 @@ class RQSFlow:
     def log_prob(self, y):
         return self._apply(y, False)[1]
 
-    def sample(self, u):
-        """u -> y and log q(y)."""
+    def sample(self, u, logq=True):
+        """u -> y and log q(y) (None with logq=False)."""
         y, ld_inv = self._apply(u, True)
-        return y, -ld_inv
+        return y, (-ld_inv if logq else None)
 
     # ----------------------------------------------------------------- training
     def grad(self, y, w, f=None, beta=0.0, need_loss=True):
@@ -196,12 +205,15 @@ class RQSFlow:
         """E_uniform[f^2 / q'] over uniform points y with integrand values f, where
         q' = (1 - mix) q + mix: the second moment of the importance weights f/q' of
         samples drawn from q' (the variance up to the fixed squared integral)."""
+        f = _lib.require_device_tensor(f, torch.float64, "f")
+        work = torch.empty(257, dtype=torch.float64, device=y.device)
         tot = 0.0
         for a in range(0, y.shape[0], chunk):
-            lq = self.log_prob(y[a:a + chunk].contiguous()).double()
-            qm = (1.0 - mix) * torch.exp(lq) + mix
-            ff = f[a:a + chunk].double()
-            tot += float((ff * ff / qm).sum())
+            lq = self.log_prob(y[a:a + chunk].contiguous())
+            ff = f[a:a + chunk]
+            _lib.check(self.lib.mfx_flow_moment(_p(lq), _p(ff), ff.shape[0], float(mix), _p(work),
+                                                _lib.stream_ptr(y.device)))
+            tot += float(work[256])
         return tot / y.shape[0]
 
     def train(self, y, w, steps=1000, batch=4096, lr=1e-3, seed=0, log=None, lr_decay=True, resample=True,
@@ -232,12 +244,14 @@ class RQSFlow:
         losses = []
         best = None
         if resample:
-            w64 = w.to(torch.float64).clamp(min=0)
-            cdf = torch.cumsum(w64, 0)
-            total = float(cdf[-1])
-            if not total > 0:
+            w64 = _lib.require_device_tensor(w, None, "w")
+            if w64.dtype != torch.float64:
+                w64 = w64.to(torch.float64)
+            cdf = torch.empty(n, dtype=torch.float64, device=w64.device)
+            ws = torch.empty(self.lib.mfx_flow_cdf_workspace(n) // 8, dtype=torch.float64, device=w64.device)
+            _lib.check(self.lib.mfx_flow_cdf(_p(w64), n, _p(cdf), _p(ws), ws.numel() * 8, _lib.stream_ptr(w64.device)))
+            if not float(ws[-1]) > 0:
                 raise ValueError("train: the weights sum to zero")
-            cdf /= total
             y = _lib.require_device_tensor(y, torch.float32, "y")
             fc = _lib.require_device_tensor(f, None, "f").to(torch.float64) if beta else None
             # device workspace of the fused step, allocated once: the minibatch is drawn
@@ -267,7 +281,7 @@ class RQSFlow:
                 break
             cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) if lr_decay else 1.0)
             show = log and val is None and (it % max(1, steps // 10) == 0 or it == steps - 1)
-            need_loss = show or it % 50 == 0
+            need_loss = show
             if resample:
                 m = self._m()
                 _lib.check(self.lib.mfx_flow_draw_forward(self.D, self.L, m, P["par"], P["pb"], P["cdf"], n, P["y"],
@@ -289,7 +303,6 @@ class RQSFlow:
             self.adam_step(grad, lr=cur_lr)
             if show:
                 log("flow step %d loss %.5f" % (it, float(loss)))
-            if it % 50 == 0:
                 losses.append(loss)
         self.steps_run = it          # optimiser steps taken (early stopping may end before steps)
         if best is not None:
@@ -309,12 +322,14 @@ class RQSFlow:
         nu = int(round(mix * n))
         y = torch.empty_like(u)
         winv = torch.empty(n, dtype=torch.float64, device=u.device)
+        st = _lib.stream_ptr(u.device)
         if nu > 0:
             yu = u_plain[:nu].contiguous()
             y[:nu] = yu
-            winv[:nu] = 1.0 / ((1.0 - mix) * torch.exp(self.log_prob(yu).double()) + mix)
+            _, ld = self._apply(yu, False)
+            _lib.check(self.lib.mfx_flow_mixture_winv(_p(ld), nu, 0, float(mix), _p(winv[:nu]), st))
         if nu < n:
-            yf, lq = self.sample(u[nu:].contiguous())
+            yf, ld_inv = self._apply(u[nu:].contiguous(), True)
             y[nu:] = yf
-            winv[nu:] = 1.0 / ((1.0 - mix) * torch.exp(lq.double()) + mix)
+            _lib.check(self.lib.mfx_flow_mixture_winv(_p(ld_inv), n - nu, 1, float(mix), _p(winv[nu:]), st))
         return y, winv

@@ -153,8 +153,8 @@ class MultiChannelIntegrand(MEIntegrand):
         s = _lib.stream_ptr(self.device)
         y_in = None
         if c == self.nch:
-            u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0).float().contiguous()
-            y_in, _ = self.flow.sample(u)
+            u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0, dtype=torch.float32)
+            y_in, _ = self.flow.sample(u, logq=False)
             y_in = y_in.contiguous()
         alpha = np.ascontiguousarray((1.0 - self.alpha_learned) * self.alpha, dtype=np.float64)
         npass = ctypes.c_int64(0)

@@ -24,11 +24,15 @@ PROCESS_KIN = {
 }
 
 
-def philox_uniforms(nev, ndim, seed, iteration, ev0=0, device="cuda"):
-    out = torch.empty((nev, ndim), dtype=torch.float64, device=device)
+def philox_uniforms(nev, ndim, seed, iteration, ev0=0, device="cuda", dtype=torch.float64):
+    """(nev, ndim) uniforms of the Philox stream (seed, iteration) for events ev0..ev0+nev-1,
+    float64 or (dtype=torch.float32) the same values rounded to float32."""
+    out = torch.empty((nev, ndim), dtype=dtype, device=device)
     if nev:
-        _lib.check(_lib.load().mfx_philox_uniforms(ev0, nev, ndim, seed, iteration, ctypes.c_void_p(out.data_ptr()),
-                                                   _lib.stream_ptr(out.device)))
+        fn = _lib.load().mfx_philox_uniforms_f32 if dtype == torch.float32 else _lib.load().mfx_philox_uniforms
+        if dtype not in (torch.float32, torch.float64):
+            raise TypeError("philox_uniforms: dtype must be float32 or float64")
+        _lib.check(fn(ev0, nev, ndim, seed, iteration, ctypes.c_void_p(out.data_ptr()), _lib.stream_ptr(out.device)))
     return out
 
 

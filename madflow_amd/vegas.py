@@ -233,7 +233,7 @@ class MEIntegrand:
         while ev0 < n_events:
             n = min(self.events_limit, n_events - ev0)
             if flow is not None:
-                u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0).float()
+                u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0, dtype=torch.float32)
                 ys, winv = flow.sample_mixture(u, u, mix)
                 _, _, f = self.chunk(ev0, n, iteration, keep_f=True, y_in=ys.contiguous(), w_in=winv.contiguous())
             else:
@@ -322,7 +322,7 @@ class MEIntegrand:
         ev0 = 0
         while ev0 < n_events:
             n = min(self.events_limit, n_events - ev0)
-            u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0).float()
+            u = philox_uniforms(n, self.ndim, seed, iteration, ev0=ev0, dtype=torch.float32)
             ys, winv = flow.sample_mixture(u, u, mix)
             r, k, _ = self.chunk(ev0, n, iteration, y_in=ys.contiguous(), w_in=winv.contiguous())
             tot = r if tot is None else _merge(tot, r)

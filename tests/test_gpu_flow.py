@@ -264,3 +264,50 @@ def test_flow_wide_layers(dev):
     cos = float(np.dot(g, g_ref) / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
     print("wide-layer grad cosine", cos, flush=True)
     assert cos > 0.999
+
+
+@pytest.mark.gpu
+def test_cdf_moment_mixture_weights(dev):
+    """mfx_flow_cdf equals the normalised running sum of max(w, 0) (a non-finite weight
+    counts as 0) with its total in the workspace's last double; mfx_flow_moment equals
+    sum f^2 / ((1 - mix) e^logq + mix); mfx_flow_mixture_winv equals 1 / ((1 - mix) q + mix)
+    for forward and inverse log-dets; the fp32 Philox uniforms are the fp64 ones rounded."""
+    import ctypes
+    from madflow_amd import _lib
+    from madflow_amd.phasespace_gpu import philox_uniforms
+    print("case flow cdf / moment / mixture weights", flush=True)
+    flow = RQSFlow(3, layers=2, seed=1, device=dev)
+    lib = flow.lib
+    rng = np.random.default_rng(41)
+    n = 100_003
+    w = rng.random(n) ** 3 - 0.1
+    w[rng.integers(0, n, 50)] = np.nan
+    wc = np.where(np.isfinite(w) & (w > 0), w, 0.0)
+    ref = np.cumsum(wc)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    w_t = t(w)
+    cdf = torch.empty(n, dtype=torch.float64, device=dev)
+    ws = torch.empty(lib.mfx_flow_cdf_workspace(n) // 8, dtype=torch.float64, device=dev)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.check(lib.mfx_flow_cdf(vp(w_t), n, vp(cdf), vp(ws), ws.numel() * 8, _lib.stream_ptr(dev)))
+    c = cdf.cpu().numpy()
+    assert abs(float(ws[-1]) / ref[-1] - 1) < 1e-12
+    np.testing.assert_allclose(c, ref / ref[-1], rtol=1e-12, atol=1e-15)
+    assert c[-1] == 1.0 and np.all(np.diff(c) >= 0)
+    lq = rng.normal(size=n).astype(np.float32)
+    f = rng.random(n) + 0.01
+    for mix in (0.0, 0.3):
+        work = torch.empty(257, dtype=torch.float64, device=dev)
+        lq_t, f_t = t(lq), t(f)
+        _lib.check(lib.mfx_flow_moment(vp(lq_t), vp(f_t), n, mix, vp(work), _lib.stream_ptr(dev)))
+        ref_m = np.sum(f * f / ((1 - mix) * np.exp(lq.astype(np.float64)) + mix))
+        assert abs(float(work[256]) / ref_m - 1) < 1e-12
+        for inv in (0, 1):
+            winv = torch.empty(n, dtype=torch.float64, device=dev)
+            _lib.check(lib.mfx_flow_mixture_winv(vp(lq_t), n, inv, mix, vp(winv), _lib.stream_ptr(dev)))
+            sgn = -1.0 if inv else 1.0
+            np.testing.assert_allclose(winv.cpu().numpy(), 1.0 / ((1 - mix) * np.exp(sgn * lq.astype(np.float64)) + mix),
+                                       rtol=1e-14)
+    u64 = philox_uniforms(1000, 7, 5, 2, ev0=33)
+    u32 = philox_uniforms(1000, 7, 5, 2, ev0=33, dtype=torch.float32)
+    assert torch.equal(u32, u64.float())

@@ -40,6 +40,7 @@ UNITS = [
     "procs/gg_ttxggg.hip",
     "procs/uux_ttx.hip",
     "procs/uux_ttxg.hip",
+    "procs/uux_ttxgg.hip",
 ]
 
 

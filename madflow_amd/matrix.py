@@ -36,6 +36,7 @@ _PROC_PARAMS = {
     "gg_ttxggg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "uux_ttx": ["mdl_MT", "mdl_WT", "GC_11"],
     "uux_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
+    "uux_ttxgg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
 }
 _GC_SLOT = {"GC_10": 0, "GC_11": 1, "GC_12": 2}
 

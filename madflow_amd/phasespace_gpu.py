@@ -21,6 +21,7 @@ PROCESS_KIN = {
     "gg_ttxggg": (5000.0, [173.0, 173.0, 0.0, 0.0, 0.0]),
     "uux_ttx": (1000.0, [173.0, 173.0]),
     "uux_ttxg": (2000.0, [173.0, 173.0, 0.0]),
+    "uux_ttxgg": (3000.0, [173.0, 173.0, 0.0, 0.0]),
 }
 
 

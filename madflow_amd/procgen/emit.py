@@ -28,7 +28,7 @@ PROC_DIR = os.path.join(PKG, "processes")
 INC_DIR = os.path.join(PKG, "csrc", "procs")
 
 NGLUONS = (0, 1, 2, 3)        # g g > t t~ + ng g
-QQ_NGLUONS = (0, 1)           # u u~ > t t~ + ng g
+QQ_NGLUONS = (0, 1, 2)        # u u~ > t t~ + ng g
 
 
 # ---------------------------------------------------------------- HIP emission
@@ -511,7 +511,7 @@ def emit_hip_unit(uname):
 # both precisions; for g g > t t~ g g g the n! jamps of the fused form do not fit the
 # register file next to a 1,890-amplitude program.
 KERNELS = {"gg_ttx": ("fused",), "gg_ttxg": ("fused",), "gg_ttxgg": ("ordered", "fused"),
-           "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",)}
+           "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",), "uux_ttxgg": ("fused",)}
 # helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
 MULTI_K = {}
 

@@ -58,6 +58,7 @@ PROCESS_SETUP = {
     "gg_ttxggg": (5, [173.0, 173.0, 0.0, 0.0, 0.0]),
     "uux_ttx": (2, [173.0, 173.0]),
     "uux_ttxg": (3, [173.0, 173.0, 0.0]),
+    "uux_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
 }
 
 

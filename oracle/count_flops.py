@@ -17,7 +17,7 @@ OUT = os.path.join(os.path.dirname(O.HERE), "madflow_amd", "processes", "flops_p
 
 def main():
     res = {}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg"):
         try:
             O.load_process(name)
         except FileNotFoundError:

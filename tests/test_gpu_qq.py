@@ -1,4 +1,4 @@
-"""GPU parity of the generated u u~ > t t~ (+ g) kernels (two fermion lines, rational
+"""GPU parity of the generated u u~ > t t~ (+ g, + g g) kernels (two fermion lines, rational
 jamp coefficients, rows that vanish by chirality skipped in the kernel) against the CPU
 oracle, which sums every row.
 
@@ -13,7 +13,7 @@ from oracle import oracle as O
 from tests.helpers import phase_space_points
 from madflow_amd.matrix import load_matrix
 
-PROCS = ["uux_ttx", "uux_ttxg"]
+PROCS = ["uux_ttx", "uux_ttxg", "uux_ttxgg"]
 
 
 def _vals(m, ref_gs=True):
@@ -21,9 +21,47 @@ def _vals(m, ref_gs=True):
     return rp, coup, [rp[0], rp[1]] + [coup[c] for c in m.params[2:]]
 
 
+def _cancellation(name, ps, rp, coup):
+    """Per event, how much the Feynman-diagram sums cancel: sum over live rows of
+    sum_ij |CF_ij| (sum_d |c_id A_d|)(sum_d |c_jd A_d|) over |M|^2 (1 when nothing
+    cancels).  The amplitudes A_d come from the oracle with one jamp per amplitude."""
+    import copy
+    from fractions import Fraction
+    d = O.load_process(name)
+    na = d["ngraphs"]
+    m = copy.deepcopy(d)
+    m["jamp"] = [[[a, "1", 0]] for a in range(na)]
+    m["ncolor"], m["color_num"], m["color_denom"] = na, np.eye(na, dtype=int).tolist(), [1] * na
+    key = name + "_amplitudes"
+    O._cache[key] = O.EncodedProcess(m)
+    try:
+        C = np.zeros((len(d["jamp"]), na), complex)
+        for c, terms in enumerate(d["jamp"]):
+            for a, re, im in terms:
+                C[c, a] += float(Fraction(re)) + 1j * float(Fraction(im))
+        cf = np.array(d["color_num"], float) / np.array(d["color_denom"], float)[:, None]
+        num = np.zeros(len(ps))
+        den = np.zeros(len(ps))
+        for h in d["helicities"]:
+            if h[0] == h[1]:
+                continue
+            _, A = O.matrix_hel(key, ps, h, rp, coup, ref_compat=True, want_jamp=True)
+            A = np.asarray(A)
+            J, Jabs = A @ C.T, np.abs(A) @ np.abs(C).T
+            num += np.einsum("ei,ij,ej->e", Jabs, np.abs(cf), Jabs)
+            den += np.einsum("ei,ij,ej->e", np.conj(J), cf, J).real
+    finally:
+        del O._cache[key]
+    return num / den
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", PROCS)
 def test_qq_fp64_parity(dev, name):
+    """1e-12 per event where the diagram sums cancel less than 100-fold, 1e-14 kappa beyond
+    (kappa the cancellation factor): rounding at 1e-16 per operation is amplified by
+    kappa whatever the order of the operations, and the kernel's order is not the
+    oracle's.  Among the 512 uncut u u~ > t t~ g g points the largest kappa is 335."""
     print("case qq fp64", name, flush=True)
     m, _ = load_matrix(name, precision=64, ref_compat=True)
     ps, _ = phase_space_points(name, 512, seed=51)
@@ -31,8 +69,12 @@ def test_qq_fp64_parity(dev, name):
     ref = O.smatrix(name, ps, rp, coup, ref_compat=True)
     got = m.smatrix(torch.from_numpy(ps).to(dev), *vals).cpu().numpy()
     rel = np.abs(got / ref - 1)
-    print(name, "fp64 max rel", rel.max(), flush=True)
-    assert rel.max() < 1e-12
+    kappa = _cancellation(name, ps, rp, coup)
+    w = int(np.argmax(rel))
+    print(name, "fp64 max rel %.3g at event %d (cancellation %.1f); events with cancellation > 100: %d, their max "
+          "rel / cancellation %.3g" % (rel.max(), w, kappa[w], int(np.sum(kappa > 100)),
+                                       float(np.max(rel / kappa, initial=0.0, where=kappa > 100))), flush=True)
+    assert np.all(rel < np.maximum(1e-12, 1e-14 * kappa))
 
 
 @pytest.mark.gpu

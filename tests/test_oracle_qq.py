@@ -1,4 +1,4 @@
-"""Pins the oracle's q q~ > t t~ (+ g) processes (two fermion lines), for which the
+"""Pins the oracle's q q~ > t t~ (+ g, + g g) processes (two fermion lines), for which the
 reference ships no vectors:
 
 * u u~ > t t~ against the textbook spin- and colour-averaged |M|^2 (Ellis, Stirling,
@@ -7,8 +7,13 @@ reference ships no vectors:
 * every helicity row of u u~ > t t~ and u u~ > t t~ g against the explicit-colour
   brute force (tests/bruteforce.py: each Feynman diagram written out with Dirac and
   Gell-Mann matrices, no colour basis, no jamps);
-* gauge (BRST) invariance of the radiated gluon, Lorentz invariance, the helicity-sum
-  denominator 4 x 9 x k!, and the rows that vanish by chirality on the massless line.
+* gauge (BRST) invariance of every radiated gluon, Lorentz invariance, the helicity-sum
+  denominator 4 x 9 x k!, and the rows that vanish by chirality on the massless line;
+* u u~ > t t~ g g: Bose symmetry of the two gluons, and its soft-gluon limit against the
+  eikonal factor applied to the explicit-colour brute force of u u~ > t t~ g (colour
+  operators on every leg, colour conservation checked), which fixes its normalisation,
+  colour factors and the diagrams that emit the soft gluon from external legs; BRST fixes
+  the rest.
 """
 
 import itertools
@@ -54,7 +59,7 @@ def test_qq_vanishing_rows():
     """Exactly the rows whose incoming quark and antiquark carry the same helicity label
     vanish (chirality on the massless line); the generated kernels skip those rows."""
     from madflow_amd.procgen.emit import zero_helicity_pairs
-    for name in ("uux_ttx", "uux_ttxg"):
+    for name in ("uux_ttx", "uux_ttxg", "uux_ttxgg"):
         d = O.load_process(name)
         assert zero_helicity_pairs(d) == [(0, 1)]
         ps, _ = phase_space_points(name, 4, seed=43)
@@ -68,23 +73,29 @@ def test_qq_vanishing_rows():
     assert zero_helicity_pairs(O.load_process("gg_ttxg")) == []
 
 
-def test_qq_brst_gauge_invariance():
-    ps, _ = phase_space_points("uux_ttxg", 3, seed=44)
-    d = O.load_process("uux_ttxg")
+@pytest.mark.parametrize("name,gluon", [("uux_ttxg", 4), ("uux_ttxgg", 4), ("uux_ttxgg", 5)])
+def test_qq_brst_gauge_invariance(name, gluon):
+    """Replacing a gluon's polarisation by its momentum (nhel = 4) makes every jamp vanish:
+    a diagram missing from the gauge-invariant set, or one with a wrong relative sign,
+    breaks this."""
+    ps, _ = phase_space_points(name, 3, seed=44)
+    d = O.load_process(name)
     rp, coup = O.default_params(wt=0.0)
-    for row in (0, 9, 22):
+    for row in (0, 9, 22, 37, 50):
+        if row >= len(d["helicities"]):
+            continue
         hel = list(d["helicities"][row])
         if hel[0] == hel[1]:
             continue
-        _, jp = O.matrix_hel("uux_ttxg", ps, hel, rp, coup, want_jamp=True)
+        _, jp = O.matrix_hel(name, ps, hel, rp, coup, want_jamp=True)
         h = list(hel)
-        h[4] = 4
-        _, jb = O.matrix_hel("uux_ttxg", ps, h, rp, coup, want_jamp=True)
+        h[gluon] = 4
+        _, jb = O.matrix_hel(name, ps, h, rp, coup, want_jamp=True)
         assert np.abs(jp).max() > 0
         assert np.abs(jb).max() < 1e-11 * np.abs(jp).max(), row
 
 
-@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg"])
+@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "uux_ttxgg"])
 def test_qq_lorentz_invariance(name):
     ps, _ = phase_space_points(name, 4, seed=45)
     rp, coup = O.default_params(wt=0.0)
@@ -93,7 +104,7 @@ def test_qq_lorentz_invariance(name):
     np.testing.assert_allclose(a, b, rtol=1e-10)
 
 
-@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1)])
+@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1), ("uux_ttxgg", 2)])
 def test_qq_helicity_sum(name, k):
     """smatrix = sum over all 2^nexternal rows / (4 spins x 9 colours x k!)."""
     ps, _ = phase_space_points(name, 3, seed=46)
@@ -127,3 +138,85 @@ def test_qq_bruteforce_catches_a_dropped_diagram():
         assert abs(mut / bf - 1) > 1e-3
     finally:
         del O._cache["uux_ttxg_mutant"]
+
+
+def test_uux_ttxgg_bose_symmetry():
+    ps, _ = phase_space_points("uux_ttxgg", 4, seed=48)
+    rp, coup = O.default_params()
+    sw = ps.copy()
+    sw[:, [4, 5]] = ps[:, [5, 4]]
+    np.testing.assert_allclose(O.smatrix("uux_ttxgg", ps, rp, coup), O.smatrix("uux_ttxgg", sw, rp, coup), rtol=1e-12)
+
+
+def _colour_ops(M):
+    """T_i^b M of the five legs of the brute-force u u~ > t t~ g tensor M[qb, q, t, tb, a]:
+    the antiquark's index transforms with T, the quark's with -T^T, the top's with T, the
+    anti-top's with -T^T, the gluon's with (T^b)_{ac} = -i f^{bac}."""
+    T, F = BF.T, BF.F
+    return [np.einsum("bij,j...->bi...", T, M),
+            -np.einsum("bji,xj...->bxi...", T, M),
+            np.einsum("bij,xyj...->bxyi...", T, M),
+            -np.einsum("bji,xyzj...->bxyzi...", T, M),
+            np.einsum("bac,xyzwc->bxyzwa", -1j * F, M)]
+
+
+def _soft_ratio(lam, seed=5, sqrts=3000.0, name="uux_ttxgg"):
+    """|M|^2 of u u~ > t t~ g g with gluon 5 of energy lam sqrt(s), summed over every
+    helicity and colour, over its eikonal limit
+        -g^2 sum_ij (p_i.p_j) / ((p_i.k)(p_j.k)) <M| T_i.T_j |M>
+    with M the explicit-colour brute force of u u~ > t t~ g at the other momenta (the
+    remaining legs recoil: the t t~ g system is a RAMBO point at the reduced mass, boosted
+    to P - k).  Returns the ratio and the colour-conservation residual |sum_i T_i M| / |M|."""
+    rng = np.random.default_rng(seed)
+    E = sqrts / 2
+    p0, p1 = np.array([E, 0, 0, E]), np.array([E, 0, 0, -E])
+    nv = rng.normal(size=3)
+    k = lam * sqrts * np.concatenate([[1.0], nv / np.linalg.norm(nv)])
+    Q = p0 + p1 - k
+    ps, _ = phase_space_points("uux_ttxg", 1, seed=seed + 100, sqrts=np.sqrt(minkowski(Q, Q)))
+    fin = _boost(ps[0, 2:], Q[1:] / Q[0])
+    p6 = np.vstack([p0, p1, fin, k])
+    p5 = np.vstack([p0, p1, fin])
+    rp, coup = O.default_params(wt=0.0)
+    full = sum(O.matrix_hel(name, p6[None], h, rp, coup)[0] for h in O.load_process("uux_ttxgg")["helicities"])
+    gs = 2 * np.sqrt(np.pi * 0.118)
+    legs = [p5[1], p5[0], p5[2], p5[3], p5[4]]          # momenta of the axes qb, q, t, tb, a
+    eik, cons = 0.0, 0.0
+    for h in itertools.product([-1, 1], repeat=5):
+        M = BF.qq_amplitude_tensor(p5, list(h), 173.0, 0.0, gs, np.sqrt(0.5))
+        if not np.any(M):
+            continue
+        ops = _colour_ops(M)
+        cons = max(cons, np.abs(sum(ops)).max() / np.abs(M).max())
+        for i in range(5):
+            for j in range(5):
+                cij = np.sum(np.conj(ops[i]) * ops[j]).real
+                eik -= gs ** 2 * minkowski(legs[i], legs[j]) / (minkowski(legs[i], k) * minkowski(legs[j], k)) * cij
+    return full / eik, cons
+
+
+def test_uux_ttxgg_soft_limit():
+    """The ratio tends to 1 linearly in the soft gluon's energy (measured: 1 + 2.9e-2 at
+    lam = 1e-3, 1 + 2.9e-4 at 1e-5)."""
+    r3, c3 = _soft_ratio(1e-3)
+    r5, c5 = _soft_ratio(1e-5)
+    assert c3 < 1e-13 and c5 < 1e-13
+    assert abs(r5 - 1) < 1e-3
+    assert 30 < (r3 - 1) / (r5 - 1) < 300
+
+
+def test_soft_limit_catches_a_dropped_emission():
+    """Dropping the amplitudes in which gluon 5 leaves the top line directly (from the
+    oracle's jamps) moves the soft ratio far from 1."""
+    import copy
+    d = copy.deepcopy(O.load_process("uux_ttxgg"))
+    wf = {op["out"] for op in d["ops"] if op["kind"] == "wf" and sorted(op.get("legs", [])) == [2, 5]}
+    amps = {op["out"] for op in d["ops"] if op["kind"] == "amp" and wf & set(op["args"])}
+    assert amps
+    d["jamp"] = [[t for t in terms if t[0] not in amps] for terms in d["jamp"]]
+    O._cache["uux_ttxgg_mutant"] = O.EncodedProcess(d)
+    try:
+        r, _ = _soft_ratio(1e-5, name="uux_ttxgg_mutant")
+        assert abs(r - 1) > 0.05
+    finally:
+        del O._cache["uux_ttxgg_mutant"]

@@ -41,6 +41,7 @@ UNITS = [
     "procs/uux_ttx.hip",
     "procs/uux_ttxg.hip",
     "procs/uux_ttxgg.hip",
+    "procs/ug_ttxu.hip",
 ]
 
 

@@ -22,6 +22,7 @@ PROCESS_KIN = {
     "uux_ttx": (1000.0, [173.0, 173.0]),
     "uux_ttxg": (2000.0, [173.0, 173.0, 0.0]),
     "uux_ttxgg": (3000.0, [173.0, 173.0, 0.0, 0.0]),
+    "ug_ttxu": (2000.0, [173.0, 173.0, 0.0]),
 }
 
 

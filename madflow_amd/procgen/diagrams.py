@@ -534,6 +534,60 @@ def build_program_qq(gluons, q_leg, qb_leg, top_leg, atop_leg, externals, nexter
     return prog, len(diags), colours
 
 
+def qg_ttxq():
+    """Process description dict for u g > t t~ u, the crossing of u u~ > t t~ g (incoming
+    gluon, outgoing quark): the same diagrams and colour algebra, with the outgoing quark's
+    row spinor (oxxxxx, nsf = +1) in place of the incoming antiquark's and the incoming
+    gluon's polarisation (vxxxxx, nsv = -1).  The same matrix element serves c, d and s
+    quarks; the mirror process has the gluon in beam 1."""
+    from .colour import colour_matrix_two_lines
+    nexternal = 5
+    kinds = ["quark", "gluon", "top", "atop", "quark_out"]
+    externals = [dict(leg=0, routine="ixxxxx", mass="ZERO", sign=+1),
+                 dict(leg=1, routine="vxxxxx", mass="ZERO", sign=-1),
+                 dict(leg=2, routine="oxxxxx", mass="mdl_MT", sign=+1),
+                 dict(leg=3, routine="ixxxxx", mass="mdl_MT", sign=-1),
+                 dict(leg=4, routine="oxxxxx", mass="ZERO", sign=+1)]
+    prog, ndiags, colours = build_program_qq([1], 0, 4, 2, 3, externals, nexternal)
+    prog.ops = schedule_ops(prog.ops, nexternal)
+    basis = sorted({key for c in colours for key in c})
+    index = {b: i for i, b in enumerate(basis)}
+    jamp = [[] for _ in basis]
+    for a, col in enumerate(colours):
+        for key, (re, im) in sorted(col.items()):
+            jamp[index[key]].append((a, _coef_json(re), _coef_json(im)))
+    cf = colour_matrix_two_lines(basis)
+    denoms, nums = line_numerators(cf)
+    name = "ug_ttxu"
+    used = {op["coup"] for op in prog.ops}
+    params = ["mdl_MT", "mdl_WT"] + [c for c in ("GC_10", "GC_11", "GC_12") if c in used]
+    return dict(
+        name=name,
+        process="u g > t t~ u",
+        class_name="Matrix_1_" + name,
+        nexternal=nexternal,
+        ninitial=2,
+        ndiags=ndiags,
+        ngraphs=prog.namps,
+        nwavefuncs=prog.nwf,
+        ncolor=len(basis),
+        ncomb=2 ** nexternal,
+        kinds=kinds,
+        externals=externals,
+        helicities=helicity_table(kinds),
+        denominator=4 * 3 * 8,
+        params=params,
+        masses=["ZERO", "ZERO", "mdl_MT", "mdl_MT", "ZERO"],
+        ops=prog.ops,
+        jamp=jamp,
+        colour_basis=[[p, list(w1), list(w2)] for p, w1, w2 in basis],
+        color_denom=denoms,
+        color_num=nums,
+        initial_states=[[2, 21], [4, 21], [1, 21], [3, 21]],
+        mirror_initial_states=True,
+    )
+
+
 def _coef_json(x):
     """A rational coefficient for the process description: an int when integral, else the
     exact fraction as a string ("-1/6")."""

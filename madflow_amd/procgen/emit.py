@@ -21,7 +21,7 @@ import sys
 
 from fractions import Fraction
 
-from .diagrams import gg_ttx_ng, qq_ttx_ng, schedule_ops
+from .diagrams import gg_ttx_ng, qg_ttxq, qq_ttx_ng, schedule_ops
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROC_DIR = os.path.join(PKG, "processes")
@@ -338,6 +338,16 @@ def zero_helicity_pairs(desc):
     return []
 
 
+def same_helicity_pairs(desc):
+    """Pairs of external legs (incoming quark, outgoing quark) of a massless quark line
+    that scatters: chirality conservation along the line makes every row in which the two
+    carry different helicity labels vanish."""
+    kinds = desc.get("kinds", [])
+    if "quark" in kinds and "quark_out" in kinds:
+        return [(kinds.index("quark"), kinds.index("quark_out"))]
+    return []
+
+
 def emit_hip(desc, uname=None):
     """HIP device code: constants + a templated amplitude program for one helicity.
 
@@ -473,14 +483,18 @@ def emit_hip(desc, uname=None):
         w("  }")
     w("  static __device__ __forceinline__ unsigned hel_bits(int h) { return HEL_BITS[h]; }")
     pairs = zero_helicity_pairs(desc)
+    same = same_helicity_pairs(desc)
     if dual:
-        assert not any(X in pq for X in legs for pq in pairs), "a paired leg decides a vanishing row"
-    if pairs:
+        assert not any(X in pq for X in legs for pq in pairs + same), "a paired leg decides a vanishing row"
+    if pairs or same:
         w("  // rows that vanish identically: a massless quark line conserves chirality, so its")
-        w("  // two external legs need opposite helicities (bit l of hb: leg l at helicity +1)")
+        w("  // two external legs need opposite helicity labels when it annihilates (quark and")
+        w("  // antiquark) and equal labels when it scatters (bit l of hb: leg l at helicity +1)")
         w("  static constexpr bool FILTERED = true;")
         w("  static __device__ __forceinline__ bool live(unsigned hb) {")
-        w("    return %s;" % " && ".join("(((hb >> %d) ^ (hb >> %d)) & 1u)" % pq for pq in pairs))
+        terms = ["(((hb >> %d) ^ (hb >> %d)) & 1u)" % pq for pq in pairs]
+        terms += ["!(((hb >> %d) ^ (hb >> %d)) & 1u)" % pq for pq in same]
+        w("    return %s;" % " && ".join(terms))
         w("  }")
     else:
         w("  static constexpr bool FILTERED = false;")
@@ -511,7 +525,8 @@ def emit_hip_unit(uname):
 # both precisions; for g g > t t~ g g g the n! jamps of the fused form do not fit the
 # register file next to a 1,890-amplitude program.
 KERNELS = {"gg_ttx": ("fused",), "gg_ttxg": ("fused",), "gg_ttxgg": ("ordered", "fused"),
-           "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",), "uux_ttxgg": ("fused",)}
+           "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",), "uux_ttxgg": ("fused",),
+           "ug_ttxu": ("fused",)}
 # helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
 MULTI_K = {}
 
@@ -698,7 +713,7 @@ def emitted_files(desc):
 
 def descriptions():
     """Process descriptions of every generated process, in emission order."""
-    return [gg_ttx_ng(ng) for ng in NGLUONS] + [qq_ttx_ng(ng) for ng in QQ_NGLUONS]
+    return [gg_ttx_ng(ng) for ng in NGLUONS] + [qq_ttx_ng(ng) for ng in QQ_NGLUONS] + [qg_ttxq()]
 
 
 def main(argv=None):

@@ -17,7 +17,7 @@ OUT = os.path.join(os.path.dirname(O.HERE), "madflow_amd", "processes", "flops_p
 
 def main():
     res = {}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"):
         try:
             O.load_process(name)
         except FileNotFoundError:
@@ -26,8 +26,11 @@ def main():
         # rows that do not vanish identically (the kernels skip the others; the count above
         # is over every row, as the reference evaluates them)
         d = O.load_process(name)
-        q = [i for i, k in enumerate(d.get("kinds", [])) if k in ("quark", "antiquark")]
-        res[name]["live_rows"] = sum(1 for h in d["helicities"] if len(q) < 2 or h[q[0]] != h[q[1]])
+        kinds = d.get("kinds", [])
+        q = [i for i, k in enumerate(kinds) if k in ("quark", "antiquark", "quark_out")]
+        same = "quark_out" in kinds               # a scattering line keeps the label, an annihilating one flips it
+        res[name]["live_rows"] = sum(1 for h in d["helicities"]
+                                     if len(q) < 2 or (h[q[0]] == h[q[1]]) == same)
         res[name]["rows"] = len(d["helicities"])
         print(name, res[name])
     with open(OUT, "w") as f:

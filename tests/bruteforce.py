@@ -431,3 +431,46 @@ def qq_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_co
     gs = 2 * np.sqrt(np.pi * alpha_s)
     sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
     return float(np.sum(np.abs(qq_amplitude_tensor(p, hel, mt, wt, gs, sqh)) ** 2))
+
+
+def qg_amplitude_tensor(p, hel, mt, wt, gs, sqh):
+    """M[c_q', c_q, c_t, c_tbar, a] of u g > t t~ u (legs 0 u in, 1 g in, 2 t, 3 t~, 4 u out):
+    the five diagrams of u u~ > t t~ g written out again with the light line's row spinor
+    that of the outgoing quark (oxxxxx, nsf = +1), its column spinor that of the incoming
+    quark, and the incoming gluon's polarisation (vxxxxx, nsv = -1); momenta are taken
+    outgoing (incoming legs negated)."""
+    GC10, GC11 = -gs, 1j * gs
+    ref = sqh != np.sqrt(0.5)
+    u = O.wavefunction("i", p[0:1], 0.0, hel[0], 1)[0][2:6]
+    ubo = O.wavefunction("o", p[4:5], 0.0, hel[4], 1)[0][2:6]
+    ub = O.wavefunction("o", p[2:3], mt, hel[2], 1)[0][2:6]
+    v = O.wavefunction("i", p[3:4], mt, hel[3], -1)[0][2:6]
+    eps = O.wavefunction("v", p[1:2], 0.0, hel[1], -1, ref_compat=ref)[0][2:6]
+    Kq, Kqo, Kt, Ktb, kg = -p[0], p[4], p[2], p[3], -p[1]
+
+    def fprop(P, m, w):
+        return 1j * (slash(P.astype(complex)) + m * np.eye(4)) / (mdot(P, P) - m * m + 1j * m * w)
+
+    KL, KT = Kq + Kqo, Kt + Ktb
+    Lprop = (-1j / mdot(KL, KL)) * GC11 * np.array([ubo @ G[m] @ u for m in range(4)])
+    Tprop = (-1j / mdot(KT, KT)) * GC11 * np.array([ub @ G[m] @ v for m in range(4)])
+    se, sL, sT = slash(eps), slash(Lprop), slash(Tprop)
+    A1 = GC11 ** 2 * ub @ se @ fprop(Kt + kg, mt, wt) @ sL @ v           # gluon on the top
+    A2 = GC11 ** 2 * ub @ sL @ fprop(Kt + KL, mt, wt) @ se @ v           # on the anti-top
+    A3 = GC11 ** 2 * ubo @ se @ fprop(Kqo + kg, 0.0, 0.0) @ sT @ u       # on the outgoing quark
+    A4 = GC11 ** 2 * ubo @ sT @ fprop(Kqo + KT, 0.0, 0.0) @ se @ u       # on the incoming quark
+    J = GC10 * _vvv(Lprop, eps, KL, kg) * (-1j / mdot(KL + kg, KL + kg))  # t-channel three-gluon vertex
+    A5 = GC11 * ub @ slash(J) @ v
+    TaTb = np.einsum("aij,bjk->abik", T, T)
+    return (A1 * np.einsum("abik,bqr->qrika", TaTb, T)
+            + A2 * np.einsum("baik,bqr->qrika", TaTb, T)
+            + A3 * np.einsum("abqr,bik->qrika", TaTb, T)
+            + A4 * np.einsum("baqr,bik->qrika", TaTb, T)
+            + A5 * np.einsum("cba,cik,bqr->qrika", F, T, T))
+
+
+def qg_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False):
+    """Colour-summed |M|^2 of one helicity row of u g > t t~ u."""
+    gs = 2 * np.sqrt(np.pi * alpha_s)
+    sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
+    return float(np.sum(np.abs(qg_amplitude_tensor(p, hel, mt, wt, gs, sqh)) ** 2))

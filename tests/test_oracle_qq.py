@@ -40,14 +40,17 @@ def test_uux_ttx_closed_form():
     np.testing.assert_allclose(me, gs ** 4 * 4 / 9 * (t1 ** 2 + t2 ** 2 + rho / 2), rtol=1e-13)
 
 
-@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg"])
+@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "ug_ttxu"])
 def test_qq_bruteforce_every_row(name):
+    """u g > t t~ u (the crossing: incoming gluon, outgoing quark) against its own brute
+    force, written out with the crossed spinors."""
     ps, _ = phase_space_points(name, 2, seed=42)
     d = O.load_process(name)
     rp, coup = O.default_params()
+    brute = BF.qg_matrix_hel if name == "ug_ttxu" else BF.qq_matrix_hel
     for e in range(2):
         for hel in d["helicities"]:
-            bf = BF.qq_matrix_hel(ps[e], hel)
+            bf = brute(ps[e], hel)
             orc = O.matrix_hel(name, ps[e:e + 1], hel, rp, coup)[0]
             if bf == 0.0:
                 assert orc == 0.0, (hel, orc)
@@ -73,7 +76,23 @@ def test_qq_vanishing_rows():
     assert zero_helicity_pairs(O.load_process("gg_ttxg")) == []
 
 
-@pytest.mark.parametrize("name,gluon", [("uux_ttxg", 4), ("uux_ttxgg", 4), ("uux_ttxgg", 5)])
+def test_qg_vanishing_rows():
+    """u g > t t~ u: the quark scatters, so the rows in which the incoming and outgoing
+    quark carry different helicity labels vanish, and only those."""
+    from madflow_amd.procgen.emit import same_helicity_pairs, zero_helicity_pairs
+    d = O.load_process("ug_ttxu")
+    assert same_helicity_pairs(d) == [(0, 4)] and zero_helicity_pairs(d) == []
+    ps, _ = phase_space_points("ug_ttxu", 4, seed=43)
+    rp, coup = O.default_params()
+    for hel in d["helicities"]:
+        v = O.matrix_hel("ug_ttxu", ps, hel, rp, coup)
+        if hel[0] != hel[4]:
+            assert np.all(v == 0.0), hel
+        else:
+            assert np.all(v > 0.0), hel
+
+
+@pytest.mark.parametrize("name,gluon", [("uux_ttxg", 4), ("uux_ttxgg", 4), ("uux_ttxgg", 5), ("ug_ttxu", 1)])
 def test_qq_brst_gauge_invariance(name, gluon):
     """Replacing a gluon's polarisation by its momentum (nhel = 4) makes every jamp vanish:
     a diagram missing from the gauge-invariant set, or one with a wrong relative sign,
@@ -85,8 +104,8 @@ def test_qq_brst_gauge_invariance(name, gluon):
         if row >= len(d["helicities"]):
             continue
         hel = list(d["helicities"][row])
-        if hel[0] == hel[1]:
-            continue
+        if (hel[0] != hel[4]) if name == "ug_ttxu" else (hel[0] == hel[1]):
+            continue                                  # a row that vanishes by chirality
         _, jp = O.matrix_hel(name, ps, hel, rp, coup, want_jamp=True)
         h = list(hel)
         h[gluon] = 4
@@ -95,7 +114,7 @@ def test_qq_brst_gauge_invariance(name, gluon):
         assert np.abs(jb).max() < 1e-11 * np.abs(jp).max(), row
 
 
-@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "uux_ttxgg"])
+@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"])
 def test_qq_lorentz_invariance(name):
     ps, _ = phase_space_points(name, 4, seed=45)
     rp, coup = O.default_params(wt=0.0)
@@ -104,15 +123,16 @@ def test_qq_lorentz_invariance(name):
     np.testing.assert_allclose(a, b, rtol=1e-10)
 
 
-@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1), ("uux_ttxgg", 2)])
+@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1), ("uux_ttxgg", 2), ("ug_ttxu", 0)])
 def test_qq_helicity_sum(name, k):
-    """smatrix = sum over all 2^nexternal rows / (4 spins x 9 colours x k!)."""
+    """smatrix = sum over all 2^nexternal rows / (4 spins x colours x k!), the colours
+    3 x 3 for q q~ and 3 x 8 for q g."""
     ps, _ = phase_space_points(name, 3, seed=46)
     d = O.load_process(name)
     n = d["nexternal"]
     rows = [list(h) for h in itertools.product([-1, 1], repeat=n)]
     assert sorted(map(tuple, d["helicities"])) == sorted(map(tuple, rows))
-    assert d["denominator"] == 4 * 9 * math.factorial(k)
+    assert d["denominator"] == 4 * (24 if name == "ug_ttxu" else 9) * math.factorial(k)
     rp, coup = O.default_params()
     tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in rows) / d["denominator"]
     np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), tot, rtol=1e-13)
@@ -220,3 +240,25 @@ def test_soft_limit_catches_a_dropped_emission():
         assert abs(r - 1) > 0.05
     finally:
         del O._cache["uux_ttxgg_mutant"]
+
+
+def test_qg_bruteforce_catches_a_dropped_diagram():
+    """The crossed brute force is sensitive to each diagram: removing the three-gluon
+    (t-channel) diagram from the oracle's u g > t t~ u jamps changes |M|^2 by far more
+    than the 1e-12 bar."""
+    import copy
+    d = copy.deepcopy(O.load_process("ug_ttxu"))
+    ggg = [op["out"] for op in d["ops"] if op["kind"] == "amp"
+           and any(o2["out"] == op["args"][2] and o2["routine"] == "VVV1P0_1" for o2 in d["ops"])]
+    assert len(ggg) == 1
+    d["jamp"] = [[t for t in terms if t[0] != ggg[0]] for terms in d["jamp"]]
+    O._cache["ug_ttxu_mutant"] = O.EncodedProcess(d)
+    try:
+        ps, _ = phase_space_points("ug_ttxu", 1, seed=47)
+        rp, coup = O.default_params()
+        hel = [h for h in d["helicities"] if h[0] == h[4]][0]
+        bf = BF.qg_matrix_hel(ps[0], hel)
+        mut = O.matrix_hel("ug_ttxu_mutant", ps, hel, rp, coup)[0]
+        assert abs(mut / bf - 1) > 1e-3
+    finally:
+        del O._cache["ug_ttxu_mutant"]

@@ -20,12 +20,17 @@ import math
 
 import numpy as np
 
-# PDG codes of the external legs of each process: g g > t t~ g ...
+# PDG codes of the external legs of each process (the u-quark member of the quark flavours a
+# process serves): g g > t t~ g, u u~ > t t~, u g > t t~ u ...
 PDG = {
     "gg_ttx": [21, 21, 6, -6],
     "gg_ttxg": [21, 21, 6, -6, 21],
     "gg_ttxgg": [21, 21, 6, -6, 21, 21],
     "gg_ttxggg": [21, 21, 6, -6, 21, 21, 21],
+    "uux_ttx": [2, -2, 6, -6],
+    "uux_ttxg": [2, -2, 6, -6, 21],
+    "uux_ttxgg": [2, -2, 6, -6, 21, 21],
+    "ug_ttxu": [2, 21, 6, -6, 2],
 }
 
 

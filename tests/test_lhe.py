@@ -59,3 +59,13 @@ def test_lhe_overweight_events_keep_their_weight(tmp_path):
     assert init[0][8] == 3.0 and all(ev[0][2] == 10.0 for ev in events)
     with pytest.raises(ValueError):
         write_lhe(path, "gg_ttxg", ps, 10.0, 0.1, 2000.0, weights=w)
+
+
+def test_every_generated_process_has_pdg_codes():
+    """write_lhe knows the external legs of every process the generator emits: one PDG code
+    per leg, gluons 21, tops +-6, light quarks +-2."""
+    from madflow_amd.lhe import PDG
+    from madflow_amd.procgen.emit import descriptions
+    code = {"gluon": 21, "top": 6, "atop": -6, "quark": 2, "antiquark": -2, "quark_out": 2}
+    for d in descriptions():
+        assert PDG[d["name"]] == [code[k] for k in d["kinds"]], d["name"]

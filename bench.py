@@ -415,6 +415,7 @@ def run_flow_compare(a, torch, main):
         return learned[k] / base[k] if base[k] > 0 else None
     mc = run_multichannel_leg(a, torch) if (a.multichannel_leg and a.mapping == "diet") else None
     mcf = run_flow_mc_leg(a, torch) if (a.flow_mc_leg and a.mapping == "diet") else None
+    mcc = run_channel_flow_leg(a, torch) if (a.channel_flow_leg and a.mapping == "diet") else None
     out = {
         "workload": "%s, %s, %d-dim %s VEGAS cube, %d points/iteration, target sigma/sigma=%g" % (
             LABEL[a.process], _kin_text(a.process), ig.ndim, a.mapping, a.nev, a.target),
@@ -449,11 +450,21 @@ def run_flow_compare(a, torch, main):
             out["gain_multichannel_per_s_to_target_vs_short_warmup"] = (
                 mc["unweighted_per_s_to_target"] / plain_short["unweighted_per_s_to_target"]
                 if plain_short["unweighted_per_s_to_target"] > 0 else None)
-        out["gain_multichannel_per_s_equal_time"] = (mc["equal_time"]["unweighted_per_s"] /
-                                                     plain["equal_time"]["unweighted_per_s"]
-                                                     if plain["equal_time"]["unweighted_per_s"] > 0 else None)
+        for k in ("equal_time", "equal_wall"):
+            out["gain_multichannel_per_s_" + k] = (mc[k]["unweighted_per_s"] / plain[k]["unweighted_per_s"]
+                                                   if plain[k]["unweighted_per_s"] > 0 else None)
         out["sigma_pull_multichannel"] = ((mc["sigma_pb"] - plain["sigma_pb"]) /
                                           math.hypot(mc["err_pb"], plain["err_pb"]))
+    if mcc is not None:
+        # a flow per dominant channel of the multi-channel phase space
+        out["multichannel_channel_flows_fp32"] = mcc
+        for k, key in (("to_target", "unweighted_per_s_to_target"),):
+            out["gain_multichannel_channel_flows_per_s_" + k] = (mcc[key] / plain[key] if plain[key] > 0 else None)
+        for k in ("equal_time", "equal_wall"):
+            out["gain_multichannel_channel_flows_per_s_" + k] = (
+                mcc[k]["unweighted_per_s"] / plain[k]["unweighted_per_s"] if plain[k]["unweighted_per_s"] > 0 else None)
+        out["sigma_pull_multichannel_channel_flows"] = ((mcc["sigma_pb"] - plain["sigma_pb"]) /
+                                                        math.hypot(mcc["err_pb"], plain["err_pb"]))
     if mcf is not None:
         # the learned proposal as a channel of the multi-channel phase space
         out["multichannel_flow_fp32"] = mcf
@@ -506,6 +517,68 @@ def run_flow_mc_leg(a, torch):
     return out
 
 
+def run_channel_flow_leg(a, torch):
+    """FP32 matrix elements on the multi-channel phase space with a flow per channel for the
+    a.channel_flows channels of largest variance share (measured on one frozen iteration):
+    each flow is trained on its channel's cube as in the flow leg (uniform points, their
+    integrand values alpha_c f / sum_j alpha_j g_j) and draws that channel's points in a
+    defensive mixture with the uniform; the channel weights stay those of the warm-up.
+    Warm-up, the probe iteration and the training are counted in the set-up."""
+    import numpy as np
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.vegas import VegasFlow
+    sync = torch.cuda.synchronize
+    ig = MultiChannelIntegrand(a.process, precision=32, seed=7, events_limit=a.events_limit, kernels=a.kernels,
+                               **ME_ONLY_KIN[a.process])
+    vf = VegasFlow(ig.ndim, a.mc_warmup_nev or a.nev)
+    vf.compile(ig)
+    sync()
+    t0 = time.perf_counter()
+    vf.run_integration(a.warmup, log=log)
+    vf.freeze_grid()
+    probe = ig.iteration(a.mc_warmup_nev or a.nev, 50, train=False)
+    sync()
+    t_warm = time.perf_counter() - t0
+    vp = np.array(probe["var_parts"])
+    chosen = [int(c) for c in np.argsort(-vp)[:a.channel_flows]]
+
+    class _Cube:
+        """Channel c's cube as the flow leg's training sees an integrand."""
+        def __init__(self, c):
+            self.c, self.ndim, self.events_limit = c, ig.ndim, ig.events_limit
+
+        def chunk(self, ev0, n, iteration, keep_f=False, y_in=None, w_in=None):
+            r, k, _, f = ig.chunk_channel(self.c, ev0, n, iteration, 1.0, keep_f=True, y_in=y_in, w_in=w_in)
+            return r, k, f
+    t_train, steps = 0.0, []
+    for c in chosen:
+        sync()
+        t1 = time.perf_counter()
+        flow, _, _ = train_flow(a, torch, _Cube(c))
+        sync()
+        t_train += time.perf_counter() - t1
+        ig.attach_channel_flow(c, flow, a.channel_flow_mix)
+        steps.append(list(getattr(flow, "stage_steps", [])))
+    setup = t_warm + t_train
+
+    def it(k):
+        return ig.iteration(a.nev, 200 + k, train=False)
+    t0 = time.perf_counter()
+    res = _run_to_target(it, a.target, a.max_iter, "multichannel + channel flows fp32")
+    sync()
+    out = _unweighted(res, setup, time.perf_counter() - t0, a.target)
+    timed = _extend_to(it, list(res), out["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup), torch)
+    out["equal_time"] = _rate_at(timed, a.equal_time_s, a.equal_time_s)
+    out["equal_wall"] = _rate_at(timed, a.equal_wall_s - setup, a.equal_wall_s)
+    vps = np.mean([r["var_parts"] for _, r in timed], axis=0)
+    out.update(warmup_s=t_warm, train_s=t_train, channels_with_flow=chosen, stage_steps_run=steps,
+               variance_share_before=[float(v) for v in vp / vp.sum()],
+               # each channel's variance after, at the probe's number of points, over the total before
+               variance_after_over_total_before=[float(v) for v in vps / vp.sum() * (a.nev / probe["n"])],
+               mixture=a.channel_flow_mix)
+    return out
+
+
 def run_multichannel_leg(a, torch):
     """FP32 matrix elements on the multi-channel phase space (flat diet, a t-channel per
     gluon, an s-channel per gluon pair, one VEGAS grid each, adapted channel weights): the
@@ -532,8 +605,9 @@ def run_multichannel_leg(a, torch):
     res = _run_to_target(it, a.target, a.max_iter, "multichannel fp32")
     sync()
     out = _unweighted(res, setup, time.perf_counter() - t0, a.target)
-    timed = _extend_to(it, list(res), out["sample_s"], a.equal_time_s, torch)
+    timed = _extend_to(it, list(res), out["sample_s"], max(a.equal_time_s, a.equal_wall_s - setup), torch)
     out["equal_time"] = _rate_at(timed, a.equal_time_s, a.equal_time_s)
+    out["equal_wall"] = _rate_at(timed, a.equal_wall_s - setup, a.equal_wall_s)
     out["warmup"] = "%d x %d points" % (a.warmup, a.mc_warmup_nev or a.nev)
     out["channels"] = [list(c) for c in ig.channels]
     out["alpha"] = [float(x) for x in ig.alpha]
@@ -640,6 +714,12 @@ def parse_args(argv=None):
                     help="add a leg with the learned proposal as a channel of the multi-channel phase space")
     ap.add_argument("--flow-mc-alpha", type=float, default=0.5,
                     help="weight of the learned channel in the multi-channel mixture")
+    ap.add_argument("--channel-flow-leg", action="store_true",
+                    help="add a multi-channel leg with a flow for each of the --channel-flows channels of largest "
+                         "variance share")
+    ap.add_argument("--channel-flows", type=int, default=2)
+    ap.add_argument("--channel-flow-mix", type=float, default=0.1,
+                    help="weight of the uniform component in each channel flow's defensive mixture")
     ap.add_argument("--flow-pairs", type=int, default=5_000_000)
     ap.add_argument("--flow-grid-warmup", type=int, default=3,
                     help="grid adaptation iterations of the flow leg's FP32 grid, counted in its set-up (the plain "

@@ -205,6 +205,15 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
                     double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
                     double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
                     void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
+/* mfx_mc_generate with the channel's cube points given (y_in, device (nev, ndim) f32; a
+ * proposal on the cube such as a flow per channel) and their weights w_in (f64 (nev,),
+ * 1 / the density of y on the cube, or NULL for 1): wps is multiplied by w_in, which keeps
+ * the stratified estimate unbiased.  No reference counterpart. */
+int mfx_mc_generate_points(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                           double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
+                           uint32_t iteration, const float* y_in, const double* w_in, double* all_ps, double* wps,
+                           uint8_t* flag, int* slot, double* dense, double* ratios, void* workspace,
+                           int64_t workspace_bytes, int64_t* npass_host, void* stream);
 
 /* A learned channel beside the analytic ones (channel 0 must be flat): its density on the
  * phase space is q(y0(p)) g_0(p), with q the learned density on channel 0's cube and y0(p)

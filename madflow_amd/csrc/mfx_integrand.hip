@@ -368,6 +368,9 @@ __device__ __forceinline__ double grid_inverse(const double* __restrict__ grid, 
 // Points of channel c: VEGAS map of grid c, channel c's map, the densities of every
 // channel at the point (ratios[e][j] = g_j / g) and the weight scale / g; then cuts, as
 // k_generate (fixed sqrt(s), c.m. frame).
+// External points (y_in, with w_in = 1 / their density on channel c's cube): the weight is
+// multiplied by w_in, so that a proposal on the cube other than the uniform one leaves the
+// estimate unbiased.
 // Learned mode (y0 != nullptr; channel 0 is the flat channel): the points of channel c
 // come from y_in when it is given (the learned channel's samples on channel 0's cube),
 // and the kernel leaves the division by g to mfx_mc_combine, once the learned density at
@@ -376,6 +379,7 @@ __device__ __forceinline__ double grid_inverse(const double* __restrict__ grid, 
 __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst mc, int c,
                                                            const double* __restrict__ grids, uint64_t ev0, int64_t nev,
                                                            uint32_t iteration, const float* __restrict__ y_in,
+                                                           const double* __restrict__ w_in,
                                                            double* __restrict__ all_ps, double* __restrict__ wps,
                                                            uint8_t* __restrict__ flag, double* __restrict__ ratios,
                                                            float* __restrict__ y0_out, double* __restrict__ gdens,
@@ -422,7 +426,9 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
     double* out = all_ps + e * (g.nexternal * 4);
     for (int i = 0; i < g.nexternal; ++i)
       for (int k = 0; k < 4; ++k) out[4 * i + k] = p[i][k];
-    wps[e] = ok ? wt * g.norm : 0.0;
+    const double wx = w_in ? w_in[e] : 1.0;    // 1 / (density of y on the cube) for external points
+    ok = ok && isfinite(wx) && wx > 0.0;
+    wps[e] = ok ? wt * g.norm * wx : 0.0;
     flag[e] = ok ? 1 : (conv ? 0 : 2);
     if (learned) {
       for (int j = 0; j < mc.nch; ++j) ratios[e * rs + j] = gj[j];
@@ -951,10 +957,11 @@ static int make_mc(const mfx_ps_params* prm, const GenConst& g, const mfx_mc_cha
   return 0;
 }
 
-int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
-                    double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
-                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
-                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+static int mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                       double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
+                       const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
+                       double* dense, double* ratios, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
+                       void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -976,7 +983,7 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
-                     (const float*)nullptr, all_ps, wps, flag, ratios, (float*)nullptr, (double*)nullptr, counts);
+                     y_in, w_in, all_ps, wps, flag, ratios, (float*)nullptr, (double*)nullptr, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);
@@ -990,6 +997,24 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
     *npass_host = tot;
   }
   return 0;
+}
+
+int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                    double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
+                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  return mc_generate(prm, channels, nch, alpha, scale, channel, grids, ev0, nev, iteration, nullptr, nullptr, all_ps,
+                     wps, flag, slot, dense, ratios, workspace, workspace_bytes, npass_host, stream);
+}
+
+int mfx_mc_generate_points(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
+                           double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
+                           uint32_t iteration, const float* y_in, const double* w_in, double* all_ps, double* wps,
+                           uint8_t* flag, int* slot, double* dense, double* ratios, void* workspace,
+                           int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  if (nev > 0 && !y_in) return fail(-3, "null pointer (y_in)%s", "");
+  return mc_generate(prm, channels, nch, alpha, scale, channel, grids, ev0, nev, iteration, y_in, w_in, all_ps, wps,
+                     flag, slot, dense, ratios, workspace, workspace_bytes, npass_host, stream);
 }
 
 int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
@@ -1022,7 +1047,7 @@ int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* chan
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
-                     y_in, all_ps, wps, flag, ratios, y0_out, gdens, counts);
+                     y_in, (const double*)nullptr, all_ps, wps, flag, ratios, y0_out, gdens, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);

@@ -39,6 +39,9 @@ def _mc_sig(lib):
     lib.mfx_mc_generate.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp, vp, vp,
                                     vp, vp, i64, ctypes.POINTER(i64), vp]
     lib.mfx_mc_generate.restype = i
+    lib.mfx_mc_generate_points.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp, vp,
+                                           vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_mc_generate_points.restype = i
     lib.mfx_mc_moments.argtypes = [i64, i, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_mc_moments.restype = i
     lib.mfx_mc_generate_learned.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp,
@@ -98,6 +101,8 @@ class MultiChannelIntegrand(MEIntegrand):
         self.history_alpha = []
         self.flow = None          # learned channel (attach_flow)
         self.alpha_learned = 0.0
+        self.channel_flows = {}   # channel -> flow on that channel's cube (attach_channel_flow)
+        self.channel_flow_mix = 0.1
 
     def _alpha_all(self):
         """Weights of every channel of the mixture: the analytic ones, then the learned one."""
@@ -139,6 +144,23 @@ class MultiChannelIntegrand(MEIntegrand):
             raise ValueError("alpha_learned must lie in (0, 1)")
         self.flow = flow
         self.alpha_learned = float(alpha_learned)
+
+    def attach_channel_flow(self, c, flow, mix=0.1):
+        """Draw channel c's points from ``flow`` on its cube (before its frozen grid) in a
+        defensive mixture q' = (1 - mix) q + mix with the uniform density; each weight is
+        multiplied by 1 / q'(y), so the stratified estimate stays unbiased while the
+        channel weights alpha_j g_j / sum alpha g keep their meaning.  Channel weights and
+        grids are no longer adapted while a channel flow is attached."""
+        if not 0 <= c < self.nch:
+            raise ValueError("channel %d out of range" % c)
+        if self.flow is not None:
+            raise ValueError("a learned channel is attached")
+        if flow.D != self.ndim:
+            raise ValueError("the flow has %d dimensions, the cube %d" % (flow.D, self.ndim))
+        if not 0.0 < mix < 1.0:
+            raise ValueError("mix must lie in (0, 1)")
+        self.channel_flows[c] = flow
+        self.channel_flow_mix = float(mix)
 
     def _chunk_learned(self, c, ev0, n, iteration, scale, keep_f):
         """One chunk of channel c of the mixture with the learned channel (c == nch)."""
@@ -190,24 +212,35 @@ class MultiChannelIntegrand(MEIntegrand):
             return me
         return self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
 
-    def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False):
-        """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f)."""
+    def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False, y_in=None, w_in=None):
+        """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f).
+        y_in / w_in: the points on channel c's cube and 1 / their density there (a proposal
+        other than the uniform one); by default the Philox uniforms."""
         b = self._alloc(n)
         if self._ratios is None or self._ratios.shape[0] < n:
             self._ratios = torch.empty((n, self.nch), dtype=torch.float64, device=self.device)
         s = _lib.stream_ptr(self.device)
         npass = ctypes.c_int64(0)
         alpha = np.ascontiguousarray(self.alpha, dtype=np.float64)
-        _lib.check(self.lib.mfx_mc_generate(
-            ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
-            alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
-            _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]), _ptr(self._ratios),
-            _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+        if y_in is None:
+            _lib.check(self.lib.mfx_mc_generate(
+                ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
+                alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
+                _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]),
+                _ptr(self._ratios), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+        else:
+            y_in = _lib.require_device_tensor(y_in, torch.float32, "y_in")
+            w_in = None if w_in is None else _lib.require_device_tensor(w_in, torch.float64, "w_in")
+            _lib.check(self.lib.mfx_mc_generate_points(
+                ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
+                alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
+                _ptr(y_in), _ptr(w_in), _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]),
+                _ptr(b["dense"]), _ptr(self._ratios), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
         npass = npass.value
         self.last_chunk_npass = npass
         me = self._smatrix_timed(b, npass)
         _lib.check(self.lib.mfx_vegas_accumulate(
-            ctypes.byref(self.p), ev0, n, iteration, None, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
             _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
         mom = torch.empty(self.nch, dtype=torch.float64, device=self.device)
         _lib.check(self.lib.mfx_mc_moments(n, self.nch, _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]),
@@ -240,6 +273,13 @@ class MultiChannelIntegrand(MEIntegrand):
                 n = min(self.events_limit, nc - ev)
                 if learned:
                     r, k, mom, f = self._chunk_learned(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
+                elif c in self.channel_flows:
+                    # the channel's own flow on its cube (defensive mixture with the uniform)
+                    from .phasespace_gpu import philox_uniforms
+                    ev0 = self.channel_offset(c) + ev
+                    u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0, dtype=torch.float32)
+                    ys, winv = self.channel_flows[c].sample_mixture(u, u, self.channel_flow_mix)
+                    r, k, mom, f = self.chunk_channel(c, ev0, n, iteration, scale, keep_f, y_in=ys, w_in=winv)
                 else:
                     r, k, mom, f = self.chunk_channel(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
                 tot = r if tot is None else _merge(tot, r)
@@ -255,13 +295,13 @@ class MultiChannelIntegrand(MEIntegrand):
             var_parts.append(var_c * (nc / N) ** 2)
             wmax = max(wmax, tot[2])
             nonzero += tot[3]
-            if train and not learned:
+            if train and not learned and c not in self.channel_flows:
                 hist = torch.from_numpy(tot[6:].copy()).to(self.device)
                 _lib.check(self.lib.mfx_vegas_refine(_ptr(self.grids[c]), _ptr(hist), self.ndim, self.nbins, ALPHA,
                                                      _lib.stream_ptr(self.device)))
         I = math.fsum(I_parts)
         var = math.fsum(var_parts)
-        if train and self.adapt_alpha and np.all(np.isfinite(moments)) and moments.sum() > 0:
+        if train and self.adapt_alpha and not self.channel_flows and np.all(np.isfinite(moments)) and moments.sum() > 0:
             a = alpha_used * np.sqrt(np.maximum(moments, 0.0))
             a = a / a.sum()
             a = np.maximum(a, self.alpha_min)
@@ -273,7 +313,8 @@ class MultiChannelIntegrand(MEIntegrand):
                 self.alpha = a
         self.history_alpha.append(alpha_used)
         out = dict(I=I, var=var, sigma=math.sqrt(var), wmax=wmax, nonzero=nonzero, npass=npass_tot, n=int(n_events),
-                   sum_f=I * N, alpha=alpha_used.tolist(), counts=counts.tolist())
+                   sum_f=I * N, alpha=alpha_used.tolist(), counts=counts.tolist(),
+                   var_parts=[float(v) for v in var_parts])
         if keep_f:
             out["f"] = torch.cat(fs)
         return out

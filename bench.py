@@ -619,9 +619,11 @@ ME_ONLY_KIN = {"gg_ttx": dict(sqrts=1000.0), "gg_ttxg": dict(sqrts=2000.0, pt_mi
                "gg_ttxggg": dict(sqrts=5000.0, pt_min=30.0, dr_min=0.4),
                "uux_ttx": dict(sqrts=1000.0), "uux_ttxg": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5),
                "uux_ttxgg": dict(sqrts=3000.0, pt_min=30.0, dr_min=0.4),
-               "ug_ttxu": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5)}
+               "ug_ttxu": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5),
+               "uxg_ttxux": dict(sqrts=2000.0, pt_min=30.0, abs_eta_max=2.5)}
 LABEL = {"gg_ttx": "gg->tt", "gg_ttxg": "gg->ttg", "gg_ttxgg": "gg->ttgg", "gg_ttxggg": "gg->ttggg",
-         "uux_ttx": "uu~->tt", "uux_ttxg": "uu~->ttg", "uux_ttxgg": "uu~->ttgg", "ug_ttxu": "ug->ttu"}
+         "uux_ttx": "uu~->tt", "uux_ttxg": "uu~->ttg", "uux_ttxgg": "uu~->ttgg", "ug_ttxu": "ug->ttu",
+         "uxg_ttxux": "u~g->ttu~"}
 
 
 def kernel_info(proc, kernels, precision):

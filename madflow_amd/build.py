@@ -42,6 +42,7 @@ UNITS = [
     "procs/uux_ttxg.hip",
     "procs/uux_ttxgg.hip",
     "procs/ug_ttxu.hip",
+    "procs/uxg_ttxux.hip",
 ]
 
 

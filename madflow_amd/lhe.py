@@ -31,6 +31,7 @@ PDG = {
     "uux_ttxg": [2, -2, 6, -6, 21],
     "uux_ttxgg": [2, -2, 6, -6, 21, 21],
     "ug_ttxu": [2, 21, 6, -6, 2],
+    "uxg_ttxux": [-2, 21, 6, -6, -2],
 }
 
 

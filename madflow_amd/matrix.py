@@ -38,6 +38,7 @@ _PROC_PARAMS = {
     "uux_ttxg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
     "uux_ttxgg": ["mdl_MT", "mdl_WT", "GC_10", "GC_11", "GC_12"],
     "ug_ttxu": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
+    "uxg_ttxux": ["mdl_MT", "mdl_WT", "GC_10", "GC_11"],
 }
 _GC_SLOT = {"GC_10": 0, "GC_11": 1, "GC_12": 2}
 

@@ -23,6 +23,7 @@ PROCESS_KIN = {
     "uux_ttxg": (2000.0, [173.0, 173.0, 0.0]),
     "uux_ttxgg": (3000.0, [173.0, 173.0, 0.0, 0.0]),
     "ug_ttxu": (2000.0, [173.0, 173.0, 0.0]),
+    "uxg_ttxux": (2000.0, [173.0, 173.0, 0.0]),
 }
 
 

@@ -326,7 +326,7 @@ def helicity_table(kinds):
     the mockup's ``helicities`` (``mockup_debug_me.py:424-442``)."""
     states = []
     for k in kinds:
-        if k in ("atop", "antiquark"):
+        if k in ("atop", "antiquark", "antiquark_out"):
             states.append((1, -1))
         else:
             states.append((-1, 1))
@@ -534,21 +534,37 @@ def build_program_qq(gluons, q_leg, qb_leg, top_leg, atop_leg, externals, nexter
     return prog, len(diags), colours
 
 
-def qg_ttxq():
+def qg_ttxq(anti=False):
     """Process description dict for u g > t t~ u, the crossing of u u~ > t t~ g (incoming
     gluon, outgoing quark): the same diagrams and colour algebra, with the outgoing quark's
     row spinor (oxxxxx, nsf = +1) in place of the incoming antiquark's and the incoming
     gluon's polarisation (vxxxxx, nsv = -1).  The same matrix element serves c, d and s
-    quarks; the mirror process has the gluon in beam 1."""
+    quarks; the mirror process has the gluon in beam 1.
+
+    anti=True gives u~ g > t t~ u~: the light line runs from the outgoing antiquark's
+    column spinor (ixxxxx, nsf = -1) to the incoming antiquark's row spinor (oxxxxx,
+    nsf = -1); diagrams, bridge and colour strings are those of the quark case with the
+    line's two ends exchanged."""
     from .colour import colour_matrix_two_lines
     nexternal = 5
-    kinds = ["quark", "gluon", "top", "atop", "quark_out"]
-    externals = [dict(leg=0, routine="ixxxxx", mass="ZERO", sign=+1),
+    if anti:
+        kinds = ["antiquark", "gluon", "top", "atop", "antiquark_out"]
+        light = [dict(leg=0, routine="oxxxxx", mass="ZERO", sign=-1),
+                 dict(leg=4, routine="ixxxxx", mass="ZERO", sign=-1)]
+        col_leg, row_leg = 4, 0
+        name, process, pdg = "uxg_ttxux", "u~ g > t t~ u~", [-2, -4, -1, -3]
+    else:
+        kinds = ["quark", "gluon", "top", "atop", "quark_out"]
+        light = [dict(leg=0, routine="ixxxxx", mass="ZERO", sign=+1),
+                 dict(leg=4, routine="oxxxxx", mass="ZERO", sign=+1)]
+        col_leg, row_leg = 0, 4
+        name, process, pdg = "ug_ttxu", "u g > t t~ u", [2, 4, 1, 3]
+    externals = [light[0],
                  dict(leg=1, routine="vxxxxx", mass="ZERO", sign=-1),
                  dict(leg=2, routine="oxxxxx", mass="mdl_MT", sign=+1),
                  dict(leg=3, routine="ixxxxx", mass="mdl_MT", sign=-1),
-                 dict(leg=4, routine="oxxxxx", mass="ZERO", sign=+1)]
-    prog, ndiags, colours = build_program_qq([1], 0, 4, 2, 3, externals, nexternal)
+                 light[1]]
+    prog, ndiags, colours = build_program_qq([1], col_leg, row_leg, 2, 3, externals, nexternal)
     prog.ops = schedule_ops(prog.ops, nexternal)
     basis = sorted({key for c in colours for key in c})
     index = {b: i for i, b in enumerate(basis)}
@@ -558,12 +574,11 @@ def qg_ttxq():
             jamp[index[key]].append((a, _coef_json(re), _coef_json(im)))
     cf = colour_matrix_two_lines(basis)
     denoms, nums = line_numerators(cf)
-    name = "ug_ttxu"
     used = {op["coup"] for op in prog.ops}
     params = ["mdl_MT", "mdl_WT"] + [c for c in ("GC_10", "GC_11", "GC_12") if c in used]
     return dict(
         name=name,
-        process="u g > t t~ u",
+        process=process,
         class_name="Matrix_1_" + name,
         nexternal=nexternal,
         ninitial=2,
@@ -583,7 +598,7 @@ def qg_ttxq():
         colour_basis=[[p, list(w1), list(w2)] for p, w1, w2 in basis],
         color_denom=denoms,
         color_num=nums,
-        initial_states=[[2, 21], [4, 21], [1, 21], [3, 21]],
+        initial_states=[[q, 21] for q in pdg],
         mirror_initial_states=True,
     )
 

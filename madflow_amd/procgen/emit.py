@@ -343,8 +343,9 @@ def same_helicity_pairs(desc):
     that scatters: chirality conservation along the line makes every row in which the two
     carry different helicity labels vanish."""
     kinds = desc.get("kinds", [])
-    if "quark" in kinds and "quark_out" in kinds:
-        return [(kinds.index("quark"), kinds.index("quark_out"))]
+    for a, b in (("quark", "quark_out"), ("antiquark", "antiquark_out")):
+        if a in kinds and b in kinds:
+            return [(kinds.index(a), kinds.index(b))]
     return []
 
 
@@ -526,7 +527,7 @@ def emit_hip_unit(uname):
 # register file next to a 1,890-amplitude program.
 KERNELS = {"gg_ttx": ("fused",), "gg_ttxg": ("fused",), "gg_ttxgg": ("ordered", "fused"),
            "gg_ttxggg": ("ordered",), "uux_ttx": ("fused",), "uux_ttxg": ("fused",), "uux_ttxgg": ("fused",),
-           "ug_ttxu": ("fused",)}
+           "ug_ttxu": ("fused",), "uxg_ttxux": ("fused",)}
 # helicity legs paired per pass of the fused kernel (2^k rows per pass), default 1
 MULTI_K = {}
 
@@ -713,7 +714,7 @@ def emitted_files(desc):
 
 def descriptions():
     """Process descriptions of every generated process, in emission order."""
-    return [gg_ttx_ng(ng) for ng in NGLUONS] + [qq_ttx_ng(ng) for ng in QQ_NGLUONS] + [qg_ttxq()]
+    return [gg_ttx_ng(ng) for ng in NGLUONS] + [qq_ttx_ng(ng) for ng in QQ_NGLUONS] + [qg_ttxq(), qg_ttxq(anti=True)]
 
 
 def main(argv=None):

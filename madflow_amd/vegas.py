@@ -60,6 +60,7 @@ PROCESS_SETUP = {
     "uux_ttxg": (3, [173.0, 173.0, 0.0]),
     "uux_ttxgg": (4, [173.0, 173.0, 0.0, 0.0]),
     "ug_ttxu": (3, [173.0, 173.0, 0.0]),
+    "uxg_ttxux": (3, [173.0, 173.0, 0.0]),
 }
 
 

@@ -17,7 +17,8 @@ OUT = os.path.join(os.path.dirname(O.HERE), "madflow_amd", "processes", "flops_p
 
 def main():
     res = {}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu",
+                 "uxg_ttxux"):
         try:
             O.load_process(name)
         except FileNotFoundError:
@@ -27,8 +28,9 @@ def main():
         # is over every row, as the reference evaluates them)
         d = O.load_process(name)
         kinds = d.get("kinds", [])
-        q = [i for i, k in enumerate(kinds) if k in ("quark", "antiquark", "quark_out")]
-        same = "quark_out" in kinds               # a scattering line keeps the label, an annihilating one flips it
+        q = [i for i, k in enumerate(kinds) if k in ("quark", "antiquark", "quark_out", "antiquark_out")]
+        # a scattering line keeps the label, an annihilating one flips it
+        same = "quark_out" in kinds or "antiquark_out" in kinds
         res[name]["live_rows"] = sum(1 for h in d["helicities"]
                                      if len(q) < 2 or (h[q[0]] == h[q[1]]) == same)
         res[name]["rows"] = len(d["helicities"])

@@ -433,25 +433,35 @@ def qq_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_co
     return float(np.sum(np.abs(qq_amplitude_tensor(p, hel, mt, wt, gs, sqh)) ** 2))
 
 
-def qg_amplitude_tensor(p, hel, mt, wt, gs, sqh):
-    """M[c_q', c_q, c_t, c_tbar, a] of u g > t t~ u (legs 0 u in, 1 g in, 2 t, 3 t~, 4 u out):
+def qg_amplitude_tensor(p, hel, mt, wt, gs, sqh, anti=False):
+    """M[c_row, c_col, c_t, c_tbar, a] of u g > t t~ u (legs 0 u in, 1 g in, 2 t, 3 t~, 4 u out):
     the five diagrams of u u~ > t t~ g written out again with the light line's row spinor
     that of the outgoing quark (oxxxxx, nsf = +1), its column spinor that of the incoming
     quark, and the incoming gluon's polarisation (vxxxxx, nsv = -1); momenta are taken
-    outgoing (incoming legs negated)."""
+    outgoing (incoming legs negated).
+
+    anti=True: u~ g > t t~ u~ (legs 0 u~ in, 4 u~ out).  The fermion arrow of the light line
+    now runs from the outgoing antiquark (column spinor v, ixxxxx nsf = -1) to the incoming
+    one (row spinor v-bar, oxxxxx nsf = -1), and the momentum along the arrow at the row
+    end is -p0 instead of p4; the propagators below are written with that momentum."""
     GC10, GC11 = -gs, 1j * gs
     ref = sqh != np.sqrt(0.5)
-    u = O.wavefunction("i", p[0:1], 0.0, hel[0], 1)[0][2:6]
-    ubo = O.wavefunction("o", p[4:5], 0.0, hel[4], 1)[0][2:6]
+    if anti:
+        u = O.wavefunction("i", p[4:5], 0.0, hel[4], -1)[0][2:6]      # column: v(p4)
+        ubo = O.wavefunction("o", p[0:1], 0.0, hel[0], -1)[0][2:6]    # row: v-bar(p0)
+    else:
+        u = O.wavefunction("i", p[0:1], 0.0, hel[0], 1)[0][2:6]
+        ubo = O.wavefunction("o", p[4:5], 0.0, hel[4], 1)[0][2:6]
     ub = O.wavefunction("o", p[2:3], mt, hel[2], 1)[0][2:6]
     v = O.wavefunction("i", p[3:4], mt, hel[3], -1)[0][2:6]
     eps = O.wavefunction("v", p[1:2], 0.0, hel[1], -1, ref_compat=ref)[0][2:6]
-    Kq, Kqo, Kt, Ktb, kg = -p[0], p[4], p[2], p[3], -p[1]
+    Kt, Ktb, kg = p[2], p[3], -p[1]
+    Kqo = -p[0] if anti else p[4]         # momentum along the light line's arrow at its row end
 
     def fprop(P, m, w):
         return 1j * (slash(P.astype(complex)) + m * np.eye(4)) / (mdot(P, P) - m * m + 1j * m * w)
 
-    KL, KT = Kq + Kqo, Kt + Ktb
+    KL, KT = -p[0] + p[4], Kt + Ktb
     Lprop = (-1j / mdot(KL, KL)) * GC11 * np.array([ubo @ G[m] @ u for m in range(4)])
     Tprop = (-1j / mdot(KT, KT)) * GC11 * np.array([ub @ G[m] @ v for m in range(4)])
     se, sL, sT = slash(eps), slash(Lprop), slash(Tprop)
@@ -469,8 +479,8 @@ def qg_amplitude_tensor(p, hel, mt, wt, gs, sqh):
             + A5 * np.einsum("cba,cik,bqr->qrika", F, T, T))
 
 
-def qg_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False):
-    """Colour-summed |M|^2 of one helicity row of u g > t t~ u."""
+def qg_matrix_hel(p, hel, mt=173.0, wt=1.4915000200271606, alpha_s=0.118, ref_compat=False, anti=False):
+    """Colour-summed |M|^2 of one helicity row of u g > t t~ u (anti: u~ g > t t~ u~)."""
     gs = 2 * np.sqrt(np.pi * alpha_s)
     sqh = (np.float32(np.sqrt(0.5)).item()) if ref_compat else np.sqrt(0.5)
-    return float(np.sum(np.abs(qg_amplitude_tensor(p, hel, mt, wt, gs, sqh)) ** 2))
+    return float(np.sum(np.abs(qg_amplitude_tensor(p, hel, mt, wt, gs, sqh, anti)) ** 2))

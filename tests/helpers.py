@@ -5,9 +5,10 @@ import numpy as np
 from oracle import oracle as O
 
 SQRTS = {"gg_ttx": 1000.0, "gg_ttxg": 2000.0, "gg_ttxgg": 3000.0, "gg_ttxggg": 5000.0,
-         "uux_ttx": 1000.0, "uux_ttxg": 2000.0, "uux_ttxgg": 3000.0, "ug_ttxu": 2000.0}
+         "uux_ttx": 1000.0, "uux_ttxg": 2000.0, "uux_ttxgg": 3000.0, "ug_ttxu": 2000.0,
+         "uxg_ttxux": 2000.0}
 NOUT = {"gg_ttx": 2, "gg_ttxg": 3, "gg_ttxgg": 4, "gg_ttxggg": 5, "uux_ttx": 2, "uux_ttxg": 3, "uux_ttxgg": 4,
-        "ug_ttxu": 3}
+        "ug_ttxu": 3, "uxg_ttxux": 3}
 MT = 173.0
 
 

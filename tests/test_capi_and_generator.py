@@ -27,8 +27,8 @@ def test_registry_and_argument_checks():
     names = [lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())]
     # sorted by (external legs, name): the process names select the default kernels,
     # gg_ttxgg_fused the fused kernel of g g > t t~ g g
-    assert names == ["gg_ttx", "uux_ttx", "gg_ttxg", "ug_ttxu", "uux_ttxg", "gg_ttxgg", "gg_ttxgg_fused",
-                     "uux_ttxgg", "gg_ttxggg"]
+    assert names == ["gg_ttx", "uux_ttx", "gg_ttxg", "ug_ttxu", "uux_ttxg", "uxg_ttxux", "gg_ttxgg",
+                     "gg_ttxgg_fused", "uux_ttxgg", "gg_ttxggg"]
     info = np.zeros(5, dtype=np.int32)
     ttgg = lib.mfx_process_index(b"gg_ttxgg")
     assert lib.mfx_process_info(ttgg, info.ctypes.data_as(ctypes.c_void_p)) == 0
@@ -217,13 +217,13 @@ def test_trace_sum_closed_forms():
 
 
 @pytest.mark.parametrize("kind,ng", [("gg", 0), ("gg", 1), ("gg", 2), ("gg", 3), ("qq", 0), ("qq", 1), ("qq", 2),
-                                     ("qg", 0)])
+                                     ("qg", 0), ("qxg", 0)])
 def test_emitted_files_are_current(kind, ng):
     """Every committed generated file (process description, drop-in module, HIP kernels of
     each kernel family) equals what the generator produces now."""
     from madflow_amd.procgen.emit import emitted_files
     from madflow_amd.procgen.diagrams import qg_ttxq, qq_ttx_ng
-    d = gg_ttx_ng(ng) if kind == "gg" else (qq_ttx_ng(ng) if kind == "qq" else qg_ttxq())
+    d = gg_ttx_ng(ng) if kind == "gg" else (qq_ttx_ng(ng) if kind == "qq" else qg_ttxq(anti=kind == "qxg"))
     files = emitted_files(d)
     assert os.path.join("processes", d["name"] + ".json") in files
     for rel, text in files.items():
@@ -241,7 +241,8 @@ def test_kernel_families_are_registered():
     import importlib
     lib = _lib.load()
     names = {lib.mfx_process_name(i).decode() for i in range(lib.mfx_process_count())}
-    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"):
+    for name in ("gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu",
+                 "uxg_ttxux"):
         mod = importlib.import_module("madflow_amd.processes.matrix_1_%s" % name)
         cls = getattr(mod, "Matrix_1_%s" % name)
         for fam in cls.kernels:
@@ -293,7 +294,7 @@ def test_dual_row_program(ng):
 
 
 @pytest.mark.parametrize("name", ["gg_ttx", "gg_ttxg", "gg_ttxgg", "gg_ttxggg", "uux_ttx", "uux_ttxg", "uux_ttxgg",
-                                  "ug_ttxu"])
+                                  "ug_ttxu", "uxg_ttxux"])
 def test_process_module_model_params(name):
     """Every drop-in module's get_model_param() evaluates exactly the parameters its
     smatrix takes, in that order."""

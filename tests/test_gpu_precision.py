@@ -16,7 +16,7 @@ from madflow_amd.matrix import load_matrix
 # g g > t t~ g g kernel beside its colour-ordered default
 CASES = [("gg_ttx", None, 8192), ("gg_ttxg", None, 8192), ("gg_ttxgg", None, 8192), ("gg_ttxgg", "fused", 8192),
          ("gg_ttxggg", None, 1024), ("uux_ttx", None, 8192), ("uux_ttxg", None, 8192), ("uux_ttxgg", None, 8192),
-         ("ug_ttxu", None, 8192)]
+         ("ug_ttxu", None, 8192), ("uxg_ttxux", None, 8192)]
 
 
 def _passes_cuts(ps, pt_min=30.0, dr_min=0.4):

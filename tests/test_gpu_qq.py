@@ -13,7 +13,7 @@ from oracle import oracle as O
 from tests.helpers import phase_space_points
 from madflow_amd.matrix import load_matrix
 
-PROCS = ["uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"]
+PROCS = ["uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu", "uxg_ttxux"]
 
 
 def _vals(m, ref_gs=True):
@@ -43,7 +43,7 @@ def _cancellation(name, ps, rp, coup):
         num = np.zeros(len(ps))
         den = np.zeros(len(ps))
         for h in d["helicities"]:
-            if (h[0] != h[4]) if name == "ug_ttxu" else (h[0] == h[1]):    # vanish by chirality
+            if (h[0] != h[4]) if name in ("ug_ttxu", "uxg_ttxux") else (h[0] == h[1]):    # vanish by chirality
                 continue
             _, A = O.matrix_hel(key, ps, h, rp, coup, ref_compat=True, want_jamp=True)
             A = np.asarray(A)
@@ -106,7 +106,7 @@ def test_qq_rows_and_jamps(dev, name):
         ref, jref = O.matrix_hel(name, ps, hel, rp, coup, ref_compat=True, want_jamp=True)
         got, jgot = m._matrix(tps, hel, tuple(vals), return_jamp=True)
         got, jgot = got.cpu().numpy(), jgot.cpu().numpy()
-        if (hel[0] != hel[4]) if name == "ug_ttxu" else (hel[0] == hel[1]):    # vanish by chirality
+        if (hel[0] != hel[4]) if name in ("ug_ttxu", "uxg_ttxux") else (hel[0] == hel[1]):    # vanish by chirality
             assert np.all(got == 0.0) and np.all(ref == 0.0)
             continue
         assert np.max(np.abs(got / ref - 1)) < 1e-12

@@ -66,6 +66,6 @@ def test_every_generated_process_has_pdg_codes():
     per leg, gluons 21, tops +-6, light quarks +-2."""
     from madflow_amd.lhe import PDG
     from madflow_amd.procgen.emit import descriptions
-    code = {"gluon": 21, "top": 6, "atop": -6, "quark": 2, "antiquark": -2, "quark_out": 2}
+    code = {"gluon": 21, "top": 6, "atop": -6, "quark": 2, "antiquark": -2, "quark_out": 2, "antiquark_out": -2}
     for d in descriptions():
         assert PDG[d["name"]] == [code[k] for k in d["kinds"]], d["name"]

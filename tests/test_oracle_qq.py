@@ -13,7 +13,9 @@ reference ships no vectors:
   eikonal factor applied to the explicit-colour brute force of u u~ > t t~ g (colour
   operators on every leg, colour conservation checked), which fixes its normalisation,
   colour factors and the diagrams that emit the soft gluon from external legs; BRST fixes
-  the rest.
+  the rest;
+* u g > t t~ u and u~ g > t t~ u~ against their brute force row by row, and against each
+  other by charge conjugation (the top and antitop momenta exchanged).
 """
 
 import itertools
@@ -28,6 +30,15 @@ from tests.helpers import phase_space_points, minkowski
 from tests.test_oracle_multijet import _boost
 
 
+QG = ("ug_ttxu", "uxg_ttxux")
+
+
+def _vanishes(name, hel):
+    """A helicity row that vanishes by chirality: the (anti)quark that scatters flips its
+    label, or the annihilating quark and antiquark share theirs."""
+    return (hel[0] != hel[4]) if name in QG else (hel[0] == hel[1])
+
+
 def test_uux_ttx_closed_form():
     ps, _ = phase_space_points("uux_ttx", 64, seed=41)
     rp, coup = O.default_params(wt=0.0)
@@ -40,14 +51,15 @@ def test_uux_ttx_closed_form():
     np.testing.assert_allclose(me, gs ** 4 * 4 / 9 * (t1 ** 2 + t2 ** 2 + rho / 2), rtol=1e-13)
 
 
-@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "ug_ttxu"])
+@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "ug_ttxu", "uxg_ttxux"])
 def test_qq_bruteforce_every_row(name):
-    """u g > t t~ u (the crossing: incoming gluon, outgoing quark) against its own brute
-    force, written out with the crossed spinors."""
+    """u g > t t~ u (the crossing: incoming gluon, outgoing quark) and u~ g > t t~ u~
+    against their own brute force, written out with the crossed spinors."""
     ps, _ = phase_space_points(name, 2, seed=42)
     d = O.load_process(name)
     rp, coup = O.default_params()
-    brute = BF.qg_matrix_hel if name == "ug_ttxu" else BF.qq_matrix_hel
+    brute = {"ug_ttxu": BF.qg_matrix_hel, "uxg_ttxux": lambda p, h: BF.qg_matrix_hel(p, h, anti=True)}.get(
+        name, BF.qq_matrix_hel)
     for e in range(2):
         for hel in d["helicities"]:
             bf = brute(ps[e], hel)
@@ -76,23 +88,25 @@ def test_qq_vanishing_rows():
     assert zero_helicity_pairs(O.load_process("gg_ttxg")) == []
 
 
-def test_qg_vanishing_rows():
-    """u g > t t~ u: the quark scatters, so the rows in which the incoming and outgoing
-    quark carry different helicity labels vanish, and only those."""
+@pytest.mark.parametrize("name", QG)
+def test_qg_vanishing_rows(name):
+    """u g > t t~ u and u~ g > t t~ u~: the (anti)quark scatters, so the rows in which the
+    incoming and outgoing one carry different helicity labels vanish, and only those."""
     from madflow_amd.procgen.emit import same_helicity_pairs, zero_helicity_pairs
-    d = O.load_process("ug_ttxu")
+    d = O.load_process(name)
     assert same_helicity_pairs(d) == [(0, 4)] and zero_helicity_pairs(d) == []
-    ps, _ = phase_space_points("ug_ttxu", 4, seed=43)
+    ps, _ = phase_space_points(name, 4, seed=43)
     rp, coup = O.default_params()
     for hel in d["helicities"]:
-        v = O.matrix_hel("ug_ttxu", ps, hel, rp, coup)
+        v = O.matrix_hel(name, ps, hel, rp, coup)
         if hel[0] != hel[4]:
             assert np.all(v == 0.0), hel
         else:
             assert np.all(v > 0.0), hel
 
 
-@pytest.mark.parametrize("name,gluon", [("uux_ttxg", 4), ("uux_ttxgg", 4), ("uux_ttxgg", 5), ("ug_ttxu", 1)])
+@pytest.mark.parametrize("name,gluon", [("uux_ttxg", 4), ("uux_ttxgg", 4), ("uux_ttxgg", 5), ("ug_ttxu", 1),
+                                        ("uxg_ttxux", 1)])
 def test_qq_brst_gauge_invariance(name, gluon):
     """Replacing a gluon's polarisation by its momentum (nhel = 4) makes every jamp vanish:
     a diagram missing from the gauge-invariant set, or one with a wrong relative sign,
@@ -104,7 +118,7 @@ def test_qq_brst_gauge_invariance(name, gluon):
         if row >= len(d["helicities"]):
             continue
         hel = list(d["helicities"][row])
-        if (hel[0] != hel[4]) if name == "ug_ttxu" else (hel[0] == hel[1]):
+        if _vanishes(name, hel):
             continue                                  # a row that vanishes by chirality
         _, jp = O.matrix_hel(name, ps, hel, rp, coup, want_jamp=True)
         h = list(hel)
@@ -114,7 +128,7 @@ def test_qq_brst_gauge_invariance(name, gluon):
         assert np.abs(jb).max() < 1e-11 * np.abs(jp).max(), row
 
 
-@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu"])
+@pytest.mark.parametrize("name", ["uux_ttx", "uux_ttxg", "uux_ttxgg", "ug_ttxu", "uxg_ttxux"])
 def test_qq_lorentz_invariance(name):
     ps, _ = phase_space_points(name, 4, seed=45)
     rp, coup = O.default_params(wt=0.0)
@@ -123,7 +137,8 @@ def test_qq_lorentz_invariance(name):
     np.testing.assert_allclose(a, b, rtol=1e-10)
 
 
-@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1), ("uux_ttxgg", 2), ("ug_ttxu", 0)])
+@pytest.mark.parametrize("name,k", [("uux_ttx", 0), ("uux_ttxg", 1), ("uux_ttxgg", 2), ("ug_ttxu", 0),
+                                    ("uxg_ttxux", 0)])
 def test_qq_helicity_sum(name, k):
     """smatrix = sum over all 2^nexternal rows / (4 spins x colours x k!), the colours
     3 x 3 for q q~ and 3 x 8 for q g."""
@@ -132,7 +147,7 @@ def test_qq_helicity_sum(name, k):
     n = d["nexternal"]
     rows = [list(h) for h in itertools.product([-1, 1], repeat=n)]
     assert sorted(map(tuple, d["helicities"])) == sorted(map(tuple, rows))
-    assert d["denominator"] == 4 * (24 if name == "ug_ttxu" else 9) * math.factorial(k)
+    assert d["denominator"] == 4 * (24 if name in QG else 9) * math.factorial(k)
     rp, coup = O.default_params()
     tot = sum(O.matrix_hel(name, ps, h, rp, coup) for h in rows) / d["denominator"]
     np.testing.assert_allclose(O.smatrix(name, ps, rp, coup), tot, rtol=1e-13)
@@ -262,3 +277,23 @@ def test_qg_bruteforce_catches_a_dropped_diagram():
         assert abs(mut / bf - 1) > 1e-3
     finally:
         del O._cache["ug_ttxu_mutant"]
+
+
+def test_qg_charge_conjugation():
+    """C invariance of QCD: |M|^2 of u~ g > t t~ u~ at (p0, p1, p2, p3, p4) equals |M|^2 of
+    u g > t t~ u with the top and antitop momenta exchanged, row by row with the same
+    helicity labels (the antitop's label moving with its momentum), and after the sum.
+    The two programs are generated with different spinors and fermion arrows, so a wrong
+    spinor kind, propagator momentum or colour assignment in either breaks this."""
+    ps, _ = phase_space_points("uxg_ttxux", 4, seed=48)
+    swapped = np.ascontiguousarray(ps[:, [0, 1, 3, 2, 4]])
+    rp, coup = O.default_params()
+    d = O.load_process("uxg_ttxux")
+    for hel in d["helicities"]:
+        a = O.matrix_hel("uxg_ttxux", ps, hel, rp, coup)
+        b = O.matrix_hel("ug_ttxu", swapped, [hel[0], hel[1], hel[3], hel[2], hel[4]], rp, coup)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=0.0)
+    np.testing.assert_allclose(O.smatrix("uxg_ttxux", ps, rp, coup), O.smatrix("ug_ttxu", swapped, rp, coup),
+                               rtol=1e-13)
+    # the unswapped momenta are a different configuration: the identity is not trivial
+    assert np.max(np.abs(O.smatrix("uxg_ttxux", ps, rp, coup) / O.smatrix("ug_ttxu", ps, rp, coup) - 1)) > 1e-3

@@ -310,8 +310,11 @@ struct FlowArgs {
   int inverse;
 };
 
+// Register budgets: at NBM = 4 (the 8-dim flow) that of 4 waves per SIMD, two 8-wave tiles
+// per CU, which the training batch needs to fill the chip; above, that of 3 waves.
 template <int NBM>
-__global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict__ P,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NBM <= 4 ? 4 : 3)))
+k_flow(FlowArgs A, const float* __restrict__ P,
                                              const unsigned short* __restrict__ Pb) {
   constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
   const int D = A.geo.D, L = A.geo.L;
@@ -349,6 +352,22 @@ __global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict
     const LayerGeom& g = A.geo.layer[l];
     const int nB = g.nB, NP = g.NP;
     char* base = A.save ? A.save + A.sl.bytes * l : nullptr;
+    // the weight operands are requested a phase ahead of their use, so that the loads (L2
+    // hits) are in flight while the earlier phases run: here the W1 column and bias of
+    // layer 1 and the W2 fragments and bias of this wave's column tile; the W3 fragments
+    // and biases of its (up to two) column tiles of the output layer before the W2 GEMM
+    static_assert(NT == 4 * H, "four samples per thread in layer 1");
+    constexpr int NU3 = (NPM / 16 + WPB - 1) / WPB;
+    constexpr int NUP = NU3 < 2 ? NU3 : 2;   // W3 tiles whose operands are prefetched
+    const int j1 = tid & (H - 1);
+    float w1[DMAX];
+#pragma unroll
+    for (int i = 0; i < DMAX; ++i) w1[i] = i < g.nA ? P[g.oW1 + i * H + j1] : 0.f;
+    const float b1 = P[g.ob1 + j1];
+    const float b2 = P[g.ob2 + 16 * wid + r];
+    bf16x8 w2f[H / 32];
+#pragma unroll
+    for (int c = 0; c < H / 32; ++c) w2f[c] = ld_bf16x8(Pb + g.oW2t + (16 * wid + r) * H + 32 * c + k8);
     __syncthreads();                       // xs holds this layer's input
     for (int q = tid; q < TILE * DMAX; q += NT) {
       const int s = q / DMAX, i = q % DMAX;
@@ -358,14 +377,10 @@ __global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict
     }
     __syncthreads();
     // layer 1 on VALU: 16 x 128 outputs; thread t computes column t % H for the samples
-    // t / H + 4 u, with that column of W1 loaded once (all rows in flight together)
+    // t / H + 4 u
     {
-      static_assert(NT == 4 * H, "four samples per thread in layer 1");
-      const int j = tid & (H - 1), sb = tid / H;
-      float w1[DMAX];
-#pragma unroll
-      for (int i = 0; i < DMAX; ++i) w1[i] = i < g.nA ? P[g.oW1 + i * H + j] : 0.f;
-      const float bj = P[g.ob1 + j];
+      const int j = j1, sb = tid / H;
+      const float bj = b1;
 #pragma unroll
       for (int u = 0; u < TILE / 4; ++u) {
         const int s = sb + 4 * u;
@@ -378,13 +393,22 @@ __global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict
     }
     __syncthreads();
     // layer 2: h2 = relu(h1 W2 + b2), B operand W2t [out][in]; wave w: column tile 16 w
+    bf16x8 w3f[NUP][H / 32];
+    float b3[NUP];
+#pragma unroll
+    for (int u = 0; u < NUP; ++u) {
+      const int n0 = 16 * (wid + WPB * u);
+      b3[u] = n0 < NP ? P[g.ob3 + n0 + r] : 0.f;
+#pragma unroll
+      for (int c = 0; c < H / 32; ++c)
+        w3f[u][c] = n0 < NP ? ld_bf16x8(Pb + g.oW3t + (n0 + r) * H + 32 * c + k8) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
     {
       const int n0 = 16 * wid;
-      const float bias = P[g.ob2 + n0 + r];
+      const float bias = b2;
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int K0 = 0; K0 < H; K0 += 32)
-        acc = mfma_k32(ld_bf16x8(h1s + r * WP + K0 + k8), ld_bf16x8(Pb + g.oW2t + (n0 + r) * H + K0 + k8), acc);
+      for (int K0 = 0; K0 < H; K0 += 32) acc = mfma_k32(ld_bf16x8(h1s + r * WP + K0 + k8), w2f[K0 / 32], acc);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = kq + i, col = n0 + r;
@@ -393,16 +417,16 @@ __global__ void __launch_bounds__(NT) k_flow(FlowArgs A, const float* __restrict
     }
     __syncthreads();
     // layer 3: raw = h2 W3 + b3, B operand W3t [NP][H]; wave w: column tiles w, w + 8, ...
-    // (unrolled to the bound, so that the operand loads of every tile are in flight together)
 #pragma unroll
-    for (int u = 0; u < (NPM / 16 + WPB - 1) / WPB; ++u) {
+    for (int u = 0; u < NU3; ++u) {
       const int n0 = 16 * (wid + WPB * u);
       if (n0 < NP) {
-        const float bias = P[g.ob3 + n0 + r];
+        const float bias = u < NUP ? b3[u < NUP ? u : 0] : P[g.ob3 + n0 + r];
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int K0 = 0; K0 < H; K0 += 32)
-          acc = mfma_k32(ld_bf16x8(h2s + r * WP + K0 + k8), ld_bf16x8(Pb + g.oW3t + (n0 + r) * H + K0 + k8), acc);
+          acc = mfma_k32(ld_bf16x8(h2s + r * WP + K0 + k8),
+                         u < NUP ? w3f[u < NUP ? u : 0][K0 / 32] : ld_bf16x8(Pb + g.oW3t + (n0 + r) * H + K0 + k8), acc);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int row = kq + i, col = n0 + r;

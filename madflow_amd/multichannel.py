@@ -253,6 +253,54 @@ class MultiChannelIntegrand(MEIntegrand):
         """Philox counter offset of channel c's points (disjoint streams per channel)."""
         return c << 40
 
+    def _chunk_any(self, c, ev0, n, iteration, scale, keep_f):
+        """One chunk of channel c of whatever the mixture holds: an analytic channel (drawn
+        uniformly on its cube, or from its attached flow), or the learned channel."""
+        if self.flow is not None:
+            return self._chunk_learned(c, ev0, n, iteration, scale, keep_f)
+        if c in self.channel_flows:
+            # the channel's own flow on its cube (defensive mixture with the uniform)
+            from .phasespace_gpu import philox_uniforms
+            u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0, dtype=torch.float32)
+            ys, winv = self.channel_flows[c].sample_mixture(u, u, self.channel_flow_mix)
+            return self.chunk_channel(c, ev0, n, iteration, scale, keep_f, y_in=ys, w_in=winv)
+        return self.chunk_channel(c, ev0, n, iteration, scale, keep_f)
+
+    def unweighted_events(self, n_events, iteration, wmax):
+        """Unweighted events of the multi-channel phase space: n_events points shared among
+        the channels in the proportions of the current channel weights (as one iteration
+        would draw them), each with its stratified weight (the weights whose mean over all
+        points is the estimate of sigma), kept with probability min(1, w / wmax) on the
+        device.  Returns (momenta of the kept events, (K, nexternal, 4) host array, their
+        weights, the number of kept events with w > wmax), as MEIntegrand.unweighted_events.
+        Points, weights and keep decisions depend only on the seed, the iteration and the
+        event index."""
+        alpha_used = self._alpha_all()
+        counts = self._counts(alpha_used, int(n_events))
+        N = float(n_events)
+        kept, wts, over = [], [], 0
+        for c in range(len(alpha_used)):
+            nc = int(counts[c])
+            scale = alpha_used[c] * N / nc
+            ev = 0
+            while ev < nc:
+                n = min(self.events_limit, nc - ev)
+                ev0 = self.channel_offset(c) + ev
+                _, _, _, f = self._chunk_any(c, ev0, n, iteration, scale, True)
+                idx, _ = self.unweight(f, wmax, iteration, ev0)
+                il = idx.long()
+                kept.append(self._bufs["all_ps"][:n].index_select(0, il).cpu().numpy())
+                fw = f.index_select(0, il)
+                wts.append(fw.cpu().numpy())
+                over += int((fw > wmax).sum())
+                ev += n
+        return np.concatenate(kept), np.concatenate(wts), over
+
+    def proposal_iteration(self, *args, **kw):
+        """Not available here: a learned density joins the multi-channel phase space as a
+        channel (attach_flow) or as one channel's proposal (attach_channel_flow)."""
+        raise NotImplementedError("MultiChannelIntegrand: use attach_flow or attach_channel_flow, then iteration()")
+
     def iteration(self, n_events, iteration, train=True, keep_f=False):
         """One iteration of n_events points over the channels; the same dict as
         MEIntegrand.iteration plus the channel weights used (``alpha``)."""
@@ -271,17 +319,7 @@ class MultiChannelIntegrand(MEIntegrand):
             ev = 0
             while ev < nc:
                 n = min(self.events_limit, nc - ev)
-                if learned:
-                    r, k, mom, f = self._chunk_learned(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
-                elif c in self.channel_flows:
-                    # the channel's own flow on its cube (defensive mixture with the uniform)
-                    from .phasespace_gpu import philox_uniforms
-                    ev0 = self.channel_offset(c) + ev
-                    u = philox_uniforms(n, self.ndim, self.p.seed, iteration, ev0=ev0, dtype=torch.float32)
-                    ys, winv = self.channel_flows[c].sample_mixture(u, u, self.channel_flow_mix)
-                    r, k, mom, f = self.chunk_channel(c, ev0, n, iteration, scale, keep_f, y_in=ys, w_in=winv)
-                else:
-                    r, k, mom, f = self.chunk_channel(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
+                r, k, mom, f = self._chunk_any(c, self.channel_offset(c) + ev, n, iteration, scale, keep_f)
                 tot = r if tot is None else _merge(tot, r)
                 moments += mom / N
                 npass_tot += k

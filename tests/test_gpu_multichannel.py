@@ -266,3 +266,49 @@ def test_mc_channel_flow_unbiased_and_reduces_variance(dev):
           flush=True)
     assert abs(I0 - I1) < 3 * np.hypot(e0, e1)
     assert vp1[c] < 0.5 * vp[c]
+
+
+def _observables(ev):
+    """(m_tt~, leading gluon pT) of events (K, nexternal, 4)."""
+    tt = ev[:, 2] + ev[:, 3]
+    mtt = np.sqrt(np.maximum(tt[:, 0] ** 2 - np.sum(tt[:, 1:] ** 2, axis=1), 0.0))
+    pt = np.hypot(ev[:, 4:, 1], ev[:, 4:, 2]).max(axis=1)
+    return mtt, pt
+
+
+@pytest.mark.gpu
+def test_mc_unweighted_events(dev):
+    """Unweighted events of the multi-channel phase space (g g > t t~ g g, FP32), drawn from
+    the same stratified points as an iteration with the same index: no kept event is over
+    the bar when wmax is the largest weight, the number kept matches sum_e min(1, w_e/wmax)
+    within 5 binomial standard deviations, and the mean t t~ mass and leading gluon pT
+    agree with unweighted events of plain RAMBO on diet (over-weight events there carrying
+    w/wmax) within 4 combined standard errors."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.vegas import MEIntegrand, VegasFlow
+    print("case mc unweighted events", flush=True)
+    nev, it = 2000000, 20
+    stats = {}
+    for tag, ig in (("diet", MEIntegrand("gg_ttxgg", SQRTS, precision=32, mapping="diet", seed=5, **CUTS)),
+                    ("multichannel", MultiChannelIntegrand("gg_ttxgg", SQRTS, precision=32, seed=5, **CUTS))):
+        vf = VegasFlow(ig.ndim, 500000)
+        vf.compile(ig)
+        vf.run_integration(3)
+        vf.freeze_grid()
+        f = ig.iteration(nev, it, train=False, keep_f=True)["f"].cpu().numpy()
+        wmax = float(f.max()) if tag == "multichannel" else float(np.quantile(f[f > 0], 0.999))
+        ev, w, over = ig.unweighted_events(nev, it, wmax)
+        p = np.minimum(1.0, f / wmax)
+        print(tag, "kept", len(ev), "expected", p.sum(), "over", over, flush=True)
+        assert abs(len(ev) - p.sum()) < 5 * np.sqrt(np.sum(p * (1 - p))) + 1
+        if tag == "multichannel":
+            assert over == 0
+        om = np.maximum(1.0, w / wmax)
+        stats[tag] = []
+        for x in _observables(ev):
+            mean = np.sum(om * x) / np.sum(om)
+            se = np.sqrt(np.sum(om ** 2 * (x - mean) ** 2)) / np.sum(om)
+            stats[tag].append((mean, se))
+        print(tag, "means", stats[tag], flush=True)
+    for (a, ea), (b, eb) in zip(stats["diet"], stats["multichannel"]):
+        assert abs(a - b) < 4 * np.hypot(ea, eb)

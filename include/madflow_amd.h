@@ -127,11 +127,14 @@ int mfx_philox_uniforms_f32(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, 
 /* Phase-space / integrand configuration (plain C struct). */
 typedef struct mfx_ps_params {
   int n_out;            /* outgoing particles (t, t~, gluons) */
-  int ndim;             /* VEGAS dimensions: 4*n_out (RAMBO), 3*n_out-4 (RAMBO on diet), 4*n_out+2 (hadronic) */
+  int ndim;             /* VEGAS dimensions: 4*n_out (RAMBO), 3*n_out-4 (RAMBO on diet), 4*n_out+2 (hadronic),
+                           3*n_out-2 (mapping 3) */
   int nbins;            /* VEGAS bins per dimension (<= 64) */
   int mapping;          /* 0 RAMBO (phasespace.py:145), 1 RAMBO on diet (minimal dimensions),
-                           2 hadronic: x1, x2 then RAMBO at sqrt(x1 x2 s) (ramboflow, phasespace.py:255-317) */
-  double sqrts;         /* partonic sqrt(s) in GeV (mappings 0, 1) or the hadronic sqrt(s) (mapping 2) */
+                           2 hadronic: x1, x2 then RAMBO at sqrt(x1 x2 s) (ramboflow, phasespace.py:255-317),
+                           3 hadronic multi-channel (mfx_mc_generate only): x1, x2 as in mapping 2, then the
+                           channels' maps at sqrt(x1 x2 s) on the other 3*n_out-4 dimensions */
+  double sqrts;         /* partonic sqrt(s) in GeV (mappings 0, 1) or the hadronic sqrt(s) (mappings 2, 3) */
   double masses[8];     /* outgoing masses */
   double gev2pb;        /* unit conversion applied with the flux 1/(2 s) (phasespace.py:315-317) */
   double pt_min;        /* cuts on outgoing legs >= cut_first_leg: pT > pt_min */
@@ -139,8 +142,8 @@ typedef struct mfx_ps_params {
   double dr_min;        /*   Delta R(i,j) > dr_min between those legs (0 disables) */
   int cut_first_leg;    /* first external leg index the cuts apply to (n_out + 2: no cuts) */
   uint32_t seed;        /* Philox key */
-  double shat_min;      /* mapping 2: lower end of shat (phasespace.py:270-273 uses (sum of masses)^2) */
-  int lab_frame;        /* mapping 2: 1 = momenta boosted to the lab frame (phasespace.py:322-357; what the
+  double shat_min;      /* mappings 2, 3: lower end of shat (phasespace.py:270-273 uses (sum of masses)^2) */
+  int lab_frame;        /* mappings 2, 3: 1 = momenta boosted to the lab frame (phasespace.py:322-357; what the
                            reference's integration script writes, com_output = False at madflow_exec.py:389),
                            0 = the partonic c.m. frame (PhaseSpaceGenerator's own default, phasespace.py:377) */
   int reserved;
@@ -200,11 +203,16 @@ typedef struct mfx_mc_channel {
  * x gev2pb / (2 s), with g_j the density of channel j at the point (its inverse map and
  * grid); ratios (device, nev x nch) receives g_j / g.  scale = alpha_channel * N / nev
  * makes the mean of f over all channels' points the stratified estimate.  The other
- * outputs, the workspace and *npass_host are those of mfx_ps_generate. */
+ * outputs, the workspace and *npass_host are those of mfx_ps_generate.
+ * Mapping 3 (hadronic): the first two coordinates give x1, x2 as in mapping 2, the channels
+ * act at sqrt(shat) with each channel's range capped by the kinematic limit there, wps
+ * carries the (1 - tau_min)(-ln tau) weight and the flux 1 / (2 shat), the momenta are in
+ * the lab frame when lab_frame is set (cuts in the partonic frame), and x12 (device,
+ * nev x 2) receives x1, x2; x12 may be NULL at fixed energy. */
 int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
                     double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
                     double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
-                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
+                    double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream);
 /* mfx_mc_generate with the channel's cube points given (y_in, device (nev, ndim) f32; a
  * proposal on the cube such as a flow per channel) and their weights w_in (f64 (nev,),
  * 1 / the density of y on the cube, or NULL for 1): wps is multiplied by w_in, which keeps
@@ -212,7 +220,7 @@ int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, in
 int mfx_mc_generate_points(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
                            double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
                            uint32_t iteration, const float* y_in, const double* w_in, double* all_ps, double* wps,
-                           uint8_t* flag, int* slot, double* dense, double* ratios, void* workspace,
+                           uint8_t* flag, int* slot, double* dense, double* ratios, double* x12, void* workspace,
                            int64_t workspace_bytes, int64_t* npass_host, void* stream);
 
 /* A learned channel beside the analytic ones (channel 0 must be flat): its density on the

@@ -237,15 +237,33 @@ __device__ __forceinline__ double grid_jacobian(const double* __restrict__ grid,
   return jac;
 }
 
-// channel map: cube point x -> outgoing momenta p (c.m. frame); returns dPhi/dx
-__device__ double mc_forward(const McChannel& ch, double sqrts, const double* x, double (*p)[4], bool& conv) {
+// Upper end of a channel's range at the partonic energy sqrts: the range given, capped by
+// the kinematic limit there (t-channel: the largest energy of the massless leg,
+// (s - T^2) / (2 sqrt(s)); s-channel: (sqrt(s) - T)^2, with T the other legs' masses).  At
+// fixed energy (mapping 1) the range given is used as it is.
+__device__ __forceinline__ double mc_hi(const McChannel& ch, int mapping, double sqrts) {
+  if (mapping != 3 || ch.type == 0) return ch.hi;
+  const double T = ch.sub.total_mass;
+  const double lim = ch.type == 1 ? (sqrts * sqrts - T * T) / (2.0 * sqrts) : (sqrts - T) * (sqrts - T);
+  return fmin(ch.hi, lim);
+}
+
+// channel map: cube point x -> outgoing momenta p (c.m. frame); returns dPhi/dx.  hi is the
+// upper end of the channel's range at this energy (mc_hi); conv = false when the range is
+// empty there
+__device__ double mc_forward(const McChannel& ch, double hi, double sqrts, const double* x, double (*p)[4],
+                             bool& conv) {
   conv = true;
   if (ch.type == 0) return diet_event(ch.sub, x, sqrts, p, conv);
+  if (!(hi > ch.lo)) {
+    conv = false;
+    return 0.0;
+  }
   double q[RAMBO_MAXN][4];
   if (ch.type == 1) {
-    const double A = 1.0 / ch.lo, B = 1.0 / ch.hi;
+    const double A = 1.0 / ch.lo, B = 1.0 / hi;
     const double pt = 1.0 / (A - x[0] * (A - B));
-    const double ymax = acosh(fmax(ch.hi / pt, 1.0));
+    const double ymax = acosh(fmax(hi / pt, 1.0));
     const double y = (2.0 * x[1] - 1.0) * ymax;
     double sp, cp;
     sincos(2.0 * MFX_PI * x[2], &sp, &cp);
@@ -261,7 +279,7 @@ __device__ double mc_forward(const McChannel& ch, double sqrts, const double* x,
     for (int c = 0; c < 4; ++c) p[ch.a][c] = k[c];
     return pt * pt * pt * (A - B) * (2.0 * ymax) / (8.0 * MFX_PI * MFX_PI) * w;
   }
-  const double L = log(ch.hi / ch.lo);
+  const double L = log(hi / ch.lo);
   const double m2 = ch.lo * exp(x[0] * L), m = sqrt(m2);
   RamboConst rc = ch.sub;
   rc.masses[ch.nsub - 1] = m;
@@ -287,7 +305,7 @@ __device__ double mc_forward(const McChannel& ch, double sqrts, const double* x,
 
 // inverse of mc_forward: outgoing momenta (c.m. frame) -> x and dPhi/dx; ok = false
 // outside the channel's support (its density is zero there)
-__device__ double mc_inverse(const McChannel& ch, double sqrts, const double (*p)[4], double* x, bool& ok) {
+__device__ double mc_inverse(const McChannel& ch, double hi, double sqrts, const double (*p)[4], double* x, bool& ok) {
   double q[RAMBO_MAXN][4];
   ok = true;
   if (ch.type == 0) {
@@ -295,13 +313,17 @@ __device__ double mc_inverse(const McChannel& ch, double sqrts, const double (*p
       for (int c = 0; c < 4; ++c) q[j][c] = p[j][c];
     return diet_inverse(ch.sub, sqrts, q, x);
   }
+  if (!(hi > ch.lo)) {
+    ok = false;
+    return 0.0;
+  }
   if (ch.type == 1) {
     const double* k = p[ch.a];
     const double pt = sqrt(k[1] * k[1] + k[2] * k[2]);
-    ok = pt >= ch.lo && pt <= ch.hi;
-    const double ptc = fmin(fmax(pt, ch.lo), ch.hi);
-    const double A = 1.0 / ch.lo, B = 1.0 / ch.hi;
-    const double ymax = acosh(fmax(ch.hi / ptc, 1.0));
+    ok = pt >= ch.lo && pt <= hi;
+    const double ptc = fmin(fmax(pt, ch.lo), hi);
+    const double A = 1.0 / ch.lo, B = 1.0 / hi;
+    const double ymax = acosh(fmax(hi / ptc, 1.0));
     const double y = asinh(k[3] / ptc);
     ok = ok && fabs(y) <= ymax;
     x[0] = (A - 1.0 / ptc) / (A - B);
@@ -320,9 +342,9 @@ __device__ double mc_inverse(const McChannel& ch, double sqrts, const double (*p
   double G[4];
   for (int c = 0; c < 4; ++c) G[c] = p[ch.a][c] + p[ch.b][c];
   const double m2 = G[0] * G[0] - G[1] * G[1] - G[2] * G[2] - G[3] * G[3];
-  ok = m2 >= ch.lo && m2 <= ch.hi;
+  ok = m2 >= ch.lo && m2 <= hi;
   if (!ok) return 0.0;
-  const double L = log(ch.hi / ch.lo), m = sqrt(m2);
+  const double L = log(hi / ch.lo), m = sqrt(m2);
   x[0] = log(m2 / ch.lo) / L;
   double kr[4];
   boost_by(p[ch.a], G, m, -1.0, kr);
@@ -383,7 +405,7 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
                                                            double* __restrict__ all_ps, double* __restrict__ wps,
                                                            uint8_t* __restrict__ flag, double* __restrict__ ratios,
                                                            float* __restrict__ y0_out, double* __restrict__ gdens,
-                                                           int* __restrict__ block_count) {
+                                                           double* __restrict__ x12, int* __restrict__ block_count) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
   bool ok = false;
   if (e < nev) {
@@ -395,9 +417,23 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
     if (y_in) y_ok = external_point(y_in + e * D, D, y);
     else uniforms(g, ev0 + e, iteration, y);
     const double jac = vmap(g, grids + (size_t)c * D * (NB + 1), y, x, nullptr);
+    // hadronic mode (mapping 3): the first two coordinates give tau = x1 x2 and x1 as in
+    // the ramboflow mapping (weight (1 - tau_min)(-ln tau), the same for every channel), the
+    // channels act at sqrt(shat) on the others
+    double rs_hat = g.sqrts, norm = g.norm, wtau = 1.0, x1 = 1.0, x2 = 1.0;
+    const int off = g.mapping == 3 ? 2 : 0;
+    if (g.mapping == 3) {
+      const double s = g.sqrts * g.sqrts, taumin = g.shat_min / s, dtau = 1.0 - taumin;
+      const double tau = x[0] * dtau + taumin;
+      x1 = pow(tau, x[1]);
+      x2 = tau / x1;
+      rs_hat = sqrt(tau * s);
+      wtau = dtau * -log(tau);
+      norm = g.gev2pb / (2.0 * tau * s);
+    }
     double p[RAMBO_MAXN + 2][4];
     bool conv = true;
-    const double w = mc_forward(mc.ch[c], g.sqrts, x, p + 2, conv);
+    const double w = mc_forward(mc.ch[c], mc_hi(mc.ch[c], g.mapping, rs_hat), rs_hat, x + off, p + 2, conv);
     double gj[MC_MAXCH], gsum = 0.0;
     for (int j = 0; j < mc.nch; ++j) {
       double v = 0.0;
@@ -408,7 +444,11 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
       } else if (conv) {
         double xj[MAXDIM];
         bool in = false;
-        const double wj = mc_inverse(mc.ch[j], g.sqrts, p + 2, xj, in);
+        if (off) {
+          xj[0] = x[0];
+          xj[1] = x[1];
+        }
+        const double wj = mc_inverse(mc.ch[j], mc_hi(mc.ch[j], g.mapping, rs_hat), rs_hat, p + 2, xj + off, in);
         const double J = grid_inverse(grids + (size_t)j * D * (NB + 1), D, NB, xj, (learned && j == 0) ? y0 : nullptr);
         if (in && wj > 0.0) v = 1.0 / (J * wj);
       } else if (learned && j == 0) {
@@ -418,17 +458,30 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_mc_generate(GenConst g, McConst m
       gj[j] = v;
       gsum += mc.alpha[j] * v;
     }
-    const double eh = 0.5 * g.sqrts;
+    const double eh = 0.5 * rs_hat;
     p[0][0] = eh; p[0][1] = 0.0; p[0][2] = 0.0; p[0][3] = eh;
     p[1][0] = eh; p[1][1] = 0.0; p[1][2] = 0.0; p[1][3] = -eh;
-    const double wt = learned ? mc.scale : mc.scale / gsum;
+    const double wt = learned ? mc.scale : mc.scale * wtau / gsum;
     ok = y_ok && conv && w > 0.0 && (learned ? gj[0] > 0.0 : gsum > 0.0) && isfinite(wt) && pass_cuts(g, p + 2);
+    if (g.mapping == 3 && g.lab_frame) {
+      // boost along z to the lab frame, rapidity -eta = (1/2) ln(x1 / x2), after the cuts
+      const double eta = -0.5 * log(x1 / x2), ch = cosh(eta), sh = sinh(eta);
+      for (int i = 0; i < g.nexternal; ++i) {
+        const double E = p[i][0], pz = p[i][3];
+        p[i][0] = ch * E - sh * pz;
+        p[i][3] = ch * pz - sh * E;
+      }
+    }
     double* out = all_ps + e * (g.nexternal * 4);
     for (int i = 0; i < g.nexternal; ++i)
       for (int k = 0; k < 4; ++k) out[4 * i + k] = p[i][k];
+    if (x12) {
+      x12[2 * e] = x1;
+      x12[2 * e + 1] = x2;
+    }
     const double wx = w_in ? w_in[e] : 1.0;    // 1 / (density of y on the cube) for external points
     ok = ok && isfinite(wx) && wx > 0.0;
-    wps[e] = ok ? wt * g.norm * wx : 0.0;
+    wps[e] = ok ? wt * norm * wx : 0.0;
     flag[e] = ok ? 1 : (conv ? 0 : 2);
     if (learned) {
       for (int j = 0; j < mc.nch; ++j) ratios[e * rs + j] = gj[j];
@@ -791,9 +844,11 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
   if (!p) return fail(-3, "null parameters%s", "");
   const int n = p->n_out;
   if (n < 2 || n > RAMBO_MAXN) return fail(-1, "n_out must be in [2, 8]%s", "");
-  if (p->mapping < 0 || p->mapping > 2) return fail(-2, "mapping must be 0 (RAMBO), 1 (diet) or 2 (hadronic)%s", "");
-  const int need = p->mapping == 0 ? 4 * n : p->mapping == 1 ? 3 * n - 4 : 4 * n + 2;
-  if (p->ndim != need) return fail(-2, "ndim must be 4*n_out (RAMBO), 3*n_out-4 (diet) or 4*n_out+2 (hadronic)%s", "");
+  if (p->mapping < 0 || p->mapping > 3)
+    return fail(-2, "mapping must be 0 (RAMBO), 1 (diet), 2 (hadronic) or 3 (hadronic, multi-channel)%s", "");
+  const int need = p->mapping == 0 ? 4 * n : p->mapping == 1 ? 3 * n - 4 : p->mapping == 2 ? 4 * n + 2 : 3 * n - 2;
+  if (p->ndim != need)
+    return fail(-2, "ndim must be 4*n_out (RAMBO), 3*n_out-4 (diet), 4*n_out+2 (hadronic) or 3*n_out-2 (mapping 3)%s", "");
   if (p->ndim > MAXDIM || p->nbins < 2 || p->nbins > MAXBINS) return fail(-2, "ndim/nbins out of range%s", "");
   memset(&g, 0, sizeof(g));
   g.rc.n = n;
@@ -805,7 +860,7 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
     if (p->masses[i] != 0.0) g.rc.massive = 1;
   }
   if (g.rc.total_mass >= p->sqrts) return fail(-2, "not enough energy for the outgoing masses%s", "");
-  if (p->mapping == 2 && !(p->shat_min >= g.rc.total_mass * g.rc.total_mass && p->shat_min < p->sqrts * p->sqrts))
+  if (p->mapping >= 2 && !(p->shat_min >= g.rc.total_mass * g.rc.total_mass && p->shat_min < p->sqrts * p->sqrts))
     return fail(-2, "hadronic mapping: shat_min must lie in [(sum of masses)^2, s)%s", "");
   g.rc.logw0 = log(MFX_PI / 2.0) * (n - 1) - 2.0 * lgamma((double)(n - 1)) - log((double)(n - 1));
   g.rc.inorm = 1.0 / pow(2.0 * MFX_PI, 3 * n - 4);
@@ -850,6 +905,7 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
+  if (g.mapping == 3) return fail(-2, "mapping 3 is the multi-channel phase space's (mfx_mc_generate)%s", "");
   if (nev < 0) return fail(-2, "nev must be >= 0%s", "");
   if (nev > 0x7fffffff) return fail(-2, "nev must fit in 31 bits%s", "");
   if (workspace_bytes < mfx_integrand_workspace_size(nev, prm->ndim, prm->nbins))
@@ -908,7 +964,8 @@ int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, ui
 
 static int make_mc(const mfx_ps_params* prm, const GenConst& g, const mfx_mc_channel* chs, int nch,
                    const double* alpha, double scale, McConst& mc) {
-  if (g.mapping != 1) return fail(-2, "multi-channel phase space needs the diet parameters (mapping 1)%s", "");
+  if (g.mapping != 1 && g.mapping != 3)
+    return fail(-2, "multi-channel phase space needs the diet parameters (mapping 1, or 3 for hadronic)%s", "");
   if (nch < 1 || nch > MC_MAXCH) return fail(-2, "between 1 and 8 channels%s", "");
   if (!chs || !alpha) return fail(-3, "null pointer%s", "");
   if (!(scale > 0.0) || !isfinite(scale)) return fail(-2, "scale must be > 0%s", "");
@@ -960,8 +1017,8 @@ static int make_mc(const mfx_ps_params* prm, const GenConst& g, const mfx_mc_cha
 static int mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
                        double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
                        const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
-                       double* dense, double* ratios, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
-                       void* stream) {
+                       double* dense, double* ratios, double* x12, void* workspace, int64_t workspace_bytes,
+                       int64_t* npass_host, void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -978,12 +1035,13 @@ static int mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels,
     return 0;
   }
   if (!grids || !all_ps || !wps || !flag || !slot || !dense || !ratios || !workspace) return fail(-3, "null pointer%s", "");
+  if (g.mapping == 3 && !x12) return fail(-3, "null pointer (x12, needed in hadronic mode)%s", "");
   hipStream_t s = (hipStream_t)stream;
   const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
-                     y_in, w_in, all_ps, wps, flag, ratios, (float*)nullptr, (double*)nullptr, counts);
+                     y_in, w_in, all_ps, wps, flag, ratios, (float*)nullptr, (double*)nullptr, x12, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);
@@ -1002,19 +1060,19 @@ static int mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels,
 int mfx_mc_generate(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
                     double scale, int channel, const double* grids, uint64_t ev0, int64_t nev, uint32_t iteration,
                     double* all_ps, double* wps, uint8_t* flag, int* slot, double* dense, double* ratios,
-                    void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+                    double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host, void* stream) {
   return mc_generate(prm, channels, nch, alpha, scale, channel, grids, ev0, nev, iteration, nullptr, nullptr, all_ps,
-                     wps, flag, slot, dense, ratios, workspace, workspace_bytes, npass_host, stream);
+                     wps, flag, slot, dense, ratios, x12, workspace, workspace_bytes, npass_host, stream);
 }
 
 int mfx_mc_generate_points(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
                            double scale, int channel, const double* grids, uint64_t ev0, int64_t nev,
                            uint32_t iteration, const float* y_in, const double* w_in, double* all_ps, double* wps,
-                           uint8_t* flag, int* slot, double* dense, double* ratios, void* workspace,
+                           uint8_t* flag, int* slot, double* dense, double* ratios, double* x12, void* workspace,
                            int64_t workspace_bytes, int64_t* npass_host, void* stream) {
   if (nev > 0 && !y_in) return fail(-3, "null pointer (y_in)%s", "");
   return mc_generate(prm, channels, nch, alpha, scale, channel, grids, ev0, nev, iteration, y_in, w_in, all_ps, wps,
-                     flag, slot, dense, ratios, workspace, workspace_bytes, npass_host, stream);
+                     flag, slot, dense, ratios, x12, workspace, workspace_bytes, npass_host, stream);
 }
 
 int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* channels, int nch, const double* alpha,
@@ -1029,6 +1087,7 @@ int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* chan
   rc = make_mc(prm, g, channels, nch, alpha, scale, mc);
   if (rc) return rc;
   if (nch + 1 > MC_MAXCH) return fail(-2, "at most 7 channels beside the learned one%s", "");
+  if (g.mapping == 3) return fail(-2, "the learned channel runs at fixed energy (mapping 1)%s", "");
   if (mc.ch[0].type != 0) return fail(-2, "the learned channel lives on channel 0's cube, which must be flat%s", "");
   if (channel < 0 || channel >= nch) return fail(-2, "channel out of range%s", "");
   if (y_in && channel != 0) return fail(-2, "learned points are mapped through channel 0%s", "");
@@ -1047,7 +1106,7 @@ int mfx_mc_generate_learned(const mfx_ps_params* prm, const mfx_mc_channel* chan
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
   hipLaunchKernelGGL(k_mc_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, mc, channel, grids, ev0, nev, iteration,
-                     y_in, (const double*)nullptr, all_ps, wps, flag, ratios, y0_out, gdens, counts);
+                     y_in, (const double*)nullptr, all_ps, wps, flag, ratios, y0_out, gdens, (double*)nullptr, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);

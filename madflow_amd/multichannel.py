@@ -37,10 +37,10 @@ def _mc_sig(lib):
     vp, i64, u64, u32, i, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
     from .vegas import PsParams
     lib.mfx_mc_generate.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp, vp, vp,
-                                    vp, vp, i64, ctypes.POINTER(i64), vp]
+                                    vp, vp, vp, i64, ctypes.POINTER(i64), vp]
     lib.mfx_mc_generate.restype = i
     lib.mfx_mc_generate_points.argtypes = [ctypes.POINTER(PsParams), vp, i, vp, d, i, vp, u64, i64, u32, vp, vp, vp, vp,
-                                           vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+                                           vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
     lib.mfx_mc_generate_points.restype = i
     lib.mfx_mc_moments.argtypes = [i64, i, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_mc_moments.restype = i
@@ -72,18 +72,31 @@ def default_channels(masses, sqrts, pt_min=0.0, dr_min=0.0):
 
 
 class MultiChannelIntegrand(MEIntegrand):
-    """MEIntegrand over the multi-channel phase space (fixed sqrt(s)); the keywords are
-    MEIntegrand's (the mapping is the diet cube of 3 n_out - 4 dimensions).  ``channels``
-    overrides default_channels; ``adapt_alpha`` switches the channel-weight adaptation;
-    ``alpha_min`` is the weight floor."""
+    """MEIntegrand over the multi-channel phase space; the keywords are MEIntegrand's.
+    mapping "diet" (the default): fixed sqrt(s), the channels on the diet cube of
+    3 n_out - 4 dimensions.  mapping "hadronic": the hadronic sqrt(s), with two more
+    dimensions in front that give x1, x2 as in MEIntegrand's hadronic mapping, the
+    channels at sqrt(x1 x2 s) (their ranges capped by the kinematic limit there), and the
+    luminosity, scale and running alpha_s of MEIntegrand's hadronic mode.  ``channels``
+    overrides default_channels (built at the given sqrt(s)); ``adapt_alpha`` switches the
+    channel-weight adaptation; ``alpha_min`` is the weight floor."""
 
     def __init__(self, process, sqrts, channels=None, adapt_alpha=True, alpha_min=0.02, **kw):
-        if kw.get("mapping", "diet") != "diet":
-            raise ValueError("the multi-channel phase space runs on the diet cube (fixed sqrt(s))")
-        kw["mapping"] = "diet"
+        mapping = kw.get("mapping", "diet")
+        if mapping not in ("diet", "hadronic"):
+            raise ValueError("the multi-channel phase space runs on the diet cube: mapping 'diet' (fixed sqrt(s)) "
+                             "or 'hadronic'")
+        kw["mapping"] = mapping
         super().__init__(process, sqrts, **kw)
         _mc_sig(self.lib)
         n_out, masses = PROCESS_SETUP[process]
+        if self.hadronic:
+            # x1, x2, then the diet cube (the library's mapping 3)
+            self.p.mapping = 3
+            self.p.ndim = 3 * n_out - 2
+            self.ndim = self.p.ndim
+            self.grid = torch.linspace(0, 1, self.nbins + 1, dtype=torch.float64,
+                                       device=self.device).repeat(self.ndim, 1).contiguous()
         pt_min, dr_min = kw.get("pt_min", 0.0), kw.get("dr_min", 0.0)
         self.channels = list(channels) if channels is not None else default_channels(masses, sqrts, pt_min, dr_min)
         self.nch = len(self.channels)
@@ -136,6 +149,8 @@ class MultiChannelIntegrand(MEIntegrand):
         proportions).  Grids are not refined while it is attached."""
         if self.channels[0][0] != 0:
             raise ValueError("the learned channel needs channel 0 to be the flat one")
+        if self.hadronic:
+            raise ValueError("the learned channel runs at fixed sqrt(s)")
         if self.nch + 1 > 8:
             raise ValueError("at most 7 analytic channels beside the learned one")
         if flow.D != self.ndim:
@@ -201,16 +216,17 @@ class MultiChannelIntegrand(MEIntegrand):
         return (b["result"].cpu().numpy().copy(), npass, mom.cpu().numpy(),
                 (b["f"][:n].clone() if keep_f else None))
 
-    def _smatrix_timed(self, b, npass):
+    def _smatrix_timed(self, b, npass, params=None):
+        params = self.params if params is None else params
         if self.me_timer is not None and npass:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-            me = self.matrix.smatrix(b["dense"][:npass], *self.params)
+            me = self.matrix.smatrix(b["dense"][:npass], *params)
             e1.record()
             self.me_timer.append((e0, e1, npass))
             return me
-        return self.matrix.smatrix(b["dense"][:npass], *self.params) if npass else b["wps"][:0]
+        return self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:0]
 
     def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False, y_in=None, w_in=None):
         """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f).
@@ -227,7 +243,7 @@ class MultiChannelIntegrand(MEIntegrand):
                 ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
                 alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
                 _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]), _ptr(b["dense"]),
-                _ptr(self._ratios), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+                _ptr(self._ratios), _ptr(b["x12"]), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
         else:
             y_in = _lib.require_device_tensor(y_in, torch.float32, "y_in")
             w_in = None if w_in is None else _lib.require_device_tensor(w_in, torch.float64, "w_in")
@@ -235,10 +251,17 @@ class MultiChannelIntegrand(MEIntegrand):
                 ctypes.byref(self.p), ctypes.cast(self._chan_arr, ctypes.c_void_p), self.nch,
                 alpha.ctypes.data_as(ctypes.c_void_p), float(scale), c, _ptr(self.grids), ev0, n, iteration,
                 _ptr(y_in), _ptr(w_in), _ptr(b["all_ps"]), _ptr(b["wps"]), _ptr(b["flag"]), _ptr(b["slot"]),
-                _ptr(b["dense"]), _ptr(self._ratios), _ptr(b["ws"]), b["ws"].numel(), ctypes.byref(npass), s))
+                _ptr(b["dense"]), _ptr(self._ratios), _ptr(b["x12"]), _ptr(b["ws"]), b["ws"].numel(),
+                ctypes.byref(npass), s))
         npass = npass.value
         self.last_chunk_npass = npass
-        me = self._smatrix_timed(b, npass)
+        if self.hadronic and npass:
+            # couplings at each event's scale, and the parton luminosity folded into the
+            # matrix elements of the passing events (compact order)
+            params, lumi = self._hadronic_factors(b, n, npass)
+            me = self._smatrix_timed(b, npass, params) * lumi
+        else:
+            me = self._smatrix_timed(b, npass)
         _lib.check(self.lib.mfx_vegas_accumulate(
             ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
             _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
@@ -295,6 +318,11 @@ class MultiChannelIntegrand(MEIntegrand):
                 over += int((fw > wmax).sum())
                 ev += n
         return np.concatenate(kept), np.concatenate(wts), over
+
+    def chunk(self, *args, **kw):
+        """Not available here: a chunk belongs to one channel (chunk_channel); the flat
+        channel's cube alone is cube0_chunk."""
+        raise NotImplementedError("MultiChannelIntegrand: use chunk_channel(c, ...) or cube0_chunk")
 
     def proposal_iteration(self, *args, **kw):
         """Not available here: a learned density joins the multi-channel phase space as a

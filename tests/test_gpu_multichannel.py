@@ -312,3 +312,96 @@ def test_mc_unweighted_events(dev):
         print(tag, "means", stats[tag], flush=True)
     for (a, ea), (b, eb) in zip(stats["diet"], stats["multichannel"]):
         assert abs(a - b) < 4 * np.hypot(ea, eb)
+
+
+def _toy_gluon(x, q2):
+    """A gluon-like x g(x) for tests (no PDF grid is available): 1.7 x^-0.2 (1 - x)^5."""
+    return 1.7 * x ** -0.2 * (1.0 - x) ** 5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lab_frame", [False, True])
+def test_mc_hadronic_matches_restatement(dev, lab_frame):
+    """Hadronic multi-channel phase space (g g > t t~ g g at sqrt(s) = 13 TeV, x1 and x2 in
+    front of the channels' dimensions): for points of every channel the kernel's momenta,
+    x1, x2, weights (with the flux 1 / (2 shat)) and density ratios equal the
+    restatement's, which builds the channels at each event's sqrt(shat), within 1e-9; cut
+    flags agree.  In the lab frame the momenta are the partonic ones boosted along z:
+    the same invariant masses, and total momentum ((x1 + x2), 0, 0, (x1 - x2)) sqrt(s) / 2."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    print("case mc hadronic parity lab_frame", lab_frame, flush=True)
+    S13 = 13000.0
+    ig = MultiChannelIntegrand("gg_ttxgg", S13, mapping="hadronic", pdf=_toy_gluon, scale=173.0, precision=64, seed=9,
+                               lab_frame=lab_frame, **CUTS)
+    assert ig.ndim == 3 * 4 - 2
+    rng = np.random.default_rng(8)
+    grids = _random_grids(ig.nch, ig.ndim, ig.nbins, rng)
+    ig.grids.copy_(torch.from_numpy(grids))
+    alpha = np.array([0.1, 0.4, 0.35, 0.15])
+    ig.alpha = alpha.copy()
+    n = 400
+    shat_min = (2 * 173.0) ** 2
+    for c in range(ig.nch):
+        ev0 = ig.channel_offset(c) + 3
+        ig.chunk_channel(c, ev0, n, 2, 1.0)
+        b = ig._bufs
+        ps = b["all_ps"][:n].cpu().numpy()
+        wps = b["wps"][:n].cpu().numpy()
+        flag = b["flag"][:n].cpu().numpy()
+        x12 = b["x12"][:n].cpu().numpy()
+        ratios = ig._ratios[:n].cpu().numpy()
+        y = O.philox_uniforms(n, ig.ndim, 9, 2, ev0=ev0)
+        P, W, Rr, X1, X2, RS, valid = R.mc_weight_hadronic(4, MASSES, S13, CUTS["pt_min"], CUTS["dr_min"], shat_min,
+                                                           grids, alpha, c, y)
+        ok_ref = valid.copy()
+        ok_ref[valid] = _cuts(P[valid])
+        print("channel", c, "valid", valid.mean(), "pass", ok_ref.mean(), flush=True)
+        assert ok_ref.mean() > 0.05
+        assert np.array_equal(flag == 1, ok_ref)
+        sel = ok_ref
+        np.testing.assert_allclose(x12[sel, 0], X1[sel], rtol=1e-12)
+        np.testing.assert_allclose(x12[sel, 1], X2[sel], rtol=1e-12)
+        norm = R.GEV2PB_REF / (2 * RS[sel] ** 2)
+        np.testing.assert_allclose(wps[sel], W[sel] * norm, rtol=1e-9)
+        np.testing.assert_allclose(ratios[sel], Rr[sel], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(ratios[sel] @ alpha, 1.0, rtol=1e-12)
+        out = ps[sel, 2:]
+        if not lab_frame:
+            assert np.max(np.abs(out - P[sel])) < 1e-9 * S13
+        else:
+            m2 = lambda q: q[..., 0] ** 2 - np.sum(q[..., 1:] ** 2, axis=-1)
+            np.testing.assert_allclose(m2(out[:, 0] + out[:, 2]), m2(P[sel][:, 0] + P[sel][:, 2]), rtol=1e-8)
+            tot = out.sum(axis=1)
+            np.testing.assert_allclose(tot[:, 0], 0.5 * S13 * (X1[sel] + X2[sel]), rtol=1e-10)
+            np.testing.assert_allclose(tot[:, 3], 0.5 * S13 * (X1[sel] - X2[sel]), atol=1e-9 * S13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,kin,nev,warm", [("gg_ttxg", dict(pt_min=30.0, abs_eta_max=2.5), 1000000, 3),
+                                               ("gg_ttxgg", dict(pt_min=30.0, dr_min=0.4), 4000000, 5)])
+def test_mc_hadronic_cross_section_against_ramboflow(dev, name, kin, nev, warm):
+    """sigma(p p > g g > t t~ g) (pT(g) > 30 GeV, |eta(g)| < 2.5) and sigma(p p > g g >
+    t t~ g g) (pT(g) > 30 GeV, Delta R(g, g) > 0.4) at sqrt(s) = 13 TeV with a toy gluon
+    density, fixed scale, FP32 matrix elements: the hadronic multi-channel phase space
+    (3 n_out - 2 dimensions) agrees with MEIntegrand's hadronic ramboflow mapping (4 n_out
+    + 2 dimensions) within 3 combined standard deviations (adapting iterations, then 3
+    frozen ones); prints both relative errors per iteration and unweighting efficiencies."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    from madflow_amd.vegas import MEIntegrand, VegasFlow
+    print("case mc hadronic sigma", name, flush=True)
+    res = {}
+    for tag, ig in (("ramboflow", MEIntegrand(name, 13000.0, precision=32, mapping="hadronic", seed=3,
+                                              pdf=_toy_gluon, scale=173.0, **kin)),
+                    ("multichannel", MultiChannelIntegrand(name, 13000.0, precision=32, mapping="hadronic",
+                                                           seed=3, pdf=_toy_gluon, scale=173.0, **kin))):
+        vf = VegasFlow(ig.ndim, nev)
+        vf.compile(ig)
+        vf.run_integration(warm)
+        vf.freeze_grid()
+        r = [ig.iteration(nev, 10 + k, train=False, keep_f=True) for k in range(3)]
+        res[tag] = (np.mean([x["I"] for x in r]), np.sqrt(np.sum([x["var"] for x in r])) / 3)
+        f = torch.cat([x["f"] for x in r])
+        print(tag, "ndim", ig.ndim, "sigma", res[tag], "rel err per iteration", r[0]["sigma"] / r[0]["I"],
+              "unweighting eff", float(f.mean() / f.max()), "alpha", getattr(ig, "alpha", None), flush=True)
+    (a, ea), (b, eb) = res["ramboflow"], res["multichannel"]
+    assert abs(a - b) < 3 * np.hypot(ea, eb)

@@ -151,6 +151,8 @@ class MultiChannelIntegrand(MEIntegrand):
             raise ValueError("the learned channel needs channel 0 to be the flat one")
         if self.hadronic:
             raise ValueError("the learned channel runs at fixed sqrt(s)")
+        if self.channel_flows:
+            raise ValueError("channel flows are attached")
         if self.nch + 1 > 8:
             raise ValueError("at most 7 analytic channels beside the learned one")
         if flow.D != self.ndim:
@@ -164,8 +166,9 @@ class MultiChannelIntegrand(MEIntegrand):
         """Draw channel c's points from ``flow`` on its cube (before its frozen grid) in a
         defensive mixture q' = (1 - mix) q + mix with the uniform density; each weight is
         multiplied by 1 / q'(y), so the stratified estimate stays unbiased while the
-        channel weights alpha_j g_j / sum alpha g keep their meaning.  Channel weights and
-        grids are no longer adapted while a channel flow is attached."""
+        channel weights alpha_j g_j / sum alpha g keep their meaning.  While a channel flow
+        is attached the channel weights stay as they are and so does that channel's grid
+        (the flow was trained on it); training iterations still refine the other grids."""
         if not 0 <= c < self.nch:
             raise ValueError("channel %d out of range" % c)
         if self.flow is not None:

@@ -124,6 +124,17 @@ int mfx_philox_uniforms_f32(uint64_t ev0, int64_t nev, int ndim, uint32_t seed, 
                             void* stream);
 
 /* ------------------------------------------------------ VEGAS integration */
+/* One cut of PhaseSpaceGenerator.register_cut (phasespace.py:424-478): an event passes
+ * when min_val < variable(particle) (if use_min) and variable(particle) < max_val (if
+ * use_max), on the outgoing external leg `particle` (2 .. n_out + 1) in the partonic c.m.
+ * frame, before any boost (phasespace.py:502-517).  variable: 0 pt (phasespace.py:416-421),
+ * 1 mt = sqrt(mt2), 2 mt2 = m^2 + pt^2 (phasespace.py:405-414). */
+#define MFX_MAX_CUTS 16
+typedef struct mfx_cut {
+  int variable, particle, use_min, use_max;
+  double min_val, max_val;
+} mfx_cut;
+
 /* Phase-space / integrand configuration (plain C struct). */
 typedef struct mfx_ps_params {
   int n_out;            /* outgoing particles (t, t~, gluons) */
@@ -147,6 +158,9 @@ typedef struct mfx_ps_params {
                            reference's integration script writes, com_output = False at madflow_exec.py:389),
                            0 = the partonic c.m. frame (PhaseSpaceGenerator's own default, phasespace.py:377) */
   int reserved;
+  int ncuts;            /* register_cut cuts in `cuts`, ANDed with the ones above (0 .. MFX_MAX_CUTS) */
+  int reserved2;
+  mfx_cut cuts[MFX_MAX_CUTS];
 } mfx_ps_params;
 
 /* Device workspace needed by mfx_ps_generate / mfx_vegas_accumulate for nev events. */

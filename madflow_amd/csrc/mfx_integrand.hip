@@ -36,6 +36,8 @@ struct GenConst {
   double pt_min, abs_eta_max, dr_min;
   int cut_first, cut_last;         // outgoing-leg range the gluon cuts apply to
   uint32_t seed;
+  int ncuts;                       // per-leg cuts of register_cut (outgoing leg index 0..)
+  mfx_cut cuts[MFX_MAX_CUTS];
 };
 
 // --------------------------------------------------------------------- helpers
@@ -120,6 +122,21 @@ __device__ __forceinline__ bool pass_cuts(const GenConst& g, double (*p)[4]) {
         if (dp > MFX_PI) dp = 2.0 * MFX_PI - dp;
         if (!(de * de + dp * dp > g.dr_min * g.dr_min)) return false;
       }
+  }
+  // register_cut's predicates (phasespace.py:457-466): strict bounds on pt, mt or mt2
+  for (int c = 0; c < g.ncuts; ++c) {
+    const mfx_cut& k = g.cuts[c];
+    const double* q = p[k.particle];
+    const double pt2 = q[1] * q[1] + q[2] * q[2];
+    double v;
+    if (k.variable == 0) {
+      v = sqrt(pt2);
+    } else {
+      const double mt2 = q[0] * q[0] - q[3] * q[3];      // m^2 + pt^2
+      v = k.variable == 1 ? sqrt(fmax(mt2, 0.0)) : mt2;
+    }
+    if (k.use_min && !(v > k.min_val)) return false;
+    if (k.use_max && !(v < k.max_val)) return false;
   }
   return true;
 }
@@ -880,6 +897,16 @@ static int make_gen(const mfx_ps_params* p, GenConst& g) {
   g.cut_last = n - 1;
   g.seed = p->seed;
   if (p->cut_first_leg < 2 || p->cut_first_leg > n + 2) return fail(-2, "cut_first_leg out of range%s", "");
+  if (p->ncuts < 0 || p->ncuts > MFX_MAX_CUTS) return fail(-2, "ncuts must lie in [0, MFX_MAX_CUTS]%s", "");
+  g.ncuts = p->ncuts;
+  for (int c = 0; c < p->ncuts; ++c) {
+    mfx_cut k = p->cuts[c];
+    if (k.variable < 0 || k.variable > 2) return fail(-2, "cut variable must be 0 (pt), 1 (mt) or 2 (mt2)%s", "");
+    if (k.particle < 2 || k.particle > n + 1) return fail(-2, "cuts apply to outgoing legs 2 .. n_out + 1%s", "");
+    if (!k.use_min && !k.use_max) return fail(-2, "a cut needs a lower or an upper bound%s", "");
+    k.particle -= 2;                       // outgoing-leg index of the momenta pass_cuts sees
+    g.cuts[c] = k;
+  }
   return 0;
 }
 

@@ -13,6 +13,7 @@ compensated accumulation of sum f, sum f^2 and the per-dimension histograms
 
 import ctypes
 import math
+import warnings
 import time
 
 import numpy as np
@@ -26,6 +27,15 @@ BINS_MAX = 50
 ALPHA = 1.5
 
 
+MAX_CUTS = 16
+CUT_VARIABLES = {"pt": 0, "mt": 1, "mt2": 2}
+
+
+class Cut(ctypes.Structure):
+    _fields_ = [("variable", ctypes.c_int), ("particle", ctypes.c_int), ("use_min", ctypes.c_int),
+                ("use_max", ctypes.c_int), ("min_val", ctypes.c_double), ("max_val", ctypes.c_double)]
+
+
 class PsParams(ctypes.Structure):
     _fields_ = [
         ("n_out", ctypes.c_int), ("ndim", ctypes.c_int), ("nbins", ctypes.c_int), ("mapping", ctypes.c_int),
@@ -33,6 +43,7 @@ class PsParams(ctypes.Structure):
         ("pt_min", ctypes.c_double), ("abs_eta_max", ctypes.c_double), ("dr_min", ctypes.c_double),
         ("cut_first_leg", ctypes.c_int), ("seed", ctypes.c_uint32),
         ("shat_min", ctypes.c_double), ("lab_frame", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("ncuts", ctypes.c_int), ("reserved2", ctypes.c_int), ("cuts", Cut * MAX_CUTS),
     ]
 
 
@@ -153,6 +164,39 @@ class MEIntegrand:
         self._bufs = None
         self.me_timer = None      # list of (start event, end event, n) around smatrix launches
         self.last_chunk_npass = 0
+
+    def register_cut(self, variable, particle=None, min_val=None, max_val=None):
+        """PhaseSpaceGenerator.register_cut (phasespace.py:424-478): keep only the events
+        with min_val < variable(particle) < max_val (either bound may be None), variable
+        "pt", "mt" or "mt2" (m^2 + pt^2) of the outgoing external leg ``particle`` (2 ..
+        n_out + 1, in the process's leg order), in the partonic c.m. frame; the cuts are
+        ANDed with each other and with the pt_min / abs_eta_max / dr_min cuts.  As in the
+        reference, an unknown variable or a particle past the last leg raises ValueError,
+        and a cut with neither bound is ignored with a warning.  A cut needs its particle
+        (the reference's particle=None form has no well-defined event mask) and an
+        outgoing one (the incoming partons have pt = 0)."""
+        if variable not in CUT_VARIABLES:
+            raise ValueError(f"{variable} is not implemented")
+        if particle is None:
+            raise ValueError("register_cut needs the particle the cut applies to")
+        if particle >= self.nexternal:
+            raise ValueError(f"Cannot apply cuts to particle {particle}, python idx starts at 0!")
+        if particle < 2:
+            raise ValueError("cuts apply to the outgoing particles, 2 .. %d" % (self.nexternal - 1))
+        if min_val is None and max_val is None:
+            warnings.warn(f"Cut for {variable} has no min or max val, ignoring")
+            return
+        if self.p.ncuts >= MAX_CUTS:
+            raise ValueError("at most %d cuts" % MAX_CUTS)
+        k = self.p.cuts[self.p.ncuts]
+        k.variable, k.particle = CUT_VARIABLES[variable], int(particle)
+        k.use_min, k.min_val = int(min_val is not None), float(min_val if min_val is not None else 0.0)
+        k.use_max, k.max_val = int(max_val is not None), float(max_val if max_val is not None else 0.0)
+        self.p.ncuts += 1
+
+    def clear_cuts(self):
+        """Remove every cut registered with register_cut (phasespace.py:396-402)."""
+        self.p.ncuts = 0
 
     def _alloc(self, n):
         if self._bufs is not None and self._bufs["n"] >= n:

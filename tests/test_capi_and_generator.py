@@ -96,6 +96,49 @@ def test_physical_input_guards():
     assert rc == -2 and b"shat_min" in lib.mfx_last_error()
 
 
+def test_register_cut_arguments_and_struct_layout(tmp_path):
+    """register_cut's cuts in mfx_ps_params: the ctypes struct has the C header's size and
+    field offsets (checked with the host compiler), and malformed cuts are rejected on the
+    host before any launch: too many, an unknown variable, an incoming or missing
+    particle, and neither bound (the pointers are never dereferenced)."""
+    import shutil
+    import subprocess
+    from madflow_amd.vegas import MAX_CUTS, PsParams, _sig
+    if shutil.which("gcc"):
+        src = tmp_path / "layout.c"
+        src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include <stdint.h>\n'
+                       '#include "%s"\nint main(void){printf("%%zu %%zu %%zu", sizeof(mfx_ps_params), '
+                       'offsetof(mfx_ps_params, ncuts), offsetof(mfx_ps_params, cuts)); return 0;}\n'
+                       % os.path.join(ROOT, "include", "madflow_amd.h"))
+        exe = tmp_path / "layout"
+        subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+        size, off_n, off_c = map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split())
+        assert (size, off_n, off_c) == (ctypes.sizeof(PsParams), PsParams.ncuts.offset, PsParams.cuts.offset)
+    lib = _lib.load()
+    _sig(lib)
+    fake = ctypes.c_void_p(1 << 20)
+
+    def run(**cut):
+        p = PsParams()
+        p.n_out, p.ndim, p.nbins, p.mapping, p.sqrts = 3, 5, 50, 1, 2000.0
+        p.masses[0], p.masses[1] = 173.0, 173.0
+        p.gev2pb, p.cut_first_leg = 1.0, 5
+        p.ncuts = cut.pop("ncuts", 1)
+        k = p.cuts[0]
+        k.variable, k.particle, k.use_min, k.use_max = cut.get("variable", 0), cut.get("particle", 3), 1, 0
+        k.min_val = 60.0
+        if cut.get("no_bounds"):
+            k.use_min = 0
+        npass = ctypes.c_int64(0)
+        return lib.mfx_ps_generate(ctypes.byref(p), fake, 0, 10, 0, None, None, fake, fake, fake, fake, fake, None,
+                                   fake, 1 << 30, ctypes.byref(npass), None)
+    assert run(ncuts=MAX_CUTS + 1) == -2 and b"ncuts" in lib.mfx_last_error()
+    assert run(variable=3) == -2 and b"variable" in lib.mfx_last_error()
+    assert run(particle=1) == -2 and b"outgoing" in lib.mfx_last_error()
+    assert run(particle=5) == -2 and b"outgoing" in lib.mfx_last_error()
+    assert run(no_bounds=True) == -2 and b"bound" in lib.mfx_last_error()
+
+
 def test_rambo_wrapper_rejects_unphysical():
     """The Python wrapper raises ValueError for one sqrt(s) below the masses before it
     touches a device (the reference's message, phasespace.py:156-158)."""

@@ -363,3 +363,43 @@ def test_hadronic_dynamic_scale_and_running_alphas(dev):
     nz = f_fix > 0
     assert nz.sum() > 0.9 * n
     np.testing.assert_allclose(f_run[nz], f_fix[nz] * ratio[nz], rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variable,particle,lo,hi", [("pt", 3, 60.0, 300.0), ("mt", 2, 400.0, None),
+                                                     ("mt2", 4, None, 250.0 ** 2)])
+def test_register_cut_matches_numpy_mask(dev, variable, particle, lo, hi):
+    """register_cut (phasespace.py:424-478) checked as the reference's own test checks it
+    (test_ps.py:40-64): on the same Philox points, the events that pass equal the uncut
+    ones with lo < variable(particle) < hi computed in numpy from their momenta, the
+    momenta of those events are unchanged, and clear_cuts restores the uncut flags."""
+    print("case register_cut", variable, particle, flush=True)
+    ig = MEIntegrand("gg_ttxg", 2000.0, precision=64, mapping="rambo", seed=4)
+    n = 20000
+    ig.chunk(0, n, 3)
+    full = ig._bufs["all_ps"][:n].cpu().numpy().copy()
+    flag0 = ig._bufs["flag"][:n].cpu().numpy().copy()
+    ig.register_cut(variable, particle=particle, min_val=lo, max_val=hi)
+    ig.chunk(0, n, 3)
+    flag = ig._bufs["flag"][:n].cpu().numpy().copy()
+    cut_ps = ig._bufs["all_ps"][:n].cpu().numpy()
+    q = full[:, particle]
+    pt = np.sqrt(q[:, 1] ** 2 + q[:, 2] ** 2)
+    mt2 = (q[:, 0] ** 2 - q[:, 1] ** 2 - q[:, 2] ** 2 - q[:, 3] ** 2) + pt ** 2
+    v = {"pt": pt, "mt": np.sqrt(mt2), "mt2": mt2}[variable]
+    mask = flag0 == 1
+    if lo is not None:
+        mask &= v > lo
+    if hi is not None:
+        mask &= v < hi
+    print("pass fraction", mask.mean(), "of", (flag0 == 1).mean(), flush=True)
+    assert 0.05 < mask.mean() < 0.95
+    assert np.array_equal(flag == 1, mask)
+    np.testing.assert_array_equal(cut_ps[mask], full[mask])
+    ig.clear_cuts()
+    ig.chunk(0, n, 3)
+    assert np.array_equal(ig._bufs["flag"][:n].cpu().numpy(), flag0)
+    with pytest.raises(ValueError):
+        ig.register_cut("eta", particle=3, min_val=0.0)
+    with pytest.raises(ValueError):
+        ig.register_cut("pt", particle=5, min_val=10.0)

@@ -183,6 +183,15 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
                     const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
                     double* dense, double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
                     void* stream);
+/* PhaseSpaceGenerator.__call__ / ramboflow (phasespace.py:255-317,480-520) on the caller's
+ * uniforms: mfx_ps_generate with the points xrand (device, nev x ndim f64) used as they
+ * are, with no VEGAS grid (jacobian 1); the same outputs, cuts and compaction.  With the
+ * hadronic mapping (2) that is ramboflow: x1, x2 from the first two coordinates, RAMBO at
+ * sqrt(x1 x2 s) on the others, the weight with the flux 1/(2 shat) and the GeV^-2 -> pb
+ * factor, and the lab-frame boost after the cuts when lab_frame is set. */
+int mfx_ps_generate_xrand(const mfx_ps_params* prm, const double* xrand, int64_t nev, double* all_ps, double* wps,
+                          uint8_t* flag, int* slot, double* dense, double* x12, void* workspace,
+                          int64_t workspace_bytes, int64_t* npass_host, void* stream);
 
 /* Accumulate one iteration: f = me[slot] * wps (or f_in if given; the reference's
  * scatter_nd back to full length, madflow_exec.py:466-468).  result (device, 6 + ndim*nbins

@@ -144,7 +144,8 @@ __device__ __forceinline__ bool pass_cuts(const GenConst& g, double (*p)[4]) {
 // ---------------------------------------------------------------- generation
 __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double* __restrict__ grid, uint64_t ev0,
                                                         int64_t nev, uint32_t iteration, const float* __restrict__ y_in,
-                                                        const double* __restrict__ w_in, double* __restrict__ all_ps,
+                                                        const double* __restrict__ w_in, const double* __restrict__ x_in,
+                                                        double* __restrict__ all_ps,
                                                         double* __restrict__ wps, uint8_t* __restrict__ flag,
                                                         double* __restrict__ x12, int* __restrict__ block_count) {
   const int64_t e = (int64_t)blockIdx.x * GEN_BLOCK + threadIdx.x;
@@ -152,12 +153,26 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_generate(GenConst g, const double
   if (e < nev) {
     double y[MAXDIM], x[MAXDIM];
     bool y_ok = true;
-    if (y_in) {
-      y_ok = external_point(y_in + e * g.ndim, g.ndim, y);
+    double jac = 1.0;
+    if (x_in) {
+      // the caller's FP64 points, used as they are (no VEGAS grid): a non-finite coordinate
+      // rejects the event, the others are clamped to [0, 1]
+      for (int d = 0; d < g.ndim; ++d) {
+        double v = x_in[e * g.ndim + d];
+        if (!isfinite(v)) {
+          y_ok = false;
+          v = 0.0;
+        }
+        x[d] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+      }
     } else {
-      uniforms(g, ev0 + e, iteration, y);
+      if (y_in) {
+        y_ok = external_point(y_in + e * g.ndim, g.ndim, y);
+      } else {
+        uniforms(g, ev0 + e, iteration, y);
+      }
+      jac = vmap(g, grid, y, x, nullptr);
     }
-    const double jac = vmap(g, grid, y, x, nullptr);
     double p[RAMBO_MAXN + 2][4];           // incoming partons, then the outgoing legs
     double w, norm = g.norm, x1 = 1.0, x2 = 1.0, eh = 0.5 * g.sqrts;
     bool converged = true;
@@ -925,10 +940,10 @@ int64_t mfx_integrand_workspace_size(int64_t nev, int ndim, int nbins) {
   return bytes + 256;
 }
 
-int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
-                    const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
-                    double* dense, double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
-                    void* stream) {
+static int ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
+                       const float* y_in, const double* w_in, const double* x_in, double* all_ps, double* wps,
+                       uint8_t* flag, int* slot, double* dense, double* x12, void* workspace, int64_t workspace_bytes,
+                       int64_t* npass_host, void* stream) {
   GenConst g;
   int rc = make_gen(prm, g);
   if (rc) return rc;
@@ -941,13 +956,14 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
     if (npass_host) *npass_host = 0;
     return 0;
   }
-  if (!grid || !all_ps || !wps || !flag || !slot || !dense || !workspace) return fail(-3, "null pointer%s", "");
+  if ((!grid && !x_in) || !all_ps || !wps || !flag || !slot || !dense || !workspace)
+    return fail(-3, "null pointer%s", "");
   hipStream_t s = (hipStream_t)stream;
   const int nblk = (int)((nev + GEN_BLOCK - 1) / GEN_BLOCK);
   int* counts = (int*)workspace;
   int* offsets = counts + nblk + 1;
-  hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, y_in, w_in, all_ps,
-                     wps, flag, x12, counts);
+  hipLaunchKernelGGL(k_generate, dim3(nblk), dim3(GEN_BLOCK), 0, s, g, grid, ev0, nev, iteration, y_in, w_in, x_in,
+                     all_ps, wps, flag, x12, counts);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nblk, offsets);
   hipLaunchKernelGGL(k_compact, dim3(nblk), dim3(GEN_BLOCK), 0, s, g.nexternal, nev, all_ps, flag, offsets, dense,
                      slot);
@@ -961,6 +977,22 @@ int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, 
     *npass_host = tot;
   }
   return 0;
+}
+
+int mfx_ps_generate(const mfx_ps_params* prm, const double* grid, uint64_t ev0, int64_t nev, uint32_t iteration,
+                    const float* y_in, const double* w_in, double* all_ps, double* wps, uint8_t* flag, int* slot,
+                    double* dense, double* x12, void* workspace, int64_t workspace_bytes, int64_t* npass_host,
+                    void* stream) {
+  return ps_generate(prm, grid, ev0, nev, iteration, y_in, w_in, nullptr, all_ps, wps, flag, slot, dense, x12,
+                     workspace, workspace_bytes, npass_host, stream);
+}
+
+int mfx_ps_generate_xrand(const mfx_ps_params* prm, const double* xrand, int64_t nev, double* all_ps, double* wps,
+                          uint8_t* flag, int* slot, double* dense, double* x12, void* workspace,
+                          int64_t workspace_bytes, int64_t* npass_host, void* stream) {
+  if (nev > 0 && !xrand) return fail(-3, "null pointer (xrand)%s", "");
+  return ps_generate(prm, nullptr, 0, nev, 0, nullptr, nullptr, xrand, all_ps, wps, flag, slot, dense, x12, workspace,
+                     workspace_bytes, npass_host, stream);
 }
 
 int mfx_vegas_accumulate(const mfx_ps_params* prm, uint64_t ev0, int64_t nev, uint32_t iteration, const float* y_in,

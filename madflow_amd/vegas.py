@@ -36,6 +36,32 @@ class Cut(ctypes.Structure):
                 ("use_max", ctypes.c_int), ("min_val", ctypes.c_double), ("max_val", ctypes.c_double)]
 
 
+def add_cut(p, nexternal, variable, particle=None, min_val=None, max_val=None):
+    """Append register_cut's cut (phasespace.py:424-478) to the PsParams p of a process
+    with nexternal legs: min_val < variable(particle) < max_val, variable "pt", "mt" or
+    "mt2", particle an outgoing leg (2 .. nexternal - 1).  The reference's errors for an
+    unknown variable or a particle past the last leg, and its warning (the cut ignored)
+    when neither bound is given."""
+    if variable not in CUT_VARIABLES:
+        raise ValueError(f"{variable} is not implemented")
+    if particle is None:
+        raise ValueError("register_cut needs the particle the cut applies to")
+    if particle >= nexternal:
+        raise ValueError(f"Cannot apply cuts to particle {particle}, python idx starts at 0!")
+    if particle < 2:
+        raise ValueError("cuts apply to the outgoing particles, 2 .. %d" % (nexternal - 1))
+    if min_val is None and max_val is None:
+        warnings.warn(f"Cut for {variable} has no min or max val, ignoring")
+        return
+    if p.ncuts >= MAX_CUTS:
+        raise ValueError("at most %d cuts" % MAX_CUTS)
+    k = p.cuts[p.ncuts]
+    k.variable, k.particle = CUT_VARIABLES[variable], int(particle)
+    k.use_min, k.min_val = int(min_val is not None), float(min_val if min_val is not None else 0.0)
+    k.use_max, k.max_val = int(max_val is not None), float(max_val if max_val is not None else 0.0)
+    p.ncuts += 1
+
+
 class PsParams(ctypes.Structure):
     _fields_ = [
         ("n_out", ctypes.c_int), ("ndim", ctypes.c_int), ("nbins", ctypes.c_int), ("mapping", ctypes.c_int),
@@ -55,6 +81,8 @@ def _sig(lib):
     lib.mfx_ps_generate.argtypes = [P, vp, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64,
                                     ctypes.POINTER(i64), vp]
     lib.mfx_ps_generate.restype = i
+    lib.mfx_ps_generate_xrand.argtypes = [P, vp, i64, vp, vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    lib.mfx_ps_generate_xrand.restype = i
     lib.mfx_vegas_accumulate.argtypes = [P, u64, i64, u32, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp]
     lib.mfx_vegas_accumulate.restype = i
     lib.mfx_vegas_refine.argtypes = [vp, vp, i, i, d, vp]
@@ -175,24 +203,7 @@ class MEIntegrand:
         and a cut with neither bound is ignored with a warning.  A cut needs its particle
         (the reference's particle=None form has no well-defined event mask) and an
         outgoing one (the incoming partons have pt = 0)."""
-        if variable not in CUT_VARIABLES:
-            raise ValueError(f"{variable} is not implemented")
-        if particle is None:
-            raise ValueError("register_cut needs the particle the cut applies to")
-        if particle >= self.nexternal:
-            raise ValueError(f"Cannot apply cuts to particle {particle}, python idx starts at 0!")
-        if particle < 2:
-            raise ValueError("cuts apply to the outgoing particles, 2 .. %d" % (self.nexternal - 1))
-        if min_val is None and max_val is None:
-            warnings.warn(f"Cut for {variable} has no min or max val, ignoring")
-            return
-        if self.p.ncuts >= MAX_CUTS:
-            raise ValueError("at most %d cuts" % MAX_CUTS)
-        k = self.p.cuts[self.p.ncuts]
-        k.variable, k.particle = CUT_VARIABLES[variable], int(particle)
-        k.use_min, k.min_val = int(min_val is not None), float(min_val if min_val is not None else 0.0)
-        k.use_max, k.max_val = int(max_val is not None), float(max_val if max_val is not None else 0.0)
-        self.p.ncuts += 1
+        add_cut(self.p, self.nexternal, variable, particle, min_val, max_val)
 
     def clear_cuts(self):
         """Remove every cut registered with register_cut (phasespace.py:396-402)."""

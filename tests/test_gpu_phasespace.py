@@ -75,3 +75,70 @@ def test_rambo_per_event_energy_guard(dev):
         assert abs(w[i] / wr[0] - 1) < 1e-12
     with pytest.raises(ValueError):
         rambo(torch.from_numpy(x).to(dev), n, torch.from_numpy(sq).to(dev), masses, check_physical=True)
+
+
+@pytest.mark.gpu
+def test_phase_space_generator_cuts(dev):
+    """PhaseSpaceGenerator and register_cut checked as the reference's own test checks them
+    (test_ps.py:40-64): 5 particles at 7 TeV, pT(particle 3) in (60, 300); the generator's
+    pt equals numpy's on the uncut momenta, and the cut output equals the uncut momenta
+    under the numpy mask, with idx the indices of the passing events."""
+    from madflow_amd.phasespace_gpu import PhaseSpaceGenerator, philox_uniforms
+    print("case PhaseSpaceGenerator cuts", flush=True)
+    gen = PhaseSpaceGenerator(5, 7e3)
+    gen.register_cut("pt", particle=3, min_val=60, max_val=300.0)
+    xrand = philox_uniforms(10000, gen.ndim, 11, 0)
+    all_ps, w, x1, x2, idx = gen(xrand)
+    gen.clear_cuts()
+    full_ps, full_w, fx1, fx2, one = gen(xrand)
+    assert one == 1
+    internal_pt = gen.pt(full_ps[:, 3, :]).cpu().numpy()
+    full_np = full_ps.cpu().numpy()
+    numpy_pt = np.sqrt(full_np[:, 3, 1] ** 2 + full_np[:, 3, 2] ** 2)
+    np.testing.assert_allclose(numpy_pt, internal_pt)
+    mask = np.all([numpy_pt > 60.0, numpy_pt < 300.0], axis=0)
+    assert 0.05 < mask.mean() < 0.95
+    np.testing.assert_allclose(all_ps.cpu().numpy(), full_np[mask])
+    np.testing.assert_allclose(w.cpu().numpy(), full_w.cpu().numpy()[mask])
+    np.testing.assert_array_equal(idx.cpu().numpy()[:, 0], np.nonzero(mask)[0])
+    np.testing.assert_allclose(x1.cpu().numpy(), fx1.cpu().numpy()[mask])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("com_output", [True, False])
+def test_ramboflow_matches_oracle(dev, com_output):
+    """ramboflow / PhaseSpaceGenerator on the caller's FP64 uniforms equal the oracle's
+    hadronic generation (mfx_ps_generate's restatement) on the same uniforms through an
+    identity VEGAS grid: momenta, weights (flux and GeV^-2 -> pb included) and x1, x2
+    within 1e-12, in the c.m. frame and boosted to the lab (com_output=False); the
+    reference's checks of test_ps.py:67-78 hold (massless incoming legs, outgoing masses
+    50 and 125 GeV on shell within 1e-8)."""
+    from madflow_amd.phasespace_gpu import PhaseSpaceGenerator, ramboflow
+    print("case ramboflow parity com_output", com_output, flush=True)
+    masses = [50.0, 125.0]
+    n, nb = 2000, 50
+    cfg = dict(n_out=2, ndim=10, nbins=nb, mapping=2, sqrts=7000.0, masses=masses, gev2pb=389379365.6, pt_min=0.0,
+               abs_eta_max=0.0, dr_min=0.0, cut_first_leg=4, seed=13, shat_min=175.0 ** 2,
+               lab_frame=0 if com_output else 1)
+    edges = np.tile(np.linspace(0.0, 1.0, nb + 1), (10, 1))
+    ref_ps, ref_w, ref_flag, ref_x12 = O.generate(cfg, edges, 0, n, 2)
+    y = O.philox_uniforms(n, 10, 13, 2)
+    xrand = torch.from_numpy(y).to(dev)
+    gen = PhaseSpaceGenerator(4, 7000.0, masses=masses, com_output=com_output)
+    ps, w, x1, x2, idx = gen(xrand)
+    assert idx == 1 and np.all(ref_flag == 1)
+    np.testing.assert_allclose(ps.cpu().numpy(), ref_ps, rtol=1e-12, atol=1e-12 * 7000.0)
+    np.testing.assert_allclose(w.cpu().numpy(), ref_w, rtol=1e-12)
+    np.testing.assert_allclose(x1.cpu().numpy(), ref_x12[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(x2.cpu().numpy(), ref_x12[:, 1], rtol=1e-12)
+    m2 = lambda q: q[..., 0] ** 2 - np.sum(q[..., 1:] ** 2, axis=-1)
+    p = ps.cpu().numpy()
+    # on shell to the massive RAMBO's Newton residual (worst 1.4e-9 here; the reference's
+    # test allows 1e-4)
+    np.testing.assert_allclose(m2(p[:, 0:2]), 0.0, atol=1e-6)
+    np.testing.assert_allclose(m2(p[:, 2]), 50.0 ** 2, rtol=1e-8)
+    np.testing.assert_allclose(m2(p[:, 3]), 125.0 ** 2, rtol=1e-8)
+    if com_output:
+        cps, cw, cx1, cx2 = ramboflow(xrand, 4, 7000.0, masses)
+        np.testing.assert_array_equal(cps.cpu().numpy(), p)
+        np.testing.assert_array_equal(cw.cpu().numpy(), w.cpu().numpy())

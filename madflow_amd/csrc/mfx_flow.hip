@@ -814,11 +814,29 @@ __global__ void __launch_bounds__(NT) k_flow_wgrad(BwdArgs A) {
       b2acc = mfma_bf16(ones4, b, b2acc);
       b1acc = mfma_bf16(ones4, c, b1acc);
     }
-    // dW1: A[i][k] = x_A of sample k (row i < 16), B[k][c] = gh1; four samples per step
-    for (int s4 = 0; s4 < nt * TILE; s4 += 4) {
-      const int sk = s4 + kf;
-      const float gv = bf2f(d_h1[(t0 + sk / TILE) * TILE * H + (16 * j + r) * TILE + sk % TILE]);
-      w1acc = mfma_f32(xl[sk * DMAX + r], gv, w1acc);
+    // dW1: A[i][k] = x_A of sample k (row i < 16), B[k][c] = gh1; four samples per step,
+    // lane quarter kf taking samples 4 kf .. 4 kf + 3 of each tile (one 8-byte load of the
+    // tile-transposed deltas).  Every tile's load is issued before the products (a tile
+    // past the split reads the last one and is skipped), so their latencies overlap.
+    {
+      uint2 gq[WG_ST];
+#pragma unroll
+      for (int t = 0; t < WG_ST; ++t) {
+        const int tc = t < nt ? t : nt - 1;
+        gq[t] = *reinterpret_cast<const uint2*>(d_h1 + (t0 + tc) * TILE * H + (16 * j + r) * TILE + 4 * kf);
+      }
+#pragma unroll
+      for (int t = 0; t < WG_ST; ++t) {
+        if (t < nt) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const unsigned wd = u < 2 ? gq[t].x : gq[t].y;
+            const float gv = __uint_as_float((u & 1 ? wd >> 16 : wd & 0xffffu) << 16);
+            const int sk = t * TILE + 4 * kf + u;
+            w1acc = mfma_f32(xl[sk * DMAX + r], gv, w1acc);
+          }
+        }
+      }
     }
 #pragma unroll
     for (int m = 0; m < H / 16; ++m)

@@ -184,6 +184,10 @@ class PhaseSpaceGenerator:
     def generate(self, xrand):
         """Every event of xrand: (momenta (nev, nparticles, 4), weights (0 for the events
         that are cut or whose massive RAMBO did not converge), x1, x2, flags (1 = pass))."""
+        return self._run(xrand)[:5]
+
+    def _run(self, xrand):
+        """generate's outputs, then the passing events' momenta in order and their count."""
         xrand = _lib.require_device_tensor(xrand, torch.float64, "xrand")
         if xrand.dim() != 2 or xrand.shape[1] != self.p.ndim:
             raise ValueError("xrand must be (nevents, %d)" % self.p.ndim)
@@ -202,17 +206,13 @@ class PhaseSpaceGenerator:
         _lib.check(self.lib.mfx_ps_generate_xrand(
             ctypes.byref(self.p), p(xrand), nev, p(all_ps), p(wps), p(flag), p(slot), p(dense), p(x12), p(ws),
             ws.numel(), ctypes.byref(npass), _lib.stream_ptr(dev)))
-        self._last = (dense, npass.value)
-        return all_ps, wps, x12[:, 0], x12[:, 1], flag
+        return all_ps, wps, x12[:, 0], x12[:, 1], flag, dense, npass.value
 
     def __call__(self, xrand):
-        all_ps, wps, x1, x2, flag = self.generate(xrand)
-        dense, npass = self._last
-        if self.p.ncuts == 0:
-            ok = flag == 1
-            if bool(ok.all()):
-                return all_ps, wps, x1, x2, 1
+        all_ps, wps, x1, x2, flag, dense, npass = self._run(xrand)
         ok = flag == 1
+        if self.p.ncuts == 0 and bool(ok.all()):
+            return all_ps, wps, x1, x2, 1
         idx = torch.nonzero(ok).to(torch.int32)
         return dense[:npass], wps[ok], x1[ok], x2[ok], idx
 

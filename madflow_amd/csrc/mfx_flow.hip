@@ -365,13 +365,17 @@ k_flow(FlowArgs A, const float* __restrict__ P,
     for (int i = 0; i < DMAX; ++i) w1[i] = i < g.nA ? P[g.oW1 + i * H + j1] : 0.f;
     const float b1 = P[g.ob1 + j1];
     const float b2 = P[g.ob2 + 16 * wid + r];
+    // the coordinate maps this thread reads with run-time indices below
+    const int ai = g.A[tid % DMAX];
+    const int bi = g.B[tid % nB];
     bf16x8 w2f[H / 32];
 #pragma unroll
     for (int c = 0; c < H / 32; ++c) w2f[c] = ld_bf16x8(Pb + g.oW2t + (16 * wid + r) * H + 32 * c + k8);
     __syncthreads();                       // xs holds this layer's input
+    static_assert(TILE * DMAX <= NT, "one input coordinate per thread");
     for (int q = tid; q < TILE * DMAX; q += NT) {
       const int s = q / DMAX, i = q % DMAX;
-      xa[q] = i < g.nA ? xs[s * DMAX + g.A[i]] : 0.f;
+      xa[q] = i < g.nA ? xs[s * DMAX + ai] : 0.f;
       const int64_t e = s0 + s;
       if (base && e < A.n && i < D) reinterpret_cast<float*>(base)[e * D + i] = xs[q];
     }
@@ -437,9 +441,9 @@ k_flow(FlowArgs A, const float* __restrict__ P,
     __syncthreads();
     // spline on every (sample, transformed dimension); the log-dets of one sample are
     // summed by one thread afterwards in a fixed order
-    for (int q = tid; q < TILE * nB; q += NT) {
+    for (int q = tid; q < TILE * nB; q += NT) {             // one item per thread (TILE nB <= NT)
       const int s = q / nB, b = q % nB;
-      const int d = g.B[b];
+      const int d = bi;
       const float* raw = raws + s * NPM + b * NPD;
       const SplineEval sp = A.inverse ? rq_inverse(raw, xs[s * DMAX + d]) : rq_forward(raw, xs[s * DMAX + d]);
       xs[s * DMAX + d] = sp.z;
@@ -506,13 +510,16 @@ struct BwdArgs {
   float* partial;          // (nsplit, nparams): weight-gradient partials
 };
 
-// The data-gradient kernel runs 4 waves per tile (each wave two column tiles of gh2 / gh1):
-// its register use allows 12 waves per CU, so three tiles are in flight on every CU.
+// The data-gradient kernel runs 4 waves per tile (each wave two column tiles of gh2 / gh1).
+// For flows of up to 4 transformed dimensions per layer its register budget is that of 2
+// waves per SIMD, two tiles per CU (the training batch of 8,192 samples is 512 tiles, two
+// per CU), which leaves room for the GEMMs' weight operands to be requested ahead of their
+// use; above that, 3 waves per SIMD.
 constexpr int WPB_B = 4;
 constexpr int NT_B = 64 * WPB_B;
 
 template <int NBM>
-__global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) k_flow_bwd(BwdArgs A, const float* __restrict__ P,
+__global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(NBM <= 4 ? 2 : 3))) k_flow_bwd(BwdArgs A, const float* __restrict__ P,
                                                    const unsigned short* __restrict__ Pb) {
   constexpr int NPM = ((NBM * NPD + 15) / 16) * 16;
   constexpr int NPP = NPM + 8;                 // padded bf16 row of go (16-byte rows, distinct banks)
@@ -529,10 +536,16 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
   __shared__ __attribute__((aligned(16))) unsigned short gob[TILE * NPP];
   // go: the tile's spline parameters, overwritten in place by their gradient
   __shared__ __attribute__((aligned(16))) float go[TILE * NPM], gh1[TILE * H], W1s[DMAX * H];
+  // the layer's coordinate maps (read per lane with run-time indices, so staged in LDS with
+  // the layer's state rather than loaded from the argument block where they are used) and
+  // the tile's d objective / d log q
+  __shared__ int Al[DMAX], Bl[DMAX];
+  __shared__ float gls[TILE];
   {
     const int s = tid / DMAX, d = tid % DMAX;
     const int64_t e = s0 + s;
     gxs[tid] = (A.gx_in && e < A.n && d < D) ? A.gx_in[e * D + d] : 0.f;
+    if (tid < TILE) gls[tid] = s0 + tid < A.n ? A.gl[s0 + tid] : 0.f;
   }
   for (int l = L - 1; l >= 0; --l) {
     const LayerGeom& g = A.geo.layer[l];
@@ -547,17 +560,31 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
     unsigned short* d_h2 = reinterpret_cast<unsigned short*>(dbase + A.dl.h2);
     unsigned short* d_h1 = reinterpret_cast<unsigned short*>(dbase + A.dl.h1);
     __syncthreads();                             // the previous layer is done with the LDS
-    // the loads of this layer's state, unrolled to their bounds so that they are all in
-    // flight together
+    // the loads of this layer's state, unrolled to their bounds and made unconditional
+    // (out-of-range lanes read an in-range address and drop the value) so that they are
+    // all in flight together
     {
       const int s = tid / DMAX, d = tid % DMAX;
       const int64_t e = s0 + s;
       xs[tid] = (e < A.n && d < D) ? sx[e * D + d] : 0.5f;
+      if (tid < DMAX) {
+        Al[tid] = g.A[tid];
+        Bl[tid] = g.B[tid];
+      }
     }
+    {
+      const int nq = g.nA * H / 4;             // float4s of W1 (g.oW1 is a multiple of 4; nA >= 1)
+      float4 v[DMAX * H / 4 / NT_B];
 #pragma unroll
-    for (int u = 0; u < DMAX * H / 4 / NT_B; ++u) {
-      const int q = tid + u * NT_B;            // four floats of W1 (g.oW1 is a multiple of 4)
-      if (q < g.nA * H / 4) *reinterpret_cast<float4*>(W1s + 4 * q) = *reinterpret_cast<const float4*>(P + g.oW1 + 4 * q);
+      for (int u = 0; u < DMAX * H / 4 / NT_B; ++u) {
+        const int q = tid + u * NT_B;
+        v[u] = *reinterpret_cast<const float4*>(P + g.oW1 + 4 * (q < nq ? q : nq - 1));
+      }
+#pragma unroll
+      for (int u = 0; u < DMAX * H / 4 / NT_B; ++u) {
+        const int q = tid + u * NT_B;
+        if (q < nq) *reinterpret_cast<float4*>(W1s + 4 * q) = v[u];
+      }
     }
 #pragma unroll
     for (int u = 0; u < TILE * H / 2 / NT_B; ++u) {
@@ -573,18 +600,28 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
     // spline parameters of the transformed dimensions, four at a time; the padding columns
     // (and samples past n) are zero and stay zero in the gradient
     const int nraw = nB * NPD, np4 = NP / 4;
+    {
+      constexpr int NR = (TILE * NPM / 4 + NT_B - 1) / NT_B;
+      float4 v[NR];
 #pragma unroll
-    for (int u = 0; u < (TILE * NPM / 4 + NT_B - 1) / NT_B; ++u) {
-      const int q = tid + u * NT_B;
-      if (q < TILE * np4) {
+      for (int u = 0; u < NR; ++u) {
+        const int q0 = tid + u * NT_B, q = q0 < TILE * np4 ? q0 : TILE * np4 - 1;
         const int s = q / np4, c = 4 * (q % np4);
-        const int64_t e = s0 + s;
-        float4 v = e < A.n ? *reinterpret_cast<const float4*>(sraw + e * A.geo.NPmax + c) : float4{0.f, 0.f, 0.f, 0.f};
-        if (c + 0 >= nraw) v.x = 0.f;
-        if (c + 1 >= nraw) v.y = 0.f;
-        if (c + 2 >= nraw) v.z = 0.f;
-        if (c + 3 >= nraw) v.w = 0.f;
-        *reinterpret_cast<float4*>(go + s * NPM + c) = v;
+        const int64_t e = s0 + s < A.n ? s0 + s : A.n - 1;
+        v[u] = *reinterpret_cast<const float4*>(sraw + e * A.geo.NPmax + c);
+      }
+#pragma unroll
+      for (int u = 0; u < NR; ++u) {
+        const int q = tid + u * NT_B;
+        if (q < TILE * np4) {
+          const int s = q / np4, c = 4 * (q % np4);
+          float4 w = s0 + s < A.n ? v[u] : float4{0.f, 0.f, 0.f, 0.f};
+          if (c + 0 >= nraw) w.x = 0.f;
+          if (c + 1 >= nraw) w.y = 0.f;
+          if (c + 2 >= nraw) w.z = 0.f;
+          if (c + 3 >= nraw) w.w = 0.f;
+          *reinterpret_cast<float4*>(go + s * NPM + c) = w;
+        }
       }
     }
     __syncthreads();
@@ -594,9 +631,9 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
       const int s = q / nB, b = q % nB;
       const int64_t e = s0 + s;
       if (e >= A.n) continue;
-      const int d = g.B[b];
+      const int d = Bl[b];
       float* gr = go + s * NPM + b * NPD;
-      gxs[s * DMAX + d] = rq_backward(gr, xs[s * DMAX + d], gxs[s * DMAX + d], A.gl[e], gr);
+      gxs[s * DMAX + d] = rq_backward(gr, xs[s * DMAX + d], gxs[s * DMAX + d], gls[s], gr);
     }
     __syncthreads();
     // go in bf16: the A operand of gh2 (LDS) and the delta of dW3 / b3 (tile-transposed)
@@ -607,17 +644,33 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
       d_o[s0 * NP + q] = v;
     }
     __syncthreads();
+    bf16x8 w2f[2][H / 32];                       // gh1's B operands, requested in the gh2 phase
     // gh2 = (go W3^T) * relu'(h2): B[k = np][n = h] = W3[h][np], the row h of W3n; wave w
     // computes the column tiles cw and cw + 4 (one A operand, two B operands per k step)
     {
       f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
       const unsigned short* w3[2] = {Pb + g.oW3n + (16 * cw + r) * NP, Pb + g.oW3n + (16 * (cw + WPB_B) + r) * NP};
+      // the B operands of every k step are requested before the products (a step past NP
+      // reads an in-range address and is skipped), and the next GEMM's W2 operands with them
+      constexpr bool PF3 = NPM / 32 <= 6;          // W3 prefetched up to the 8-dim flow's width
+      bf16x8 w3f[2][PF3 ? NPM / 32 : 1];
+      if constexpr (PF3) {
+#pragma unroll
+        for (int K0 = 0; K0 + 32 <= NPM; K0 += 32)
+#pragma unroll
+          for (int u = 0; u < 2; ++u) w3f[u][K0 / 32] = ld_bf16x8(w3[u] + (K0 + 32 <= NP ? K0 : 0) + k8);
+      }
+#pragma unroll
+      for (int K0 = 0; K0 < H; K0 += 32)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) w2f[u][K0 / 32] = ld_bf16x8(Pb + g.oW2n + (16 * (cw + WPB_B * u) + r) * H + K0 + k8);
 #pragma unroll
       for (int K0 = 0; K0 + 32 <= NPM; K0 += 32)
         if (K0 + 32 <= NP) {
           const bf16x8 a = ld_bf16x8(gob + r * NPP + K0 + k8);
 #pragma unroll
-          for (int u = 0; u < 2; ++u) acc[u] = mfma_k32(a, ld_bf16x8(w3[u] + K0 + k8), acc[u]);
+          for (int u = 0; u < 2; ++u)
+            acc[u] = mfma_k32(a, PF3 ? w3f[u][PF3 ? K0 / 32 : 0] : ld_bf16x8(w3[u] + K0 + k8), acc[u]);
         }
       if (NP % 32) {                               // NP = 32 m + 16: the last 16 k
         const int k0 = NP - 16;
@@ -649,8 +702,7 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
       for (int K0 = 0; K0 < H; K0 += 32) {
         const bf16x8 a = ld_bf16x8(ghb + r * WP + K0 + k8);
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-          acc[u] = mfma_k32(a, ld_bf16x8(Pb + g.oW2n + (16 * (cw + WPB_B * u) + r) * H + K0 + k8), acc[u]);
+        for (int u = 0; u < 2; ++u) acc[u] = mfma_k32(a, w2f[u][K0 / 32], acc[u]);
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
@@ -670,6 +722,9 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
     __syncthreads();
     // d x_A += gh1 W1^T (fp32): 8 lanes per (sample, input), 16 terms each, summed by
     // shuffles within aligned groups of 8 lanes (the summation order is fixed)
+    static_assert((NT_B / 8) % DMAX == 0, "a thread's input i is the same in every pass");
+    const int iA = (tid >> 3) % DMAX;
+    const int dA = Al[iA];
     for (int q0 = 0; q0 < TILE * DMAX * 8; q0 += NT_B) {
       const int q = q0 + tid, o = q >> 3, part = q & 7;
       const int s = o / DMAX, i = o % DMAX;
@@ -681,7 +736,7 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(3))) 
       v += __shfl_xor(v, 1, 64);
       v += __shfl_xor(v, 2, 64);
       v += __shfl_xor(v, 4, 64);
-      if (part == 0 && i < g.nA) gxs[s * DMAX + g.A[i]] += v;
+      if (part == 0 && i < g.nA) gxs[s * DMAX + dA] += v;
     }
   }
   __syncthreads();

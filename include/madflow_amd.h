@@ -306,11 +306,13 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
  * sum of the npool training weights, f64), u_e the first 53-bit uniform of the Philox
  * stream (counter (e, 0, 0, draw), key (seed, 0x6d66)).  logdet (batch,) and save as in
  * mfx_flow_apply; y_out (batch, D) and f_out (batch,) optional (f_out needs pool_f).
+ * guide (optional, nguide entries, mfx_flow_cdf_guide of the same cdf) finds the same k in
+ * two or three loads instead of a bisection; NULL bisects.
  * No reference counterpart (the learned proposal is new). */
 int mfx_flow_draw_forward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16,
-                          const double* cdf, int64_t npool, const float* pool_y, const double* pool_f, uint32_t seed,
-                          uint32_t draw, int64_t batch, float* y_out, double* f_out, float* logdet, float* save,
-                          void* stream);
+                          const double* cdf, const int* guide, int64_t nguide, int64_t npool, const float* pool_y,
+                          const double* pool_f, uint32_t seed, uint32_t draw, int64_t batch, float* y_out,
+                          double* f_out, float* logdet, float* save, void* stream);
 int64_t mfx_flow_save_bytes(int D, int L, const int* masks, int64_t n);
 int64_t mfx_flow_partial_bytes(int D, int L, const int* masks, int64_t n);
 /* Gradient of sum_i gl_i * logdet_i (+ gx . u) w.r.t. the parameters (grad, fp32) and
@@ -337,6 +339,10 @@ int mfx_flow_adam(float* params, const float* grad, float* m1, float* m2, int64_
  * double receives the total (0: every weight is 0, and cdf is 0).  No reference counterpart. */
 int64_t mfx_flow_cdf_workspace(int64_t n);
 int mfx_flow_cdf(const double* w, int64_t n, double* cdf, double* workspace, int64_t workspace_bytes, void* stream);
+/* Guide table of a cdf for mfx_flow_draw_forward: guide[b] (device int, m entries) = the
+ * first index k with cdf[k] >= b / m.  With it a draw takes two or three dependent loads
+ * instead of a bisection of log2(n) and returns the same index. */
+int mfx_flow_cdf_guide(const double* cdf, int64_t n, int* guide, int64_t m, void* stream);
 /* sum_i f_i^2 / ((1 - mix) exp(logq_i) + mix) in f64, fixed order: the second moment of the
  * importance weights of the defensive mixture at n points.  work: 257 doubles of device
  * memory; the sum lands in work[256]. */

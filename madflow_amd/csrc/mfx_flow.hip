@@ -290,6 +290,34 @@ __device__ __forceinline__ int64_t cdf_search(const double* __restrict__ cdf, in
   return lo;
 }
 
+// The same index from a guide table: guide[m] = the first k with cdf[k] >= m / M, so the
+// answer for u lies at or after guide[floor(u M)] (with that bucket's threshold at most
+// u); usually it is that entry, else a doubling search from it brackets it for a
+// bisection.  Two or three dependent loads instead of log2(n).
+__device__ __forceinline__ int64_t cdf_search_guided(const double* __restrict__ cdf, int64_t n,
+                                                     const int* __restrict__ guide, int64_t M, double u) {
+  int64_t m = (int64_t)(u * (double)M);
+  if (m >= M) m = M - 1;
+  while (m > 0 && (double)m / (double)M > u) --m;
+  int64_t lo = guide[m];
+  if (lo >= n - 1) return n - 1;
+  if (cdf[lo] >= u) return lo;
+  // cdf[lo] < u: the answer lies in (lo, n - 1]
+  int64_t step = 1, hi = lo + 1;
+  while (hi < n - 1 && cdf[hi] < u) {
+    lo = hi;
+    step *= 2;
+    hi = lo + step < n - 1 ? lo + step : n - 1;
+  }
+  lo = lo + 1;                                 // answer in [lo, hi]
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] >= u) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
 struct FlowArgs {
   FlowGeom geo;
   int64_t n;
@@ -298,6 +326,8 @@ struct FlowArgs {
   // is the first 53-bit Philox uniform of counter (e, 0, 0, draw) under key (seed, 0x6d66)
   const double* cdf;
   int64_t npool;
+  const int* guide;        // optional guide table of cdf (nguide buckets, mfx_flow_cdf_guide)
+  int64_t nguide;
   const float* pool_y;     // (npool, D)
   const double* pool_f;    // optional (npool): integrand values
   uint32_t seed, draw;
@@ -332,7 +362,7 @@ k_flow(FlowArgs A, const float* __restrict__ P,
     if (A.cdf && e < A.n) {
       double u0, u1;
       philox_pair((uint64_t)e, 0u, A.seed, A.draw, u0, u1);
-      k = cdf_search(A.cdf, A.npool, u0);
+      k = A.guide ? cdf_search_guided(A.cdf, A.npool, A.guide, A.nguide, u0) : cdf_search(A.cdf, A.npool, u0);
       if (A.f_out) A.f_out[e] = A.pool_f[k];
     }
     pick[tid] = k;
@@ -1130,6 +1160,14 @@ __global__ void k_cdf_finish(double* __restrict__ cdf, int64_t n, const double* 
   cdf[i] = tot > 0.0 ? (i == n - 1 ? 1.0 : v / tot) : 0.0;
 }
 
+// guide[m] = the first k with cdf[k] >= m / M (n - 1 when there is none), one bisection per
+// bucket: the table of cdf_search_guided.
+__global__ void k_cdf_guide(const double* __restrict__ cdf, int64_t n, int* __restrict__ guide, int64_t M) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  guide[m] = (int)cdf_search(cdf, n, (double)m / (double)M);
+}
+
 // Second moment of defensive-mixture importance weights: sum_i f_i^2 / ((1 - mix) e^{logq_i} + mix),
 // in fp64: MOM_G blocks each sum a contiguous range (thread partials in order, then the
 // block's threads in order), one thread sums the block partials in order.
@@ -1259,6 +1297,8 @@ static FlowArgs flow_args(const FlowGeom& G, int64_t n) {
   A.x_in = nullptr;
   A.cdf = nullptr;
   A.npool = 0;
+  A.guide = nullptr;
+  A.nguide = 0;
   A.pool_y = nullptr;
   A.pool_f = nullptr;
   A.seed = A.draw = 0;
@@ -1317,19 +1357,22 @@ int mfx_flow_apply(int D, int L, const int* masks, const float* params, const ui
 }
 
 int mfx_flow_draw_forward(int D, int L, const int* masks, const float* params, const uint16_t* params_bf16,
-                          const double* cdf, int64_t npool, const float* pool_y, const double* pool_f, uint32_t seed,
-                          uint32_t draw, int64_t batch, float* y_out, double* f_out, float* logdet, float* save,
-                          void* stream) {
+                          const double* cdf, const int* guide, int64_t nguide, int64_t npool, const float* pool_y,
+                          const double* pool_f, uint32_t seed, uint32_t draw, int64_t batch, float* y_out,
+                          double* f_out, float* logdet, float* save, void* stream) {
   FlowGeom G;
   int rc = make_geom(D, L, masks, G);
   if (rc) return rc;
   if (npool <= 0 || batch < 0) return fail(-2, "npool > 0 and batch >= 0 required%s", "");
+  if (guide && nguide <= 0) return fail(-2, "nguide > 0 required with a guide table%s", "");
   if (batch == 0) return 0;
   if (!params || !params_bf16 || !cdf || !pool_y || !logdet || (f_out && !pool_f))
     return fail(-3, "null pointer (pool_f is needed for f_out)%s", "");
   FlowArgs A = flow_args(G, batch);
   A.cdf = cdf;
   A.npool = npool;
+  A.guide = guide;
+  A.nguide = guide ? nguide : 0;
   A.pool_y = pool_y;
   A.pool_f = pool_f;
   A.seed = seed;
@@ -1449,6 +1492,14 @@ int mfx_flow_cdf(const double* w, int64_t n, double* cdf, double* workspace, int
   hipLaunchKernelGGL(k_cdf_totals, dim3(1), dim3(64), 0, s, workspace, (int)nb);
   hipLaunchKernelGGL(k_cdf_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cdf, n, (const double*)workspace,
                      (int)nb);
+  return hip_status(hipGetLastError());
+}
+
+int mfx_flow_cdf_guide(const double* cdf, int64_t n, int* guide, int64_t m, void* stream) {
+  if (n <= 0 || m <= 0) return fail(-2, "n > 0 and m > 0 required%s", "");
+  if (n > 0x7fffffff) return fail(-2, "n must fit in 31 bits%s", "");
+  if (!cdf || !guide) return fail(-3, "null pointer%s", "");
+  hipLaunchKernelGGL(k_cdf_guide, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream, cdf, n, guide, m);
   return hip_status(hipGetLastError());
 }
 

@@ -63,7 +63,8 @@ def _sig(lib):
     lib.mfx_flow_seed.argtypes = [vp, vp, vp, i64, f, vp, vp]
     lib.mfx_flow_seed.restype = i
     u32 = ctypes.c_uint32
-    lib.mfx_flow_draw_forward.argtypes = [i, i, vp, vp, vp, vp, i64, vp, vp, u32, u32, i64, vp, vp, vp, vp, vp]
+    lib.mfx_flow_draw_forward.argtypes = [i, i, vp, vp, vp, vp, vp, i64, i64, vp, vp, u32, u32, i64, vp, vp, vp, vp,
+                                          vp]
     lib.mfx_flow_draw_forward.restype = i
     lib.mfx_flow_adam_cast.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, f, f, f, i, f, vp]
     lib.mfx_flow_adam_cast.restype = i
@@ -72,6 +73,8 @@ def _sig(lib):
     lib.mfx_flow_cdf_workspace.restype = i64
     lib.mfx_flow_cdf.argtypes = [vp, i64, vp, vp, i64, vp]
     lib.mfx_flow_cdf.restype = i
+    lib.mfx_flow_cdf_guide.argtypes = [vp, i64, vp, i64, vp]
+    lib.mfx_flow_cdf_guide.restype = i
     lib.mfx_flow_moment.argtypes = [vp, vp, i64, d, vp, vp]
     lib.mfx_flow_moment.restype = i
     lib.mfx_flow_mixture_winv.argtypes = [vp, i64, i, d, vp, vp]
@@ -252,6 +255,9 @@ class RQSFlow:
             _lib.check(self.lib.mfx_flow_cdf(_p(w64), n, _p(cdf), _p(ws), ws.numel() * 8, _lib.stream_ptr(w64.device)))
             if not float(ws[-1]) > 0:
                 raise ValueError("train: the weights sum to zero")
+            # guide table of the cdf: a draw then takes two or three loads, not a bisection
+            guide = torch.empty(n, dtype=torch.int32, device=w64.device)
+            _lib.check(self.lib.mfx_flow_cdf_guide(_p(cdf), n, _p(guide), n, _lib.stream_ptr(w64.device)))
             y = _lib.require_device_tensor(y, torch.float32, "y")
             fc = _lib.require_device_tensor(f, None, "f").to(torch.float64) if beta else None
             # device workspace of the fused step, allocated once: the minibatch is drawn
@@ -265,7 +271,8 @@ class RQSFlow:
             partial = torch.empty(self.lib.mfx_flow_partial_bytes(self.D, self.L, self._m(), batch) // 4 + 1,
                                   dtype=torch.float32, device=dev)
             gbuf = torch.empty_like(self.params)
-            P = dict(par=_p(self.params), pb=_p(self.pbf16), cdf=_p(cdf), y=_p(y), fc=_p(fc), fb=_p(fb), ld=_p(ld),
+            P = dict(par=_p(self.params), pb=_p(self.pbf16), cdf=_p(cdf), guide=_p(guide), y=_p(y), fc=_p(fc), fb=_p(fb),
+                     ld=_p(ld),
                      gl=_p(gl), save=_p(save), partial=_p(partial), grad=_p(gbuf), st=_lib.stream_ptr(dev))
             seed32 = int(seed) & 0xFFFFFFFF
         for it in range(steps + 1):
@@ -284,9 +291,9 @@ class RQSFlow:
             need_loss = show
             if resample:
                 m = self._m()
-                _lib.check(self.lib.mfx_flow_draw_forward(self.D, self.L, m, P["par"], P["pb"], P["cdf"], n, P["y"],
-                                                          P["fc"], seed32, it, batch, None, P["fb"], P["ld"],
-                                                          P["save"], P["st"]))
+                _lib.check(self.lib.mfx_flow_draw_forward(self.D, self.L, m, P["par"], P["pb"], P["cdf"], P["guide"], n,
+                                                          n, P["y"], P["fc"], seed32, it, batch, None, P["fb"],
+                                                          P["ld"], P["save"], P["st"]))
                 _lib.check(self.lib.mfx_flow_seed(None, P["fb"], P["ld"], batch, float(beta), P["gl"], P["st"]))
                 _lib.check(self.lib.mfx_flow_backward(self.D, self.L, m, P["par"], P["pb"], batch, P["save"], P["gl"],
                                                       None, P["partial"], P["grad"], P["st"]))

@@ -181,7 +181,9 @@ def test_draw_forward_and_fused_step(dev, beta):
     weight reaches the Philox uniform of counter (e, 0, 0, draw) (the oracle's stream),
     returns the point's integrand value, and its log q and saved state equal those of a
     forward pass over the drawn points: the gradient from its save equals flow.grad of the
-    drawn batch bitwise.  mfx_flow_adam_cast equals mfx_flow_adam then mfx_flow_cast bitwise."""
+    drawn batch bitwise.  The draw through a guide table (mfx_flow_cdf_guide) picks the same
+    indices as the bisection.  mfx_flow_adam_cast equals mfx_flow_adam then mfx_flow_cast
+    bitwise."""
     import ctypes
     from madflow_amd import _lib
     from oracle import oracle as O
@@ -203,14 +205,24 @@ def test_draw_forward_and_fused_step(dev, beta):
     ld = torch.empty(batch, dtype=torch.float32, device=dev)
     save = torch.empty(lib.mfx_flow_save_bytes(D, 8, flow._m(), batch) // 4 + 1, dtype=torch.float32, device=dev)
     vp = lambda x: ctypes.c_void_p(x.data_ptr())
-    _lib.check(lib.mfx_flow_draw_forward(D, 8, flow._m(), vp(flow.params), vp(flow.pbf16), vp(cdf_t), npool, vp(y_t),
-                                         vp(f_t), seed, draw, batch, vp(yo), vp(fo), vp(ld), vp(save),
-                                         _lib.stream_ptr(dev)))
     u = O.philox_uniforms(batch, 2, seed, draw)[:, 0]
     idx = np.minimum(np.searchsorted(cdf, u, side="left"), npool - 1)
     assert np.all(w[idx] > 0)
-    np.testing.assert_array_equal(yo.cpu().numpy(), y[idx])
-    np.testing.assert_array_equal(fo.cpu().numpy(), f[idx])
+    # the bisection, then guide tables of several sizes (a table as long as the pool, and
+    # coarse ones whose buckets hold many entries), all drawing the same indices
+    for m_guide in (0, npool, 64, 1):
+        guide = None
+        if m_guide:
+            guide = torch.empty(m_guide, dtype=torch.int32, device=dev)
+            _lib.check(lib.mfx_flow_cdf_guide(vp(cdf_t), npool, vp(guide), m_guide, _lib.stream_ptr(dev)))
+            b = np.arange(m_guide) / m_guide
+            np.testing.assert_array_equal(guide.cpu().numpy(), np.minimum(np.searchsorted(cdf, b, side="left"), npool - 1))
+        _lib.check(lib.mfx_flow_draw_forward(D, 8, flow._m(), vp(flow.params), vp(flow.pbf16), vp(cdf_t),
+                                             vp(guide) if guide is not None else None, m_guide, npool, vp(y_t),
+                                             vp(f_t), seed, draw, batch, vp(yo), vp(fo), vp(ld), vp(save),
+                                             _lib.stream_ptr(dev)))
+        np.testing.assert_array_equal(yo.cpu().numpy(), y[idx])
+        np.testing.assert_array_equal(fo.cpu().numpy(), f[idx])
     assert torch.equal(ld, flow.log_prob(yo))
     # the gradient from the draw's saved state equals that of a separate forward pass
     gl = torch.empty(batch, dtype=torch.float32, device=dev)

@@ -1006,9 +1006,25 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
   const int t = threadIdx.x;
   // f_i <= 0 (or not a number) gives v_i = 0
   auto lv = [&](int i) { return f[i] > 0.0 ? (double)beta * (log(f[i]) - (double)ld[i]) : -HUGE_VAL; };
+  // up to SEED_C items per thread (a training batch of up to 8,192 samples) keep their
+  // log-ratio and weight in registers, so each log and exp is evaluated once; larger n
+  // evaluates them again in the later passes (the same values)
+  constexpr int SEED_C = 8;
+  const bool cached = n <= SEED_C * SEED_T;
+  double lc[SEED_C];
   double mx = -HUGE_VAL;
-  if (beta != 0.f)
-    for (int i = t; i < n; i += SEED_T) mx = fmax(mx, lv(i));
+  if (beta != 0.f) {
+    if (cached) {
+#pragma unroll
+      for (int c = 0; c < SEED_C; ++c) {
+        const int i = t + c * SEED_T;
+        lc[c] = i < n ? lv(i) : -HUGE_VAL;
+        mx = fmax(mx, lc[c]);
+      }
+    } else {
+      for (int i = t; i < n; i += SEED_T) mx = fmax(mx, lv(i));
+    }
+  }
   red[t] = mx;
   __syncthreads();
   for (int s = SEED_T / 2; s > 0; s >>= 1) {
@@ -1019,7 +1035,16 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
   if (!(mx > -HUGE_VAL)) mx = 0.0;     // every f_i = 0: all v_i = 0, the seed is zero
   __syncthreads();
   double sum = 0.0;
-  for (int i = t; i < n; i += SEED_T) sum += (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
+  if (cached) {
+#pragma unroll
+    for (int c = 0; c < SEED_C; ++c) {
+      const int i = t + c * SEED_T;
+      lc[c] = i < n ? (w ? w[i] : 1.0) * (beta != 0.f ? exp(lc[c] - mx) : 1.0) : 0.0;   // now v_i
+      sum += lc[c];
+    }
+  } else {
+    for (int i = t; i < n; i += SEED_T) sum += (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
+  }
   red[t] = sum;
   __syncthreads();
   for (int s = SEED_T / 2; s > 0; s >>= 1) {
@@ -1027,9 +1052,17 @@ __global__ void __launch_bounds__(SEED_T) k_seed(const double* __restrict__ w, c
     __syncthreads();
   }
   const double tot = red[0];
-  for (int i = t; i < n; i += SEED_T) {
-    const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
-    gl[i] = (float)(tot > 0.0 ? -v / tot : 0.0);
+  if (cached) {
+#pragma unroll
+    for (int c = 0; c < SEED_C; ++c) {
+      const int i = t + c * SEED_T;
+      if (i < n) gl[i] = (float)(tot > 0.0 ? -lc[c] / tot : 0.0);
+    }
+  } else {
+    for (int i = t; i < n; i += SEED_T) {
+      const double v = (w ? w[i] : 1.0) * (beta != 0.f ? exp(lv(i) - mx) : 1.0);
+      gl[i] = (float)(tot > 0.0 ? -v / tot : 0.0);
+    }
   }
 }
 

@@ -230,6 +230,68 @@ __device__ __forceinline__ float rq_backward(const float* raw, float x, float gz
   return inside ? Gxi / wk : 0.f;
 }
 
+// Softmax over the 16 bins held 4 per lane by the 4 consecutive lanes of an item, with a
+// minimum value minv per bin; and the running sum below each of the lane's bins (the sums
+// of the lanes below, then this lane's in order): the pieces of rq_backward4.
+__device__ __forceinline__ void softmax4(const float* r, float minv, float* out) {
+  float m = fmaxf(fmaxf(r[0], r[1]), fmaxf(r[2], r[3]));
+  m = fmaxf(m, __shfl_xor(m, 1, 4));
+  m = fmaxf(m, __shfl_xor(m, 2, 4));
+  float e[4], sl = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    e[i] = __expf(r[i] - m);
+    sl += e[i];
+  }
+  sl += __shfl_xor(sl, 1, 4);
+  sl += __shfl_xor(sl, 2, 4);
+  const float sc = (1.f - K * minv) / sl;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = minv + sc * e[i];
+}
+__device__ __forceinline__ void below4(const float* v, int sub, float* c) {
+  const float t = (v[0] + v[1]) + (v[2] + v[3]);
+  float inc = t, u = __shfl_up(inc, 1, 4);
+  if (sub >= 1) inc += u;
+  u = __shfl_up(inc, 2, 4);
+  if (sub >= 2) inc += u;
+  c[0] = inc - t;
+#pragma unroll
+  for (int i = 1; i < 4; ++i) c[i] = c[i - 1] + v[i - 1];
+}
+// bin k of x (the edges c[j], j = 1 .. K - 1, at or below it), and its lower edge and width
+// in the widths' and heights' sums, from the lane that holds it
+struct Bin4 {
+  int k;
+  float lo_w, w, lo_h, h;
+};
+__device__ __forceinline__ Bin4 bin4(const float* cw, const float* w, const float* ch, const float* h, float x,
+                                     int sub) {
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (4 * sub + i >= 1 && x >= cw[i]) ++cnt;
+  cnt += __shfl_xor(cnt, 1, 4);
+  cnt += __shfl_xor(cnt, 2, 4);
+  const int ik = cnt & 3;
+  float sw = cw[0], swk = w[0], sh = ch[0], shk = h[0];
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+    if (ik == i) {
+      sw = cw[i];
+      swk = w[i];
+      sh = ch[i];
+      shk = h[i];
+    }
+  Bin4 b;
+  b.k = cnt;
+  b.lo_w = __shfl(sw, cnt >> 2, 4);
+  b.w = __shfl(swk, cnt >> 2, 4);
+  b.lo_h = __shfl(sh, cnt >> 2, 4);
+  b.h = __shfl(shk, cnt >> 2, 4);
+  return b;
+}
+
 // rq_backward spread over the 4 consecutive lanes of an item (sub = lane & 3 holds bins
 // 4 sub .. 4 sub + 3): the softmax maxima and sums, the running sums of the bin widths and
 // heights and the bin count reduced across the 4 lanes, the bin's edges taken from the lane
@@ -237,75 +299,22 @@ __device__ __forceinline__ float rq_backward(const float* raw, float x, float gz
 // gradient of its own bins (lane 3 also the last derivative).  The same formulas as
 // rq_backward; the sums are formed in another order.  All 4 lanes of an item must call it.
 __device__ __forceinline__ float rq_backward4(float* gr, float x, float gz, float gl, int sub) {
-  float rw[4], rh[4];
+  float rw[4], rh[4], w[4], h[4], cw[4], chh[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     rw[i] = gr[4 * sub + i];
     rh[i] = gr[K + 4 * sub + i];
   }
-  // softmax with a minimum value per bin, over the 16 bins of the 4 lanes
-  auto softmax4 = [&](const float* r, float minv, float* out) {
-    float m = fmaxf(fmaxf(r[0], r[1]), fmaxf(r[2], r[3]));
-    m = fmaxf(m, __shfl_xor(m, 1, 4));
-    m = fmaxf(m, __shfl_xor(m, 2, 4));
-    float e[4], sl = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      e[i] = __expf(r[i] - m);
-      sl += e[i];
-    }
-    sl += __shfl_xor(sl, 1, 4);
-    sl += __shfl_xor(sl, 2, 4);
-    const float sc = (1.f - K * minv) / sl;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = minv + sc * e[i];
-  };
-  float w[4], h[4];
   softmax4(rw, MINW, w);
   softmax4(rh, MINH, h);
-  // running sums: c[4 sub + i] = sum of the bins below, from the lanes below plus this lane's
-  auto excl_scan = [&](const float* v) {
-    const float t = (v[0] + v[1]) + (v[2] + v[3]);
-    float inc = t, u = __shfl_up(inc, 1, 4);
-    if (sub >= 1) inc += u;
-    u = __shfl_up(inc, 2, 4);
-    if (sub >= 2) inc += u;
-    return inc - t;
-  };
-  const float cw0 = excl_scan(w), ch0 = excl_scan(h);
-  float cw[4], chh[4];
-  cw[0] = cw0;
-  chh[0] = ch0;
-#pragma unroll
-  for (int i = 1; i < 4; ++i) {
-    cw[i] = cw[i - 1] + w[i - 1];
-    chh[i] = chh[i - 1] + h[i - 1];
-  }
+  below4(w, sub, cw);
+  below4(h, sub, chh);
   const bool inside = x > 0.f && x < 1.f;
   x = fminf(fmaxf(x, 0.f), 1.f);
-  // bin k = the number of edges c[j], j = 1 .. K - 1, at or below x
-  int cnt = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = 4 * sub + i;
-    if (j >= 1 && x >= cw[i]) ++cnt;
-  }
-  cnt += __shfl_xor(cnt, 1, 4);
-  cnt += __shfl_xor(cnt, 2, 4);
-  const int k = cnt, ik = k & 3;
-  // the lane holding bin k supplies its lower edges and widths
-  float selw = cw[0], selwk = w[0], selh = chh[0], selhk = h[0];
-#pragma unroll
-  for (int i = 1; i < 4; ++i)
-    if (ik == i) {
-      selw = cw[i];
-      selwk = w[i];
-      selh = chh[i];
-      selhk = h[i];
-    }
-  const float lo_w = __shfl(selw, k >> 2, 4), wbin = __shfl(selwk, k >> 2, 4);
-  const float lo_h = __shfl(selh, k >> 2, 4), hbin = __shfl(selhk, k >> 2, 4);
-  const float hi_w = k == K - 1 ? 1.f : lo_w + wbin, hi_h = k == K - 1 ? 1.f : lo_h + hbin;
+  const Bin4 bn = bin4(cw, w, chh, h, x, sub);
+  const int k = bn.k;
+  const float lo_w = bn.lo_w, lo_h = bn.lo_h;
+  const float hi_w = k == K - 1 ? 1.f : lo_w + bn.w, hi_h = k == K - 1 ? 1.f : lo_h + bn.h;
   const float ra = gr[2 * K + k], rc = gr[2 * K + k + 1];
   const float a = MIND + softplus(ra), c = MIND + softplus(rc);
   const float wk = hi_w - lo_w, hk = hi_h - lo_h;

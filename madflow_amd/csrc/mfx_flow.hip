@@ -230,142 +230,6 @@ __device__ __forceinline__ float rq_backward(const float* raw, float x, float gz
   return inside ? Gxi / wk : 0.f;
 }
 
-// Softmax over the 16 bins held 4 per lane by the 4 consecutive lanes of an item, with a
-// minimum value minv per bin; and the running sum below each of the lane's bins (the sums
-// of the lanes below, then this lane's in order): the pieces of rq_backward4.
-__device__ __forceinline__ void softmax4(const float* r, float minv, float* out) {
-  float m = fmaxf(fmaxf(r[0], r[1]), fmaxf(r[2], r[3]));
-  m = fmaxf(m, __shfl_xor(m, 1, 4));
-  m = fmaxf(m, __shfl_xor(m, 2, 4));
-  float e[4], sl = 0.f;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    e[i] = __expf(r[i] - m);
-    sl += e[i];
-  }
-  sl += __shfl_xor(sl, 1, 4);
-  sl += __shfl_xor(sl, 2, 4);
-  const float sc = (1.f - K * minv) / sl;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) out[i] = minv + sc * e[i];
-}
-__device__ __forceinline__ void below4(const float* v, int sub, float* c) {
-  const float t = (v[0] + v[1]) + (v[2] + v[3]);
-  float inc = t, u = __shfl_up(inc, 1, 4);
-  if (sub >= 1) inc += u;
-  u = __shfl_up(inc, 2, 4);
-  if (sub >= 2) inc += u;
-  c[0] = inc - t;
-#pragma unroll
-  for (int i = 1; i < 4; ++i) c[i] = c[i - 1] + v[i - 1];
-}
-// bin k of x (the edges c[j], j = 1 .. K - 1, at or below it), and its lower edge and width
-// in the widths' and heights' sums, from the lane that holds it
-struct Bin4 {
-  int k;
-  float lo_w, w, lo_h, h;
-};
-__device__ __forceinline__ Bin4 bin4(const float* cw, const float* w, const float* ch, const float* h, float x,
-                                     int sub) {
-  int cnt = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (4 * sub + i >= 1 && x >= cw[i]) ++cnt;
-  cnt += __shfl_xor(cnt, 1, 4);
-  cnt += __shfl_xor(cnt, 2, 4);
-  const int ik = cnt & 3;
-  float sw = cw[0], swk = w[0], sh = ch[0], shk = h[0];
-#pragma unroll
-  for (int i = 1; i < 4; ++i)
-    if (ik == i) {
-      sw = cw[i];
-      swk = w[i];
-      sh = ch[i];
-      shk = h[i];
-    }
-  Bin4 b;
-  b.k = cnt;
-  b.lo_w = __shfl(sw, cnt >> 2, 4);
-  b.w = __shfl(swk, cnt >> 2, 4);
-  b.lo_h = __shfl(sh, cnt >> 2, 4);
-  b.h = __shfl(shk, cnt >> 2, 4);
-  return b;
-}
-
-// rq_backward spread over the 4 consecutive lanes of an item (sub = lane & 3 holds bins
-// 4 sub .. 4 sub + 3): the softmax maxima and sums, the running sums of the bin widths and
-// heights and the bin count reduced across the 4 lanes, the bin's edges taken from the lane
-// that holds bin k, the scalar part computed by every lane, and each lane writing the
-// gradient of its own bins (lane 3 also the last derivative).  The same formulas as
-// rq_backward; the sums are formed in another order.  All 4 lanes of an item must call it.
-__device__ __forceinline__ float rq_backward4(float* gr, float x, float gz, float gl, int sub) {
-  float rw[4], rh[4], w[4], h[4], cw[4], chh[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    rw[i] = gr[4 * sub + i];
-    rh[i] = gr[K + 4 * sub + i];
-  }
-  softmax4(rw, MINW, w);
-  softmax4(rh, MINH, h);
-  below4(w, sub, cw);
-  below4(h, sub, chh);
-  const bool inside = x > 0.f && x < 1.f;
-  x = fminf(fmaxf(x, 0.f), 1.f);
-  const Bin4 bn = bin4(cw, w, chh, h, x, sub);
-  const int k = bn.k;
-  const float lo_w = bn.lo_w, lo_h = bn.lo_h;
-  const float hi_w = k == K - 1 ? 1.f : lo_w + bn.w, hi_h = k == K - 1 ? 1.f : lo_h + bn.h;
-  const float ra = gr[2 * K + k], rc = gr[2 * K + k + 1];
-  const float a = MIND + softplus(ra), c = MIND + softplus(rc);
-  const float wk = hi_w - lo_w, hk = hi_h - lo_h;
-  const float xi = fminf(fmaxf((x - lo_w) / wk, 0.f), 1.f);
-  const float s = hk / wk;
-  const float t = xi * (1.f - xi);
-  const float g2 = a + c - 2.f * s;
-  const float N = hk * (s * xi * xi + a * t);
-  const float Dn = s + g2 * t;
-  const float P = c * xi * xi + 2.f * s * t + a * (1.f - xi) * (1.f - xi);
-  const float iD = 1.f / Dn, iD2 = iD * iD, iP = 1.f / P;
-  const float dt = 1.f - 2.f * xi;
-  const float Nxi = hk * (2.f * s * xi + a * dt), Ns = hk * xi * xi, Nh = s * xi * xi + a * t, Na = hk * t;
-  const float Dxi = g2 * dt, Ds = 1.f - 2.f * t, Da = t, Dc = t;
-  const float Pxi = 2.f * c * xi + 2.f * s * dt - 2.f * a * (1.f - xi), Ps = 2.f * t, Pa = (1.f - xi) * (1.f - xi),
-              Pc = xi * xi;
-  const float Gxi = gz * (Nxi * Dn - N * Dxi) * iD2 + gl * (Pxi * iP - 2.f * Dxi * iD);
-  const float Gs = gz * (Ns * Dn - N * Ds) * iD2 + gl * (2.f / s + Ps * iP - 2.f * Ds * iD);
-  const float Gh = gz * Nh * iD;
-  const float GY = gz;
-  const float Ga = gz * (Na * Dn - N * Da) * iD2 + gl * (Pa * iP - 2.f * Da * iD);
-  const float Gc = gz * (-N * Dc) * iD2 + gl * (Pc * iP - 2.f * Dc * iD);
-  const float GX = -Gxi / wk;
-  const float Gw = -Gxi * xi / wk - Gs * s / wk;
-  const float Ghk = Gh + Gs / wk;
-  float gw[4], gh[4], sw = 0.f, sh = 0.f;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = 4 * sub + i;
-    gw[i] = (j < k ? GX : 0.f) + (j == k ? Gw : 0.f);
-    gh[i] = (j < k ? GY : 0.f) + (j == k ? Ghk : 0.f);
-    sw += (w[i] - MINW) * gw[i];
-    sh += (h[i] - MINH) * gh[i];
-  }
-  sw += __shfl_xor(sw, 1, 4);
-  sw += __shfl_xor(sw, 2, 4);
-  sh += __shfl_xor(sh, 1, 4);
-  sh += __shfl_xor(sh, 2, 4);
-  const float nw = 1.f - K * MINW, nh = 1.f - K * MINH;
-  const float da = Ga * sigmoid(ra), dc = Gc * sigmoid(rc);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int j = 4 * sub + i;
-    gr[j] = (w[i] - MINW) * (gw[i] - sw / nw);
-    gr[K + j] = (h[i] - MINH) * (gh[i] - sh / nh);
-    gr[2 * K + j] = j == k ? da : (j == k + 1 ? dc : 0.f);
-  }
-  if (sub == 3) gr[3 * K] = k + 1 == K ? dc : 0.f;
-  return inside ? Gxi / wk : 0.f;
-}
-
 // ------------------------------------------------------------------ MFMA helpers
 // v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16];
 // the 16x16 result has D[4(l/16)+i][l%16] in element i of lane l.
@@ -793,29 +657,13 @@ __global__ void __launch_bounds__(NT_B) __attribute__((amdgpu_waves_per_eu(NBM <
     __syncthreads();
     // spline backward: d raw (in place in go) and d x_B (replaces the incoming gradient of
     // x_B); each item reads its parameters before it writes their gradient
-    if constexpr (NBM <= 4) {
-      // four lanes per item (TILE x nB <= 64 items on NT_B = 256 lanes)
-      static_assert(TILE * 4 * 4 <= NT_B, "four lanes for each of up to 64 items");
-      const int q = tid >> 2, sub = tid & 3;
-      if (q < TILE * nB) {
-        const int s = q / nB, b = q % nB;
-        const int64_t e = s0 + s;
-        if (e < A.n) {
-          const int d = Bl[b];
-          float* gr = go + s * NPM + b * NPD;
-          const float gx = rq_backward4(gr, xs[s * DMAX + d], gxs[s * DMAX + d], gls[s], sub);
-          if (sub == 0) gxs[s * DMAX + d] = gx;
-        }
-      }
-    } else {
-      for (int q = tid; q < TILE * nB; q += NT_B) {
-        const int s = q / nB, b = q % nB;
-        const int64_t e = s0 + s;
-        if (e >= A.n) continue;
-        const int d = Bl[b];
-        float* gr = go + s * NPM + b * NPD;
-        gxs[s * DMAX + d] = rq_backward(gr, xs[s * DMAX + d], gxs[s * DMAX + d], gls[s], gr);
-      }
+    for (int q = tid; q < TILE * nB; q += NT_B) {
+      const int s = q / nB, b = q % nB;
+      const int64_t e = s0 + s;
+      if (e >= A.n) continue;
+      const int d = Bl[b];
+      float* gr = go + s * NPM + b * NPD;
+      gxs[s * DMAX + d] = rq_backward(gr, xs[s * DMAX + d], gxs[s * DMAX + d], gls[s], gr);
     }
     __syncthreads();
     // go in bf16: the A operand of gh2 (LDS) and the delta of dW3 / b3 (tile-transposed)

@@ -405,3 +405,28 @@ def test_mc_hadronic_cross_section_against_ramboflow(dev, name, kin, nev, warm):
               "unweighting eff", float(f.mean() / f.max()), "alpha", getattr(ig, "alpha", None), flush=True)
     (a, ea), (b, eb) = res["ramboflow"], res["multichannel"]
     assert abs(a - b) < 3 * np.hypot(ea, eb)
+
+
+@pytest.mark.gpu
+def test_mc_register_cut(dev):
+    """register_cut's cuts act in the multi-channel generation too: for points of every
+    channel, the events that pass with a cut on the top's pT (100 < pT < 600) equal the
+    uncut passing events under the numpy mask of their momenta."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    print("case mc register_cut", flush=True)
+    ig = MultiChannelIntegrand("gg_ttxgg", SQRTS, precision=64, seed=7, **CUTS)
+    n = 4000
+    for c in range(ig.nch):
+        ev0 = ig.channel_offset(c) + 1
+        ig.clear_cuts()
+        ig.chunk_channel(c, ev0, n, 4, 1.0)
+        full = ig._bufs["all_ps"][:n].cpu().numpy().copy()
+        flag0 = ig._bufs["flag"][:n].cpu().numpy().copy()
+        ig.register_cut("pt", particle=2, min_val=100.0, max_val=600.0)
+        ig.chunk_channel(c, ev0, n, 4, 1.0)
+        flag = ig._bufs["flag"][:n].cpu().numpy()
+        pt = np.hypot(full[:, 2, 1], full[:, 2, 2])
+        mask = (flag0 == 1) & (pt > 100.0) & (pt < 600.0)
+        print("channel", c, "pass", (flag0 == 1).mean(), "with the top cut", mask.mean(), flush=True)
+        assert 0.05 < mask.mean() < (flag0 == 1).mean()
+        assert np.array_equal(flag == 1, mask)

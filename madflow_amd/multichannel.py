@@ -82,6 +82,8 @@ class MultiChannelIntegrand(MEIntegrand):
     channel-weight adaptation; ``alpha_min`` is the weight floor."""
 
     def __init__(self, process, sqrts, channels=None, adapt_alpha=True, alpha_min=0.02, **kw):
+        if not isinstance(process, str):
+            raise NotImplementedError("the multi-channel phase space takes one process")
         mapping = kw.get("mapping", "diet")
         if mapping not in ("diet", "hadronic"):
             raise ValueError("the multi-channel phase space runs on the diet cube: mapping 'diet' (fixed sqrt(s)) "

@@ -107,12 +107,12 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def initial_flavours(matrix):
+def initial_flavours(matrix, explicit=False):
     """(beam-1 flavours, beam-2 flavours) of a drop-in class's initial states, the mirror
     pairs appended when mirror_initial_states is set (madflow_exec.py:141-155); None for
-    processes with two incoming gluons only."""
+    processes with two incoming gluons only, unless ``explicit``."""
     states = [tuple(s) for s in matrix.initial_states]
-    if all(s == (21, 21) for s in states):
+    if all(s == (21, 21) for s in states) and not explicit:
         return None
     f1, f2 = [a for a, _ in states], [b for _, b in states]
     if matrix.mirror_initial_states:
@@ -140,30 +140,46 @@ class MEIntegrand:
     madflow_exec.py:389, so its LHE momenta are lab-frame), while the phase-space class
     alone defaults to the partonic frame (phasespace.py:377).  The cuts act in the partonic
     frame either way.  ``kernels`` selects the matrix element's kernel family (the
-    drop-in class's default when None)."""
+    drop-in class's default when None).
+
+    ``process`` may also be a list of processes with the same outgoing masses: the
+    integrand is then the sum over them of luminosity x |M|^2 at each phase-space point,
+    as the reference's integrand sums its matrices (madflow_exec.py:453-465); in hadronic
+    mode the pdf then takes (x, q2, pid) for every one of them, gluons as pid 21.  The
+    first process sets the leg order of the momenta (the cuts and event output)."""
 
     def __init__(self, process, sqrts, precision=64, mapping="rambo", pt_min=0.0, abs_eta_max=0.0, dr_min=0.0,
                  seed=1, nbins=BINS_MAX, alpha_s=0.118, device="cuda", events_limit=1 << 22, ref_compat=True,
                  width=None, pdf=None, scale=None, alphas=None, lab_frame=None, shat_min=None, kernels=None):
-        n_out, masses = PROCESS_SETUP[process]
+        procs = [process] if isinstance(process, str) else list(process)
+        if not procs:
+            raise ValueError("MEIntegrand needs a process")
+        n_out, masses = PROCESS_SETUP[procs[0]]
+        for q in procs[1:]:
+            if PROCESS_SETUP[q] != (n_out, masses):
+                raise ValueError("the processes of one integrand share its phase space: %s and %s have different "
+                                 "outgoing masses" % (procs[0], q))
         self.process = process
         self.device = torch.device(device)
-        self.matrix, mod = load_matrix(process, precision=precision, ref_compat=ref_compat, kernels=kernels)
-        model = mod.get_model_param()
-        self.model = model
         self.hadronic = mapping == "hadronic"
         if self.hadronic and pdf is None:
             raise ValueError("the hadronic mapping needs a pdf(x, q2) -> x f(x, Q^2)")
         self.pdf, self.scale, self.alphas = pdf, scale, alphas
-        self.flavours = initial_flavours(self.matrix)
-        if alphas is None:
-            model.freeze_alpha_s(alpha_s)
-            self.params = list(model.evaluate())
-        else:
-            self.params = list(model.evaluate(torch.tensor([float(alpha_s)], dtype=torch.float64)))
-        if width is not None:
-            self.params[1] = float(width)
         self.width = width
+        # one entry per process: (matrix, model, parameters, initial flavours)
+        self.subprocesses = []
+        for q in procs:
+            matrix, mod = load_matrix(q, precision=precision, ref_compat=ref_compat, kernels=kernels)
+            model = mod.get_model_param()
+            if alphas is None:
+                model.freeze_alpha_s(alpha_s)
+                params = list(model.evaluate())
+            else:
+                params = list(model.evaluate(torch.tensor([float(alpha_s)], dtype=torch.float64)))
+            if width is not None:
+                params[1] = float(width)
+            self.subprocesses.append((matrix, model, params, initial_flavours(matrix, explicit=len(procs) > 1)))
+        self.matrix, self.model, self.params, self.flavours = self.subprocesses[0]
         p = PsParams()
         p.n_out = n_out
         p.mapping = {"rambo": 0, "diet": 1, "hadronic": 2}[mapping]
@@ -242,6 +258,8 @@ class MEIntegrand:
             b["ws"].numel(), ctypes.byref(npass), s))
         npass = npass.value
         self.last_chunk_npass = npass
+        if len(self.subprocesses) > 1:
+            return self._chunk_sum(b, ev0, n, iteration, keep_f, y_in, npass, s)
         params, f_in = self.params, None
         if self.hadronic and npass:
             params, lumi = self._hadronic_factors(b, n, npass)
@@ -332,10 +350,35 @@ class MEIntegrand:
             out["f"] = torch.cat(fs)
         return out
 
-    def _hadronic_factors(self, b, n, npass):
+    def _chunk_sum(self, b, ev0, n, iteration, keep_f, y_in, npass, s):
+        """The rest of chunk() for several processes: f = sum over them of luminosity x
+        |M|^2 (1 at fixed energy) at each passing point, times its weight."""
+        me = torch.zeros(npass, dtype=torch.float64, device=self.device)
+        if npass:
+            if self.me_timer is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            for sub in self.subprocesses:
+                if self.hadronic:
+                    params, lumi = self._hadronic_factors(b, n, npass, sub)
+                    me += sub[0].smatrix(b["dense"][:npass], *params) * lumi
+                else:
+                    me += sub[0].smatrix(b["dense"][:npass], *sub[2])
+            if self.me_timer is not None:
+                e1.record()
+                self.me_timer.append((e0, e1, npass))
+        _lib.check(self.lib.mfx_vegas_accumulate(
+            ctypes.byref(self.p), ev0, n, iteration, _ptr(y_in), _ptr(b["slot"]), _ptr(me), _ptr(b["wps"]), None,
+            _ptr(b["f"]) if keep_f else None, _ptr(b["ws"]), b["ws"].numel(), _ptr(b["result"]), s))
+        return b["result"].cpu().numpy().copy(), npass, (b["f"][:n] if keep_f else None)
+
+    def _hadronic_factors(self, b, n, npass, sub=None):
         """Couplings and parton luminosity of the passing events of a chunk: the scale Q
         (fixed, or the dynamic sum of outgoing transverse masses / 2), alpha_s(Q^2) if a
-        running alpha_s is given, and x1 f(x1) x2 f(x2) / (x1 x2) at Q^2."""
+        running alpha_s is given, and x1 f(x1) x2 f(x2) / (x1 x2) at Q^2, for the process
+        ``sub`` (an entry of subprocesses; the first by default)."""
+        _, model, params0, flavours = sub if sub is not None else self.subprocesses[0]
         d = b["dense"][:npass]
         x = b["x12"][:n][b["flag"][:n] == 1]
         if self.scale is None:
@@ -344,18 +387,18 @@ class MEIntegrand:
             q2 = (0.5 * mt) ** 2
         else:
             q2 = torch.full((npass,), float(self.scale) ** 2, dtype=torch.float64, device=self.device)
-        params = self.params
+        params = params0
         if self.alphas is not None:
-            params = list(self.model.evaluate(self.alphas(q2).to(torch.float64).reshape(-1)))
+            params = list(model.evaluate(self.alphas(q2).to(torch.float64).reshape(-1)))
             if self.width is not None:
                 params[1] = float(self.width)
         x1, x2 = x[:, 0], x[:, 1]
-        if self.flavours is None:
+        if flavours is None:
             lumi = self.pdf(x1, q2) * self.pdf(x2, q2) / (x1 * x2)
         else:
             # sum over the initial-state flavour pairs of the process, mirror pairs included
             # (madflow_exec.py:141-155,440-454; pdfflow's xfxQ2(pid, x, q2) per beam)
-            f1, f2 = self.flavours
+            f1, f2 = flavours
             cache = {}
 
             def xf(pid, xv, side):

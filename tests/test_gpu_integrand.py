@@ -403,3 +403,38 @@ def test_register_cut_matches_numpy_mask(dev, variable, particle, lo, hi):
         ig.register_cut("eta", particle=3, min_val=0.0)
     with pytest.raises(ValueError):
         ig.register_cut("pt", particle=5, min_val=10.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hadronic", [False, True])
+def test_integrand_sums_subprocesses(dev, hadronic):
+    """Several processes in one integrand, as the reference's integrand sums its matrices
+    (madflow_exec.py:453-465): on the same points, the per-event f of g g > t t~ g,
+    u u~ > t t~ g, u g > t t~ u and u~ g > t t~ u~ together equals the sum of the four
+    single-process integrands' f within 1e-12, at fixed energy and in hadronic mode
+    (each process with its own flavour luminosity)."""
+    print("case integrand subprocess sum hadronic", hadronic, flush=True)
+    procs = ["gg_ttxg", "uux_ttxg", "ug_ttxu", "uxg_ttxux"]
+
+    def pdf(x, q2, pid=21):
+        return _toy_pdf(x, q2) * (1.0 if pid == 21 else (0.6 if pid > 0 else 0.25))
+    kw = dict(precision=64, seed=6, pt_min=30.0, abs_eta_max=2.5)
+    if hadronic:
+        kw.update(mapping="hadronic", pdf=pdf, scale=173.0)
+    else:
+        kw.update(mapping="rambo")
+    sqrts = 13000.0 if hadronic else 2000.0
+    n = 20000
+    ig = MEIntegrand(procs, sqrts, **kw)
+    _, npass, f = ig.chunk(0, n, 3, keep_f=True)
+    f = f.cpu().numpy().copy()
+    tot = np.zeros(n)
+    for q in procs:
+        _, _, fq = MEIntegrand(q, sqrts, **kw).chunk(0, n, 3, keep_f=True)
+        fq = fq.cpu().numpy()
+        assert np.any(fq > 0), q
+        tot += fq
+    print("passing", npass, "sum of f", f.sum(), flush=True)
+    np.testing.assert_allclose(f, tot, rtol=1e-12, atol=0.0)
+    with pytest.raises(ValueError):
+        MEIntegrand(["gg_ttxg", "gg_ttxgg"], sqrts, **kw)

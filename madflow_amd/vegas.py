@@ -120,6 +120,24 @@ def initial_flavours(matrix, explicit=False):
     return f1, f2
 
 
+def unweight(w, wmax, seed, iteration=0, ev0=0):
+    """Hit-or-miss unweighting on the device (mfx_unweight): event e of the weights w
+    (device f64) is kept with probability min(1, w_e / wmax), from its own Philox uniform
+    (key seed, counter ev0 + e, iteration).  Returns (indices of the kept events in order,
+    keep mask)."""
+    lib = _lib.load()
+    _sig(lib)
+    w = _lib.require_device_tensor(w, torch.float64, "w")
+    n = w.shape[0]
+    keep = torch.empty(n, dtype=torch.uint8, device=w.device)
+    idx = torch.empty(n, dtype=torch.int32, device=w.device)
+    ws = torch.empty(lib.mfx_unweight_workspace_size(n), dtype=torch.uint8, device=w.device)
+    nk = ctypes.c_int64(0)
+    _lib.check(lib.mfx_unweight(_ptr(w), n, float(wmax), int(seed) & 0xFFFFFFFF, iteration, ev0, _ptr(keep), _ptr(idx),
+                                _ptr(ws), ws.numel(), ctypes.byref(nk), _lib.stream_ptr(w.device)))
+    return idx[:nk.value], keep
+
+
 class MEIntegrand:
     """sigma = int dPhi |M|^2 / (2 s) * GeV^-2->pb for one process at fixed sqrt(s)
     (mapping "rambo" or "diet"), or the hadronic
@@ -286,15 +304,7 @@ class MEIntegrand:
     def unweight(self, w, wmax, iteration, ev0=0):
         """Hit-or-miss unweighting of the weights w (device f64) against wmax on the
         device: returns (indices of the kept events in order, keep mask)."""
-        w = _lib.require_device_tensor(w, torch.float64, "w")
-        n = w.shape[0]
-        keep = torch.empty(n, dtype=torch.uint8, device=w.device)
-        idx = torch.empty(n, dtype=torch.int32, device=w.device)
-        ws = torch.empty(self.lib.mfx_unweight_workspace_size(n), dtype=torch.uint8, device=w.device)
-        nk = ctypes.c_int64(0)
-        _lib.check(self.lib.mfx_unweight(_ptr(w), n, float(wmax), self.p.seed, iteration, ev0, _ptr(keep), _ptr(idx),
-                                         _ptr(ws), ws.numel(), ctypes.byref(nk), _lib.stream_ptr(w.device)))
-        return idx[:nk.value], keep
+        return unweight(w, wmax, self.p.seed, iteration, ev0)
 
     def unweighted_events(self, n_events, iteration, wmax, flow=None, mix=0.2, seed=1234):
         """Generate n_events points on the frozen grid (from the flow's defensive mixture

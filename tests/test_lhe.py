@@ -69,3 +69,78 @@ def test_every_generated_process_has_pdg_codes():
     code = {"gluon": 21, "top": 6, "atop": -6, "quark": 2, "antiquark": -2, "quark_out": 2, "antiquark_out": -2}
     for d in descriptions():
         assert PDG[d["name"]] == [code[k] for k in d["kinds"]], d["name"]
+
+
+def _writer_batches(writer, name, nev, seed):
+    """Two batches of weighted events handed to lhe_parser as the integrand does."""
+    ps, _ = phase_space_points(name, nev, seed=seed)
+    w = np.random.default_rng(seed).random(nev) ** 3
+    half = nev // 2
+    assert writer.lhe_parser(ps[:half], w[:half]) == 0.0
+    assert writer.lhe_parser(ps[half:], w[half:]) == 0.0
+    return ps, w
+
+
+def test_lhe_writer_weighted_events(tmp_path):
+    """LheWriter(no_unweight=True): the weighted events of every lhe_parser call reach
+    Events/<run>/weighted_events.lhe.gz in order, each with its own weight (IDWTUP 4), and
+    the stored result heads the file; dump_result writes (sigma, err)."""
+    from madflow_amd.lhe import LheWriter
+    with LheWriter(tmp_path, run="run_07", no_unweight=True, process="gg_ttxg", sqrts=2000.0) as wr:
+        ps, w = _writer_batches(wr, "gg_ttxg", 40, seed=5)
+        wr.store_result((3.25, 0.125))
+    assert wr.cross == 3.25 and wr.err == 0.125 and wr.stats is None
+    path = tmp_path / "Events" / "run_07" / "weighted_events.lhe.gz"
+    assert path == wr.lhe_path and path.exists() and not wr.unweighted_path.exists()
+    init, events = read_lhe(path)
+    assert init[0][8] == 4.0 and init[1] == [3.25, 0.125, 3.25, 1.0]
+    assert len(events) == 40
+    np.testing.assert_allclose([ev[0][2] for ev in events], w, rtol=1e-10)
+    got = np.array([[[p[5], p[2], p[3], p[4]] for p in parts] for _, parts in events])
+    np.testing.assert_allclose(got, ps, rtol=1e-11, atol=1e-9)
+    wr.dump_result(tmp_path / "result.txt")
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "result.txt"), [3.25, 0.125])
+    with pytest.raises(ValueError):
+        LheWriter(tmp_path, process="pp_ttx", sqrts=2000.0)
+    with pytest.raises(ValueError):
+        LheWriter(tmp_path, process="gg_ttx", no_unweight=True)
+    with pytest.raises(ValueError):
+        with LheWriter(tmp_path, process="gg_ttx", sqrts=1000.0, no_unweight=True) as wr:
+            wr.lhe_parser(np.zeros((3, 4, 4)), np.zeros(2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("target", [0, 3000, 9000])
+def test_lhe_writer_unweights_on_the_device(tmp_path, dev, target):
+    """On leaving the block the weighted events are unweighted on the device: the kept
+    count is within 5 sigma of the hit-or-miss expectation sum min(1, w / wmax), where wmax
+    is the largest weight or (event_target) the weight whose expectation is the target;
+    every kept event is one of the weighted events and carries the cross section."""
+    from madflow_amd.lhe import LheWriter
+    nev = 20000
+    print("case LheWriter unweighting target=%d" % target, flush=True)
+    with LheWriter(tmp_path, event_target=target, process="gg_ttx", sqrts=1000.0, seed=11) as wr:
+        ps, w = _writer_batches(wr, "gg_ttx", nev, seed=9)
+        wr.store_result((7.5, 0.5))
+    kept, total = wr.stats
+    assert total == nev
+    init, events = read_lhe(wr.unweighted_path)
+    assert len(events) == kept and init[1][0] == 7.5
+    if target:
+        at_max = float((w / w.max()).sum())
+        # 3000 is below what the largest weight keeps (about 5000): wmax rises above it and
+        # every event is unweighted; 9000 lowers wmax and leaves over-weight events
+        assert init[0][8] == (4.0 if target > at_max else 3.0)
+    else:
+        expect = float((w / w.max()).sum())
+        assert init[0][8] == 3.0 and all(ev[0][2] == 7.5 for ev in events)
+        target = expect
+    assert abs(kept - target) < 5 * np.sqrt(target)
+    assert all(ev[0][2] >= 7.5 for ev in events)
+    order = np.argsort(ps[:, 2, 3])                # the top's pz (its energy is fixed for 2 -> 2)
+    pz_sorted = ps[order, 2, 3]
+    for _, parts in events:
+        mom = np.array([[p[5], p[2], p[3], p[4]] for p in parts])
+        i = int(np.searchsorted(pz_sorted, mom[2, 3]))
+        near = order[max(0, i - 2):i + 2]
+        assert any(np.allclose(ps[j], mom, rtol=1e-11, atol=1e-9) for j in near)

@@ -691,15 +691,22 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_compact(int nexternal, int64_t ne
     }
   }
   __syncthreads();
-  if (e >= nev) return;
-  if (ok) {
-    const int rank = offsets[blockIdx.x] + wave_base[wave] + __popcll(bal & ((1ull << lane) - 1ull));
-    slot[e] = rank;
-    const double* src = all_ps + e * (nexternal * 4);
-    double* dst = dense + (int64_t)rank * (nexternal * 4);
-    for (int k = 0; k < nexternal * 4; ++k) dst[k] = src[k];
-  } else {
-    slot[e] = -1;
+  // The passing events of a wave land on consecutive ranks, so the wave copies their rows
+  // together: lane l moves doubles l, l + 64, ... of the wave's dense range, which makes the
+  // stores contiguous and the loads contiguous within each source row.
+  __shared__ int src_lane[GEN_BLOCK];
+  const int r = __popcll(bal & ((1ull << lane) - 1ull));
+  const int64_t rank0 = (int64_t)offsets[blockIdx.x] + wave_base[wave];
+  if (e < nev) slot[e] = ok ? (int)(rank0 + r) : -1;
+  if (ok) src_lane[wave * 64 + r] = lane;
+  __syncthreads();
+  const int W = nexternal * 4, cnt = __popcll(bal);
+  const int64_t e0 = (int64_t)blockIdx.x * GEN_BLOCK + wave * 64;
+  const double* src = all_ps + e0 * W;
+  double* dst = dense + rank0 * W;
+  for (int j = lane; j < cnt * W; j += 64) {
+    const int row = j / W;
+    dst[j] = src[(int64_t)src_lane[wave * 64 + row] * W + (j - row * W)];
   }
 }
 

@@ -231,7 +231,9 @@ class MultiChannelIntegrand(MEIntegrand):
             e1.record()
             self.me_timer.append((e0, e1, npass))
             return me
-        return self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:0]
+        # no passing event: every slot is -1 and the matrix elements are not read, but the
+        # library needs a device pointer, which an empty tensor does not have
+        return self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:1]
 
     def chunk_channel(self, c, ev0, n, iteration, scale, keep_f=False, y_in=None, w_in=None):
         """One chunk of channel c: (accumulated result vector, npass, moments (nch,), f).

@@ -289,7 +289,9 @@ class MEIntegrand:
             e1.record()
             self.me_timer.append((e0, e1, npass))
         else:
-            me = self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:0]
+            # with no passing event every slot is -1 and nothing reads the matrix elements, but
+            # the library still needs a device pointer, which an empty tensor does not have
+            me = self.matrix.smatrix(b["dense"][:npass], *params) if npass else b["wps"][:1]
         if self.hadronic:
             # f = luminosity x |M|^2 x weight (madflow_exec.py:455-459), scattered to full length
             f_in = torch.zeros(n, dtype=torch.float64, device=self.device)
@@ -363,7 +365,7 @@ class MEIntegrand:
     def _chunk_sum(self, b, ev0, n, iteration, keep_f, y_in, npass, s):
         """The rest of chunk() for several processes: f = sum over them of luminosity x
         |M|^2 (1 at fixed energy) at each passing point, times its weight."""
-        me = torch.zeros(npass, dtype=torch.float64, device=self.device)
+        me = torch.zeros(max(npass, 1), dtype=torch.float64, device=self.device)   # a pointer also for npass 0
         if npass:
             if self.me_timer is not None:
                 e0 = torch.cuda.Event(enable_timing=True)

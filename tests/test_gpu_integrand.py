@@ -76,6 +76,31 @@ def test_generate_parity(dev, name, kw, mapping):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 63, 65, 4173])
+def test_compaction_ragged_and_sparse(dev, n):
+    """Compaction at sizes that end inside a wave and a block, under cuts hard enough that
+    many waves hold few or no passing events: the flags equal the oracle's, the passing
+    events get slots 0, 1, ... in event order, the others -1, and the dense rows are the
+    passing rows of all_ps, bit for bit."""
+    print("case compaction n=%d" % n, flush=True)
+    ig = MEIntegrand("gg_ttxgg", precision=64, mapping="diet", seed=9, sqrts=3000.0, pt_min=400.0, dr_min=0.4)
+    edges = _warp_grid(ig, np.random.default_rng(2))
+    ev0, it = 777, 2
+    ig._alloc(n)
+    ig.chunk(ev0, n, it)
+    b = ig._bufs
+    flag = b["flag"][:n].cpu().numpy()
+    assert np.array_equal(flag, O.generate(_cfg(ig), edges, ev0, n, it)[2])
+    idx = np.nonzero(flag == 1)[0]
+    if n == 4173:
+        assert 0.05 * n < len(idx) < 0.8 * n
+    slot = b["slot"][:n].cpu().numpy()
+    assert np.array_equal(slot[idx], np.arange(len(idx)))
+    assert np.all(slot[flag != 1] == -1)
+    np.testing.assert_array_equal(b["dense"][:len(idx)].cpu().numpy(), b["all_ps"][:n].cpu().numpy()[idx])
+
+
+@pytest.mark.gpu
 def test_accumulate_and_refine(dev):
     print("case accumulate", flush=True)
     ig = MEIntegrand("gg_ttxgg", precision=32, mapping="diet", seed=9, sqrts=3000.0, pt_min=30.0, dr_min=0.4)

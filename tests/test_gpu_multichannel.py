@@ -430,3 +430,16 @@ def test_mc_register_cut(dev):
         print("channel", c, "pass", (flag0 == 1).mean(), "with the top cut", mask.mean(), flush=True)
         assert 0.05 < mask.mean() < (flag0 == 1).mean()
         assert np.array_equal(flag == 1, mask)
+
+
+@pytest.mark.gpu
+def test_mc_chunk_with_no_passing_event(dev):
+    """A chunk of one point that fails a hard cut (pT(g) > 1400 GeV at 3 TeV leaves almost no
+    phase space): every channel returns npass 0, a zero result vector and zero moments."""
+    from madflow_amd.multichannel import MultiChannelIntegrand
+    print("case mc chunk with no passing event", flush=True)
+    ig = MultiChannelIntegrand("gg_ttxgg", SQRTS, precision=32, seed=5, pt_min=1400.0, dr_min=0.4)
+    for c in range(ig.nch):
+        res, npass, mom, _ = ig.chunk_channel(c, 0, 1, 0, 1.0)
+        assert npass == 0
+        assert np.all(res == 0) and np.all(mom == 0)

@@ -126,6 +126,27 @@ def read_lhe(path):
     return init, events
 
 
+def wmax_for_target(w, event_target=0):
+    """The maximum weight that hit-or-miss unweighting of the positive weights w is run
+    against: the largest weight, or for 0 < event_target < len(w) the w_max at which the
+    expected number of kept events, sum min(1, w / w_max), equals event_target.  That sum
+    falls as w_max grows; above the largest weight it is sum(w) / w_max."""
+    pos = np.asarray(w, dtype=np.float64)
+    wmax = float(pos.max())
+    if 0 < event_target <= pos.sum() / wmax:
+        return float(pos.sum()) / event_target
+    if 0 < event_target < len(pos):
+        lo, hi = float(pos.min()), wmax
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if np.minimum(1.0, pos / mid).sum() > event_target:
+                lo = mid
+            else:
+                hi = mid
+        return hi
+    return wmax
+
+
 class LheWriter:
     """madflow.lhe_writer.LheWriter (lhe_writer.py:94-356): a context manager around an
     integration.  The integrand hands each batch of weighted events to lhe_parser(all_ps,
@@ -205,18 +226,7 @@ class LheWriter:
         if not len(pos):
             write_lhe(self.unweighted_path, self.process, ps[:0], self.cross, self.err or 0.0, self.sqrts)
             return 0, len(w)
-        wmax = float(pos.max())
-        if 0 < event_target <= pos.sum() / wmax:
-            wmax = float(pos.sum()) / event_target   # above the largest weight: sum(w) / wmax kept
-        elif 0 < event_target < len(pos):
-            lo, hi = float(pos.min()), wmax      # expected kept count sum(min(1, w / wmax)) falls with wmax
-            for _ in range(200):
-                mid = 0.5 * (lo + hi)
-                if np.minimum(1.0, pos / mid).sum() > event_target:
-                    lo = mid
-                else:
-                    hi = mid
-            wmax = hi
+        wmax = wmax_for_target(pos, event_target)
         wt = torch.from_numpy(w).to("cuda")
         idx, _ = unweight(wt, wmax, self.seed)
         k = idx.long().cpu().numpy()

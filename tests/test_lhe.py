@@ -144,3 +144,18 @@ def test_lhe_writer_unweights_on_the_device(tmp_path, dev, target):
         i = int(np.searchsorted(pz_sorted, mom[2, 3]))
         near = order[max(0, i - 2):i + 2]
         assert any(np.allclose(ps[j], mom, rtol=1e-11, atol=1e-9) for j in near)
+
+
+def test_wmax_for_target_hand_solved():
+    """w = (1, 2, 3, 4): the largest weight keeps 1/4 + 2/4 + 3/4 + 1 = 2.5 events on
+    average.  A target of 2 needs w_max above 4, where the count is 10 / w_max: w_max = 5.
+    A target of 3 puts w_max in [3, 4], where it is 1/w + 2/w + 1 + 1 = 3 at w_max = 3; a
+    target of 3.5 gives 3/w + 2 = 3.5 in [2, 3], w_max = 2.  No target (or one of at least the
+    number of weights) keeps the largest weight."""
+    from madflow_amd.lhe import wmax_for_target
+    w = np.array([1.0, 2.0, 3.0, 4.0])
+    assert wmax_for_target(w) == 4.0 and wmax_for_target(w, 4) == 4.0 and wmax_for_target(w, 9) == 4.0
+    assert wmax_for_target(w, 2) == pytest.approx(5.0, rel=1e-14)
+    assert wmax_for_target(w, 2.5) == pytest.approx(4.0, rel=1e-14)
+    assert wmax_for_target(w, 3) == pytest.approx(3.0, rel=1e-12)
+    assert wmax_for_target(w, 3.5) == pytest.approx(2.0, rel=1e-12)
